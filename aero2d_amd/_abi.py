@@ -1,0 +1,190 @@
+"""ctypes binding of libaero2d_hip.so (the C ABI in include/aero2d_hip.h).
+
+The library is built in-tree by __graft_entry__.build() (make in
+aero2d_amd/csrc). There is no fallback: when the library is missing or no HIP
+device is present, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libaero2d_hip.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64 = ctypes.c_int64
+
+
+class Knobs(ctypes.Structure):
+    """a2d_knobs (include/aero2d_hip.h)."""
+    _fields_ = [("cfl_explicit", ctypes.c_double), ("lim_k", ctypes.c_double),
+                ("max_update", ctypes.c_double), ("gmres_tol", ctypes.c_double),
+                ("vortex", ctypes.c_int), ("order", ctypes.c_int),
+                ("gmres_restart", ctypes.c_int), ("jacobi_sweeps", ctypes.c_int),
+                ("early_exit", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+# name -> (restype, argtypes); the exported surface of include/aero2d_hip.h
+SIGNATURES = {
+    "a2d_init": (ctypes.c_int, [ctypes.c_int]),
+    "a2d_last_error": (ctypes.c_char_p, []),
+    "a2d_version": (ctypes.c_int, []),
+    "a2d_default_knobs": (None, [ctypes.POINTER(Knobs)]),
+    "a2d_pool_create": (ctypes.c_int, [_i64, ctypes.POINTER(_i64)]),
+    "a2d_pool_destroy": (ctypes.c_int, [_i64]),
+    "a2d_pool_set_knobs": (ctypes.c_int, [_i64, ctypes.POINTER(Knobs)]),
+    "a2d_pool_get_knobs": (ctypes.c_int, [_i64, ctypes.POINTER(Knobs)]),
+    "a2d_pool_used_bytes": (_i64, [_i64]),
+    "a2d_case_bytes": (_i64, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "a2d_case_add": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, _ip]),
+    "a2d_case_remove": (ctypes.c_int, [_i64, ctypes.c_int]),
+    "a2d_case_set_freestream": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "a2d_case_set_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
+    "a2d_case_get_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
+    "a2d_case_set_shape": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int]),
+    "a2d_stencil_shapes": (ctypes.c_int, [_ip, _ip, ctypes.c_int]),
+    "a2d_residual": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double, _dp, _dp]),
+    "a2d_forces": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp, _dp]),
+    "a2d_local_dt": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
+    "a2d_jvp": (ctypes.c_int, [_i64, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _dp]),
+    "a2d_assemble": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]),
+    "a2d_precond_setup": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double]),
+    "a2d_precond_apply": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp]),
+    "a2d_newton_step": (ctypes.c_int, [_i64, _ip, ctypes.c_int, _dp, _dp]),
+    "a2d_hllc_flux": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
+}
+
+_lib = None
+_init_lock = threading.Lock()
+_device = None
+
+
+class HipError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, name, code, msg):
+        super().__init__(f"{name} failed ({code}): {msg}")
+        self.code = code
+
+
+class BudgetExceeded(HipError):
+    """a2d_case_add refused a case: the pool's HBM budget would be exceeded."""
+
+
+def load():
+    """Load the shared library (no device work). Raises when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def lib(device: int | None = None):
+    """The library, initialised on a HIP device (default: the current one)."""
+    global _device
+    L = load()
+    with _init_lock:
+        if _device is None:
+            dev = 0 if device is None else device
+            call("a2d_init", dev)
+            _device = dev
+    return L
+
+
+def call(name, *args):
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        msg = L.a2d_last_error().decode(errors="replace")
+        cls = BudgetExceeded if (name == "a2d_case_add" and rc == -2) else HipError
+        raise cls(name, rc, msg)
+    return rc
+
+
+def dptr(a: np.ndarray):
+    """ctypes double* of a C-contiguous float64 array."""
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def default_knobs() -> Knobs:
+    k = Knobs()
+    load().a2d_default_knobs(ctypes.byref(k))
+    return k
+
+
+def case_bytes(ni: int, nc: int, precision: int, restart: int = 50) -> int:
+    return int(load().a2d_case_bytes(ni, nc, precision, restart))
+
+
+def stencil_shapes():
+    ti = (ctypes.c_int * 16)()
+    tj = (ctypes.c_int * 16)()
+    n = load().a2d_stencil_shapes(ti, tj, 16)
+    return [(ti[k], tj[k]) for k in range(n)]
+
+
+class Pool:
+    """Owner of a HIP stream and device-resident cases (a2d_pool_*)."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, knobs: Knobs | None = None):
+        lib()
+        h = _i64(0)
+        call("a2d_pool_create", int(hbm_budget_bytes), ctypes.byref(h))
+        self.handle = h.value
+        if knobs is not None:
+            self.set_knobs(knobs)
+
+    def set_knobs(self, k: Knobs):
+        call("a2d_pool_set_knobs", self.handle, ctypes.byref(k))
+
+    def knobs(self) -> Knobs:
+        k = Knobs()
+        call("a2d_pool_get_knobs", self.handle, ctypes.byref(k))
+        return k
+
+    def used_bytes(self) -> int:
+        return int(load().a2d_pool_used_bytes(self.handle))
+
+    def add_case(self, x: np.ndarray, y: np.ndarray, mach: float, alpha: float, precision: int) -> int:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        ni, nj = x.shape
+        cid = ctypes.c_int(-1)
+        call("a2d_case_add", self.handle, ni, nj - 1, dptr(x), dptr(y), float(mach), float(alpha),
+             int(precision), ctypes.byref(cid))
+        return cid.value
+
+    def remove_case(self, cid: int):
+        call("a2d_case_remove", self.handle, cid)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            call("a2d_pool_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,150 @@
+// Device-side case descriptors, batch plans and small helpers shared by every
+// kernel of the implicit Euler path.
+//
+// HBM layout of one case (ni cells around, nc cells wall-normal, n = ni*nc):
+// every field is structure-of-arrays with i (around the contour) fastest,
+// [var][j][i]. A wavefront then walks 64 neighbouring i-lines in lock step,
+// which is what the j-marching line solves need for coalesced loads.
+//   U, R, dU              double [4][nc][ni]
+//   vol                   double [nc][ni]
+//   Si                    double [2][nc][ni]     face between cells i-1 and i
+//   Sj                    double [2][nc+1][ni]   face j; j = 0 wall, j = nc far
+//   xf, yf, xw, yw        double [ni]            outer / wall face midpoints
+//   Ut, volt, Sit, Sjt... T      copies in the inner precision (alias the
+//                                double arrays when T = double)
+//   blocks                T [5][16][nc][ni]       inv, c, A(=Bjm), Bim, Bip
+//   V                     T [K+1][4][nc][ni]      raw Krylov basis
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace a2d {
+
+constexpr int KMAX = 64;          // largest Krylov dimension a case may use
+constexpr int WAVE = 64;
+
+// Per-case scalars kept in HBM next to the case's arrays. Everything the
+// Newton step decides per case lives here, so a whole step can be launched
+// (or replayed from a graph) without host round trips.
+struct CaseScal {
+  double winf[4];      // rho, u, v, p of the freestream
+  double cfl;          // pseudo-time CFL for this Newton step
+  double circ;         // Gamma = Cl |V_inf| / 2 at the step's base state
+  double cl, cd, cm;
+  double dcirc;        // tangent of Gamma for the vector in flight
+  double rn;           // sqrt(mean((R_rho / vol)^2))
+  double rel;          // max relative change of rho, p under the full step
+  double omega;        // step limiter
+  double beta;         // |M(b - A x0)|
+  double mbnorm;       // |M b|
+  double bnorm;        // |b|
+  double gres;         // GMRES residual estimate of the current iterate
+  double vn0;          // |w| before orthogonalisation
+  int run;             // the Krylov cycle runs (|M(b - A x0)| > ptol)
+  int active;          // still taking Arnoldi steps
+  int kdone;           // Arnoldi steps completed
+  int accept;          // the Newton update was applied
+  int nonfinite;       // count of non-finite entries in the update
+  int frozen;          // host says: do not update this case
+  int pad[2];
+  double s[KMAX + 1];          // scale of raw basis vector j (V_true = s_j V_raw)
+  double H[KMAX + 1][KMAX];    // Hessenberg, H[row][col]
+  double gc[KMAX], gs[KMAX];   // Givens rotations
+  double g[KMAX + 1];          // rotated right-hand side
+  double y[KMAX];              // least-squares solution
+};
+
+struct CaseDev {
+  int ni, nc, n;        // n = ni * nc
+  int prec;             // 0 = fp64 inner, 1 = fp32 inner
+  int kmax;             // Krylov dimension allocated
+  int npart;            // partial-sum slots available per reduction
+  int ntiles;           // stencil tiles of this case's tile shape
+  int nvecb;            // vector-kernel blocks (VEC_CHUNK cells each)
+  int nlineb;           // line-kernel blocks (64 lines each)
+  int shape;            // stencil tile shape index
+  double* U;
+  double* R;
+  double* dU;
+  const double* vol;
+  const double* Si;
+  const double* Sj;
+  const double* xf;
+  const double* yf;
+  const double* xw;
+  const double* yw;
+  void* Ut;             // inner-precision copies (T*)
+  void* volt;
+  void* Sit;
+  void* Sjt;
+  void* xft;
+  void* yft;
+  void* diag;           // T [nc][ni]: vol / dtau
+  void* blk;            // T [5][16][nc][ni]
+  void* V;              // T [kmax+1][4][nc][ni]
+  void* x0;             // T [4][nc][ni]
+  void* w;              // T [4][nc][ni]
+  void* d;              // T [4][nc][ni] line-solve forward sweep
+  void* xa;             // T [4][nc][ni] sweep ping-pong
+  void* xb;
+  double* part;         // [npart][kmax + 2] reduction partials
+  CaseScal* sc;
+};
+
+// A batch of cases and the block ranges of one kernel family over it.
+// blk_prefix[c] is the first block of batch entry c; blk_prefix[nb] the total.
+struct Plan {
+  const CaseDev* cases;   // device array indexed by slot
+  const int* slots;       // nb slot indices
+  const int* blk_prefix;  // nb + 1
+  int nb;
+};
+
+// Map a flat block index to (batch entry, local block) by binary search over
+// the prefix array; every lane of the block computes the same answer.
+__device__ __forceinline__ int plan_entry(const Plan& p, int b) {
+  int lo = 0, hi = p.nb;  // invariant: prefix[lo] <= b < prefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (p.blk_prefix[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <class T> __device__ __forceinline__ T* tp(void* p) { return reinterpret_cast<T*>(p); }
+template <class T> __device__ __forceinline__ const T* ctp(const void* p) { return reinterpret_cast<const T*>(p); }
+
+// Wavefront sum with cross-lane exchanges (fixed order, deterministic).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, WAVE));
+  return x;
+}
+
+// Named per-case vectors (ids >= 0 select raw Krylov basis vector V_id).
+enum VecId : int { VEC_X0 = -1, VEC_W = -2, VEC_D = -3, VEC_XA = -4, VEC_XB = -5, VEC_NEG_R = -6 };
+
+template <class T>
+__device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
+  switch (id) {
+    case VEC_X0: return tp<T>(cd.x0);
+    case VEC_W: return tp<T>(cd.w);
+    case VEC_D: return tp<T>(cd.d);
+    case VEC_XA: return tp<T>(cd.xa);
+    case VEC_XB: return tp<T>(cd.xb);
+    default: return tp<T>(cd.V) + (size_t)id * 4 * cd.n;
+  }
+}
+
+// Positive modulo for the periodic i direction.
+__device__ __forceinline__ int wrap(int i, int ni) {
+  i %= ni;
+  return i < 0 ? i + ni : i;
+}
+
+}  // namespace a2d

@@ -1,0 +1,907 @@
+// Host engine of the implicit Euler library: pools of device-resident cases,
+// batch plans, the launch sequence of one Newton step (optionally captured
+// into a HIP graph and replayed), and the C ABI declared in
+// include/aero2d_hip.h.
+#include "../../include/aero2d_hip.h"
+#include "common.hpp"
+#include "launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace a2d {
+namespace {
+
+thread_local std::string g_err;
+int g_device = -1;
+std::recursive_mutex g_mu;
+
+struct Fail {
+  int code;
+};
+
+void fail(int code, const std::string& msg) {
+  g_err = msg;
+  throw Fail{code};
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(-10, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HC(x) hip_check((x), #x)
+
+constexpr int MAX_SLOTS = 4096;
+constexpr size_t ALIGN = 256;
+
+size_t align_up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
+int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+KnobsDev to_dev(const a2d_knobs& k) {
+  KnobsDev d;
+  d.cfl_explicit = k.cfl_explicit;
+  d.lim_k = k.lim_k;
+  d.max_update = k.max_update;
+  d.gtol = k.gmres_tol;
+  d.vortex = k.vortex;
+  d.order = k.order;
+  d.early_exit = k.early_exit;
+  d.restart = k.gmres_restart;
+  return d;
+}
+
+// Byte layout of one case's slab; offsets of every sub-buffer.
+struct Layout {
+  size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
+  size_t Ut, volt, Sit, Sjt, xft, yft;
+  size_t diag, blk, V, x0, w, d, xa, xb, part, sc;
+  size_t total;
+  int npart;
+};
+
+int max_tiles(int ni, int nc) {
+  int m = 0;
+  for (int s = 0; s < stencil_num_shapes(); ++s) {
+    StencilShape sh = stencil_shape(s);
+    m = std::max(m, cdiv(ni, sh.ti) * cdiv(nc, sh.tj));
+  }
+  return m;
+}
+
+Layout layout(int ni, int nc, int prec, int kmax) {
+  Layout L{};
+  const size_t n = size_t(ni) * nc;
+  const size_t t = prec == 0 ? 8 : 4;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = o;
+    o = align_up(o + bytes);
+    return at;
+  };
+  L.U = take(4 * n * 8);
+  L.R = take(4 * n * 8);
+  L.dU = take(4 * n * 8);
+  L.vol = take(n * 8);
+  L.Si = take(2 * n * 8);
+  L.Sj = take(2 * (n + ni) * 8);
+  L.xf = take(ni * 8);
+  L.yf = take(ni * 8);
+  L.xw = take(ni * 8);
+  L.yw = take(ni * 8);
+  if (prec == 1) {
+    L.Ut = take(4 * n * 4);
+    L.volt = take(n * 4);
+    L.Sit = take(2 * n * 4);
+    L.Sjt = take(2 * (n + ni) * 4);
+    L.xft = take(ni * 4);
+    L.yft = take(ni * 4);
+  }
+  L.diag = take(n * t);
+  L.blk = take(80 * n * t);
+  L.V = take(size_t(kmax + 1) * 4 * n * t);
+  L.x0 = take(4 * n * t);
+  L.w = take(4 * n * t);
+  L.d = take(4 * n * t);
+  L.xa = take(4 * n * t);
+  L.xb = take(4 * n * t);
+  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), cdiv(ni, LINES_PER_BLOCK), 1});
+  L.part = take(size_t(L.npart) * (kmax + 2) * 8);
+  L.sc = take(sizeof(CaseScal));
+  L.total = o;
+  return L;
+}
+
+struct CaseHost {
+  bool live = false;
+  int ni = 0, nc = 0, prec = 0, kmax = 0;
+  char* slab = nullptr;
+  size_t bytes = 0;
+  CaseDev dev{};
+  double winf[4];
+};
+
+struct BatchPlan {
+  std::vector<int> slots;
+  int prec = 0, nb = 0;
+  int* dbuf = nullptr;
+  double* dcfl = nullptr;
+  double* dout = nullptr;
+  double* hcfl = nullptr;
+  double* hout = nullptr;
+  Plan pc{}, pcell{}, pvec{}, pline{};
+  int ncell = 0, nvec = 0, nline = 0;
+  struct Group {
+    Plan p;
+    int shape, nblocks;
+  };
+  std::vector<Group> sten;
+  hipGraphExec_t exec = nullptr;
+  int uses = 0;
+  uint64_t knob_version = 0;
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (dbuf) (void)hipFree(dbuf);
+    if (dcfl) (void)hipFree(dcfl);
+    if (dout) (void)hipFree(dout);
+    if (hcfl) (void)hipHostFree(hcfl);
+    if (hout) (void)hipHostFree(hout);
+  }
+};
+
+struct Pool {
+  int64_t id = 0;
+  hipStream_t stream = nullptr;
+  int64_t budget = 0, used = 0;
+  std::vector<CaseHost> cases;
+  CaseDev* dcases = nullptr;
+  a2d_knobs knobs{};
+  uint64_t knob_version = 1;
+  std::map<std::vector<int>, std::unique_ptr<BatchPlan>> plans;
+  std::vector<double> stage_d;
+  std::vector<float> stage_f;
+};
+
+std::map<int64_t, std::unique_ptr<Pool>> g_pools;
+int64_t g_next_pool = 1;
+
+void default_knobs(a2d_knobs* k) {
+  k->cfl_explicit = 2.5;
+  k->lim_k = 0.02;
+  k->max_update = 0.2;
+  k->gmres_tol = 0.01;
+  k->vortex = 1;
+  k->order = 2;
+  k->gmres_restart = 50;
+  k->jacobi_sweeps = 4;
+  k->early_exit = 0;
+  k->use_graph = 1;
+}
+
+Pool& get_pool(int64_t id) {
+  auto it = g_pools.find(id);
+  if (it == g_pools.end()) fail(-3, "unknown pool handle");
+  return *it->second;
+}
+
+CaseHost& get_case(Pool& p, int id) {
+  if (id < 0 || id >= int(p.cases.size()) || !p.cases[id].live) fail(-4, "unknown case id");
+  return p.cases[id];
+}
+
+void ensure_init() {
+  if (g_device < 0) fail(-1, "a2d_init has not been called");
+  HC(hipSetDevice(g_device));
+}
+
+// ---- plans --------------------------------------------------------------------
+void drop_plans_with(Pool& p, int slot) {
+  for (auto it = p.plans.begin(); it != p.plans.end();) {
+    if (std::find(it->first.begin(), it->first.end(), slot) != it->first.end()) it = p.plans.erase(it);
+    else ++it;
+  }
+}
+
+BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
+  auto it = p.plans.find(slots);
+  if (it != p.plans.end()) return *it->second;
+  if (p.plans.size() >= 64) p.plans.clear();
+  auto bp = std::make_unique<BatchPlan>();
+  bp->slots = slots;
+  bp->nb = int(slots.size());
+  bp->prec = p.cases[slots[0]].prec;
+  for (int s : slots)
+    if (p.cases[s].prec != bp->prec) fail(-5, "a batch must hold cases of one precision");
+  const int nb = bp->nb;
+  // host image: slots | pc | pcell | pvec | pline | per group: slots_g | prefix_g
+  std::vector<int> h;
+  h.insert(h.end(), slots.begin(), slots.end());
+  auto add_prefix = [&](auto count) {
+    size_t at = h.size();
+    int acc = 0;
+    for (int e = 0; e < nb; ++e) {
+      h.push_back(acc);
+      acc += count(p.cases[slots[e]]);
+    }
+    h.push_back(acc);
+    return std::make_pair(at, acc);
+  };
+  auto pcase = add_prefix([](const CaseHost&) { return 1; });
+  auto pcell = add_prefix([](const CaseHost& c) { return cdiv(c.dev.n, CELL_BLOCK); });
+  auto pvec = add_prefix([](const CaseHost& c) { return c.dev.nvecb; });
+  auto pline = add_prefix([](const CaseHost& c) { return c.dev.nlineb; });
+  std::map<int, std::vector<int>> groups;
+  for (int s : slots) groups[p.cases[s].dev.shape].push_back(s);
+  struct G {
+    size_t slots_at, prefix_at;
+    int nb, shape, total;
+  };
+  std::vector<G> gs;
+  for (auto& kv : groups) {
+    G g;
+    g.shape = kv.first;
+    g.nb = int(kv.second.size());
+    g.slots_at = h.size();
+    h.insert(h.end(), kv.second.begin(), kv.second.end());
+    g.prefix_at = h.size();
+    int acc = 0;
+    for (int s : kv.second) {
+      h.push_back(acc);
+      acc += p.cases[s].dev.ntiles;
+    }
+    h.push_back(acc);
+    g.total = acc;
+    gs.push_back(g);
+  }
+  HC(hipMalloc(&bp->dbuf, h.size() * sizeof(int)));
+  HC(hipMemcpy(bp->dbuf, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  HC(hipMalloc(&bp->dcfl, nb * sizeof(double)));
+  HC(hipMalloc(&bp->dout, nb * 10 * sizeof(double)));
+  HC(hipHostMalloc(&bp->hcfl, nb * sizeof(double), hipHostMallocDefault));
+  HC(hipHostMalloc(&bp->hout, nb * 10 * sizeof(double), hipHostMallocDefault));
+  auto mk = [&](size_t prefix_at) {
+    Plan q;
+    q.cases = p.dcases;
+    q.slots = bp->dbuf;
+    q.blk_prefix = bp->dbuf + prefix_at;
+    q.nb = nb;
+    return q;
+  };
+  bp->pc = mk(pcase.first);
+  bp->pcell = mk(pcell.first);
+  bp->ncell = pcell.second;
+  bp->pvec = mk(pvec.first);
+  bp->nvec = pvec.second;
+  bp->pline = mk(pline.first);
+  bp->nline = pline.second;
+  for (auto& g : gs) {
+    BatchPlan::Group gr;
+    gr.p.cases = p.dcases;
+    gr.p.slots = bp->dbuf + g.slots_at;
+    gr.p.blk_prefix = bp->dbuf + g.prefix_at;
+    gr.p.nb = g.nb;
+    gr.shape = g.shape;
+    gr.nblocks = g.total;
+    bp->sten.push_back(gr);
+  }
+  BatchPlan& ref = *bp;
+  p.plans[slots] = std::move(bp);
+  return ref;
+}
+
+// ---- launch sequences ---------------------------------------------------------
+void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int gate) {
+  const int S = std::max(1, p.knobs.jacobi_sweeps);
+  for (int s = 0; s < S; ++s) {
+    const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
+    const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
+    launch_line_sweep(b.pline, b.nline, src, prev, out, s, (s == S - 1) ? want_norm : 0, gate,
+                      b.prec, p.stream);
+  }
+}
+
+// Residual with the local time step at each case's CFL (sc->cfl), rn.
+void enq_prologue(Pool& p, BatchPlan& b, const KnobsDev& kn) {
+  launch_forces(b.pc, nullptr, p.stream);
+  for (auto& g : b.sten)
+    launch_residual(g.p, g.shape, g.nblocks, kn, 1, NAN, 1, p.stream);
+  launch_rn_finalize(b.pc, p.stream);
+}
+
+void enq_newton(Pool& p, BatchPlan& b) {
+  const KnobsDev kn = to_dev(p.knobs);
+  const int prec = b.prec;
+  hipStream_t s = p.stream;
+  launch_set_cfl(b.pc, b.dcfl, s);
+  enq_prologue(p, b, kn);
+  launch_assemble(b.pcell, b.ncell, kn, prec, s);
+  launch_factor(b.pline, b.nline, prec, s);
+  // x0 = M b with b = -R
+  enq_precond(p, b, VEC_NEG_R, VEC_X0, 1, 0);
+  launch_krylov_begin(b.pc, 0, kn, prec, s);
+  // V_0 = M(b - A x0)
+  launch_dcirc(b.pc, VEC_X0, 0, prec, s);
+  for (auto& g : b.sten)
+    launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 1, -1, 1, 0, prec, s);
+  enq_precond(p, b, VEC_W, 0, 1, 0);
+  launch_krylov_begin(b.pc, 1, kn, prec, s);
+  const int K = std::min(p.knobs.gmres_restart, KMAX);
+  for (int k = 0; k < K; ++k) {
+    launch_dcirc(b.pc, k, 1, prec, s);
+    for (auto& g : b.sten) launch_jvp(g.p, g.shape, g.nblocks, kn, k, VEC_W, 0, k, 1, 1, prec, s);
+    enq_precond(p, b, VEC_W, k + 1, 0, 1);
+    launch_dots(b.pvec, b.nvec, k, k + 1, 1, prec, s);
+    launch_reduce_dots(b.pc, k, s);
+    launch_update(b.pvec, b.nvec, k, k + 1, 1, prec, s);
+    launch_normalize(b.pc, k, kn, prec, s);
+  }
+  launch_lstsq(b.pc, s);
+  launch_dupdate(b.pvec, b.nvec, prec, s);
+  launch_limiter(b.pc, kn, s);
+  launch_apply(b.pvec, b.nvec, s);
+  launch_forces(b.pc, nullptr, s);
+  launch_gather(b.pc, b.dout, s);
+}
+
+// ---- host <-> device field transfer -------------------------------------------
+// Reference layout (4, ni, nc) j-fastest <-> device [4][nc][ni] i-fastest.
+void upload_field(Pool& p, const CaseHost& c, const double* src, void* dst, int prec, int nv) {
+  const int ni = c.ni, nc = c.nc;
+  const size_t n = size_t(ni) * nc;
+  if (prec == 0) {
+    p.stage_d.resize(nv * n);
+    for (int v = 0; v < nv; ++v)
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nc; ++j) p.stage_d[v * n + size_t(j) * ni + i] = src[v * n + size_t(i) * nc + j];
+    HC(hipMemcpyAsync(dst, p.stage_d.data(), nv * n * 8, hipMemcpyHostToDevice, p.stream));
+  } else {
+    p.stage_f.resize(nv * n);
+    for (int v = 0; v < nv; ++v)
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nc; ++j)
+          p.stage_f[v * n + size_t(j) * ni + i] = float(src[v * n + size_t(i) * nc + j]);
+    HC(hipMemcpyAsync(dst, p.stage_f.data(), nv * n * 4, hipMemcpyHostToDevice, p.stream));
+  }
+  HC(hipStreamSynchronize(p.stream));
+}
+
+void download_field(Pool& p, const CaseHost& c, const void* src, double* dst, int prec, int nv) {
+  const int ni = c.ni, nc = c.nc;
+  const size_t n = size_t(ni) * nc;
+  if (prec == 0) {
+    p.stage_d.resize(nv * n);
+    HC(hipMemcpyAsync(p.stage_d.data(), src, nv * n * 8, hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    for (int v = 0; v < nv; ++v)
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nc; ++j) dst[v * n + size_t(i) * nc + j] = p.stage_d[v * n + size_t(j) * ni + i];
+  } else {
+    p.stage_f.resize(nv * n);
+    HC(hipMemcpyAsync(p.stage_f.data(), src, nv * n * 4, hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    for (int v = 0; v < nv; ++v)
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nc; ++j) dst[v * n + size_t(i) * nc + j] = p.stage_f[v * n + size_t(j) * ni + i];
+  }
+}
+
+void set_cfl_one(Pool& p, BatchPlan& b, double cfl) {
+  b.hcfl[0] = cfl;
+  HC(hipMemcpyAsync(b.dcfl, b.hcfl, sizeof(double), hipMemcpyHostToDevice, p.stream));
+  launch_set_cfl(b.pc, b.dcfl, p.stream);
+}
+
+template <class F>
+int guarded(F&& f) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  try {
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -99;
+  }
+}
+
+}  // namespace
+}  // namespace a2d
+
+using namespace a2d;
+
+extern "C" {
+
+int a2d_version(void) { return 1; }
+
+const char* a2d_last_error(void) { return g_err.c_str(); }
+
+void a2d_default_knobs(a2d_knobs* k) { default_knobs(k); }
+
+int a2d_init(int device) {
+  return guarded([&] {
+    int count = 0;
+    HC(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) fail(-1, "no such HIP device");
+    HC(hipSetDevice(device));
+    g_device = device;
+  });
+}
+
+int a2d_pool_create(int64_t hbm_budget_bytes, int64_t* pool_out) {
+  return guarded([&] {
+    ensure_init();
+    auto p = std::make_unique<Pool>();
+    p->id = g_next_pool++;
+    p->budget = hbm_budget_bytes;
+    default_knobs(&p->knobs);
+    HC(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    HC(hipMalloc(&p->dcases, MAX_SLOTS * sizeof(CaseDev)));
+    *pool_out = p->id;
+    g_pools[p->id] = std::move(p);
+  });
+}
+
+int a2d_pool_destroy(int64_t pool) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    (void)hipStreamSynchronize(p.stream);
+    p.plans.clear();
+    for (auto& c : p.cases)
+      if (c.live) (void)hipFree(c.slab);
+    (void)hipFree(p.dcases);
+    (void)hipStreamDestroy(p.stream);
+    g_pools.erase(pool);
+  });
+}
+
+int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    if (k->gmres_restart < 0 || k->gmres_restart > KMAX)
+      fail(-6, "gmres_restart must be in [0, " + std::to_string(KMAX) + "]");
+    if (k->jacobi_sweeps < 1) fail(-6, "jacobi_sweeps must be >= 1");
+    if (k->order != 1 && k->order != 2) fail(-6, "order must be 1 or 2");
+    for (auto& c : p.cases)
+      if (c.live && c.kmax < k->gmres_restart)
+        fail(-6, "a live case was allocated for a smaller Krylov dimension");
+    p.knobs = *k;
+    p.knob_version++;
+    for (auto& kv : p.plans)
+      if (kv.second->exec) {
+        (void)hipGraphExecDestroy(kv.second->exec);
+        kv.second->exec = nullptr;
+      }
+  });
+}
+
+int a2d_pool_get_knobs(int64_t pool, a2d_knobs* k) {
+  return guarded([&] { *k = get_pool(pool).knobs; });
+}
+
+int64_t a2d_pool_used_bytes(int64_t pool) {
+  int64_t v = -1;
+  guarded([&] { v = get_pool(pool).used; });
+  return v;
+}
+
+int64_t a2d_case_bytes(int ni, int nc, int precision, int restart) {
+  return int64_t(layout(ni, nc, precision, std::max(restart, 1)).total);
+}
+
+int a2d_stencil_shapes(int* ti, int* tj, int max_shapes) {
+  const int n = stencil_num_shapes();
+  for (int s = 0; s < std::min(n, max_shapes); ++s) {
+    ti[s] = stencil_shape(s).ti;
+    tj[s] = stencil_shape(s).tj;
+  }
+  return n;
+}
+
+int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y, double mach,
+                 double alpha_deg, int precision, int* case_id) {
+  return guarded([&] {
+    ensure_init();
+    Pool& p = get_pool(pool);
+    if (ni < 4 || nc < 2) fail(-6, "grid too small (need ni >= 4, nc >= 2)");
+    if (precision != 0 && precision != 1) fail(-6, "precision must be 0 (fp64) or 1 (mixed)");
+    const int kmax = std::max(1, std::min(p.knobs.gmres_restart, KMAX));
+    const Layout L = layout(ni, nc, precision, kmax);
+    if (p.budget > 0 && p.used + int64_t(L.total) > p.budget)
+      fail(-2, "HBM budget exceeded: case needs " + std::to_string(L.total) + " bytes, " +
+                   std::to_string(p.budget - p.used) + " free");
+    int slot = -1;
+    for (int s = 0; s < int(p.cases.size()); ++s)
+      if (!p.cases[s].live) { slot = s; break; }
+    if (slot < 0) {
+      if (int(p.cases.size()) >= MAX_SLOTS) fail(-7, "pool is full");
+      p.cases.emplace_back();
+      slot = int(p.cases.size()) - 1;
+    }
+    CaseHost& c = p.cases[slot];
+    c = CaseHost{};
+    c.ni = ni;
+    c.nc = nc;
+    c.prec = precision;
+    c.kmax = kmax;
+    c.bytes = L.total;
+    HC(hipMalloc(&c.slab, L.total));
+    c.live = true;
+    p.used += int64_t(L.total);
+    const size_t n = size_t(ni) * nc;
+    char* b = c.slab;
+    CaseDev& d = c.dev;
+    d.ni = ni;
+    d.nc = nc;
+    d.n = int(n);
+    d.prec = precision;
+    d.kmax = kmax;
+    d.npart = L.npart;
+    d.shape = 0;
+    d.ntiles = cdiv(ni, stencil_shape(0).ti) * cdiv(nc, stencil_shape(0).tj);
+    d.nvecb = cdiv(int(n), VEC_CHUNK);
+    d.nlineb = cdiv(ni, LINES_PER_BLOCK);
+    d.U = reinterpret_cast<double*>(b + L.U);
+    d.R = reinterpret_cast<double*>(b + L.R);
+    d.dU = reinterpret_cast<double*>(b + L.dU);
+    double* vol = reinterpret_cast<double*>(b + L.vol);
+    double* Si = reinterpret_cast<double*>(b + L.Si);
+    double* Sj = reinterpret_cast<double*>(b + L.Sj);
+    double* xf = reinterpret_cast<double*>(b + L.xf);
+    double* yf = reinterpret_cast<double*>(b + L.yf);
+    double* xw = reinterpret_cast<double*>(b + L.xw);
+    double* yw = reinterpret_cast<double*>(b + L.yw);
+    d.vol = vol;
+    d.Si = Si;
+    d.Sj = Sj;
+    d.xf = xf;
+    d.yf = yf;
+    d.xw = xw;
+    d.yw = yw;
+    if (precision == 1) {
+      d.Ut = b + L.Ut;
+      d.volt = b + L.volt;
+      d.Sit = b + L.Sit;
+      d.Sjt = b + L.Sjt;
+      d.xft = b + L.xft;
+      d.yft = b + L.yft;
+    } else {
+      d.Ut = d.U;
+      d.volt = vol;
+      d.Sit = Si;
+      d.Sjt = Sj;
+      d.xft = xf;
+      d.yft = yf;
+    }
+    d.diag = b + L.diag;
+    d.blk = b + L.blk;
+    d.V = b + L.V;
+    d.x0 = b + L.x0;
+    d.w = b + L.w;
+    d.d = b + L.d;
+    d.xa = b + L.xa;
+    d.xb = b + L.xb;
+    d.part = reinterpret_cast<double*>(b + L.part);
+    d.sc = reinterpret_cast<CaseScal*>(b + L.sc);
+
+    // metrics from the nodes, FP64 (aero2d/euler.py:44-60); node (i, j) at x[i*(nc+1)+j]
+    const int nj = nc + 1;
+    auto X = [&](int i, int j) { return x[size_t(i) * nj + j]; };
+    auto Y = [&](int i, int j) { return y[size_t(i) * nj + j]; };
+    std::vector<double> hvol(n), hSi(2 * n), hSj(2 * (n + ni)), hxf(ni), hyf(ni), hxw(ni), hyw(ni);
+    for (int i = 0; i < ni; ++i) {
+      const int i1 = (i + 1) % ni;
+      for (int j = 0; j < nc; ++j) {
+        const double d1x = X(i1, j + 1) - X(i, j), d1y = Y(i1, j + 1) - Y(i, j);
+        const double d2x = X(i, j + 1) - X(i1, j), d2y = Y(i, j + 1) - Y(i1, j);
+        hvol[size_t(j) * ni + i] = -0.5 * (d1x * d2y - d1y * d2x);
+        hSi[size_t(j) * ni + i] = -(Y(i, j + 1) - Y(i, j));
+        hSi[n + size_t(j) * ni + i] = X(i, j + 1) - X(i, j);
+      }
+      for (int j = 0; j < nj; ++j) {
+        hSj[size_t(j) * ni + i] = Y(i1, j) - Y(i, j);
+        hSj[n + ni + size_t(j) * ni + i] = -(X(i1, j) - X(i, j));
+      }
+      hxw[i] = 0.5 * (X(i, 0) + X(i1, 0));
+      hyw[i] = 0.5 * (Y(i, 0) + Y(i1, 0));
+      hxf[i] = 0.5 * (X(i, nc) + X(i1, nc));
+      hyf[i] = 0.5 * (Y(i, nc) + Y(i1, nc));
+    }
+    for (double v : hvol)
+      if (!(v > 0)) {
+        (void)hipFree(c.slab);
+        p.used -= int64_t(c.bytes);
+        c = CaseHost{};
+        fail(-8, "grid has a cell of non-positive area");
+      }
+    auto put = [&](void* dst, const std::vector<double>& src) {
+      HC(hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+    };
+    auto putf = [&](void* dst, const std::vector<double>& src) {
+      std::vector<float> f(src.begin(), src.end());
+      HC(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    };
+    put(vol, hvol);
+    put(Si, hSi);
+    put(Sj, hSj);
+    put(xf, hxf);
+    put(yf, hyf);
+    put(xw, hxw);
+    put(yw, hyw);
+    if (precision == 1) {
+      putf(d.volt, hvol);
+      putf(d.Sit, hSi);
+      putf(d.Sjt, hSj);
+      putf(d.xft, hxf);
+      putf(d.yft, hyf);
+    }
+    // freestream and initial state (aero2d/euler.py:253-256, 383-384)
+    const double a = alpha_deg * (3.141592653589793 / 180.0);
+    c.winf[0] = 1.0;
+    c.winf[1] = mach * std::cos(a);
+    c.winf[2] = mach * std::sin(a);
+    c.winf[3] = 1.0 / 1.4;
+    CaseScal sc{};
+    for (int k = 0; k < 4; ++k) sc.winf[k] = c.winf[k];
+    sc.cfl = p.knobs.cfl_explicit;
+    HC(hipMemcpy(d.sc, &sc, sizeof(CaseScal), hipMemcpyHostToDevice));
+    const double r = c.winf[0], u = c.winf[1], v = c.winf[2], pr = c.winf[3];
+    const double Uc[4] = {r, r * u, r * v, pr / (1.4 - 1.0) + 0.5 * r * (u * u + v * v)};
+    std::vector<double> hU(4 * n);
+    for (int k = 0; k < 4; ++k) std::fill(hU.begin() + k * n, hU.begin() + (k + 1) * n, Uc[k]);
+    put(d.U, hU);
+    if (precision == 1) {
+      std::vector<float> f(hU.begin(), hU.end());
+      HC(hipMemcpy(d.Ut, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    HC(hipMemcpy(p.dcases + slot, &d, sizeof(CaseDev), hipMemcpyHostToDevice));
+    *case_id = slot;
+  });
+}
+
+int a2d_case_remove(int64_t pool, int case_id) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    HC(hipStreamSynchronize(p.stream));
+    drop_plans_with(p, case_id);
+    (void)hipFree(c.slab);
+    p.used -= int64_t(c.bytes);
+    c = CaseHost{};
+  });
+}
+
+int a2d_case_set_freestream(int64_t pool, int case_id, double mach, double alpha_deg) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    const double a = alpha_deg * (3.141592653589793 / 180.0);
+    c.winf[0] = 1.0;
+    c.winf[1] = mach * std::cos(a);
+    c.winf[2] = mach * std::sin(a);
+    c.winf[3] = 1.0 / 1.4;
+    HC(hipMemcpy(c.dev.sc->winf, c.winf, sizeof(c.winf), hipMemcpyHostToDevice));
+  });
+}
+
+int a2d_case_set_shape(int64_t pool, int case_id, int shape) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    if (shape < 0 || shape >= stencil_num_shapes()) fail(-6, "no such stencil shape");
+    c.dev.shape = shape;
+    c.dev.ntiles = cdiv(c.ni, stencil_shape(shape).ti) * cdiv(c.nc, stencil_shape(shape).tj);
+    HC(hipMemcpy(p.dcases + case_id, &c.dev, sizeof(CaseDev), hipMemcpyHostToDevice));
+    drop_plans_with(p, case_id);
+  });
+}
+
+int a2d_case_set_state(int64_t pool, int case_id, const double* U) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    upload_field(p, c, U, c.dev.U, 0, 4);
+    if (c.prec == 1) upload_field(p, c, U, c.dev.Ut, 1, 4);
+  });
+}
+
+int a2d_case_get_state(int64_t pool, int case_id, double* U) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    download_field(p, c, c.dev.U, U, 0, 4);
+  });
+}
+
+int a2d_residual(int64_t pool, int case_id, double circ, double* R, double* rn) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    const KnobsDev kn = to_dev(p.knobs);
+    if (std::isnan(circ)) launch_forces(b.pc, nullptr, p.stream);
+    for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 0, circ, 0, p.stream);
+    launch_rn_finalize(b.pc, p.stream);
+    if (R) download_field(p, c, c.dev.R, R, 0, 4);
+    if (rn) {
+      HC(hipMemcpyAsync(rn, &c.dev.sc->rn, sizeof(double), hipMemcpyDeviceToHost, p.stream));
+      HC(hipStreamSynchronize(p.stream));
+    }
+  });
+}
+
+int a2d_forces(int64_t pool, int case_id, double* clcdcm, double* cp, double* circ) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    double* dcp = nullptr;
+    HC(hipMalloc(&dcp, c.ni * sizeof(double)));
+    launch_forces(b.pc, dcp, p.stream);
+    CaseScal sc;
+    HC(hipMemcpyAsync(&sc, c.dev.sc, offsetof(CaseScal, s), hipMemcpyDeviceToHost, p.stream));
+    if (cp) HC(hipMemcpyAsync(cp, dcp, c.ni * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    (void)hipFree(dcp);
+    if (clcdcm) {
+      clcdcm[0] = sc.cl;
+      clcdcm[1] = sc.cd;
+      clcdcm[2] = sc.cm;
+    }
+    if (circ) *circ = sc.circ;
+  });
+}
+
+int a2d_local_dt(int64_t pool, int case_id, double* dt) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    const KnobsDev kn = to_dev(p.knobs);
+    set_cfl_one(p, b, p.knobs.cfl_explicit);
+    enq_prologue(p, b, kn);
+    const size_t n = size_t(c.ni) * c.nc;
+    std::vector<double> diag(n), vol(n);
+    if (c.prec == 0) {
+      HC(hipMemcpyAsync(diag.data(), c.dev.diag, n * 8, hipMemcpyDeviceToHost, p.stream));
+    } else {
+      std::vector<float> f(n);
+      HC(hipMemcpyAsync(f.data(), c.dev.diag, n * 4, hipMemcpyDeviceToHost, p.stream));
+      HC(hipStreamSynchronize(p.stream));
+      std::copy(f.begin(), f.end(), diag.begin());
+    }
+    HC(hipMemcpyAsync(vol.data(), c.dev.vol, n * 8, hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    // diag = vol / dt at CFL = cfl_explicit
+    for (int i = 0; i < c.ni; ++i)
+      for (int j = 0; j < c.nc; ++j) dt[size_t(i) * c.nc + j] = vol[size_t(j) * c.ni + i] / diag[size_t(j) * c.ni + i];
+  });
+}
+
+int a2d_jvp(int64_t pool, int case_id, const double* v, double cfl, int with_diag, double* out) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    const KnobsDev kn = to_dev(p.knobs);
+    set_cfl_one(p, b, cfl);
+    enq_prologue(p, b, kn);
+    upload_field(p, c, v, c.dev.xa, c.prec, 4);
+    launch_dcirc(b.pc, VEC_XA, 0, c.prec, p.stream);
+    for (auto& g : b.sten)
+      launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_XA, VEC_W, 0, -1, with_diag, 0, c.prec, p.stream);
+    download_field(p, c, c.dev.w, out, c.prec, 4);
+  });
+}
+
+int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, double* Bip,
+                 double* Bjm, double* Bjp) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    const KnobsDev kn = to_dev(p.knobs);
+    set_cfl_one(p, b, cfl);
+    enq_prologue(p, b, kn);
+    launch_assemble(b.pcell, b.ncell, kn, c.prec, p.stream);
+    const size_t n = size_t(c.ni) * c.nc;
+    const size_t t = c.prec == 0 ? 8 : 4;
+    std::vector<char> raw(80 * n * t);
+    HC(hipMemcpyAsync(raw.data(), c.dev.blk, raw.size(), hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    auto get = [&](int slot, int m, size_t idx) -> double {
+      const size_t o = (size_t(slot) * 16 + m) * n + idx;
+      return c.prec == 0 ? reinterpret_cast<double*>(raw.data())[o] : reinterpret_cast<float*>(raw.data())[o];
+    };
+    double* outs[5] = {D, Bjp, Bjm, Bim, Bip};
+    for (int slot = 0; slot < 5; ++slot) {
+      if (!outs[slot]) continue;
+      for (int i = 0; i < c.ni; ++i)
+        for (int j = 0; j < c.nc; ++j)
+          for (int m = 0; m < 16; ++m)
+            outs[slot][(size_t(i) * c.nc + j) * 16 + m] = get(slot, m, size_t(j) * c.ni + i);
+    }
+  });
+}
+
+int a2d_precond_setup(int64_t pool, int case_id, double cfl) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    const KnobsDev kn = to_dev(p.knobs);
+    set_cfl_one(p, b, cfl);
+    enq_prologue(p, b, kn);
+    launch_assemble(b.pcell, b.ncell, kn, c.prec, p.stream);
+    launch_factor(b.pline, b.nline, c.prec, p.stream);
+    HC(hipStreamSynchronize(p.stream));
+  });
+}
+
+int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    BatchPlan& b = get_plan(p, {case_id});
+    upload_field(p, c, r, c.dev.w, c.prec, 4);
+    enq_precond(p, b, VEC_W, VEC_X0, 0, 0);
+    download_field(p, c, c.dev.x0, x, c.prec, 4);
+  });
+}
+
+int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl, double* out) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    if (n <= 0) fail(-6, "empty batch");
+    std::vector<int> slots(case_ids, case_ids + n);
+    for (int s : slots) get_case(p, s);
+    BatchPlan& b = get_plan(p, slots);
+    std::memcpy(b.hcfl, cfl, n * sizeof(double));
+    HC(hipMemcpyAsync(b.dcfl, b.hcfl, n * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    b.uses++;
+    const bool graph = p.knobs.use_graph && b.uses >= 2;
+    if (graph && (!b.exec || b.knob_version != p.knob_version)) {
+      if (b.exec) {
+        (void)hipGraphExecDestroy(b.exec);
+        b.exec = nullptr;
+      }
+      hipGraph_t gr;
+      HC(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+      enq_newton(p, b);
+      HC(hipStreamEndCapture(p.stream, &gr));
+      HC(hipGraphInstantiate(&b.exec, gr, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(gr);
+      b.knob_version = p.knob_version;
+    }
+    if (graph) HC(hipGraphLaunch(b.exec, p.stream));
+    else enq_newton(p, b);
+    HC(hipMemcpyAsync(b.hout, b.dout, n * 10 * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    std::memcpy(out, b.hout, n * 10 * sizeof(double));
+  });
+}
+
+int a2d_hllc_flux(int n, const double* WL, const double* WR, const double* S, double* F) {
+  return guarded([&] {
+    ensure_init();
+    double* d = nullptr;
+    const size_t bytes = size_t(n) * 14 * sizeof(double);
+    HC(hipMalloc(&d, bytes));
+    HC(hipMemcpy(d, WL, n * 4 * 8, hipMemcpyHostToDevice));
+    HC(hipMemcpy(d + 4 * size_t(n), WR, n * 4 * 8, hipMemcpyHostToDevice));
+    HC(hipMemcpy(d + 8 * size_t(n), S, n * 2 * 8, hipMemcpyHostToDevice));
+    launch_hllc_test(n, d, d + 4 * size_t(n), d + 8 * size_t(n), d + 10 * size_t(n), nullptr);
+    HC(hipGetLastError());
+    HC(hipMemcpy(F, d + 10 * size_t(n), n * 4 * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(d);
+  });
+}
+
+}  // extern "C"

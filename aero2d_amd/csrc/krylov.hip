@@ -1,0 +1,404 @@
+// GMRES and Newton-update kernels (aero2d/implicit.py:184-191 and the
+// "batched" GMRES of jax.scipy.sparse.linalg that it calls).
+//
+// Basis vectors are stored raw: V_true_j = s_j * V_raw_j with the scale s_j
+// kept in the case scalars. Normalising is then a scalar update, the JVP
+// applies s_j to its output, and no separate scaling pass touches HBM.
+//
+// Reductions are two-level and case-local: every block writes its partial
+// sums to the case's partial array, then one small kernel per case adds them
+// in block order. Results never depend on which other cases share a launch.
+#include "common.hpp"
+#include "launch.hpp"
+#include "physics.hpp"
+
+#include <cfloat>
+#include <type_traits>
+
+namespace a2d {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int PER = VEC_CHUNK / NT;   // cells per thread in the vector kernels
+
+template <class T> __device__ __forceinline__ double eps_of() {
+  return std::is_same<T, double>::value ? DBL_EPSILON : double(FLT_EPSILON);
+}
+
+// h_j = <V_j, w> partials for j = 0..k (times s_j at reduction), and <w, w>.
+template <class T>
+__global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gate) {
+  __shared__ double sred[KMAX + 2][NT / WAVE];
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  if (gate && !cd.sc->active) return;
+  const int n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  const int c0 = lb * VEC_CHUNK + threadIdx.x;
+  const T* w = vec_ptr<T>(cd, wvec);
+  T wv[4][PER];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int c = c0 + m * NT;
+      wv[v][m] = (c < n) ? w[(size_t)v * n + c] : T(0);
+    }
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  for (int j = 0; j <= k + 1; ++j) {
+    double acc = 0.0;
+    if (j <= k) {
+      const T* V = vec_ptr<T>(cd, j);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+          const int c = c0 + m * NT;
+          if (c < n) acc += double(V[(size_t)v * n + c]) * double(wv[v][m]);
+        }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) acc += double(wv[v][m]) * double(wv[v][m]);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) sred[j][wave] = acc;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j <= k + 1; j += NT) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / WAVE; ++q) s += sred[j][q];
+    cd.part[(size_t)lb * (cd.kmax + 2) + j] = s;
+  }
+}
+
+// One block per case: H[j][k] = s_j * sum_b part[b][j]; vn0 = |w|.
+__global__ void k_reduce_dots(Plan pc, int k) {
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
+  CaseScal* sc = cd.sc;
+  if (!sc->active) return;
+  const int nblk = cd.nvecb;
+  for (int j = threadIdx.x; j <= k + 1; j += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2) + j];
+    if (j <= k) sc->H[j][k] = sc->s[j] * s;
+    else sc->vn0 = sqrt(s);
+  }
+}
+
+// w <- w - sum_j H[j][k] s_j V_j (classical Gram-Schmidt), in place in V_{k+1};
+// partial |w|^2 per block.
+template <class T>
+__global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int gate) {
+  __shared__ double sred[NT / WAVE];
+  __shared__ double coef[KMAX + 1];
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  if (gate && !sc->active) return;
+  const int n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  const int c0 = lb * VEC_CHUNK + threadIdx.x;
+  for (int j = threadIdx.x; j <= k; j += NT) coef[j] = sc->H[j][k] * sc->s[j];
+  __syncthreads();
+  T* w = vec_ptr<T>(cd, wvec);
+  double nrm = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int c = c0 + m * NT;
+      if (c >= n) continue;
+      const size_t o = (size_t)v * n + c;
+      double acc = double(w[o]);
+      for (int j = 0; j <= k; ++j) acc -= coef[j] * double(vec_ptr<T>(cd, j)[o]);
+      const T r = T(acc);
+      w[o] = r;
+      nrm += double(r) * double(r);
+    }
+  nrm = wave_sum(nrm);
+  if ((threadIdx.x & (WAVE - 1)) == 0) sred[threadIdx.x / WAVE] = nrm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / WAVE; ++q) s += sred[q];
+    cd.part[(size_t)lb * (cd.kmax + 2)] = s;
+  }
+}
+
+// One thread per case: finish Arnoldi step k (norm, breakdown test with
+// threshold eps * |w| as in JAX, Givens update of the Hessenberg column and
+// the residual estimate, optional early exit).
+template <class T>
+__global__ void k_normalize(Plan pc, int k, KnobsDev kn) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  const CaseDev& cd = pc.cases[pc.slots[e]];
+  CaseScal* sc = cd.sc;
+  if (!sc->active) return;
+  const int nblk = cd.nvecb;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
+  const double nrm = sqrt(s);
+  const bool brk = !(nrm > eps_of<T>() * sc->vn0);
+  sc->H[k + 1][k] = brk ? 0.0 : nrm;
+  sc->s[k + 1] = brk ? 0.0 : 1.0 / nrm;
+  // apply the previous rotations to column k
+  for (int i = 0; i < k; ++i) {
+    const double a = sc->H[i][k], b = sc->H[i + 1][k];
+    sc->H[i][k] = sc->gc[i] * a + sc->gs[i] * b;
+    sc->H[i + 1][k] = -sc->gs[i] * a + sc->gc[i] * b;
+  }
+  const double a = sc->H[k][k], b = sc->H[k + 1][k];
+  const double r = hypot(a, b);
+  const double c = (r > 0) ? a / r : 1.0, sn = (r > 0) ? b / r : 0.0;
+  sc->gc[k] = c;
+  sc->gs[k] = sn;
+  sc->H[k][k] = r;
+  sc->H[k + 1][k] = 0.0;
+  sc->g[k + 1] = -sn * sc->g[k];
+  sc->g[k] = c * sc->g[k];
+  sc->gres = fabs(sc->g[k + 1]);
+  sc->kdone = k + 1;
+  bool stop = brk || (k + 1 >= kn.restart);
+  if (kn.early_exit && sc->gres <= fmin(1.0, kn.gtol) * sc->mbnorm) stop = true;
+  if (stop) sc->active = 0;
+}
+
+// stage 0: |M b| from the x0 sweep's partials.
+// stage 1: beta = |M(b - A x0)| from the V_0 sweep's partials; run flag.
+template <class T>
+__global__ void k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  const CaseDev& cd = pc.cases[pc.slots[e]];
+  CaseScal* sc = cd.sc;
+  const int nblk = cd.nlineb;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
+  const double nrm = sqrt(s);
+  if (stage == 0) {
+    sc->mbnorm = nrm;
+    return;
+  }
+  const double beta = (nrm > eps_of<T>()) ? nrm : 0.0;
+  const double ptol = sc->mbnorm * fmin(1.0, kn.gtol);
+  sc->beta = beta;
+  sc->s[0] = beta > 0 ? 1.0 / beta : 0.0;
+  sc->g[0] = beta;
+  sc->gres = beta;
+  sc->run = beta > ptol;
+  sc->active = sc->run && kn.restart > 0;
+  sc->kdone = 0;
+}
+
+// y = R^-1 g by back substitution over the completed Arnoldi steps.
+__global__ void k_lstsq(Plan pc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  CaseScal* sc = pc.cases[pc.slots[e]].sc;
+  const int kd = sc->run ? sc->kdone : 0;
+  for (int i = kd - 1; i >= 0; --i) {
+    double acc = sc->g[i];
+    for (int l = i + 1; l < kd; ++l) acc -= sc->H[i][l] * sc->y[l];
+    sc->y[i] = acc / sc->H[i][i];
+  }
+  sc->kdone = kd;
+}
+
+// dU = x0 + sum_j y_j s_j V_j (stored FP64), with the step-limiter partials:
+// max over cells of |rho_new - rho|/rho and |p_new - p|/p for U + dU, and the
+// count of non-finite entries of dU (aero2d/implicit.py:186-189).
+template <class T>
+__global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
+  __shared__ double coef[KMAX];
+  __shared__ double smax[NT / WAVE], scnt[NT / WAVE];
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  const int n = cd.n, kd = sc->kdone;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  for (int j = threadIdx.x; j < kd; j += NT) coef[j] = sc->y[j] * sc->s[j];
+  __syncthreads();
+  const T* x0 = tp<T>(cd.x0);
+  const double* U = cd.U;
+  double* dU = cd.dU;
+  double rel = 0.0, bad = 0.0;
+  for (int m = 0; m < PER; ++m) {
+    const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
+    if (c >= n) continue;
+    double du[4], u[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const size_t o = (size_t)v * n + c;
+      double acc = double(x0[o]);
+      for (int j = 0; j < kd; ++j) acc += coef[j] * double(vec_ptr<T>(cd, j)[o]);
+      du[v] = acc;
+      dU[o] = acc;
+      u[v] = U[o];
+      if (!isfinite(acc)) bad += 1.0;
+    }
+    const W4<double> w0 = cons_to_prim<double>(u[0], u[1], u[2], u[3]);
+    const W4<double> w1 =
+        cons_to_prim<double>(u[0] + du[0], u[1] + du[1], u[2] + du[2], u[3] + du[3]);
+    const double q = fmax(fabs(w1.r - w0.r) / w0.r, fabs(w1.p - w0.p) / w0.p);
+    if (q == q) rel = fmax(rel, q); else bad += 1.0;
+  }
+  rel = wave_max(rel);
+  bad = wave_sum(bad);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    smax[threadIdx.x / WAVE] = rel;
+    scnt[threadIdx.x / WAVE] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = 0.0, ct = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / WAVE; ++q) {
+      mx = fmax(mx, smax[q]);
+      ct += scnt[q];
+    }
+    cd.part[(size_t)lb * (cd.kmax + 2)] = mx;
+    cd.part[(size_t)lb * (cd.kmax + 2) + 1] = ct;
+  }
+}
+
+// One thread per case: omega = min(1, max_update / (rel + 1e-30)); accept the
+// step when rn, rel and dU are finite (aero2d/implicit.py:190, 218-219).
+__global__ void k_limiter(Plan pc, KnobsDev kn) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  const CaseDev& cd = pc.cases[pc.slots[e]];
+  CaseScal* sc = cd.sc;
+  const int nblk = cd.nvecb;
+  double mx = 0.0, ct = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    mx = fmax(mx, cd.part[(size_t)b * (cd.kmax + 2)]);
+    ct += cd.part[(size_t)b * (cd.kmax + 2) + 1];
+  }
+  sc->rel = mx;
+  sc->nonfinite = int(ct);
+  const double om = fmin(1.0, kn.max_update / (mx + 1e-30));
+  sc->omega = om;
+  sc->accept = (ct == 0.0) && isfinite(mx) && isfinite(sc->rn) && !sc->frozen;
+}
+
+// U += omega dU for accepted cases.
+__global__ void __launch_bounds__(NT) k_apply(Plan plan) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  if (!sc->accept) return;
+  const double om = sc->omega;
+  const int n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  for (int m = 0; m < PER; ++m) {
+    const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
+    if (c >= n) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cd.U[(size_t)v * n + c] += om * cd.dU[(size_t)v * n + c];
+  }
+}
+
+// Start of a Newton step: per-case CFL from the host, flags reset.
+__global__ void k_set_cfl(Plan pc, const double* cfl) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  CaseScal* sc = pc.cases[pc.slots[e]].sc;
+  sc->cfl = cfl[e];
+  sc->active = 1;
+  sc->run = 0;
+  sc->kdone = 0;
+  sc->accept = 0;
+  sc->nonfinite = 0;
+  sc->dcirc = 0.0;
+}
+
+// Per-case results of a step, 10 doubles per batch entry.
+__global__ void k_gather(Plan pc, double* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  const CaseScal* sc = pc.cases[pc.slots[e]].sc;
+  double* o = out + (size_t)e * 10;
+  o[0] = sc->rn;
+  o[1] = sc->omega;
+  o[2] = sc->accept;
+  o[3] = sc->cl;
+  o[4] = sc->cd;
+  o[5] = sc->cm;
+  o[6] = sc->kdone;
+  o[7] = sc->rel;
+  o[8] = sc->beta;
+  o[9] = sc->gres;
+}
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_copy(Plan plan, int src, int dst) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const int n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  const T* a = vec_ptr<T>(cd, src);
+  T* b = vec_ptr<T>(cd, dst);
+  for (int m = 0; m < PER; ++m) {
+    const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
+    if (c >= n) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) b[(size_t)v * n + c] = a[(size_t)v * n + c];
+  }
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_dots<double>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else hipLaunchKernelGGL(k_dots<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+}
+void launch_reduce_dots(const Plan& pc, int k, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(64), 0, s, pc, k);
+}
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_update<double>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else hipLaunchKernelGGL(k_update<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+}
+void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_normalize<double>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, k, kn);
+  else hipLaunchKernelGGL(k_normalize<float>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, k, kn);
+}
+void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL(k_krylov_begin<double>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, stage, kn);
+  else
+    hipLaunchKernelGGL(k_krylov_begin<float>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, stage, kn);
+}
+void launch_lstsq(const Plan& pc, hipStream_t s) {
+  hipLaunchKernelGGL(k_lstsq, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc);
+}
+void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_dupdate<double>, dim3(nblocks), dim3(NT), 0, s, pv);
+  else hipLaunchKernelGGL(k_dupdate<float>, dim3(nblocks), dim3(NT), 0, s, pv);
+}
+void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s) {
+  hipLaunchKernelGGL(k_limiter, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, kn);
+}
+void launch_apply(const Plan& pv, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply, dim3(nblocks), dim3(NT), 0, s, pv);
+}
+void launch_set_cfl(const Plan& pc, const double* cfl, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_cfl, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, cfl);
+}
+void launch_gather(const Plan& pc, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, out);
+}
+void launch_copy_vec(const Plan& pv, int nblocks, int src, int dst, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_copy<double>, dim3(nblocks), dim3(NT), 0, s, pv, src, dst);
+  else hipLaunchKernelGGL(k_copy<float>, dim3(nblocks), dim3(NT), 0, s, pv, src, dst);
+}
+
+}  // namespace a2d

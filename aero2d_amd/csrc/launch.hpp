@@ -1,0 +1,73 @@
+// Host-side launch wrappers for the device kernels (one translation unit per
+// kernel family; the engine only sees these declarations).
+//
+// Plans: every wrapper takes a Plan whose blk_prefix lists, per batch entry,
+// the blocks of that kernel family: per case (1), stencil tiles (per shape),
+// cells (256 per block), vector chunks (VEC_CHUNK cells per block) or lines
+// (LINES_PER_BLOCK per block).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.hpp"
+
+namespace a2d {
+
+constexpr int VEC_CHUNK = 1024;      // cells per block of the vector kernels
+constexpr int CELL_BLOCK = 256;      // cells per block of the assembly kernel
+constexpr int LINES_PER_BLOCK = 64;  // i-lines per block of the line kernels
+
+struct KnobsDev {        // solver knobs the kernels read
+  double cfl_explicit;   // EulerSolver.cfl (2.5)
+  double lim_k;          // limiter scale factor (0.02)
+  double max_update;     // step limiter (0.2)
+  double gtol;           // GMRES tolerance (0.01)
+  int vortex;            // far-field vortex correction on/off
+  int order;             // 1 or 2
+  int early_exit;        // stop Arnoldi once the estimate meets gtol
+  int restart;           // Arnoldi steps per Newton step
+};
+
+// ---- stencil.hip ------------------------------------------------------------
+struct StencilShape { int ti, tj; };
+int stencil_num_shapes();
+StencilShape stencil_shape(int idx);
+
+// FP64 residual R(U); want_diag: also diag = vol/dtau at the case CFL;
+// refresh_ut: also the FP32 copy of U for mixed-precision cases; rn partials
+// always. circ_override: Gamma to use (NaN = the case's stored Gamma).
+void launch_residual(const Plan& ps, int shape, int nblocks, const KnobsDev& k, int want_diag,
+                     double circ_override, int refresh_ut, hipStream_t s);
+// Tangent-linear in precision prec (0 fp64, 1 fp32), J including dGamma/dU:
+// mode 0: out = scale * (diag v + J v), scale = s[kidx] (kidx >= 0) or 1;
+// mode 1: out = -R - (diag v + J v). use_diag = 0 drops diag v.
+void launch_jvp(const Plan& ps, int shape, int nblocks, const KnobsDev& k, int vin, int vout,
+                int mode, int kidx, int use_diag, int gate, int prec, hipStream_t s);
+void launch_forces(const Plan& pc, double* cp_out, hipStream_t s);
+void launch_dcirc(const Plan& pc, int vin, int gate, int prec, hipStream_t s);
+void launch_rn_finalize(const Plan& pc, hipStream_t s);
+void launch_hllc_test(int n, const double* WL, const double* WR, const double* S, double* F,
+                      hipStream_t s);
+
+// ---- precond.hip --------------------------------------------------------------
+void launch_assemble(const Plan& pcell, int nblocks, const KnobsDev& k, int prec, hipStream_t s);
+void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
+// One line-Jacobi sweep: rhs = src - (Bim x(i-1) + Bip x(i+1)) with x = xprev
+// (sweep > 0), xout = L^-1 rhs. src = VEC_NEG_R reads -R. want_norm: per-block
+// partials of |xout|^2.
+void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int sweep,
+                       int want_norm, int gate, int prec, hipStream_t s);
+
+// ---- krylov.hip -----------------------------------------------------------------
+void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);
+void launch_reduce_dots(const Plan& pc, int k, hipStream_t s);
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);
+void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s);
+void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s);
+void launch_lstsq(const Plan& pc, hipStream_t s);
+void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s);
+void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s);
+void launch_apply(const Plan& pv, int nblocks, hipStream_t s);
+void launch_set_cfl(const Plan& pc, const double* cfl, hipStream_t s);
+void launch_gather(const Plan& pc, double* out, hipStream_t s);
+void launch_copy_vec(const Plan& pv, int nblocks, int src, int dst, int prec, hipStream_t s);
+
+}  // namespace a2d

@@ -1,0 +1,421 @@
+// Block-tridiagonal wall-normal line preconditioner (aero2d/implicit.py:81-157).
+//
+//   k_assemble  first-order HLLC face Jacobians by forward-mode duals (one unit
+//               tangent per conserved variable, what jax.jacfwd computes at
+//               aero2d/implicit.py:36-59), assembled into the blocks D, Bim,
+//               Bip, Bjm, Bjp of each cell plus vol/dtau on the diagonal
+//               (aero2d/implicit.py:81-110). Thread per cell.
+//   k_factor    block Thomas elimination along j for every i-line
+//               (aero2d/implicit.py:112-128): den = D - A c_prev, inv = den^-1,
+//               c = inv C. Thread per line, 4x4 blocks in registers.
+//   k_sweep     one line-Jacobi sweep (aero2d/implicit.py:130-157): forward
+//               d_j = inv_j (rhs_j - A_j d_{j-1}), backward x_j = d_j - c_j x_{j+1},
+//               rhs = r - Bim x(i-1) - Bip x(i+1) after the first sweep. Four lanes
+//               (a quad) per line, lane r owns row r of every 4x4 product; rows
+//               are exchanged inside the quad.
+//
+// Block storage: T blk[5][16][nc][ni], slot 0 = D then inv, 1 = Bjp then c,
+// 2 = A = Bjm, 3 = Bim, 4 = Bip; entry m = row * 4 + col.
+#include "common.hpp"
+#include "launch.hpp"
+#include "physics.hpp"
+
+#include <type_traits>
+
+namespace a2d {
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ const T* base_U(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) return cd.U;
+  else return ctp<T>(cd.Ut);
+}
+template <class T>
+__device__ __forceinline__ const T* met_Si(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) return cd.Si;
+  else return ctp<T>(cd.Sit);
+}
+template <class T>
+__device__ __forceinline__ const T* met_Sj(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) return cd.Sj;
+  else return ctp<T>(cd.Sjt);
+}
+template <class T>
+__device__ __forceinline__ const T* met_xf(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) return cd.xf;
+  else return ctp<T>(cd.xft);
+}
+template <class T>
+__device__ __forceinline__ const T* met_yf(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) return cd.yf;
+  else return ctp<T>(cd.yft);
+}
+
+template <class T>
+__device__ __forceinline__ void load_cons(const T* U, int n, int idx, T q[4]) {
+  q[0] = U[idx];
+  q[1] = U[n + idx];
+  q[2] = U[2 * n + idx];
+  q[3] = U[3 * n + idx];
+}
+
+// Interior first-order face: flux of (qL, qR); tangent e_b on side `right`.
+template <class T>
+__device__ __forceinline__ void face_col(const T qL[4], const T qR[4], T sx, T sy, int b,
+                                         bool right, T col[4]) {
+  using D = Dual<T>;
+  D l[4], r[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    l[m] = D(qL[m], (!right && m == b) ? T(1) : T(0));
+    r[m] = D(qR[m], (right && m == b) ? T(1) : T(0));
+  }
+  const W4<D> WL = cons_to_prim<D>(l[0], l[1], l[2], l[3]);
+  const W4<D> WR = cons_to_prim<D>(r[0], r[1], r[2], r[3]);
+  D F[4];
+  hllc_flux<D, T>(WL, WR, sx, sy, F);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) col[m] = F[m].d;
+}
+
+// Wall face: flux(mirror(w), w) with w = prim(q) (aero2d/implicit.py:44-51).
+template <class T>
+__device__ __forceinline__ void wall_col(const T q[4], T sx, T sy, int b, T col[4]) {
+  using D = Dual<T>;
+  D u[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) u[m] = D(q[m], m == b ? T(1) : T(0));
+  const W4<D> w = cons_to_prim<D>(u[0], u[1], u[2], u[3]);
+  const T area = sqrt(sx * sx + sy * sy);
+  const W4<D> g = mirror(w, sx / area, sy / area);
+  D F[4];
+  hllc_flux<D, T>(g, w, sx, sy, F);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) col[m] = F[m].d;
+}
+
+// Far-field face: flux(w, ghost(w)) with the circulation frozen
+// (aero2d/implicit.py:54-59).
+template <class T>
+__device__ __forceinline__ void far_col(const T q[4], T sx, T sy, const Inf<T>& inf, T xf, T yf,
+                                        T circ, int b, T col[4]) {
+  using D = Dual<T>;
+  D u[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) u[m] = D(q[m], m == b ? T(1) : T(0));
+  const W4<D> w = cons_to_prim<D>(u[0], u[1], u[2], u[3]);
+  const W4<D> g = farfield_ghost<D, T>(w, sx, sy, inf, xf, yf, D(circ));
+  D F[4];
+  hllc_flux<D, T>(w, g, sx, sy, F);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) col[m] = F[m].d;
+}
+
+constexpr int NT = 256;
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int idx = (blockIdx.x - plan.blk_prefix[e]) * NT + threadIdx.x;
+  if (idx >= n) return;
+  const int i = idx % ni, j = idx / ni;
+  const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
+  const T* U = base_U<T>(cd);
+  const T* Si = met_Si<T>(cd);
+  const T* Sj = met_Sj<T>(cd);
+  const size_t sj1 = (size_t)(nc + 1) * ni;
+  T* blk = tp<T>(cd.blk);
+  const size_t plane = (size_t)n;
+  auto at = [&](int slot, int m) -> T& { return blk[((size_t)slot * 16 + m) * plane + idx]; };
+
+  T qc[4], qim[4], qip[4];
+  load_cons(U, n, idx, qc);
+  load_cons(U, n, j * ni + im, qim);
+  load_cons(U, n, j * ni + ip, qip);
+  T Dm[16];
+  const T dg = ctp<T>(cd.diag)[idx];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) Dm[m] = (m % 5 == 0) ? dg : T(0);
+
+  // i-face i (cells im | i) and i-face i+1 (cells i | ip)
+  const T six0 = Si[j * ni + i], siy0 = Si[n + j * ni + i];
+  const T six1 = Si[j * ni + ip], siy1 = Si[n + j * ni + ip];
+  for (int b = 0; b < 4; ++b) {
+    T col[4];
+    face_col<T>(qim, qc, six0, siy0, b, false, col);   // AL_i -> Bim = -AL_i
+#pragma unroll
+    for (int r = 0; r < 4; ++r) at(3, r * 4 + b) = -col[r];
+    face_col<T>(qim, qc, six0, siy0, b, true, col);    // AR_i -> D -= AR_i
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
+    face_col<T>(qc, qip, six1, siy1, b, false, col);   // AL_{i+1} -> D += AL_{i+1}
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
+    face_col<T>(qc, qip, six1, siy1, b, true, col);    // AR_{i+1} -> Bip
+#pragma unroll
+    for (int r = 0; r < 4; ++r) at(4, r * 4 + b) = col[r];
+  }
+  // j-face j (cells j-1 | j) or the wall
+  const T sjx0 = Sj[j * ni + i], sjy0 = Sj[sj1 + j * ni + i];
+  if (j >= 1) {
+    T qjm[4];
+    load_cons(U, n, idx - ni, qjm);
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      face_col<T>(qjm, qc, sjx0, sjy0, b, false, col);  // AL_j -> Bjm = -AL_j
+#pragma unroll
+      for (int r = 0; r < 4; ++r) at(2, r * 4 + b) = -col[r];
+      face_col<T>(qjm, qc, sjx0, sjy0, b, true, col);   // AR_j -> D -= AR_j
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
+    }
+  } else {
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      wall_col<T>(qc, sjx0, sjy0, b, col);              // D -= Aw
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Dm[r * 4 + b] -= col[r];
+        at(2, r * 4 + b) = T(0);
+      }
+    }
+  }
+  // j-face j+1 (cells j | j+1) or the far field
+  const T sjx1 = Sj[(j + 1) * ni + i], sjy1 = Sj[sj1 + (j + 1) * ni + i];
+  if (j + 1 < nc) {
+    T qjp[4];
+    load_cons(U, n, idx + ni, qjp);
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      face_col<T>(qc, qjp, sjx1, sjy1, b, false, col);  // AL_{j+1} -> D += AL_{j+1}
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
+      face_col<T>(qc, qjp, sjx1, sjy1, b, true, col);   // AR_{j+1} -> Bjp
+#pragma unroll
+      for (int r = 0; r < 4; ++r) at(1, r * 4 + b) = col[r];
+    }
+  } else {
+    const CaseScal* sc = cd.sc;
+    const Inf<T> inf = {T(sc->winf[0]), T(sc->winf[1]), T(sc->winf[2]), T(sc->winf[3])};
+    const T circ = kn.vortex ? T(sc->circ) : T(0);
+    const T xf = met_xf<T>(cd)[i], yf = met_yf<T>(cd)[i];
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      far_col<T>(qc, sjx1, sjy1, inf, xf, yf, circ, b, col);  // D += Af
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Dm[r * 4 + b] += col[r];
+        at(1, r * 4 + b) = T(0);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) at(0, m) = Dm[m];
+}
+
+// 4x4 inverse by Gauss-Jordan elimination with partial pivoting; all indices
+// compile-time so the blocks stay in registers.
+template <class T>
+__device__ __forceinline__ void inv4(const T a_in[16], T out[16]) {
+  T a[16], b[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    a[m] = a_in[m];
+    b[m] = (m % 5 == 0) ? T(1) : T(0);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    // pivot search in column c among rows c..3
+    int p = c;
+    T best = fabs(a[c * 4 + c]);
+#pragma unroll
+    for (int r = c + 1; r < 4; ++r) {
+      const T v = fabs(a[r * 4 + c]);
+      if (v > best) { best = v; p = r; }
+    }
+#pragma unroll
+    for (int r = c + 1; r < 4; ++r) {
+      const bool sw = (p == r);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const T ta = a[c * 4 + k], tb = b[c * 4 + k];
+        a[c * 4 + k] = sw ? a[r * 4 + k] : ta;
+        a[r * 4 + k] = sw ? ta : a[r * 4 + k];
+        b[c * 4 + k] = sw ? b[r * 4 + k] : tb;
+        b[r * 4 + k] = sw ? tb : b[r * 4 + k];
+      }
+    }
+    const T piv = T(1) / a[c * 4 + c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[c * 4 + k] *= piv;
+      b[c * 4 + k] *= piv;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const T f = a[r * 4 + c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        a[r * 4 + k] -= f * a[c * 4 + k];
+        b[r * 4 + k] -= f * b[c * 4 + k];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) out[m] = b[m];
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_factor(Plan plan) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int i = (blockIdx.x - plan.blk_prefix[e]) * 64 + threadIdx.x;
+  if (i >= ni) return;
+  T* blk = tp<T>(cd.blk);
+  const size_t plane = (size_t)n;
+  T cprev[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) cprev[m] = T(0);
+  for (int j = 0; j < nc; ++j) {
+    const size_t idx = (size_t)j * ni + i;
+    T den[16], A[16], C[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      den[m] = blk[(0 * 16 + m) * plane + idx];
+      A[m] = blk[(2 * 16 + m) * plane + idx];
+      C[m] = blk[(1 * 16 + m) * plane + idx];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += A[r * 4 + k] * cprev[k * 4 + c];
+        den[r * 4 + c] -= s;
+      }
+    T inv[16];
+    inv4<T>(den, inv);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += inv[r * 4 + k] * C[k * 4 + c];
+        cprev[r * 4 + c] = s;
+      }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      blk[(0 * 16 + m) * plane + idx] = inv[m];
+      blk[(1 * 16 + m) * plane + idx] = cprev[m];
+    }
+  }
+}
+
+// Value of x held by lane `c` of this lane's quad.
+template <class T>
+__device__ __forceinline__ T quad_get(T x, int c) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  return __shfl(x, (lane & ~3) | c, WAVE);
+}
+
+// One line-Jacobi sweep. Block = 64 lines x 4 rows.
+template <class T>
+__global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xprev, int xout, int sweep,
+                                               int want_norm, int gate) {
+  __shared__ double red[4];
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  if (gate && !cd.sc->active) return;
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  const int r = threadIdx.x & 3;
+  const int i = lb * 64 + (threadIdx.x >> 2);
+  const bool live = i < ni;
+  const int ic = live ? i : ni - 1;     // dead lanes shadow a real line, never store
+  const int im = (ic == 0) ? ni - 1 : ic - 1, ip = (ic + 1 == ni) ? 0 : ic + 1;
+  const T* blk = ctp<T>(cd.blk);
+  const size_t plane = (size_t)n;
+  const T* inv = blk + (size_t)(0 * 16 + r * 4) * plane;
+  const T* cf = blk + (size_t)(1 * 16 + r * 4) * plane;
+  const T* A = blk + (size_t)(2 * 16 + r * 4) * plane;
+  const T* Bim = blk + (size_t)(3 * 16 + r * 4) * plane;
+  const T* Bip = blk + (size_t)(4 * 16 + r * 4) * plane;
+  const double* Rn = cd.R;
+  const T* b = (src == VEC_NEG_R) ? nullptr : vec_ptr<T>(cd, src);
+  const T* xp = (sweep > 0) ? vec_ptr<T>(cd, xprev) : nullptr;
+  T* d = tp<T>(cd.d);
+  T* x = vec_ptr<T>(cd, xout);
+
+  // forward elimination
+  T dr = T(0);
+  for (int j = 0; j < nc; ++j) {
+    const size_t idx = (size_t)j * ni + ic;
+    T rhs = b ? b[(size_t)r * n + idx] : T(-Rn[(size_t)r * n + idx]);
+    if (sweep > 0) {
+      const size_t jm = (size_t)j * ni + im, jp = (size_t)j * ni + ip;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        rhs -= Bim[(size_t)c * plane + idx] * xp[(size_t)c * n + jm] +
+               Bip[(size_t)c * plane + idx] * xp[(size_t)c * n + jp];
+    }
+    T t = rhs;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) t -= A[(size_t)c * plane + idx] * quad_get(dr, c);
+    T dn = T(0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dn += inv[(size_t)c * plane + idx] * quad_get(t, c);
+    dr = dn;
+    if (live) d[(size_t)r * n + idx] = dr;
+  }
+  // back substitution
+  T xr = T(0);
+  double nrm = 0.0;
+  for (int j = nc - 1; j >= 0; --j) {
+    const size_t idx = (size_t)j * ni + ic;
+    T s = live ? d[(size_t)r * n + idx] : dr;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s -= cf[(size_t)c * plane + idx] * quad_get(xr, c);
+    xr = s;
+    if (live) {
+      x[(size_t)r * n + idx] = xr;
+      nrm += double(xr) * double(xr);
+    }
+  }
+  if (want_norm) {
+    nrm = wave_sum(nrm);
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) cd.part[(size_t)lb * (cd.kmax + 2)] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+}  // namespace
+
+void launch_assemble(const Plan& pv, int nblocks, const KnobsDev& k, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_assemble<double>, dim3(nblocks), dim3(NT), 0, s, pv, k);
+  else hipLaunchKernelGGL(k_assemble<float>, dim3(nblocks), dim3(NT), 0, s, pv, k);
+}
+
+void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_factor<double>, dim3(nblocks), dim3(64), 0, s, pl);
+  else hipLaunchKernelGGL(k_factor<float>, dim3(nblocks), dim3(64), 0, s, pl);
+}
+
+void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int sweep,
+                       int want_norm, int gate, int prec, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL(k_sweep<double>, dim3(nblocks), dim3(256), 0, s, pl, src, xprev, xout, sweep,
+                       want_norm, gate);
+  else
+    hipLaunchKernelGGL(k_sweep<float>, dim3(nblocks), dim3(256), 0, s, pl, src, xprev, xout, sweep,
+                       want_norm, gate);
+}
+
+}  // namespace a2d

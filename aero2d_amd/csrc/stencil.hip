@@ -1,0 +1,464 @@
+// Cell-face stencil kernels of the implicit Euler path:
+//   * the second-order finite-volume residual R(U) (aero2d/euler.py:259-287),
+//     FP64, fused with the local pseudo-time step (aero2d/euler.py:289-296,
+//     aero2d/implicit.py:173-177) and the density-residual norm partials;
+//   * its exact tangent-linear J v (replaces jax.jvp at aero2d/implicit.py:180)
+//     from the same code instantiated over Dual<T>;
+//   * per-case wall reductions: forces / circulation (aero2d/euler.py:318-331,
+//     aero2d/implicit.py:160-164) and the circulation tangent.
+//
+// Tiling: a workgroup of 256 threads owns TI x TJ cells. It stages the
+// primitive state of the (TI+4) x (TJ+4) halo in LDS (MUSCL needs two cells
+// each side; wall mirror ghosts and far-field ghosts are produced while
+// loading), evaluates every i-face and j-face flux of the tile exactly once
+// into LDS, then differences them. HBM traffic is one read of U (and v),
+// the face metrics and one write of the result; neighbouring tiles re-read
+// their 2-cell halos from L2.
+#include "common.hpp"
+#include "launch.hpp"
+#include "physics.hpp"
+
+#include <cmath>
+#include <type_traits>
+
+namespace a2d {
+
+namespace {
+
+constexpr int NT = 256;
+
+template <class T>
+struct CaseView {
+  const T* U;     // base state (conserved)
+  const T* vol;
+  const T* Si;
+  const T* Sj;
+  const T* xf;
+  const T* yf;
+};
+
+template <class T>
+__device__ __forceinline__ CaseView<T> view(const CaseDev& cd) {
+  if constexpr (std::is_same<T, double>::value) {
+    return {cd.U, cd.vol, cd.Si, cd.Sj, cd.xf, cd.yf};
+  } else {
+    return {ctp<T>(cd.Ut), ctp<T>(cd.volt), ctp<T>(cd.Sit), ctp<T>(cd.Sjt), ctp<T>(cd.xft),
+            ctp<T>(cd.yft)};
+  }
+}
+
+// Primitive state of cell (i, j) as scalar type S (tangent from v when dual).
+template <class S, class T>
+__device__ __forceinline__ W4<S> load_prim(const T* U, const T* v, int n, int idx) {
+  if constexpr (std::is_same<S, T>::value) {
+    return cons_to_prim<S>(U[idx], U[n + idx], U[2 * n + idx], U[3 * n + idx]);
+  } else {
+    return cons_to_prim<S>(S(U[idx], v[idx]), S(U[n + idx], v[n + idx]),
+                           S(U[2 * n + idx], v[2 * n + idx]), S(U[3 * n + idx], v[3 * n + idx]));
+  }
+}
+
+// The stencil kernel. TAN = false: FP64 residual. TAN = true: tangent-linear
+// in precision T.
+template <class T, bool TAN, int TI, int TJ>
+__global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin, int vout,
+                                                int mode, int kidx, int use_diag, int want_diag,
+                                                double circ_override, int refresh_ut, int gate) {
+  using S = typename std::conditional<TAN, Dual<T>, T>::type;
+  constexpr int HI = TI + 4, HJ = TJ + 4, NH = HI * HJ;
+  constexpr int NFI = (TI + 1) * TJ, NFJ = TI * (TJ + 1);
+  __shared__ T sW[4][HJ][HI];
+  __shared__ T sD[TAN ? 4 : 1][TAN ? HJ : 1][TAN ? HI : 1];
+  __shared__ T sFi[4][TJ][TI + 1];
+  __shared__ T sFj[4][TJ + 1][TI];
+  __shared__ double sred[NT / WAVE];
+
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  if (gate && !sc->active) return;
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int tiles_i = (ni + TI - 1) / TI;
+  const int lt = blockIdx.x - plan.blk_prefix[e];
+  const int i0 = (lt % tiles_i) * TI, j0 = (lt / tiles_i) * TJ;
+  const int tid = threadIdx.x;
+
+  const CaseView<T> cv = view<T>(cd);
+  const T* vt = TAN ? vec_ptr<T>(cd, vin) : nullptr;
+  const Inf<T> inf = {T(sc->winf[0]), T(sc->winf[1]), T(sc->winf[2]), T(sc->winf[3])};
+  const double circ_d = (circ_override == circ_override) ? circ_override : sc->circ;
+  const T vort = kn.vortex ? T(1) : T(0);
+  S circ;
+  if constexpr (TAN) circ = S(T(circ_d) * vort, T(sc->dcirc) * vort);
+  else circ = S(T(circ_d) * vort);
+
+  // ---- stage primitives of the halo in LDS --------------------------------
+  for (int c = tid; c < NH; c += NT) {
+    const int hi = c % HI, hj = c / HI;
+    const int i = wrap(i0 - 2 + hi, ni);
+    const int j = j0 - 2 + hj;
+    W4<S> w;
+    if (j >= 0 && j < nc) {
+      w = load_prim<S, T>(cv.U, vt, n, j * ni + i);
+    } else if (j < 0) {
+      // mirror ghosts: j = -1 mirrors cell 0, j = -2 mirrors cell 1
+      int jm = (j == -1) ? 0 : 1;
+      if (jm >= nc) jm = nc - 1;
+      const T sx = cv.Sj[i], sy = cv.Sj[(nc + 1) * ni + i];
+      const T area = sqrt(sx * sx + sy * sy);
+      w = mirror(load_prim<S, T>(cv.U, vt, n, jm * ni + i), sx / area, sy / area);
+    } else if (j <= nc + 1) {
+      // both far-field ghost layers equal the characteristic boundary state
+      const T sx = cv.Sj[nc * ni + i], sy = cv.Sj[(nc + 1) * ni + nc * ni + i];
+      w = farfield_ghost<S, T>(load_prim<S, T>(cv.U, vt, n, (nc - 1) * ni + i), sx, sy, inf,
+                               cv.xf[i], cv.yf[i], circ);
+    } else {
+      w = {K<S>(1.0), K<S>(0.0), K<S>(0.0), K<S>(1.0)};  // outside the grid, never read
+    }
+    sW[0][hj][hi] = val(w.r);
+    sW[1][hj][hi] = val(w.u);
+    sW[2][hj][hi] = val(w.v);
+    sW[3][hj][hi] = val(w.p);
+    if constexpr (TAN) {
+      sD[0][hj][hi] = tan_of(w.r);
+      sD[1][hj][hi] = tan_of(w.u);
+      sD[2][hj][hi] = tan_of(w.v);
+      sD[3][hj][hi] = tan_of(w.p);
+    }
+  }
+  __syncthreads();
+
+  // limiter thresholds eps_k = (lim_k * scale_k)^2 + 1e-20, scale = (rho, M, M, p)_inf
+  const T mach = sqrt(inf.u * inf.u + inf.v * inf.v);
+  T eps[4];
+  {
+    const T scl[4] = {inf.r, mach, mach, inf.p};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T a = T(kn.lim_k) * scl[k];
+      eps[k] = a * a + T(1e-20);
+    }
+  }
+
+  // ---- face fluxes ----------------------------------------------------------
+  for (int f = tid; f < NFI + NFJ; f += NT) {
+    const bool isI = f < NFI;
+    int li, lj, h0i, h0j, di, dj;
+    T sx, sy;
+    bool valid;
+    if (isI) {
+      li = f % (TI + 1);
+      lj = f / (TI + 1);
+      const int j = j0 + lj, i = i0 + li;
+      valid = (j < nc) && (i <= ni);
+      const int iw = (i >= ni) ? i - ni : i;
+      h0i = li; h0j = lj + 2; di = 1; dj = 0;
+      sx = valid ? cv.Si[j * ni + iw] : T(1);
+      sy = valid ? cv.Si[n + j * ni + iw] : T(0);
+    } else {
+      const int g = f - NFI;
+      li = g % TI;
+      lj = g / TI;
+      const int j = j0 + lj, i = i0 + li;
+      valid = (i < ni) && (j <= nc);
+      h0i = li + 2; h0j = lj; di = 0; dj = 1;
+      sx = valid ? cv.Sj[j * ni + i] : T(1);
+      sy = valid ? cv.Sj[(nc + 1) * ni + j * ni + i] : T(0);
+    }
+    if (!valid) continue;
+    S wl[4], wr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      S q[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int hi = h0i + m * di, hj = h0j + m * dj;
+        if constexpr (TAN) q[m] = S(sW[k][hj][hi], sD[k][hj][hi]);
+        else q[m] = sW[k][hj][hi];
+      }
+      if (kn.order == 2) {
+        const S a = q[1] - q[0], b = q[2] - q[1], c = q[3] - q[2];
+        wl[k] = q[1] + K<S>(0.5) * van_albada(a, b, eps[k]);
+        wr[k] = q[2] - K<S>(0.5) * van_albada(b, c, eps[k]);
+      } else {
+        wl[k] = q[1];
+        wr[k] = q[2];
+      }
+    }
+    const W4<S> L = {dmaxc(wl[0], T(1e-6)), wl[1], wl[2], dmaxc(wl[3], T(1e-6))};
+    const W4<S> R = {dmaxc(wr[0], T(1e-6)), wr[1], wr[2], dmaxc(wr[3], T(1e-6))};
+    S F[4];
+    hllc_flux<S, T>(L, R, sx, sy, F);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T fv = TAN ? tan_of(F[k]) : val(F[k]);
+      if (isI) sFi[k][lj][li] = fv;
+      else sFj[k][lj][li] = fv;
+    }
+  }
+  __syncthreads();
+
+  // ---- cell differences and outputs ----------------------------------------
+  double rn_part = 0.0;
+  for (int c = tid; c < TI * TJ; c += NT) {
+    const int li = c % TI, lj = c / TI;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= ni || j >= nc) continue;
+    const int idx = j * ni + i;
+    T r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      r[k] = (sFi[k][lj][li + 1] - sFi[k][lj][li]) + (sFj[k][lj + 1][li] - sFj[k][lj][li]);
+    if constexpr (!TAN) {
+      double* R = cd.R;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) R[k * n + idx] = r[k];
+      const double vol = cd.vol[idx];
+      const double q = r[0] / vol;
+      rn_part += q * q;
+      if (want_diag) {
+        // local pseudo-time step from this cell's primitives (aero2d/euler.py:289-296)
+        const double rho = sW[0][lj + 2][li + 2], u = sW[1][lj + 2][li + 2];
+        const double v = sW[2][lj + 2][li + 2], p = sW[3][lj + 2][li + 2];
+        const double cs = sqrt(GAMMA * p / rho);
+        const int ip = (i + 1 == ni) ? 0 : i + 1;
+        const double six = 0.5 * (cd.Si[idx] + cd.Si[j * ni + ip]);
+        const double siy = 0.5 * (cd.Si[n + idx] + cd.Si[n + j * ni + ip]);
+        const double sjx = 0.5 * (cd.Sj[(j + 1) * ni + i] + cd.Sj[idx]);
+        const double sjy = 0.5 * (cd.Sj[(nc + 1) * ni + (j + 1) * ni + i] + cd.Sj[(nc + 1) * ni + idx]);
+        const double li_ = fabs(u * six + v * siy) + cs * sqrt(six * six + siy * siy);
+        const double lj_ = fabs(u * sjx + v * sjy) + cs * sqrt(sjx * sjx + sjy * sjy);
+        const double dt = kn.cfl_explicit * vol / (li_ + lj_);
+        const double dtl = dt / kn.cfl_explicit * sc->cfl;
+        const double dti = 1.0 / dtl;
+        if (cd.prec == 0) tp<double>(cd.diag)[idx] = vol * dti;
+        else tp<float>(cd.diag)[idx] = float(vol * dti);
+      }
+      if (refresh_ut && cd.prec == 1) {
+        float* Ut = tp<float>(cd.Ut);
+        const double* U = cd.U;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Ut[k * n + idx] = float(U[k * n + idx]);
+      }
+    } else {
+      T* out = vec_ptr<T>(cd, vout);
+      const T dg = use_diag ? ctp<T>(cd.diag)[idx] : T(0);
+      if (mode == 0) {
+        const T s = (kidx >= 0) ? T(sc->s[kidx]) : T(1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k * n + idx] = s * (dg * vt[k * n + idx] + r[k]);
+      } else {
+        const double* R = cd.R;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          out[k * n + idx] = T(-R[k * n + idx]) - (dg * vt[k * n + idx] + r[k]);
+      }
+    }
+  }
+  if constexpr (!TAN) {
+    rn_part = wave_sum(rn_part);
+    if ((tid & (WAVE - 1)) == 0) sred[tid / WAVE] = rn_part;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT / WAVE; ++w) s += sred[w];
+      cd.part[(size_t)lt * (cd.kmax + 2)] = s;
+    }
+  }
+}
+
+// ---- per-case wall reductions (one workgroup per case) ----------------------
+
+// Cl, Cd, Cm and wall Cp of the FP64 state; Gamma = Cl |V_inf| / 2.
+__global__ void __launch_bounds__(NT) k_forces(Plan plan, double* cp_out) {
+  __shared__ double red[3][NT / WAVE];
+  const int e = blockIdx.x;
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  CaseScal* sc = cd.sc;
+  const int ni = cd.ni, n = cd.n, nc = cd.nc;
+  const double* U = cd.U;
+  const double q = 0.5 * (sc->winf[1] * sc->winf[1] + sc->winf[2] * sc->winf[2]);
+  double fx = 0.0, fy = 0.0, cm = 0.0;
+  for (int i = threadIdx.x; i < ni; i += NT) {
+    const W4<double> w0 = cons_to_prim<double>(U[i], U[n + i], U[2 * n + i], U[3 * n + i]);
+    const int i1 = ni + i;
+    const W4<double> w1 = cons_to_prim<double>(U[i1], U[n + i1], U[2 * n + i1], U[3 * n + i1]);
+    const double pw = 1.5 * w0.p - 0.5 * w1.p;
+    const double cp = (pw - sc->winf[3]) / q;
+    const double sx = cd.Sj[i], sy = cd.Sj[(nc + 1) * ni + i];
+    fx += cp * sx;
+    fy += cp * sy;
+    cm += cp * ((cd.xw[i] - 0.25) * sy - cd.yw[i] * sx);
+    if (cp_out) cp_out[(size_t)e * ni + i] = cp;  // caller sizes cp_out for the widest case
+  }
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  cm = wave_sum(cm);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    red[0][w] = fx;
+    red[1][w] = fy;
+    red[2][w] = cm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sx = 0, sy = 0, sm = 0;
+    for (int k = 0; k < NT / WAVE; ++k) {
+      sx += red[0][k];
+      sy += red[1][k];
+      sm += red[2][k];
+    }
+    fx = -sx;
+    fy = -sy;
+    const double mach = sqrt(2.0 * q);
+    const double ca = sc->winf[1] / mach, sa = sc->winf[2] / mach;
+    const double cl = fy * ca - fx * sa;
+    sc->cl = cl;
+    sc->cd = fx * ca + fy * sa;
+    sc->cm = sm;
+    sc->circ = 0.5 * cl * sqrt(sc->winf[1] * sc->winf[1] + sc->winf[2] * sc->winf[2]);
+  }
+}
+
+// Tangent of Gamma along vector v: dGamma = |V_inf|/2 * dCl, dCl from the wall
+// pressure extrapolation 1.5 p_0 - 0.5 p_1 (the circulation coupling of the
+// implicit residual, aero2d/implicit.py:160-167).
+template <class T>
+__global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate) {
+  __shared__ double red[2][NT / WAVE];
+  const int e = blockIdx.x;
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  CaseScal* sc = cd.sc;
+  if (gate && !sc->active) return;
+  const int ni = cd.ni, n = cd.n, nc = cd.nc;
+  const CaseView<T> cv = view<T>(cd);
+  const T* v = vec_ptr<T>(cd, vin);
+  const double q = 0.5 * (sc->winf[1] * sc->winf[1] + sc->winf[2] * sc->winf[2]);
+  double dfx = 0.0, dfy = 0.0;
+  for (int i = threadIdx.x; i < ni; i += NT) {
+    using D = Dual<T>;
+    const W4<D> w0 = load_prim<D, T>(cv.U, v, n, i);
+    const W4<D> w1 = load_prim<D, T>(cv.U, v, n, ni + i);
+    const double dpw = 1.5 * double(w0.p.d) - 0.5 * double(w1.p.d);
+    const double dcp = dpw / q;
+    dfx += dcp * cd.Sj[i];
+    dfy += dcp * cd.Sj[(nc + 1) * ni + i];
+  }
+  dfx = wave_sum(dfx);
+  dfy = wave_sum(dfy);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    red[0][w] = dfx;
+    red[1][w] = dfy;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sx = 0, sy = 0;
+    for (int k = 0; k < NT / WAVE; ++k) {
+      sx += red[0][k];
+      sy += red[1][k];
+    }
+    const double mach = sqrt(2.0 * q);
+    const double ca = sc->winf[1] / mach, sa = sc->winf[2] / mach;
+    const double dcl = (-sy) * ca - (-sx) * sa;
+    sc->dcirc = 0.5 * dcl * mach;
+  }
+}
+
+// rn = sqrt(sum of the residual kernel's partials / n), one thread per case.
+__global__ void k_rn_finalize(Plan pc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pc.nb) return;
+  const CaseDev& cd = pc.cases[pc.slots[e]];
+  const int nblk = cd.ntiles;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
+  cd.sc->rn = sqrt(s / cd.n);
+}
+
+// ---- shape table --------------------------------------------------------------
+#define A2D_SHAPES(X) X(16, 16) X(32, 8) X(64, 4) X(8, 32)
+
+constexpr StencilShape kShapes[] = {
+#define X(a, b) {a, b},
+    A2D_SHAPES(X)
+#undef X
+};
+
+template <int TI, int TJ>
+void run_residual(const Plan& p, int nblocks, const KnobsDev& k, int want_diag, double circ_override,
+                  int refresh_ut, hipStream_t s) {
+  hipLaunchKernelGGL((k_stencil<double, false, TI, TJ>), dim3(nblocks), dim3(NT), 0, s, p, k, 0, 0,
+                     0, -1, 0, want_diag, circ_override, refresh_ut, 0);
+}
+
+template <class T, int TI, int TJ>
+void run_jvp(const Plan& p, int nblocks, const KnobsDev& k, int vin, int vout, int mode, int kidx,
+             int use_diag, int gate, hipStream_t s) {
+  hipLaunchKernelGGL((k_stencil<T, true, TI, TJ>), dim3(nblocks), dim3(NT), 0, s, p, k, vin, vout,
+                     mode, kidx, use_diag, 0, 0.0, 0, gate);
+}
+
+}  // namespace
+
+int stencil_num_shapes() { return int(sizeof(kShapes) / sizeof(kShapes[0])); }
+StencilShape stencil_shape(int idx) { return kShapes[idx]; }
+
+void launch_residual(const Plan& p, int shape, int nblocks, const KnobsDev& k, int want_diag,
+                     double circ_override, int refresh_ut, hipStream_t s) {
+  int idx = 0;
+#define X(a, b)                                                                    \
+  if (shape == idx) {                                                              \
+    run_residual<a, b>(p, nblocks, k, want_diag, circ_override, refresh_ut, s);    \
+    return;                                                                        \
+  }                                                                                \
+  ++idx;
+  A2D_SHAPES(X)
+#undef X
+}
+
+void launch_jvp(const Plan& p, int shape, int nblocks, const KnobsDev& k, int vin, int vout,
+                int mode, int kidx, int use_diag, int gate, int prec, hipStream_t s) {
+  int idx = 0;
+#define X(a, b)                                                                      \
+  if (shape == idx) {                                                                \
+    if (prec == 0) run_jvp<double, a, b>(p, nblocks, k, vin, vout, mode, kidx, use_diag, gate, s); \
+    else run_jvp<float, a, b>(p, nblocks, k, vin, vout, mode, kidx, use_diag, gate, s);    \
+    return;                                                                          \
+  }                                                                                  \
+  ++idx;
+  A2D_SHAPES(X)
+#undef X
+}
+
+void launch_forces(const Plan& p, double* cp_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_forces, dim3(p.nb), dim3(NT), 0, s, p, cp_out);
+}
+
+void launch_dcirc(const Plan& p, int vin, int gate, int prec, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_dcirc<double>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate);
+  else hipLaunchKernelGGL(k_dcirc<float>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate);
+}
+
+void launch_rn_finalize(const Plan& pc, hipStream_t s) {
+  hipLaunchKernelGGL(k_rn_finalize, dim3((pc.nb + 63) / 64), dim3(64), 0, s, pc);
+}
+
+// Test hook: HLLC flux on n independent faces, arrays (4, n) / (2, n).
+__global__ void k_hllc_test(int n, const double* WL, const double* WR, const double* S, double* F) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const W4<double> L = {WL[f], WL[n + f], WL[2 * n + f], WL[3 * n + f]};
+  const W4<double> R = {WR[f], WR[n + f], WR[2 * n + f], WR[3 * n + f]};
+  double out[4];
+  hllc_flux<double, double>(L, R, S[f], S[n + f], out);
+  for (int k = 0; k < 4; ++k) F[k * n + f] = out[k];
+}
+
+void launch_hllc_test(int n, const double* WL, const double* WR, const double* S, double* F,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_hllc_test, dim3((n + 255) / 256), dim3(256), 0, s, n, WL, WR, S, F);
+}
+
+}  // namespace a2d

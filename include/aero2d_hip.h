@@ -1,0 +1,123 @@
+/* aero2d_hip.h — C ABI of the MI355X implicit Euler library (libaero2d_hip.so).
+ *
+ * Drop-in numerics for aero2d's batched implicit Euler path. The reference has
+ * no FFI: its boundary is a Python class API (aero2d/euler.py EulerSolver,
+ * aero2d/implicit.py ImplicitEuler). Each entry point below replaces the
+ * device-side work behind one of those methods; the package aero2d_amd keeps
+ * the Python classes and binds these symbols with ctypes.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    a2d_last_error() then describes the failure (thread-local string).
+ *  - Host arrays are caller-owned and copied. Field arrays use the
+ *    reference's layout: (4, ni, nc) C order, j (wall-normal) fastest; node
+ *    arrays (ni, nc + 1). Device memory is owned by the library.
+ *  - A pool owns one HIP stream and the device memory of its cases. A pool is
+ *    not thread-safe: drive it from one host thread. Pools are independent.
+ *  - precision: 0 = FP64 throughout; 1 = mixed (FP32 Jacobian-vector product,
+ *    line preconditioner and Krylov basis; FP64 residual, Newton update and
+ *    force integration).
+ */
+#ifndef AERO2D_HIP_H
+#define AERO2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define A2D_PREC_FP64 0
+#define A2D_PREC_MIXED 1
+
+/* Solver knobs. Defaults (a2d_default_knobs) are the reference's:
+ * EulerSolver(cfl=2.5, order=2, vortex_correction=True, lim_k=0.02)
+ * (aero2d/euler.py:219-221) and ImplicitEuler(gmres_restart=50,
+ * gmres_tol=0.01, jacobi_sweeps=4, max_update=0.2) (aero2d/implicit.py:68-70).
+ * early_exit = 0 keeps JAX's "batched" GMRES semantics (a fixed `restart`
+ * Arnoldi steps, aero2d/implicit.py:184-185); 1 stops a case's Arnoldi loop once
+ * its preconditioned residual estimate is below gmres_tol * |M b|. */
+typedef struct a2d_knobs {
+  double cfl_explicit;
+  double lim_k;
+  double max_update;
+  double gmres_tol;
+  int vortex;
+  int order;
+  int gmres_restart;
+  int jacobi_sweeps;
+  int early_exit;
+  int use_graph;      /* replay captured HIP graphs for repeated batches */
+} a2d_knobs;
+
+/* Library set-up on one HIP device; idempotent. */
+int a2d_init(int device);
+const char* a2d_last_error(void);
+int a2d_version(void);
+void a2d_default_knobs(a2d_knobs* k);
+
+/* Pools. hbm_budget_bytes bounds the sum of the pool's case footprints
+ * (admission by HBM footprint for the serving loop). */
+int a2d_pool_create(int64_t hbm_budget_bytes, int64_t* pool_out);
+int a2d_pool_destroy(int64_t pool);
+int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k);
+int a2d_pool_get_knobs(int64_t pool, a2d_knobs* k);
+int64_t a2d_pool_used_bytes(int64_t pool);
+/* Device bytes one case of this size occupies (the admission footprint). */
+int64_t a2d_case_bytes(int ni, int nc, int precision, int restart);
+
+/* Cases. Replaces EulerSolver(grid, dtype) (aero2d/euler.py:219-250) for one
+ * (grid, M, alpha): metrics are computed from the nodes (FP64,
+ * aero2d/euler.py:44-60), freestream winf = (1, M cos a, M sin a, 1/gamma)
+ * (aero2d/euler.py:253-256) and U = freestream (aero2d/euler.py:383-384).
+ * x, y: node coordinates (ni, nc + 1). Fails with -2 when the pool's HBM
+ * budget would be exceeded. */
+int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y, double mach,
+                 double alpha_deg, int precision, int* case_id);
+int a2d_case_remove(int64_t pool, int case_id);
+/* Replace the case's freestream (M, alpha); the state is left as it is. */
+int a2d_case_set_freestream(int64_t pool, int case_id, double mach, double alpha_deg);
+int a2d_case_set_state(int64_t pool, int case_id, const double* U);  /* (4, ni, nc) */
+int a2d_case_get_state(int64_t pool, int case_id, double* U);
+/* Stencil tile shape used by this case's residual/JVP (index into the shape
+ * table of a2d_stencil_shapes). */
+int a2d_case_set_shape(int64_t pool, int case_id, int shape);
+int a2d_stencil_shapes(int* ti, int* tj, int max_shapes);
+
+/* Single-case operators.
+ * a2d_residual: EulerSolver.residual(U, winf, circ) (aero2d/euler.py:259-287);
+ *   circ NaN means Gamma = Cl |V_inf|/2 of U (ImplicitEuler._res,
+ *   aero2d/implicit.py:166-167). rn = sqrt(mean((R_rho/vol)^2)).
+ * a2d_forces: EulerSolver.forces (aero2d/euler.py:318-331) -> cl, cd, cm, cp(ni).
+ * a2d_local_dt: EulerSolver.local_dt(W(U)) (aero2d/euler.py:289-296), (ni, nc).
+ * a2d_jvp: d/de residual_with_circ(U + e v) (jax.jvp at aero2d/implicit.py:180),
+ *   plus diag*v when with_diag (diag = vol/dtau at CFL `cfl`).
+ * a2d_assemble: first-order blocks D, Bim, Bip, Bjm, Bjp at CFL `cfl`
+ *   (ImplicitEuler._assemble, aero2d/implicit.py:81-110), each (ni, nc, 4, 4).
+ * a2d_precond_setup / a2d_precond_apply: factor the lines at CFL `cfl`
+ *   (aero2d/implicit.py:112-128), then x = M r (aero2d/implicit.py:150-157). */
+int a2d_residual(int64_t pool, int case_id, double circ, double* R, double* rn);
+int a2d_forces(int64_t pool, int case_id, double* clcdcm, double* cp, double* circ);
+int a2d_local_dt(int64_t pool, int case_id, double* dt);
+int a2d_jvp(int64_t pool, int case_id, const double* v, double cfl, int with_diag, double* out);
+int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, double* Bip,
+                 double* Bjm, double* Bjp);
+int a2d_precond_setup(int64_t pool, int case_id, double cfl);
+int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x);
+
+/* One pseudo-transient Newton step (ImplicitEuler.step, aero2d/implicit.py:169-192)
+ * for n cases of one precision, each at its own CFL. The update is applied
+ * on the device only when rn, the step and its limiter are finite
+ * (aero2d/implicit.py:218-225). out: 10 doubles per case:
+ *   rn (before the update), omega, accepted (0/1), cl, cd, cm (after),
+ *   Arnoldi steps taken, max relative change, beta, GMRES residual estimate. */
+int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl, double* out);
+
+/* Test hook: the device HLLC flux on n faces. WL, WR (4, n), S (2, n), F (4, n). */
+int a2d_hllc_flux(int n, const double* WL, const double* WR, const double* S, double* F);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AERO2D_HIP_H */

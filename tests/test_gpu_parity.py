@@ -1,0 +1,158 @@
+"""HIP kernels vs the CPU oracle, FP64, on small grids (partial tiles included).
+
+Tolerances: the FP64 kernels evaluate the same formulas as the oracle in a
+different order (fused multiply-adds, reciprocal reuse), so agreement is at
+round-off: 1e-11 relative to the field's magnitude for residual, Jacobian-vector
+product and blocks, 1e-9 for the line solves (they amplify round-off by the
+conditioning of the 4x4 blocks)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+from torch.func import jvp
+
+from aero2d_amd import EulerSolver, _abi
+from oracle import flow, newton
+from tests.helpers import oracle_geo, perturbed_state, rel_err, small_grid, winf_t
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (n_side, nj, mach, alpha, seed)
+    (17, 9, 0.5, 2.0, 1),
+    (25, 13, 0.8, 1.25, 2),
+    (33, 21, 0.3, -3.0, 3),
+]
+
+
+def _setup(n_side, nj, mach, alpha, seed, precision="fp64", shape=None):
+    g = small_grid(n_side, nj)
+    s = EulerSolver(g, dtype=np.float64, precision=precision)
+    U = perturbed_state(s.ni, s.nc, mach, alpha, seed)
+    h, c = s._scratch(s.freestream(mach, alpha), U)
+    if shape is not None:
+        _abi.call("a2d_case_set_shape", h, c, shape)
+    return g, s, U, h, c
+
+
+def test_hllc_device_matches_oracle():
+    rng = np.random.default_rng(7)
+    n = 4096
+    WL = np.stack([rng.uniform(0.3, 2.0, n), rng.normal(0, 0.8, n), rng.normal(0, 0.8, n),
+                   rng.uniform(0.2, 1.5, n)])
+    WR = np.stack([rng.uniform(0.3, 2.0, n), rng.normal(0, 0.8, n), rng.normal(0, 0.8, n),
+                   rng.uniform(0.2, 1.5, n)])
+    S = rng.normal(size=(2, n))
+    from aero2d_amd.euler import hllc_flux
+    F = hllc_flux(WL, WR, S)
+    Fo = flow.hllc_flux(torch.tensor(WL), torch.tensor(WR), torch.tensor(S)).numpy()
+    assert rel_err(F, Fo) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("case", CASES)
+def test_residual_matches_oracle(case, shape):
+    g, s, U, h, c = _setup(*case, shape=shape)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    for circ in (0.0, 0.137, float("nan")):
+        R = np.empty((4, s.ni, s.nc))
+        rn = np.zeros(1)
+        _abi.call("a2d_residual", h, c, circ, _abi.dptr(R), _abi.dptr(rn))
+        Ut = torch.tensor(U)
+        ci = flow.circulation(Ut, w, geo) if math.isnan(circ) else torch.tensor(circ, dtype=torch.float64)
+        Ro, _ = flow.residual(Ut, w, ci, geo)
+        assert rel_err(R, Ro.numpy()) < 1e-11, (case, shape, circ)
+        rno = torch.sqrt(torch.mean((Ro[0] / geo["vol"]) ** 2)).item()
+        assert abs(rn[0] - rno) <= 1e-11 * rno
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_forces_and_local_dt_match_oracle(case):
+    g, s, U, h, c = _setup(*case)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    cl, cd, cm, cp = s.forces(U, s.freestream(mach, alpha))
+    clo, cdo, cmo, cpo = flow.forces(torch.tensor(U), w, geo)
+    assert abs(cl - clo.item()) < 1e-12 and abs(cd - cdo.item()) < 1e-12 and abs(cm - cmo.item()) < 1e-12
+    assert rel_err(cp, cpo.numpy()) < 1e-12
+    from aero2d_amd.euler import cons_to_prim
+    dt = s.local_dt(cons_to_prim(U))
+    dto = flow.local_dt(flow.cons_to_prim(torch.tensor(U)), geo).numpy()
+    assert rel_err(dt, dto) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [0, 2])
+@pytest.mark.parametrize("case", CASES)
+def test_jvp_matches_oracle(case, shape):
+    g, s, U, h, c = _setup(*case, shape=shape)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    rng = np.random.default_rng(case[4] + 100)
+    v = rng.normal(size=U.shape) * np.array([1, 0.5, 0.5, 2.0])[:, None, None]
+    out = np.empty_like(U)
+    _abi.call("a2d_jvp", h, c, _abi.dptr(np.ascontiguousarray(v)), 10.0, 0, _abi.dptr(out))
+    res = lambda u: flow.residual_with_circ(u, w, geo)[0]
+    jo = jvp(res, (torch.tensor(U),), (torch.tensor(v),))[1].numpy()
+    assert rel_err(out, jo) < 1e-11
+    # with the pseudo-time diagonal at CFL 10
+    _abi.call("a2d_jvp", h, c, _abi.dptr(np.ascontiguousarray(v)), 10.0, 1, _abi.dptr(out))
+    Wt = flow.cons_to_prim(torch.tensor(U))
+    diag = (geo["vol"] / (flow.local_dt(Wt, geo) / 2.5 * 10.0)).numpy()
+    assert rel_err(out, diag[None] * v + jo) < 1e-11
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_assembly_matches_oracle(case):
+    g, s, U, h, c = _setup(*case)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    ni, nc = s.ni, s.nc
+    blocks = [np.empty((ni, nc, 4, 4)) for _ in range(5)]
+    _abi.call("a2d_assemble", h, c, 10.0, *[_abi.dptr(b) for b in blocks])
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    Wt = flow.cons_to_prim(Ut)
+    dti = 1.0 / (flow.local_dt(Wt, geo) / 2.5 * 10.0)
+    ref = newton.assemble(Ut, w, circ, dti, geo)
+    for got, exp in zip(blocks, ref):
+        assert rel_err(got, exp.numpy()) < 1e-11
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_preconditioner_matches_oracle(case):
+    g, s, U, h, c = _setup(*case)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    _abi.call("a2d_precond_setup", h, c, 25.0)
+    rng = np.random.default_rng(5)
+    r = np.ascontiguousarray(rng.normal(size=U.shape))
+    x = np.empty_like(r)
+    _abi.call("a2d_precond_apply", h, c, _abi.dptr(r), _abi.dptr(x))
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 25.0)
+    D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
+    fac = newton.factor_lines(D, Bjm, Bjp)
+    xo = newton.precond(fac, Bim, Bip, torch.tensor(r), 4).numpy()
+    assert rel_err(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_newton_step_matches_oracle(case):
+    """Three full Newton steps (50 Arnoldi steps each) vs the oracle."""
+    n_side, nj, mach, alpha, seed = case
+    g = small_grid(n_side, nj)
+    from aero2d_amd import ImplicitEuler
+    s = EulerSolver(g, dtype=np.float64)
+    r = ImplicitEuler(s).solve([mach], [alpha], n_iter=3, verbose=False, tol=0.0)
+    geo = flow.stack_geometry([oracle_geo(g)])
+    ro = newton.solve(geo, [mach], [alpha], n_iter=3, tol=0.0)
+    assert rel_err(r.U[0], ro.U[..., 0].numpy()) < 1e-8
+    assert np.allclose(r.res_hist[:, 0], ro.res_hist[:, 0], rtol=1e-8, atol=0)
+    assert abs(r.cl[0] - ro.cl[0]) < 1e-9

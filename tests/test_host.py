@@ -1,0 +1,81 @@
+"""Host-side input generation (geometry, grid) and the C ABI surface, on CPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from aero2d_amd import _abi, growth_ratio, naca4, naca4_params, ogrid, self_similar_spacing
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_naca4_closed_form_properties():
+    af = naca4("0012", 65)
+    assert af.n == 129
+    assert af.x[0] == 1.0 and af.x[-1] == 1.0 and af.y[0] == 0.0 and af.y[-1] == 0.0
+    assert abs(af.x[64]) < 1e-15 and abs(af.y[64]) < 1e-15          # leading edge
+    assert np.allclose(af.y[:64], -af.y[::-1][:64], atol=1e-15)    # symmetric section
+    # max thickness of a 4-digit section: 0.12 at x ~ 0.30 chord (NACA TR-460)
+    t = 2 * np.max(af.y)
+    assert abs(t - 0.12) < 1.5e-3
+    cam = naca4("2412", 101)
+    assert np.max((cam.y[:101][::-1] + cam.y[100:]) / 2) == pytest.approx(0.02, abs=1e-3)
+
+
+def test_naca4_params_reduces_to_digit_code():
+    for code in ("0012", "2412", "4415"):
+        a = naca4(code, 41)
+        b = naca4_params(int(code[0]) / 100, int(code[1]) / 10, int(code[2:]) / 100, 41)
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_growth_ratio_closed_form():
+    for ds0, tot, n in ((2e-3, 40.0, 48), (4.4965e-3, 20.0, 64), (1e-3, 20.0, 256)):
+        r = growth_ratio(ds0, tot, n)
+        assert abs(ds0 * (r ** n - 1) / (r - 1) - tot) < 1e-9 * tot
+
+
+def test_self_similar_spacing_table():
+    assert self_similar_spacing(64) == pytest.approx(4.4965e-3, rel=1e-4)
+    assert self_similar_spacing(128) == pytest.approx(2.1947e-3, rel=1e-4)
+    assert self_similar_spacing(256) == pytest.approx(1.0843e-3, rel=1e-4)
+
+
+def test_ogrid_invariants_and_inverted_cells():
+    g = ogrid(naca4("0012", 65), nj=49, ds0=2e-3, radius=40.0)
+    assert g.shape == (128, 49)
+    assert g.cell_areas().min() > 0
+    r = np.hypot(g.x[:, -1] - 0.5, g.y[:, -1])
+    assert np.ptp(r) < 1e-12 * r.mean()
+    d1 = np.hypot(g.x[:, 1] - g.x[:, 0], g.y[:, 1] - g.y[:, 0])
+    assert abs(np.median(d1) / 2e-3 - 1) < 0.1
+    with pytest.raises(RuntimeError):
+        ogrid(naca4("0012", 33), nj=129, ds0=1e-5, radius=20.0)
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "aero2d_hip.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|int64_t|void|const char\*)\s+(a2d_\w+)\(", hdr, re.M))
+    assert len(declared) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (a2d_\w+)", out))
+    assert declared <= exported, declared - exported
+    assert declared == set(_abi.SIGNATURES), declared ^ set(_abi.SIGNATURES)
+    _abi.load()
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "aero2d_amd")
+    for f in os.listdir(pkg):
+        if f.endswith(".py"):
+            src = open(os.path.join(pkg, f)).read()
+            assert not re.search(r"^\s*(from|import)\s+oracle", src, re.M), f
+
+
+def test_case_footprint_scales_with_size_and_precision():
+    a64 = _abi.case_bytes(256, 128, 0)
+    a32 = _abi.case_bytes(256, 128, 1)
+    assert 0.45 < a32 / a64 < 0.65
+    assert _abi.case_bytes(512, 256, 0) > 3.9 * a64
