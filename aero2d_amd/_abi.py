@@ -60,6 +60,7 @@ SIGNATURES = {
     "a2d_precond_apply": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp]),
     "a2d_newton_step": (ctypes.c_int, [_i64, _ip, ctypes.c_int, _dp, _dp]),
     "a2d_hllc_flux": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "a2d_time_kernel": (ctypes.c_int, [_i64, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
 }
 
 _lib = None
@@ -128,8 +129,13 @@ def iptr(a: np.ndarray):
 
 
 def default_knobs() -> Knobs:
+    """The reference's solver defaults. AERO2D_USE_GRAPH=0 in the environment
+    switches HIP graph replay off (every Newton step then launches its kernels
+    directly; the kernels are the same)."""
     k = Knobs()
     load().a2d_default_knobs(ctypes.byref(k))
+    if os.environ.get("AERO2D_USE_GRAPH", "1") == "0":
+        k.use_graph = 0
     return k
 
 

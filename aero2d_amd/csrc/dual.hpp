@@ -27,70 +27,70 @@ template <class T> struct base_of<Dual<T>> { using type = T; };
 template <class S> using base_t = typename base_of<S>::type;
 
 // A constant of scalar type S (plain or dual) from a double literal.
-template <class S> __device__ __forceinline__ S K(double x) { return S(base_t<S>(x)); }
+template <class S> __host__ __device__ __forceinline__ S K(double x) { return S(base_t<S>(x)); }
 
-template <class T> __device__ __forceinline__ T val(T x) { return x; }
-template <class T> __device__ __forceinline__ T val(Dual<T> x) { return x.v; }
-template <class T> __device__ __forceinline__ T tan_of(T) { return T(0); }
-template <class T> __device__ __forceinline__ T tan_of(Dual<T> x) { return x.d; }
+template <class T> __host__ __device__ __forceinline__ T val(T x) { return x; }
+template <class T> __host__ __device__ __forceinline__ T val(Dual<T> x) { return x.v; }
+template <class T> __host__ __device__ __forceinline__ T tan_of(T) { return T(0); }
+template <class T> __host__ __device__ __forceinline__ T tan_of(Dual<T> x) { return x.d; }
 
 // ---- arithmetic -------------------------------------------------------------
-template <class T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
   T q = a.v / b.v;
   return {q, (a.d - q * b.d) / b.v};
 }
-template <class T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
 
-template <class T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator+(T a, Dual<T> b) { return {a + b.v, b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator-(T a, Dual<T> b) { return {a - b.v, -b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
-template <class T> __device__ __forceinline__ Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
-template <class T> __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
-template <class T> __device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator+(T a, Dual<T> b) { return {a + b.v, b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(T a, Dual<T> b) { return {a - b.v, -b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
   T q = a / b.v;
   return {q, -q * b.d / b.v};
 }
 
 // ---- elementary functions ---------------------------------------------------
-__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
-template <class T> __device__ __forceinline__ Dual<T> dsqrt(Dual<T> x) {
+__host__ __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+template <class T> __host__ __device__ __forceinline__ Dual<T> dsqrt(Dual<T> x) {
   T s = dsqrt(x.v);
   return {s, x.d / (T(2) * s)};
 }
 
-__device__ __forceinline__ double dpow(double x, double a) { return pow(x, a); }
-__device__ __forceinline__ float dpow(float x, double a) { return powf(x, float(a)); }
+__host__ __device__ __forceinline__ double dpow(double x, double a) { return pow(x, a); }
+__host__ __device__ __forceinline__ float dpow(float x, double a) { return powf(x, float(a)); }
 // d(x^a) = a x^(a-1) dx, written as a (x^a / x) dx.
-template <class T> __device__ __forceinline__ Dual<T> dpow(Dual<T> x, double a) {
+template <class T> __host__ __device__ __forceinline__ Dual<T> dpow(Dual<T> x, double a) {
   T p = dpow(x.v, a);
   return {p, T(a) * (p / x.v) * x.d};
 }
 
 // min / max with JAX's balanced tie rule.
-template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
-template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
-template <class T> __device__ __forceinline__ Dual<T> dmin(Dual<T> a, Dual<T> b) {
+template <class T> __host__ __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+template <class T> __host__ __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> dmin(Dual<T> a, Dual<T> b) {
   T dd = a.v < b.v ? a.d : (b.v < a.v ? b.d : T(0.5) * (a.d + b.d));
   return {a.v < b.v ? a.v : b.v, dd};
 }
-template <class T> __device__ __forceinline__ Dual<T> dmax(Dual<T> a, Dual<T> b) {
+template <class T> __host__ __device__ __forceinline__ Dual<T> dmax(Dual<T> a, Dual<T> b) {
   T dd = a.v > b.v ? a.d : (b.v > a.v ? b.d : T(0.5) * (a.d + b.d));
   return {a.v > b.v ? a.v : b.v, dd};
 }
 // max(x, c) for a constant c.
-template <class T> __device__ __forceinline__ T dmaxc(T a, T c) { return a > c ? a : c; }
-template <class T> __device__ __forceinline__ Dual<T> dmaxc(Dual<T> a, T c) {
+template <class T> __host__ __device__ __forceinline__ T dmaxc(T a, T c) { return a > c ? a : c; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> dmaxc(Dual<T> a, T c) {
   T dd = a.v > c ? a.d : (a.v < c ? T(0) : T(0.5) * a.d);
   return {a.v > c ? a.v : c, dd};
 }
 
 // Branch select on a value-only predicate.
-template <class S> __device__ __forceinline__ S sel(bool c, S a, S b) { return c ? a : b; }
+template <class S> __host__ __device__ __forceinline__ S sel(bool c, S a, S b) { return c ? a : b; }
 
 }  // namespace a2d

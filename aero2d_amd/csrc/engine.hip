@@ -302,8 +302,8 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
-    launch_line_sweep(b.pline, b.nline, src, prev, out, s, (s == S - 1) ? want_norm : 0, gate,
-                      b.prec, p.stream);
+    launch_line_sweep(b.pcell, b.ncell, b.pline, b.nline, src, prev, out, s,
+                      (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
   }
 }
 
@@ -885,6 +885,58 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
     HC(hipMemcpyAsync(b.hout, b.dout, n * 10 * sizeof(double), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     std::memcpy(out, b.hout, n * 10 * sizeof(double));
+  });
+}
+
+int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    if (n <= 0 || reps <= 0) fail(-6, "empty batch or no repetitions");
+    std::vector<int> slots(case_ids, case_ids + n);
+    for (int s : slots) get_case(p, s);
+    BatchPlan& b = get_plan(p, slots);
+    const KnobsDev kn = to_dev(p.knobs);
+    const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));
+    auto launch = [&] {
+      switch (which) {
+        case 0:
+          for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 0, NAN, 0, p.stream);
+          break;
+        case 1:
+          for (auto& g : b.sten)
+            launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
+          break;
+        case 2:
+          launch_line_sweep(b.pcell, b.ncell, b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 0, 0, b.prec,
+                            p.stream);
+          break;
+        case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
+        case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
+        case 5: launch_factor(b.pline, b.nline, b.prec, p.stream); break;
+        case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
+        default: fail(-6, "unknown kernel id");
+      }
+    };
+    launch();  // warm
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipEventRecord(e0, p.stream));
+    for (int r = 0; r < reps; ++r) launch();
+    HC(hipEventRecord(e1, p.stream));
+    HC(hipEventSynchronize(e1));
+    float t = 0.f;
+    HC(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = double(t) / reps;
+    if (which == 4 || which == 5) {
+      // assembly/factorisation overwrote the factored blocks: refactor so the
+      // cases stay consistent for a following Newton step
+      launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream);
+      launch_factor(b.pline, b.nline, b.prec, p.stream);
+      HC(hipStreamSynchronize(p.stream));
+    }
   });
 }
 

@@ -185,7 +185,11 @@ __global__ void k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
     sc->mbnorm = nrm;
     return;
   }
-  const double beta = (nrm > eps_of<T>()) ? nrm : 0.0;
+  // FP64 keeps JAX's absolute threshold (machine eps on the norm); the FP32
+  // Krylov of the mixed form scales it by |M b| so that deep convergence of the
+  // FP64 outer residual is not cut off by the size of its FP32 corrections.
+  const double thresh = std::is_same<T, double>::value ? eps_of<T>() : eps_of<T>() * sc->mbnorm;
+  const double beta = (nrm > thresh) ? nrm : 0.0;
   const double ptol = sc->mbnorm * fmin(1.0, kn.gtol);
   sc->beta = beta;
   sc->s[0] = beta > 0 ? 1.0 / beta : 0.0;

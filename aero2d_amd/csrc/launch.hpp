@@ -52,9 +52,10 @@ void launch_assemble(const Plan& pcell, int nblocks, const KnobsDev& k, int prec
 void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // One line-Jacobi sweep: rhs = src - (Bim x(i-1) + Bip x(i+1)) with x = xprev
 // (sweep > 0), xout = L^-1 rhs. src = VEC_NEG_R reads -R. want_norm: per-block
-// partials of |xout|^2.
-void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int sweep,
-                       int want_norm, int gate, int prec, hipStream_t s);
+// partials of |xout|^2. Two launches: a per-cell pass (pcell, ncell blocks)
+// and the line recurrences (pl, nblocks).
+void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
+                       int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);

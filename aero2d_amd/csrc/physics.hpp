@@ -19,7 +19,7 @@ struct W4 {  // primitive state (rho, u, v, p)
 
 // (rho, rho u, rho v, E) -> (rho, u, v, p)   (aero2d/euler.py:66-71)
 template <class S>
-__device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
+__host__ __device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
   S u = q1 / q0;
   S v = q2 / q0;
   S p = K<S>(GM1) * (q3 - K<S>(0.5) * q0 * (u * u + v * v));
@@ -33,7 +33,7 @@ __device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
 // star correction unless that side is supersonic. Only the chosen side's flux
 // is formed, so every lane runs one path.
 template <class S, class T>
-__device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& R, T sx, T sy, S F[4]) {
+__host__ __device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& R, T sx, T sy, S F[4]) {
   const T area = sqrt(sx * sx + sy * sy);
   const T nx = sx / area, ny = sy / area;
   const S qL = L.u * nx + L.v * ny;
@@ -83,13 +83,13 @@ __device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& R, T sx, 
 // Smooth van Albada limiter, LIMITER="smooth" (aero2d/euler.py:124-137);
 // eps already includes the +1e-20.
 template <class S, class T>
-__device__ __forceinline__ S van_albada(S a, S b, T eps) {
+__host__ __device__ __forceinline__ S van_albada(S a, S b, T eps) {
   return ((b * b + eps) * a + (a * a + eps) * b) / (a * a + b * b + T(2) * eps);
 }
 
 // Mirror ghost about the wall normal (aero2d/euler.py:143-153).
 template <class S, class T>
-__device__ __forceinline__ W4<S> mirror(const W4<S>& w, T nx, T ny) {
+__host__ __device__ __forceinline__ W4<S> mirror(const W4<S>& w, T nx, T ny) {
   S vn = w.u * nx + w.v * ny;
   return {w.r, w.u - K<S>(2.0) * vn * nx, w.v - K<S>(2.0) * vn * ny, w.p};
 }
@@ -105,7 +105,7 @@ struct Inf {
 // freestream are constants; the interior state and the circulation carry
 // tangents in the dual instantiation.
 template <class S, class T>
-__device__ __forceinline__ W4<S> farfield_ghost(const W4<S>& wi, T sx, T sy, const Inf<T>& f,
+__host__ __device__ __forceinline__ W4<S> farfield_ghost(const W4<S>& wi, T sx, T sy, const Inf<T>& f,
                                                 T xf, T yf, S circ) {
   const T area = sqrt(sx * sx + sy * sy);
   const T nx = sx / area, ny = sy / area;

@@ -8,14 +8,20 @@
 //   k_factor    block Thomas elimination along j for every i-line
 //               (aero2d/implicit.py:112-128): den = D - A c_prev, inv = den^-1,
 //               c = inv C. Thread per line, 4x4 blocks in registers.
-//   k_sweep     one line-Jacobi sweep (aero2d/implicit.py:130-157): forward
-//               d_j = inv_j (rhs_j - A_j d_{j-1}), backward x_j = d_j - c_j x_{j+1},
-//               rhs = r - Bim x(i-1) - Bip x(i+1) after the first sweep. Four lanes
-//               (a quad) per line, lane r owns row r of every 4x4 product; rows
-//               are exchanged inside the quad.
+//   one line-Jacobi sweep (aero2d/implicit.py:130-157) is two kernels. The
+//   reference's forward step d_j = inv_j (rhs_j - A_j d_{j-1}) is split as
+//   d_j = g_j - P_j d_{j-1} with g_j = inv_j rhs_j and P_j = inv_j A_j:
+//   k_sweep_rhs  fully parallel, thread per cell: rhs = r - Bim x(i-1) - Bip x(i+1)
+//                (r alone in the first sweep), g = inv rhs.
+//   k_sweep_seq  the j recurrences only: forward d_j = g_j - P_j d_{j-1}, backward
+//                x_j = d_j - c_j x_{j+1}. Four lanes (a quad) per line, lane r
+//                owns row r of each 4x4 product, rows exchanged in the quad;
+//                loads for the next PF steps are in flight while the current PF
+//                steps compute (double-buffered registers).
+//   The same arithmetic as the reference in another association order.
 //
 // Block storage: T blk[5][16][nc][ni], slot 0 = D then inv, 1 = Bjp then c,
-// 2 = A = Bjm, 3 = Bim, 4 = Bip; entry m = row * 4 + col.
+// 2 = A = Bjm then P = inv A, 3 = Bim, 4 = Bip; entry m = row * 4 + col.
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -310,10 +316,21 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
         for (int k = 0; k < 4; ++k) s += inv[r * 4 + k] * C[k * 4 + c];
         cprev[r * 4 + c] = s;
       }
+    T P[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += inv[r * 4 + k] * A[k * 4 + c];
+        P[r * 4 + c] = s;
+      }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       blk[(0 * 16 + m) * plane + idx] = inv[m];
       blk[(1 * 16 + m) * plane + idx] = cprev[m];
+      blk[(2 * 16 + m) * plane + idx] = P[m];
     }
   }
 }
@@ -325,10 +342,57 @@ __device__ __forceinline__ T quad_get(T x, int c) {
   return __shfl(x, (lane & ~3) | c, WAVE);
 }
 
-// One line-Jacobi sweep. Block = 64 lines x 4 rows.
+// Sweep part 1, thread per cell: g = inv (src - Bim x(i-1) - Bip x(i+1)) into d.
 template <class T>
-__global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xprev, int xout, int sweep,
-                                               int want_norm, int gate) {
+__global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev, int sweep,
+                                                   int gate) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  if (gate && !cd.sc->active) return;
+  const int ni = cd.ni, n = cd.n;
+  const int idx = (blockIdx.x - plan.blk_prefix[e]) * 256 + threadIdx.x;
+  if (idx >= n) return;
+  const int i = idx % ni, j = idx / ni;
+  const T* __restrict__ blk = ctp<T>(cd.blk);
+  const size_t plane = (size_t)n;
+  T rhs[4];
+  if (src == VEC_NEG_R) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rhs[r] = T(-cd.R[(size_t)r * n + idx]);
+  } else {
+    const T* __restrict__ b = vec_ptr<T>(cd, src);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rhs[r] = b[(size_t)r * n + idx];
+  }
+  if (sweep > 0) {
+    const T* __restrict__ xp = vec_ptr<T>(cd, xprev);
+    const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
+    T xm[4], xq[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      xm[c] = xp[(size_t)c * n + j * ni + im];
+      xq[c] = xp[(size_t)c * n + j * ni + ip];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        rhs[r] -= blk[(size_t)(3 * 16 + r * 4 + c) * plane + idx] * xm[c] +
+                  blk[(size_t)(4 * 16 + r * 4 + c) * plane + idx] * xq[c];
+  }
+  T* __restrict__ g = tp<T>(cd.d);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s += blk[(size_t)(r * 4 + c) * plane + idx] * rhs[c];
+    g[(size_t)r * n + idx] = s;
+  }
+}
+
+// Sweep part 2: the two j recurrences of every line. Block = 64 lines x 4 rows.
+template <class T, int PF>
+__global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want_norm, int gate) {
   __shared__ double red[4];
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
@@ -338,55 +402,81 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xprev, in
   const int r = threadIdx.x & 3;
   const int i = lb * 64 + (threadIdx.x >> 2);
   const bool live = i < ni;
-  const int ic = live ? i : ni - 1;     // dead lanes shadow a real line, never store
-  const int im = (ic == 0) ? ni - 1 : ic - 1, ip = (ic + 1 == ni) ? 0 : ic + 1;
-  const T* blk = ctp<T>(cd.blk);
+  const int ic = live ? i : ni - 1;     // dead lanes shadow a real line and never store
   const size_t plane = (size_t)n;
-  const T* inv = blk + (size_t)(0 * 16 + r * 4) * plane;
-  const T* cf = blk + (size_t)(1 * 16 + r * 4) * plane;
-  const T* A = blk + (size_t)(2 * 16 + r * 4) * plane;
-  const T* Bim = blk + (size_t)(3 * 16 + r * 4) * plane;
-  const T* Bip = blk + (size_t)(4 * 16 + r * 4) * plane;
-  const double* Rn = cd.R;
-  const T* b = (src == VEC_NEG_R) ? nullptr : vec_ptr<T>(cd, src);
-  const T* xp = (sweep > 0) ? vec_ptr<T>(cd, xprev) : nullptr;
-  T* d = tp<T>(cd.d);
-  T* x = vec_ptr<T>(cd, xout);
+  const T* __restrict__ P = ctp<T>(cd.blk) + (size_t)(2 * 16 + r * 4) * plane + ic;
+  const T* __restrict__ C = ctp<T>(cd.blk) + (size_t)(1 * 16 + r * 4) * plane + ic;
+  T* d = tp<T>(cd.d) + (size_t)r * n + ic;
+  T* __restrict__ x = vec_ptr<T>(cd, xout) + (size_t)r * n + ic;
 
-  // forward elimination
-  T dr = T(0);
-  for (int j = 0; j < nc; ++j) {
-    const size_t idx = (size_t)j * ni + ic;
-    T rhs = b ? b[(size_t)r * n + idx] : T(-Rn[(size_t)r * n + idx]);
-    if (sweep > 0) {
-      const size_t jm = (size_t)j * ni + im, jp = (size_t)j * ni + ip;
+  // forward: d_j = g_j - P_j d_{j-1}, in place over g
+  T bm[2][PF][4], bv[2][PF];
+  auto fload = [&](int b, int j0) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        rhs -= Bim[(size_t)c * plane + idx] * xp[(size_t)c * n + jm] +
-               Bip[(size_t)c * plane + idx] * xp[(size_t)c * n + jp];
+    for (int q = 0; q < PF; ++q) {
+      const int j = min(j0 + q, nc - 1);
+      const size_t o = (size_t)j * ni;
+      bv[b][q] = d[o];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bm[b][q][c] = P[(size_t)c * plane + o];
     }
-    T t = rhs;
+  };
+  T dr = T(0);
+  auto fcomp = [&](int b, int j0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) t -= A[(size_t)c * plane + idx] * quad_get(dr, c);
-    T dn = T(0);
+    for (int q = 0; q < PF; ++q) {
+      const int j = j0 + q;
+      if (j < nc) {
+        T t = bv[b][q];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) dn += inv[(size_t)c * plane + idx] * quad_get(t, c);
-    dr = dn;
-    if (live) d[(size_t)r * n + idx] = dr;
+        for (int c = 0; c < 4; ++c) t -= bm[b][q][c] * quad_get(dr, c);
+        dr = t;
+        if (live) d[(size_t)j * ni] = dr;
+      }
+    }
+  };
+  fload(0, 0);
+  for (int j0 = 0; j0 < nc; j0 += 2 * PF) {
+    fload(1, j0 + PF);
+    fcomp(0, j0);
+    fload(0, j0 + 2 * PF);
+    fcomp(1, j0 + PF);
   }
-  // back substitution
+  // backward: x_j = d_j - c_j x_{j+1}
+  auto bload = [&](int b, int j0) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int j = max(j0 - q, 0);
+      const size_t o = (size_t)j * ni;
+      bv[b][q] = d[o];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bm[b][q][c] = C[(size_t)c * plane + o];
+    }
+  };
   T xr = T(0);
   double nrm = 0.0;
-  for (int j = nc - 1; j >= 0; --j) {
-    const size_t idx = (size_t)j * ni + ic;
-    T s = live ? d[(size_t)r * n + idx] : dr;
+  auto bcomp = [&](int b, int j0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s -= cf[(size_t)c * plane + idx] * quad_get(xr, c);
-    xr = s;
-    if (live) {
-      x[(size_t)r * n + idx] = xr;
-      nrm += double(xr) * double(xr);
+    for (int q = 0; q < PF; ++q) {
+      const int j = j0 - q;
+      if (j >= 0) {
+        T t = bv[b][q];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) t -= bm[b][q][c] * quad_get(xr, c);
+        xr = t;
+        if (live) {
+          x[(size_t)j * ni] = xr;
+          nrm += double(xr) * double(xr);
+        }
+      }
     }
+  };
+  bload(0, nc - 1);
+  for (int j0 = nc - 1; j0 >= 0; j0 -= 2 * PF) {
+    bload(1, j0 - PF);
+    bcomp(0, j0);
+    bload(0, j0 - 2 * PF);
+    bcomp(1, j0 - PF);
   }
   if (want_norm) {
     nrm = wave_sum(nrm);
@@ -395,6 +485,8 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xprev, in
     if (threadIdx.x == 0) cd.part[(size_t)lb * (cd.kmax + 2)] = red[0] + red[1] + red[2] + red[3];
   }
 }
+
+constexpr int SWEEP_PF = 8;
 
 }  // namespace
 
@@ -408,14 +500,17 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
   else hipLaunchKernelGGL(k_factor<float>, dim3(nblocks), dim3(64), 0, s, pl);
 }
 
-void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int sweep,
-                       int want_norm, int gate, int prec, hipStream_t s) {
-  if (prec == 0)
-    hipLaunchKernelGGL(k_sweep<double>, dim3(nblocks), dim3(256), 0, s, pl, src, xprev, xout, sweep,
+void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
+                       int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s) {
+  if (prec == 0) {
+    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate);
+    hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
                        want_norm, gate);
-  else
-    hipLaunchKernelGGL(k_sweep<float>, dim3(nblocks), dim3(256), 0, s, pl, src, xprev, xout, sweep,
+  } else {
+    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate);
+    hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
                        want_norm, gate);
+  }
 }
 
 }  // namespace a2d

@@ -397,7 +397,7 @@ template <class T, int TI, int TJ>
 void run_jvp(const Plan& p, int nblocks, const KnobsDev& k, int vin, int vout, int mode, int kidx,
              int use_diag, int gate, hipStream_t s) {
   hipLaunchKernelGGL((k_stencil<T, true, TI, TJ>), dim3(nblocks), dim3(NT), 0, s, p, k, vin, vout,
-                     mode, kidx, use_diag, 0, 0.0, 0, gate);
+                     mode, kidx, use_diag, 0, double(NAN), 0, gate);
 }
 
 }  // namespace

@@ -113,6 +113,16 @@ int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x);
  *   Arnoldi steps taken, max relative change, beta, GMRES residual estimate. */
 int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl, double* out);
 
+/* Kernel timing (no counterpart in the reference; for bench.py). Launches
+ * kernel family `which` over the batch `reps` times back to back on the pool's
+ * stream between two HIP events and returns the mean duration in ms:
+ *   0 residual (FP64)   1 Jacobian-vector product   2 line sweep with i-coupling
+ *   3 Gram-Schmidt dots (k = 25)   4 block assembly   5 line factorisation
+ *   6 Gram-Schmidt update (k = 25)
+ * The cases must have taken a Newton step (their blocks and Krylov vectors
+ * hold data); the state is not changed. */
+int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms);
+
 /* Test hook: the device HLLC flux on n faces. WL, WR (4, n), S (2, n), F (4, n). */
 int a2d_hllc_flux(int n, const double* WL, const double* WR, const double* S, double* F);
 
