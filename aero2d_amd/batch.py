@@ -20,7 +20,7 @@ from . import _abi
 from .grid import OGrid
 from .implicit import NewtonLoop
 
-__all__ = ["Case", "BatchResult", "solve_batch", "default_knobs"]
+__all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs"]
 
 
 @dataclass
@@ -55,69 +55,114 @@ def default_knobs(precision: str, restart: int = 50, early_exit: bool | None = N
     return k
 
 
+class BatchSolve:
+    """A batch resident on the device: upload() places every case (grid
+    metrics, freestream state, buffers) in one pool; run() then iterates the
+    Newton loop, timing only device-resident work plus the host control."""
+
+    def __init__(self, cases: list[Case], precision: str = "fp64", knobs: _abi.Knobs | None = None,
+                 shapes: dict | None = None, pool_budget: int = 0):
+        self.cases = cases
+        self.precision = precision
+        self.prec = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED
+        self.knobs = knobs if knobs is not None else default_knobs(precision)
+        self.shapes = shapes
+        self.pool_budget = pool_budget
+        self.pool = None
+        self.ids = None
+        self.upload_s = 0.0
+
+    def upload(self):
+        t0 = time.perf_counter()
+        self.pool = _abi.Pool(self.pool_budget, self.knobs)
+        self.ids = np.empty(len(self.cases), np.int32)
+        for k, c in enumerate(self.cases):
+            self.ids[k] = self.pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, self.prec)
+            if self.shapes:
+                sh = self.shapes.get((c.grid.x.shape[0], c.grid.x.shape[1] - 1))
+                if sh is not None:
+                    _abi.call("a2d_case_set_shape", self.pool.handle, int(self.ids[k]), int(sh))
+        self.upload_s = time.perf_counter() - t0
+        return self
+
+    def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float = 300.0,
+            keep_state: bool = False, on_step=None, cfl_late: float | None = None,
+            late_below: float = 0.0, record_history: bool = False) -> BatchResult:
+        """Newton loop to rn/r0 <= tol per case (or n_iter steps); the pool is
+        released at the end."""
+        if self.pool is None:
+            self.upload()
+        cases, ids, pool = self.cases, self.ids, self.pool
+        nb = len(cases)
+        t0 = time.perf_counter()
+        try:
+            loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below)
+            hist = []
+            done = np.zeros(nb, bool)
+            t_done = np.full(nb, np.nan)
+            arn = np.zeros(nb)
+            for it in range(n_iter):
+                act = np.nonzero(~done)[0]
+                if len(act) == 0:
+                    break
+                sub = np.ascontiguousarray(ids[act])
+                cfl = np.ascontiguousarray(loop.cfl[act])
+                out = np.empty((len(act), 10))
+                _abi.call("a2d_newton_step", pool.handle, _abi.iptr(sub), len(act), _abi.dptr(cfl),
+                          _abi.dptr(out))
+                rn, _ = loop.update(act, out)
+                arn[act] += out[:, 6]
+                if record_history:
+                    h = np.full(nb, np.nan)
+                    h[act] = rn / loop.r0[act]
+                    hist.append(h)
+                fin = (rn / loop.r0[act] <= tol) | loop.frozen[act]
+                now = time.perf_counter() - t0
+                t_done[act[fin]] = now
+                done[act[fin]] = True
+                if on_step is not None:
+                    on_step(it, len(act))
+            t_done[np.isnan(t_done)] = time.perf_counter() - t0
+            cl = np.empty(nb)
+            cd = np.empty(nb)
+            cm = np.empty(nb)
+            states = []
+            for k in range(nb):
+                f3 = np.empty(3)
+                c = cases[k]
+                cp = np.empty(c.grid.x.shape[0])
+                circ = np.empty(1)
+                _abi.call("a2d_forces", pool.handle, int(ids[k]), _abi.dptr(f3), _abi.dptr(cp),
+                          _abi.dptr(circ))
+                cl[k], cd[k], cm[k] = f3
+                if keep_state:
+                    ni, nj = c.grid.x.shape
+                    Uk = np.empty((4, ni, nj - 1))
+                    _abi.call("a2d_case_get_state", pool.handle, int(ids[k]), _abi.dptr(Uk))
+                    states.append(Uk)
+            wall = time.perf_counter() - t0
+        finally:
+            pool.close()
+            self.pool = None
+        rel = loop.last_rn / loop.r0
+        conv = (~loop.frozen) & (rel <= tol)
+        res = BatchResult(cl, cd, cm, conv, loop.frozen.copy(), loop.iters.copy(), rel, loop.last_rn.copy(),
+                          t_done, wall, arn / np.maximum(loop.iters, 1), states)
+        res.history = np.array(hist) if record_history else None
+        res.upload_s = self.upload_s
+        return res
+
+
 def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, n_iter: int = 200,
                 cfl0: float = 10.0, cfl_max: float = 300.0, knobs: _abi.Knobs | None = None,
                 shapes: dict | None = None, keep_state: bool = False, pool_budget: int = 0,
-                on_step=None) -> BatchResult:
-    """Solve every case to rn/r0 <= tol (or n_iter Newton steps).
-
-    precision: "fp64" or "mixed". shapes: optional {(ni, nc): stencil shape
-    index} from the autotuner. on_step(it, n_active) is called after each step."""
-    prec = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED
-    kn = knobs if knobs is not None else default_knobs(precision)
-    t0 = time.perf_counter()
-    pool = _abi.Pool(pool_budget, kn)
-    nb = len(cases)
-    try:
-        ids = np.empty(nb, np.int32)
-        for k, c in enumerate(cases):
-            ids[k] = pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, prec)
-            if shapes:
-                sh = shapes.get(tuple(c.grid.x.shape[:1]) + (c.grid.x.shape[1] - 1,))
-                if sh is not None:
-                    _abi.call("a2d_case_set_shape", pool.handle, int(ids[k]), int(sh))
-        loop = NewtonLoop(nb, cfl0, cfl_max)
-        done = np.zeros(nb, bool)
-        t_done = np.full(nb, np.nan)
-        arn = np.zeros(nb)
-        for it in range(n_iter):
-            act = np.nonzero(~done)[0]
-            if len(act) == 0:
-                break
-            sub = np.ascontiguousarray(ids[act])
-            cfl = np.ascontiguousarray(loop.cfl[act])
-            out = np.empty((len(act), 10))
-            _abi.call("a2d_newton_step", pool.handle, _abi.iptr(sub), len(act), _abi.dptr(cfl),
-                      _abi.dptr(out))
-            rn, _ = loop.update(act, out)
-            arn[act] += out[:, 6]
-            fin = (rn / loop.r0[act] <= tol) | loop.frozen[act]
-            now = time.perf_counter() - t0
-            t_done[act[fin]] = now
-            done[act[fin]] = True
-            if on_step is not None:
-                on_step(it, len(act))
-        t_done[np.isnan(t_done)] = time.perf_counter() - t0
-        cl = np.empty(nb)
-        cd = np.empty(nb)
-        cm = np.empty(nb)
-        states = []
-        for k in range(nb):
-            f3 = np.empty(3)
-            c = cases[k]
-            cp = np.empty(c.grid.x.shape[0])
-            circ = np.empty(1)
-            _abi.call("a2d_forces", pool.handle, int(ids[k]), _abi.dptr(f3), _abi.dptr(cp), _abi.dptr(circ))
-            cl[k], cd[k], cm[k] = f3
-            if keep_state:
-                ni, nj = c.grid.x.shape
-                Uk = np.empty((4, ni, nj - 1))
-                _abi.call("a2d_case_get_state", pool.handle, int(ids[k]), _abi.dptr(Uk))
-                states.append(Uk)
-    finally:
-        pool.close()
-    wall = time.perf_counter() - t0
-    rel = loop.last_rn / loop.r0
-    conv = (~loop.frozen) & (rel <= tol)
-    return BatchResult(cl, cd, cm, conv, loop.frozen.copy(), loop.iters.copy(), rel, loop.last_rn.copy(),
-                       t_done, wall, arn / np.maximum(loop.iters, 1), states)
+                on_step=None, cfl_late: float | None = None, late_below: float = 0.0,
+                record_history: bool = False) -> BatchResult:
+    """Upload every case, then solve each to rn/r0 <= tol (or n_iter Newton
+    steps). precision: "fp64" or "mixed". shapes: optional {(ni, nc): stencil
+    shape index} from the autotuner. on_step(it, n_active) after each step.
+    cfl_late / late_below: raise the CFL cap to cfl_late once rn/r0 < late_below.
+    BatchResult.wall covers the Newton loop (inputs resident); .upload_s the
+    host-to-device set-up before it."""
+    bs = BatchSolve(cases, precision, knobs, shapes, pool_budget).upload()
+    return bs.run(tol, n_iter, cfl0, cfl_max, keep_state, on_step, cfl_late, late_below, record_history)

@@ -14,6 +14,9 @@
 //                                double arrays when T = double)
 //   blocks                T [5][16][nc][ni]       inv, c, A(=Bjm), Bim, Bip
 //   V                     T [K+1][4][nc][ni]      raw Krylov basis
+// Consecutive block planes and basis vectors sit pstride / vstride elements
+// apart: the sizes plus a pad, so that the many arrays a kernel walks in step
+// do not all start at the same offset modulo a large power of two.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -63,6 +66,8 @@ struct CaseDev {
   int nvecb;            // vector-kernel blocks (VEC_CHUNK cells each)
   int nlineb;           // line-kernel blocks (64 lines each)
   int shape;            // stencil tile shape index
+  int vstride;          // elements between consecutive Krylov basis vectors (>= 4n)
+  int pstride;          // elements between consecutive block planes (>= n)
   double* U;
   double* R;
   double* dU;
@@ -114,6 +119,35 @@ __device__ __forceinline__ int plan_entry(const Plan& p, int b) {
 template <class T> __device__ __forceinline__ T* tp(void* p) { return reinterpret_cast<T*>(p); }
 template <class T> __device__ __forceinline__ const T* ctp(const void* p) { return reinterpret_cast<const T*>(p); }
 
+// Global-address-space views of device pointers. Pointers read out of the
+// case descriptors are generic to the compiler, which then emits flat memory
+// instructions; those share the wait counter of LDS and cross-lane traffic,
+// so a shuffle would also wait for every outstanding prefetch. Casting to the
+// global address space gives global_load/global_store instead.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define A2D_GLOBAL __attribute__((address_space(1)))
+#else
+#define A2D_GLOBAL
+#endif
+template <class T> using gptr = A2D_GLOBAL T*;
+template <class T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+template <class T> __device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
+
+// Value of x held by lane C of this lane's quad (DPP quad permutation: a VALU
+// operation, no LDS traffic).
+template <int C>
+__device__ __forceinline__ double quad_bcast(double x) {
+  constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int C>
+__device__ __forceinline__ float quad_bcast(float x) {
+  constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, 0xF, 0xF, false));
+}
+
 // Wavefront sum with cross-lane exchanges (fixed order, deterministic).
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -130,6 +164,11 @@ __device__ __forceinline__ double wave_max(double x) {
 enum VecId : int { VEC_X0 = -1, VEC_W = -2, VEC_D = -3, VEC_XA = -4, VEC_XB = -5, VEC_NEG_R = -6 };
 
 template <class T>
+__device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id);
+template <class T>
+__device__ __forceinline__ gptr<T> vec_gptr(const CaseDev& cd, int id) { return gp(vec_ptr<T>(cd, id)); }
+
+template <class T>
 __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
   switch (id) {
     case VEC_X0: return tp<T>(cd.x0);
@@ -137,7 +176,7 @@ __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
     case VEC_D: return tp<T>(cd.d);
     case VEC_XA: return tp<T>(cd.xa);
     case VEC_XB: return tp<T>(cd.xb);
-    default: return tp<T>(cd.V) + (size_t)id * 4 * cd.n;
+    default: return tp<T>(cd.V) + (size_t)id * cd.vstride;
   }
 }
 

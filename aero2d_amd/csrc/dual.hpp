@@ -39,8 +39,9 @@ template <class T> __host__ __device__ __forceinline__ Dual<T> operator+(Dual<T>
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
-  T q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
+  const T r = T(1) / b.v;
+  const T q = a.v * r;
+  return {q, (a.d - q * b.d) * r};
 }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
 
@@ -52,8 +53,17 @@ template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T>
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
-  T q = a / b.v;
-  return {q, -q * b.d / b.v};
+  const T r = T(1) / b.v;
+  const T q = a * r;
+  return {q, -q * b.d * r};
+}
+
+// Reciprocal 1/x (one division for value and tangent).
+__host__ __device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
+__host__ __device__ __forceinline__ float drcp(float x) { return 1.0f / x; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> drcp(Dual<T> x) {
+  const T r = T(1) / x.v;
+  return {r, -r * r * x.d};
 }
 
 // ---- elementary functions ---------------------------------------------------

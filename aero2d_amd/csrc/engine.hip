@@ -76,6 +76,13 @@ int max_tiles(int ni, int nc) {
   return m;
 }
 
+#ifndef A2D_PAD_BYTES
+#define A2D_PAD_BYTES 0
+#endif
+// Padded strides of the block planes and basis vectors (elements).
+size_t plane_stride(size_t n, size_t t) { return n + A2D_PAD_BYTES / t; }
+size_t vec_stride(size_t n, size_t t) { return 4 * n + A2D_PAD_BYTES / t; }
+
 Layout layout(int ni, int nc, int prec, int kmax) {
   Layout L{};
   const size_t n = size_t(ni) * nc;
@@ -105,8 +112,8 @@ Layout layout(int ni, int nc, int prec, int kmax) {
     L.yft = take(ni * 4);
   }
   L.diag = take(n * t);
-  L.blk = take(80 * n * t);
-  L.V = take(size_t(kmax + 1) * 4 * n * t);
+  L.blk = take(80 * plane_stride(n, t) * t);
+  L.V = take(size_t(kmax + 1) * vec_stride(n, t) * t);
   L.x0 = take(4 * n * t);
   L.w = take(4 * n * t);
   L.d = take(4 * n * t);
@@ -548,6 +555,8 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.ntiles = cdiv(ni, stencil_shape(0).ti) * cdiv(nc, stencil_shape(0).tj);
     d.nvecb = cdiv(int(n), VEC_CHUNK);
     d.nlineb = cdiv(ni, LINES_PER_BLOCK);
+    d.vstride = int(vec_stride(n, precision == 0 ? 8 : 4));
+    d.pstride = int(plane_stride(n, precision == 0 ? 8 : 4));
     d.U = reinterpret_cast<double*>(b + L.U);
     d.R = reinterpret_cast<double*>(b + L.R);
     d.dU = reinterpret_cast<double*>(b + L.dU);
@@ -813,11 +822,12 @@ int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, 
     launch_assemble(b.pcell, b.ncell, kn, c.prec, p.stream);
     const size_t n = size_t(c.ni) * c.nc;
     const size_t t = c.prec == 0 ? 8 : 4;
-    std::vector<char> raw(80 * n * t);
+    const size_t ps = size_t(c.dev.pstride);
+    std::vector<char> raw(80 * ps * t);
     HC(hipMemcpyAsync(raw.data(), c.dev.blk, raw.size(), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     auto get = [&](int slot, int m, size_t idx) -> double {
-      const size_t o = (size_t(slot) * 16 + m) * n + idx;
+      const size_t o = (size_t(slot) * 16 + m) * ps + idx;
       return c.prec == 0 ? reinterpret_cast<double*>(raw.data())[o] : reinterpret_cast<float*>(raw.data())[o];
     };
     double* outs[5] = {D, Bjp, Bjm, Bim, Bip};
@@ -899,8 +909,8 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));
     auto launch = [&] {
       switch (which) {
-        case 0:
-          for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 0, NAN, 0, p.stream);
+        case 0:  // as launched in a Newton step: with the local time step and U copy
+          for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 1, NAN, 1, p.stream);
           break;
         case 1:
           for (auto& g : b.sten)
@@ -914,6 +924,9 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
         case 5: launch_factor(b.pline, b.nline, b.prec, p.stream); break;
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
+        case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
+        case 8: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 0, b.prec, p.stream); break;
+        case 9: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 1, b.prec, p.stream); break;
         default: fail(-6, "unknown kernel id");
       }
     };

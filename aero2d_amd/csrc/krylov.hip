@@ -36,35 +36,56 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
   const int n = cd.n;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   const int c0 = lb * VEC_CHUNK + threadIdx.x;
-  const T* w = vec_ptr<T>(cd, wvec);
+  const gptr<const T> w = vec_gptr<T>(cd, wvec);
+  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const size_t vs = (size_t)cd.vstride;
+  // indices are clamped into the case and out-of-range cells contribute zero
+  // through selects, so every load is unconditional (no per-element branches)
+  int cc[PER];
+  bool in[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int c = c0 + m * NT;
+    in[m] = c < n;
+    cc[m] = in[m] ? c : n - 1;
+  }
   T wv[4][PER];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-      const int c = c0 + m * NT;
-      wv[v][m] = (c < n) ? w[(size_t)v * n + c] : T(0);
+      const T x = w[(size_t)v * n + cc[m]];
+      wv[v][m] = in[m] ? x : T(0);
     }
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  for (int j = 0; j <= k + 1; ++j) {
-    double acc = 0.0;
-    if (j <= k) {
-      const T* V = vec_ptr<T>(cd, j);
+  // four basis vectors per pass so that their loads are in flight together
+  constexpr int JB = 4;
+  for (int j0 = 0; j0 <= k; j0 += JB) {
+    double acc[JB];
+#pragma unroll
+    for (int q = 0; q < JB; ++q) {
+      acc[q] = 0.0;
+      const int j = min(j0 + q, k);
+      const gptr<const T> V = Vb + (size_t)j * vs;
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-          const int c = c0 + m * NT;
-          if (c < n) acc += double(V[(size_t)v * n + c]) * double(wv[v][m]);
-        }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int m = 0; m < PER; ++m) acc += double(wv[v][m]) * double(wv[v][m]);
+        for (int m = 0; m < PER; ++m) acc[q] += double(V[(size_t)v * n + cc[m]]) * double(wv[v][m]);
     }
+#pragma unroll
+    for (int q = 0; q < JB; ++q) {
+      const double a = wave_sum(acc[q]);
+      if (lane == 0 && j0 + q <= k) sred[j0 + q][wave] = a;
+    }
+  }
+  {
+    double acc = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int m = 0; m < PER; ++m) acc += double(wv[v][m]) * double(wv[v][m]);
     acc = wave_sum(acc);
-    if (lane == 0) sred[j][wave] = acc;
+    if (lane == 0) sred[k + 1][wave] = acc;
   }
   __syncthreads();
   for (int j = threadIdx.x; j <= k + 1; j += NT) {
@@ -104,21 +125,46 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   const int c0 = lb * VEC_CHUNK + threadIdx.x;
   for (int j = threadIdx.x; j <= k; j += NT) coef[j] = sc->H[j][k] * sc->s[j];
   __syncthreads();
-  T* w = vec_ptr<T>(cd, wvec);
+  const gptr<T> w = vec_gptr<T>(cd, wvec);
+  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const size_t vs = (size_t)cd.vstride;
   double nrm = 0.0;
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
+  auto body = [&](auto check) {
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
       const int c = c0 + m * NT;
-      if (c >= n) continue;
-      const size_t o = (size_t)v * n + c;
-      double acc = double(w[o]);
-      for (int j = 0; j <= k; ++j) acc -= coef[j] * double(vec_ptr<T>(cd, j)[o]);
-      const T r = T(acc);
-      w[o] = r;
-      nrm += double(r) * double(r);
+      if (decltype(check)::value && c >= n) continue;
+      double acc[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = double(w[(size_t)v * n + c]);
+      // four basis vectors per pass: 16 loads in flight per element
+      int j = 0;
+      for (; j + 3 <= k; j += 4) {
+        T vv[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) vv[q][v] = Vb[(size_t)(j + q) * vs + (size_t)v * n + c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[v] -= coef[j + q] * double(vv[q][v]);
+      }
+      for (; j <= k; ++j) {
+        const double cj = coef[j];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] -= cj * double(Vb[(size_t)j * vs + (size_t)v * n + c]);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T r = T(acc[v]);
+        w[(size_t)v * n + c] = r;
+        nrm += double(r) * double(r);
+      }
     }
+  };
+  if ((lb + 1) * VEC_CHUNK <= n) body(std::false_type{});
+  else body(std::true_type{});
   nrm = wave_sum(nrm);
   if ((threadIdx.x & (WAVE - 1)) == 0) sred[threadIdx.x / WAVE] = nrm;
   __syncthreads();
@@ -228,23 +274,41 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   const int lb = blockIdx.x - plan.blk_prefix[e];
   for (int j = threadIdx.x; j < kd; j += NT) coef[j] = sc->y[j] * sc->s[j];
   __syncthreads();
-  const T* x0 = tp<T>(cd.x0);
-  const double* U = cd.U;
-  double* dU = cd.dU;
+  const gptr<const T> x0 = gp(ctp<T>(cd.x0));
+  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const size_t vs = (size_t)cd.vstride;
+  const gptr<const double> U = gp(cd.U);
+  const gptr<double> dU = gp(cd.dU);
   double rel = 0.0, bad = 0.0;
   for (int m = 0; m < PER; ++m) {
     const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
     if (c >= n) continue;
     double du[4], u[4];
 #pragma unroll
+    for (int v = 0; v < 4; ++v) du[v] = double(x0[(size_t)v * n + c]);
+    int j = 0;
+    for (; j + 4 <= kd; j += 4) {
+      T vv[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) vv[q][v] = Vb[(size_t)(j + q) * vs + (size_t)v * n + c];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) du[v] += coef[j + q] * double(vv[q][v]);
+    }
+    for (; j < kd; ++j) {
+      const double cj = coef[j];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) du[v] += cj * double(Vb[(size_t)j * vs + (size_t)v * n + c]);
+    }
+#pragma unroll
     for (int v = 0; v < 4; ++v) {
       const size_t o = (size_t)v * n + c;
-      double acc = double(x0[o]);
-      for (int j = 0; j < kd; ++j) acc += coef[j] * double(vec_ptr<T>(cd, j)[o]);
-      du[v] = acc;
-      dU[o] = acc;
+      dU[o] = du[v];
       u[v] = U[o];
-      if (!isfinite(acc)) bad += 1.0;
+      if (!isfinite(du[v])) bad += 1.0;
     }
     const W4<double> w0 = cons_to_prim<double>(u[0], u[1], u[2], u[3]);
     const W4<double> w1 =

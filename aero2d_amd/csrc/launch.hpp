@@ -11,7 +11,10 @@
 
 namespace a2d {
 
-constexpr int VEC_CHUNK = 1024;      // cells per block of the vector kernels
+#ifndef A2D_VEC_CHUNK
+#define A2D_VEC_CHUNK 1024
+#endif
+constexpr int VEC_CHUNK = A2D_VEC_CHUNK;  // cells per block of the vector kernels
 constexpr int CELL_BLOCK = 256;      // cells per block of the assembly kernel
 constexpr int LINES_PER_BLOCK = 64;  // i-lines per block of the line kernels
 
@@ -56,6 +59,11 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // and the line recurrences (pl, nblocks).
 void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
                        int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s);
+
+// Timing hook: part 0 = the per-cell rhs pass of a coupled sweep, part 1 = the
+// line recurrences alone (reading the d buffer left by a previous sweep).
+void launch_sweep_part(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int part, int prec,
+                       hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);

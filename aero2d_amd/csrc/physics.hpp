@@ -20,8 +20,9 @@ struct W4 {  // primitive state (rho, u, v, p)
 // (rho, rho u, rho v, E) -> (rho, u, v, p)   (aero2d/euler.py:66-71)
 template <class S>
 __host__ __device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
-  S u = q1 / q0;
-  S v = q2 / q0;
+  const S ir = drcp(q0);
+  S u = q1 * ir;
+  S v = q2 * ir;
   S p = K<S>(GM1) * (q3 - K<S>(0.5) * q0 * (u * u + v * v));
   return {q0, u, v, p};
 }
@@ -32,24 +33,32 @@ __host__ __device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
 // is evaluated as: pick side L when SL >= 0 or SM >= 0 (else R), then add the
 // star correction unless that side is supersonic. Only the chosen side's flux
 // is formed, so every lane runs one path.
+//
+// Divisions are the expensive FP64 operations on this path, so the flux is
+// arranged around six of them: 1/area, 1/rho_L, 1/rho_R, the Roe-average
+// denominator, S_M, and 1/(S_k - S_M), which gives the whole star state: with
+// d = 1/(S_k - S_M) and f = rho (S_k - q) d,
+//   f E/rho = (S_k - q) E d,   f p/(rho (S_k - q)) = p d.
 template <class S, class T>
 __host__ __device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& R, T sx, T sy, S F[4]) {
   const T area = sqrt(sx * sx + sy * sy);
-  const T nx = sx / area, ny = sy / area;
+  const T ia = T(1) / area;
+  const T nx = sx * ia, ny = sy * ia;
+  const S irL = drcp(L.r), irR = drcp(R.r);
   const S qL = L.u * nx + L.v * ny;
   const S qR = R.u * nx + R.v * ny;
-  const S cL = dsqrt(K<S>(GAMMA) * L.p / L.r);
-  const S cR = dsqrt(K<S>(GAMMA) * R.p / R.r);
-  const S EL = L.p / K<S>(GM1) + K<S>(0.5) * L.r * (L.u * L.u + L.v * L.v);
-  const S ER = R.p / K<S>(GM1) + K<S>(0.5) * R.r * (R.u * R.u + R.v * R.v);
+  const S cL = dsqrt(K<S>(GAMMA) * L.p * irL);
+  const S cR = dsqrt(K<S>(GAMMA) * R.p * irR);
+  const S EL = L.p * K<S>(1.0 / GM1) + K<S>(0.5) * L.r * (L.u * L.u + L.v * L.v);
+  const S ER = R.p * K<S>(1.0 / GM1) + K<S>(0.5) * R.r * (R.u * R.u + R.v * R.v);
   const S sl = dsqrt(L.r), sr = dsqrt(R.r);
-  const S den = sl + sr;
-  const S qt = (sl * qL + sr * qR) / den;
-  const S HL = (EL + L.p) / L.r;
-  const S HR = (ER + R.p) / R.r;
-  const S Ht = (sl * HL + sr * HR) / den;
-  const S ut = (sl * L.u + sr * R.u) / den;
-  const S vt = (sl * L.v + sr * R.v) / den;
+  const S iden = drcp(sl + sr);
+  const S qt = (sl * qL + sr * qR) * iden;
+  const S HL = (EL + L.p) * irL;
+  const S HR = (ER + R.p) * irR;
+  const S Ht = (sl * HL + sr * HR) * iden;
+  const S ut = (sl * L.u + sr * R.u) * iden;
+  const S vt = (sl * L.v + sr * R.v) * iden;
   const S ct = dsqrt(dmaxc(K<S>(GM1) * (Ht - K<S>(0.5) * (ut * ut + vt * vt)), base_t<S>(1e-12)));
   const S SL = dmin(qL - cL, qt - ct);
   const S SR = dmax(qR + cR, qt + ct);
@@ -67,12 +76,13 @@ __host__ __device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& 
   S f3 = (E + p) * q;
   if (star) {
     const S skq = Sk - q;
-    const S f = r * skq / (Sk - SM);
+    const S d = drcp(Sk - SM);
+    const S f = r * skq * d;
     const S smq = SM - q;
     f0 = f0 + Sk * (f - r);
     f1 = f1 + Sk * (f * (u + smq * nx) - r * u);
     f2 = f2 + Sk * (f * (v + smq * ny) - r * v);
-    f3 = f3 + Sk * (f * (E / r + smq * (SM + p / (r * skq))) - E);
+    f3 = f3 + Sk * ((skq * E * d + f * smq * SM + smq * p * d) - E);
   }
   F[0] = f0 * area;
   F[1] = f1 * area;

@@ -59,7 +59,7 @@ __device__ __forceinline__ const T* met_yf(const CaseDev& cd) {
 }
 
 template <class T>
-__device__ __forceinline__ void load_cons(const T* U, int n, int idx, T q[4]) {
+__device__ __forceinline__ void load_cons(gptr<const T> U, int n, int idx, T q[4]) {
   q[0] = U[idx];
   q[1] = U[n + idx];
   q[2] = U[2 * n + idx];
@@ -129,12 +129,12 @@ __global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
   if (idx >= n) return;
   const int i = idx % ni, j = idx / ni;
   const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
-  const T* U = base_U<T>(cd);
-  const T* Si = met_Si<T>(cd);
-  const T* Sj = met_Sj<T>(cd);
+  const gptr<const T> U = gp(base_U<T>(cd));
+  const gptr<const T> Si = gp(met_Si<T>(cd));
+  const gptr<const T> Sj = gp(met_Sj<T>(cd));
   const size_t sj1 = (size_t)(nc + 1) * ni;
-  T* blk = tp<T>(cd.blk);
-  const size_t plane = (size_t)n;
+  const gptr<T> blk = gp(tp<T>(cd.blk));
+  const size_t plane = (size_t)cd.pstride;
   auto at = [&](int slot, int m) -> T& { return blk[((size_t)slot * 16 + m) * plane + idx]; };
 
   T qc[4], qim[4], qip[4];
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
   load_cons(U, n, j * ni + im, qim);
   load_cons(U, n, j * ni + ip, qip);
   T Dm[16];
-  const T dg = ctp<T>(cd.diag)[idx];
+  const T dg = gp(ctp<T>(cd.diag))[idx];
 #pragma unroll
   for (int m = 0; m < 16; ++m) Dm[m] = (m % 5 == 0) ? dg : T(0);
 
@@ -282,8 +282,8 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int i = (blockIdx.x - plan.blk_prefix[e]) * 64 + threadIdx.x;
   if (i >= ni) return;
-  T* blk = tp<T>(cd.blk);
-  const size_t plane = (size_t)n;
+  const gptr<T> blk = gp(tp<T>(cd.blk));
+  const size_t plane = (size_t)cd.pstride;
   T cprev[16];
 #pragma unroll
   for (int m = 0; m < 16; ++m) cprev[m] = T(0);
@@ -335,13 +335,6 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   }
 }
 
-// Value of x held by lane `c` of this lane's quad.
-template <class T>
-__device__ __forceinline__ T quad_get(T x, int c) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  return __shfl(x, (lane & ~3) | c, WAVE);
-}
-
 // Sweep part 1, thread per cell: g = inv (src - Bim x(i-1) - Bip x(i+1)) into d.
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev, int sweep,
@@ -353,19 +346,20 @@ __global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev
   const int idx = (blockIdx.x - plan.blk_prefix[e]) * 256 + threadIdx.x;
   if (idx >= n) return;
   const int i = idx % ni, j = idx / ni;
-  const T* __restrict__ blk = ctp<T>(cd.blk);
-  const size_t plane = (size_t)n;
+  const gptr<const T> blk = gp(ctp<T>(cd.blk));
+  const size_t plane = (size_t)cd.pstride;
   T rhs[4];
   if (src == VEC_NEG_R) {
+    const gptr<const double> Rg = gp(cd.R);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) rhs[r] = T(-cd.R[(size_t)r * n + idx]);
+    for (int r = 0; r < 4; ++r) rhs[r] = T(-Rg[(size_t)r * n + idx]);
   } else {
-    const T* __restrict__ b = vec_ptr<T>(cd, src);
+    const gptr<const T> b = vec_gptr<T>(cd, src);
 #pragma unroll
     for (int r = 0; r < 4; ++r) rhs[r] = b[(size_t)r * n + idx];
   }
   if (sweep > 0) {
-    const T* __restrict__ xp = vec_ptr<T>(cd, xprev);
+    const gptr<const T> xp = vec_gptr<T>(cd, xprev);
     const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
     T xm[4], xq[4];
 #pragma unroll
@@ -380,7 +374,7 @@ __global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev
         rhs[r] -= blk[(size_t)(3 * 16 + r * 4 + c) * plane + idx] * xm[c] +
                   blk[(size_t)(4 * 16 + r * 4 + c) * plane + idx] * xq[c];
   }
-  T* __restrict__ g = tp<T>(cd.d);
+  const gptr<T> g = gp(tp<T>(cd.d));
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     T s = T(0);
@@ -403,11 +397,11 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
   const int i = lb * 64 + (threadIdx.x >> 2);
   const bool live = i < ni;
   const int ic = live ? i : ni - 1;     // dead lanes shadow a real line and never store
-  const size_t plane = (size_t)n;
-  const T* __restrict__ P = ctp<T>(cd.blk) + (size_t)(2 * 16 + r * 4) * plane + ic;
-  const T* __restrict__ C = ctp<T>(cd.blk) + (size_t)(1 * 16 + r * 4) * plane + ic;
-  T* d = tp<T>(cd.d) + (size_t)r * n + ic;
-  T* __restrict__ x = vec_ptr<T>(cd, xout) + (size_t)r * n + ic;
+  const size_t plane = (size_t)cd.pstride;
+  const gptr<const T> P = gp(ctp<T>(cd.blk) + (size_t)(2 * 16 + r * 4) * plane + ic);
+  const gptr<const T> C = gp(ctp<T>(cd.blk) + (size_t)(1 * 16 + r * 4) * plane + ic);
+  const gptr<T> d = gp(tp<T>(cd.d) + (size_t)r * n + ic);
+  const gptr<T> x = gp(vec_ptr<T>(cd, xout) + (size_t)r * n + ic);
 
   // forward: d_j = g_j - P_j d_{j-1}, in place over g
   T bm[2][PF][4], bv[2][PF];
@@ -427,9 +421,8 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
     for (int q = 0; q < PF; ++q) {
       const int j = j0 + q;
       if (j < nc) {
-        T t = bv[b][q];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) t -= bm[b][q][c] * quad_get(dr, c);
+        const T t = bv[b][q] - bm[b][q][0] * quad_bcast<0>(dr) - bm[b][q][1] * quad_bcast<1>(dr) -
+                    bm[b][q][2] * quad_bcast<2>(dr) - bm[b][q][3] * quad_bcast<3>(dr);
         dr = t;
         if (live) d[(size_t)j * ni] = dr;
       }
@@ -460,9 +453,8 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
     for (int q = 0; q < PF; ++q) {
       const int j = j0 - q;
       if (j >= 0) {
-        T t = bv[b][q];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) t -= bm[b][q][c] * quad_get(xr, c);
+        const T t = bv[b][q] - bm[b][q][0] * quad_bcast<0>(xr) - bm[b][q][1] * quad_bcast<1>(xr) -
+                    bm[b][q][2] * quad_bcast<2>(xr) - bm[b][q][3] * quad_bcast<3>(xr);
         xr = t;
         if (live) {
           x[(size_t)j * ni] = xr;
@@ -486,7 +478,10 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
   }
 }
 
-constexpr int SWEEP_PF = 8;
+#ifndef A2D_SWEEP_PF
+#define A2D_SWEEP_PF 8
+#endif
+constexpr int SWEEP_PF = A2D_SWEEP_PF;   // j steps loaded ahead by the line recurrences
 
 }  // namespace
 
@@ -510,6 +505,19 @@ void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks
     hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate);
     hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
                        want_norm, gate);
+  }
+}
+
+void launch_sweep_part(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int part, int prec,
+                       hipStream_t s) {
+  if (part == 0) {
+    if (prec == 0) hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0);
+    else hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0);
+  } else {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0);
+    else
+      hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0);
   }
 }
 

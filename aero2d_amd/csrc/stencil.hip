@@ -29,27 +29,27 @@ constexpr int NT = 256;
 
 template <class T>
 struct CaseView {
-  const T* U;     // base state (conserved)
-  const T* vol;
-  const T* Si;
-  const T* Sj;
-  const T* xf;
-  const T* yf;
+  gptr<const T> U;     // base state (conserved)
+  gptr<const T> vol;
+  gptr<const T> Si;
+  gptr<const T> Sj;
+  gptr<const T> xf;
+  gptr<const T> yf;
 };
 
 template <class T>
 __device__ __forceinline__ CaseView<T> view(const CaseDev& cd) {
   if constexpr (std::is_same<T, double>::value) {
-    return {cd.U, cd.vol, cd.Si, cd.Sj, cd.xf, cd.yf};
+    return {gp(cd.U), gp(cd.vol), gp(cd.Si), gp(cd.Sj), gp(cd.xf), gp(cd.yf)};
   } else {
-    return {ctp<T>(cd.Ut), ctp<T>(cd.volt), ctp<T>(cd.Sit), ctp<T>(cd.Sjt), ctp<T>(cd.xft),
-            ctp<T>(cd.yft)};
+    return {gp(ctp<T>(cd.Ut)), gp(ctp<T>(cd.volt)), gp(ctp<T>(cd.Sit)), gp(ctp<T>(cd.Sjt)),
+            gp(ctp<T>(cd.xft)), gp(ctp<T>(cd.yft))};
   }
 }
 
 // Primitive state of cell (i, j) as scalar type S (tangent from v when dual).
 template <class S, class T>
-__device__ __forceinline__ W4<S> load_prim(const T* U, const T* v, int n, int idx) {
+__device__ __forceinline__ W4<S> load_prim(gptr<const T> U, gptr<const T> v, int n, int idx) {
   if constexpr (std::is_same<S, T>::value) {
     return cons_to_prim<S>(U[idx], U[n + idx], U[2 * n + idx], U[3 * n + idx]);
   } else {
@@ -71,6 +71,10 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   __shared__ T sD[TAN ? 4 : 1][TAN ? HJ : 1][TAN ? HI : 1];
   __shared__ T sFi[4][TJ][TI + 1];
   __shared__ T sFj[4][TJ + 1][TI];
+  // limited slopes per cell: i-direction for cells i0-1..i0+TI of the tile's
+  // rows, j-direction for rows j0-1..j0+TJ of its columns (value, tangent)
+  __shared__ T sSi[TAN ? 2 : 1][4][TJ][TI + 2];
+  __shared__ T sSj[TAN ? 2 : 1][4][TJ + 2][TI];
   __shared__ double sred[NT / WAVE];
 
   const int e = plan_entry(plan, blockIdx.x);
@@ -84,7 +88,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   const int tid = threadIdx.x;
 
   const CaseView<T> cv = view<T>(cd);
-  const T* vt = TAN ? vec_ptr<T>(cd, vin) : nullptr;
+  const gptr<const T> vt = TAN ? vec_gptr<T>(cd, vin) : gptr<const T>(nullptr);
   const Inf<T> inf = {T(sc->winf[0]), T(sc->winf[1]), T(sc->winf[2]), T(sc->winf[3])};
   const double circ_d = (circ_override == circ_override) ? circ_override : sc->circ;
   const T vort = kn.vortex ? T(1) : T(0);
@@ -140,6 +144,53 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
     }
   }
 
+  // ---- limited slopes, once per cell and direction (MUSCL, van Albada) -------
+  // The left state of a face is W0 + slope(W0)/2 and its right state
+  // Wp - slope(Wp)/2 (aero2d/euler.py:211-215); each cell's slope serves the
+  // faces on both of its sides.
+  if (kn.order == 2) {
+    constexpr int NSI = (TI + 2) * TJ, NSJ = TI * (TJ + 2);
+    for (int c = tid; c < NSI + NSJ; c += NT) {
+      const bool isI = c < NSI;
+      int hi, hj, a0, a1;
+      if (isI) {
+        a0 = c % (TI + 2);       // slope index, cell hi = a0 + 1
+        a1 = c / (TI + 2);       // tile row, hj = a1 + 2
+        hi = a0 + 1;
+        hj = a1 + 2;
+      } else {
+        const int g = c - NSI;
+        a0 = g % TI;             // tile column, hi = a0 + 2
+        a1 = g / TI;             // slope index, cell hj = a1 + 1
+        hi = a0 + 2;
+        hj = a1 + 1;
+      }
+      const int di = isI ? 1 : 0, dj = isI ? 0 : 1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        S wm, w0, wp;
+        if constexpr (TAN) {
+          wm = S(sW[k][hj - dj][hi - di], sD[k][hj - dj][hi - di]);
+          w0 = S(sW[k][hj][hi], sD[k][hj][hi]);
+          wp = S(sW[k][hj + dj][hi + di], sD[k][hj + dj][hi + di]);
+        } else {
+          wm = sW[k][hj - dj][hi - di];
+          w0 = sW[k][hj][hi];
+          wp = sW[k][hj + dj][hi + di];
+        }
+        const S sl = van_albada(w0 - wm, wp - w0, eps[k]);
+        if (isI) {
+          sSi[0][k][a1][a0] = val(sl);
+          if constexpr (TAN) sSi[TAN ? 1 : 0][k][a1][a0] = tan_of(sl);
+        } else {
+          sSj[0][k][a1][a0] = val(sl);
+          if constexpr (TAN) sSj[TAN ? 1 : 0][k][a1][a0] = tan_of(sl);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
   // ---- face fluxes ----------------------------------------------------------
   for (int f = tid; f < NFI + NFJ; f += NT) {
     const bool isI = f < NFI;
@@ -167,22 +218,44 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
     }
     if (!valid) continue;
     S wl[4], wr[4];
+    const int hLi = h0i + di, hLj = h0j + dj;          // cell left of the face
+    const int hRi = h0i + 2 * di, hRj = h0j + 2 * dj;  // cell right of the face
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      S q[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int hi = h0i + m * di, hj = h0j + m * dj;
-        if constexpr (TAN) q[m] = S(sW[k][hj][hi], sD[k][hj][hi]);
-        else q[m] = sW[k][hj][hi];
+      S q1, q2;
+      if constexpr (TAN) {
+        q1 = S(sW[k][hLj][hLi], sD[k][hLj][hLi]);
+        q2 = S(sW[k][hRj][hRi], sD[k][hRj][hRi]);
+      } else {
+        q1 = sW[k][hLj][hLi];
+        q2 = sW[k][hRj][hRi];
       }
       if (kn.order == 2) {
-        const S a = q[1] - q[0], b = q[2] - q[1], c = q[3] - q[2];
-        wl[k] = q[1] + K<S>(0.5) * van_albada(a, b, eps[k]);
-        wr[k] = q[2] - K<S>(0.5) * van_albada(b, c, eps[k]);
+        // slope indices: i-face li uses i-slopes li (left) and li+1 (right) of
+        // row lj; j-face lj uses j-slopes lj and lj+1 of column li
+        S s1, s2;
+        if (isI) {
+          if constexpr (TAN) {
+            s1 = S(sSi[0][k][lj][li], sSi[TAN ? 1 : 0][k][lj][li]);
+            s2 = S(sSi[0][k][lj][li + 1], sSi[TAN ? 1 : 0][k][lj][li + 1]);
+          } else {
+            s1 = sSi[0][k][lj][li];
+            s2 = sSi[0][k][lj][li + 1];
+          }
+        } else {
+          if constexpr (TAN) {
+            s1 = S(sSj[0][k][lj][li], sSj[TAN ? 1 : 0][k][lj][li]);
+            s2 = S(sSj[0][k][lj + 1][li], sSj[TAN ? 1 : 0][k][lj + 1][li]);
+          } else {
+            s1 = sSj[0][k][lj][li];
+            s2 = sSj[0][k][lj + 1][li];
+          }
+        }
+        wl[k] = q1 + K<S>(0.5) * s1;
+        wr[k] = q2 - K<S>(0.5) * s2;
       } else {
-        wl[k] = q[1];
-        wr[k] = q[2];
+        wl[k] = q1;
+        wr[k] = q2;
       }
     }
     const W4<S> L = {dmaxc(wl[0], T(1e-6)), wl[1], wl[2], dmaxc(wl[3], T(1e-6))};
@@ -210,7 +283,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
     for (int k = 0; k < 4; ++k)
       r[k] = (sFi[k][lj][li + 1] - sFi[k][lj][li]) + (sFj[k][lj + 1][li] - sFj[k][lj][li]);
     if constexpr (!TAN) {
-      double* R = cd.R;
+      const gptr<double> R = gp(cd.R);
 #pragma unroll
       for (int k = 0; k < 4; ++k) R[k * n + idx] = r[k];
       const double vol = cd.vol[idx];
@@ -241,14 +314,14 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
         for (int k = 0; k < 4; ++k) Ut[k * n + idx] = float(U[k * n + idx]);
       }
     } else {
-      T* out = vec_ptr<T>(cd, vout);
-      const T dg = use_diag ? ctp<T>(cd.diag)[idx] : T(0);
+      const gptr<T> out = vec_gptr<T>(cd, vout);
+      const T dg = use_diag ? gp(ctp<T>(cd.diag))[idx] : T(0);
       if (mode == 0) {
         const T s = (kidx >= 0) ? T(sc->s[kidx]) : T(1);
 #pragma unroll
         for (int k = 0; k < 4; ++k) out[k * n + idx] = s * (dg * vt[k * n + idx] + r[k]);
       } else {
-        const double* R = cd.R;
+        const gptr<const double> R = gp(cd.R);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           out[k * n + idx] = T(-R[k * n + idx]) - (dg * vt[k * n + idx] + r[k]);
@@ -333,7 +406,7 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate) {
   if (gate && !sc->active) return;
   const int ni = cd.ni, n = cd.n, nc = cd.nc;
   const CaseView<T> cv = view<T>(cd);
-  const T* v = vec_ptr<T>(cd, vin);
+  const gptr<const T> v = vec_gptr<T>(cd, vin);
   const double q = 0.5 * (sc->winf[1] * sc->winf[1] + sc->winf[2] * sc->winf[2]);
   double dfx = 0.0, dfy = 0.0;
   for (int i = threadIdx.x; i < ni; i += NT) {

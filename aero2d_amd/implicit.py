@@ -25,14 +25,27 @@ class NewtonLoop:
     (aero2d/implicit.py:211-244), one object per solve. `step_out` rows come
     from a2d_newton_step."""
 
-    def __init__(self, nb, cfl0, cfl_max):
+    def __init__(self, nb, cfl0, cfl_max, cfl_late=None, late_below=0.0):
+        """cfl_late / late_below: once a case's rn/r0 falls below late_below its
+        CFL cap rises from cfl_max to cfl_late (None keeps the reference's
+        single cap)."""
         self.cfl0, self.cfl_max = cfl0, cfl_max
+        self.cfl_late, self.late_below = cfl_late, late_below
         self.cfl = np.full(nb, float(cfl0))
         self.r0 = np.full(nb, np.nan)
         self.fails = np.zeros(nb, int)
         self.frozen = np.zeros(nb, bool)
         self.iters = np.zeros(nb, int)
         self.last_rn = np.full(nb, np.nan)
+
+    def reset(self, idx):
+        """Fresh per-case state for entries idx (a new case in a reused slot)."""
+        self.cfl[idx] = float(self.cfl0)
+        self.r0[idx] = np.nan
+        self.fails[idx] = 0
+        self.frozen[idx] = False
+        self.iters[idx] = 0
+        self.last_rn[idx] = np.nan
 
     def update(self, idx, out):
         """Apply one step's results for batch entries idx; returns rn (NaN when
@@ -47,7 +60,10 @@ class NewtonLoop:
         self.r0[idx] = np.where(first, r0, self.r0[idx])
         r0 = self.r0[idx]
         ser = self.cfl0 * (r0 / np.where(np.isfinite(rn), np.maximum(rn, 1e-30), r0))
-        cfl = np.clip(ser, self.cfl0, self.cfl_max)
+        cap = np.full(len(idx), float(self.cfl_max))
+        if self.cfl_late is not None:
+            cap = np.where(np.isfinite(rn) & (rn / r0 < self.late_below), float(self.cfl_late), cap)
+        cfl = np.clip(ser, self.cfl0, cap)
         cfl = np.where(om < 1.0, np.maximum(cfl * 0.5, 1.0), cfl)
         cfl = np.where(~finite, 1.0, cfl)
         self.cfl[idx] = cfl

@@ -1,0 +1,216 @@
+"""Serving loop: continuous batching of single-case solves on one GPU.
+
+Requests (an O-grid and a freestream) arrive at any time. A request is
+admitted when the device pool's HBM budget has room for its footprint
+(a2d_case_bytes: state, metrics, Krylov basis and preconditioner blocks);
+its state then stays on the device until it finishes. Every Newton step runs
+over all admitted cases at once (one a2d_newton_step call), so requests that
+arrive while others are in flight join the next step and their work overlaps
+on the device; a case leaves as soon as it converges (rn/r0 <= tol), freezes
+or reaches its step limit, and its memory admits the next request.
+
+Each case's result depends only on that case: kernels reduce per case in a
+fixed order and the host keeps per-case CFL/failure state, so a request gives
+the same answer alone and in any mix (tests/test_gpu_serving.py).
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .batch import Case
+from .implicit import NewtonLoop
+
+__all__ = ["ServeResult", "SolveServer", "serve_stream", "MIXED_SERVING_KNOBS"]
+
+# Mixed-form serving defaults: GMRES(30) with early exit at gmres_tol, CFL cap
+# 1000 (measured on the transonic NACA 0012/2412 workload: all cases converge
+# to the FP64 answer within 1e-8 in Cl with these).
+MIXED_SERVING_KNOBS = dict(restart=30, early_exit=1, cfl_max=1000.0)
+FP64_SERVING_KNOBS = dict(restart=50, early_exit=0, cfl_max=300.0)
+
+
+@dataclass
+class ServeResult:
+    name: str
+    cl: float
+    cd: float
+    cm: float
+    converged: bool
+    frozen: bool
+    iterations: int
+    rn_over_r0: float
+    t_arrive: float
+    t_admit: float
+    t_done: float
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_arrive
+
+
+class _Slot:
+    __slots__ = ("case", "t_arrive", "t_admit", "future", "cid")
+
+    def __init__(self, case, t_arrive, t_admit, future, cid):
+        self.case, self.t_arrive, self.t_admit, self.future, self.cid = case, t_arrive, t_admit, future, cid
+
+
+class SolveServer:
+    """Continuous-batching solver over one device pool.
+
+    precision "mixed" (default) or "fp64"; hbm_budget in bytes bounds the
+    admitted cases' total footprint; max_batch bounds their count; tol and
+    n_iter are per request. Use submit() from any thread with start(), or
+    drive it synchronously with step() (serve_stream does)."""
+
+    def __init__(self, precision: str = "mixed", hbm_budget: int = 64 << 30, max_batch: int = 1024,
+                 tol: float = 1e-10, n_iter: int = 800, cfl0: float = 10.0,
+                 restart: int | None = None, early_exit: int | None = None,
+                 cfl_max: float | None = None):
+        defaults = MIXED_SERVING_KNOBS if precision == "mixed" else FP64_SERVING_KNOBS
+        self.precision = precision
+        self.prec = _abi.PREC_MIXED if precision == "mixed" else _abi.PREC_FP64
+        self.restart = defaults["restart"] if restart is None else restart
+        kn = _abi.default_knobs()
+        kn.gmres_restart = self.restart
+        kn.early_exit = defaults["early_exit"] if early_exit is None else int(early_exit)
+        self.pool = _abi.Pool(hbm_budget, kn)
+        self.budget = hbm_budget
+        self.max_batch = max_batch
+        self.tol, self.n_iter = tol, n_iter
+        self.loop = NewtonLoop(4096, cfl0, defaults["cfl_max"] if cfl_max is None else cfl_max)
+        self.active: dict[int, _Slot] = {}
+        self.waiting: list[tuple[Case, float, Future]] = []
+        self.t0 = time.perf_counter()
+        self._inbox: queue.Queue = queue.Queue()
+        self._thread = None
+        self._stop = threading.Event()
+        self.steps = 0
+
+    # ---- clock -------------------------------------------------------------------
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    # ---- admission -------------------------------------------------------------------
+    def footprint(self, case: Case) -> int:
+        ni, nj = case.grid.x.shape
+        return _abi.case_bytes(ni, nj - 1, self.prec, self.restart)
+
+    def enqueue(self, case: Case, t_arrive: float | None = None, future: Future | None = None) -> Future:
+        """Queue a request (not thread-safe; use submit() across threads)."""
+        f = future if future is not None else Future()
+        if self.footprint(case) > self.budget:
+            f.set_exception(_abi.BudgetExceeded("a2d_case_add", -2, "request larger than the HBM budget"))
+            return f
+        self.waiting.append((case, self.now() if t_arrive is None else t_arrive, f))
+        return f
+
+    def _admit(self):
+        while self.waiting and len(self.active) < self.max_batch:
+            case, t_arr, fut = self.waiting[0]
+            if self.pool.used_bytes() + self.footprint(case) > self.budget:
+                break                                   # wait for memory, keep arrival order
+            self.waiting.pop(0)
+            cid = self.pool.add_case(case.grid.x, case.grid.y, case.mach, case.alpha, self.prec)
+            self.loop.reset([cid])
+            self.active[cid] = _Slot(case, t_arr, self.now(), fut, cid)
+
+    def _finish(self, cid: int):
+        s = self.active.pop(cid)
+        f3 = np.empty(3)
+        cp = np.empty(s.case.grid.x.shape[0])
+        circ = np.empty(1)
+        _abi.call("a2d_forces", self.pool.handle, cid, _abi.dptr(f3), _abi.dptr(cp), _abi.dptr(circ))
+        self.pool.remove_case(cid)
+        rel = self.loop.last_rn[cid] / self.loop.r0[cid]
+        res = ServeResult(s.case.name, float(f3[0]), float(f3[1]), float(f3[2]),
+                          bool(rel <= self.tol and not self.loop.frozen[cid]), bool(self.loop.frozen[cid]),
+                          int(self.loop.iters[cid]), float(rel), s.t_arrive, s.t_admit, self.now())
+        s.future.set_result(res)
+
+    # ---- one Newton step over everything admitted --------------------------------------------
+    def step(self) -> int:
+        """Admit what fits, take one Newton step over all active cases and
+        retire the finished ones. Returns the number of active cases stepped."""
+        self._admit()
+        if not self.active:
+            return 0
+        ids = np.array(sorted(self.active), np.int32)
+        cfl = np.ascontiguousarray(self.loop.cfl[ids])
+        out = np.empty((len(ids), 10))
+        _abi.call("a2d_newton_step", self.pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl),
+                  _abi.dptr(out))
+        rn, _ = self.loop.update(ids, out)
+        self.steps += 1
+        done = (rn / self.loop.r0[ids] <= self.tol) | self.loop.frozen[ids] | (self.loop.iters[ids] >= self.n_iter)
+        for cid in ids[done]:
+            self._finish(int(cid))
+        return len(ids)
+
+    # ---- threaded API ----------------------------------------------------------------------------
+    def submit(self, case: Case) -> Future:
+        """Thread-safe: queue a request for the background loop (start() first)."""
+        f = Future()
+        self._inbox.put((case, self.now(), f))
+        return f
+
+    def start(self):
+        def run():
+            while not self._stop.is_set():
+                while True:
+                    try:
+                        case, t, f = self._inbox.get_nowait()
+                    except queue.Empty:
+                        break
+                    self.enqueue(case, t, f)
+                if self.step() == 0:
+                    time.sleep(0.001)
+        self._thread = threading.Thread(target=run, daemon=True)
+        self._thread.start()
+        return self
+
+    def close(self):
+        if self._thread is not None:
+            self._stop.set()
+            self._thread.join()
+            self._thread = None
+        self.pool.close()
+
+
+def serve_stream(requests, precision: str = "mixed", hbm_budget: int = 64 << 30, tol: float = 1e-10,
+                 horizon: float | None = None, **kw):
+    """Replay an open-loop request stream (objects with .t_arrive and .case, t
+    in seconds from the start) against a SolveServer in real time. Requests are
+    released when the wall clock reaches their arrival time; the run ends when
+    all released requests finish or, with `horizon`, at that many seconds
+    (unfinished requests are reported). Returns (results, server, wall)."""
+    srv = SolveServer(precision, hbm_budget, tol=tol, **kw)
+    futures = []
+    k = 0
+    reqs = sorted(requests, key=lambda r: r.t_arrive)
+    try:
+        srv.t0 = time.perf_counter()
+        while True:
+            now = srv.now()
+            while k < len(reqs) and reqs[k].t_arrive <= now:
+                futures.append(srv.enqueue(reqs[k].case, reqs[k].t_arrive))
+                k += 1
+            if horizon is not None and now >= horizon:
+                break
+            if srv.step() == 0:
+                if k >= len(reqs) and not srv.waiting:
+                    break
+                if k < len(reqs):
+                    time.sleep(max(0.0, min(0.01, reqs[k].t_arrive - srv.now())))
+        wall = srv.now()
+    finally:
+        srv.close()
+    results = [f.result() for f in futures if f.done() and f.exception() is None]
+    return results, srv, wall

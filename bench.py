@@ -5,7 +5,7 @@ solved to a relative density residual rn/r0 <= 1e-10 — the uniform FP64 batch
 whose throughput is the headline metric. Airfoils, Mach numbers and angles
 are drawn from a fixed seed: m ~ U[0, 4]%, p ~ U[0.3, 0.5], t ~ U[9, 15]%,
 M ~ U[0.3, 0.7], alpha ~ U[-4, 6] deg; grids ogrid(nj = 129, radius 20,
-ds0 = 20 (q - 1)/(q^128 - 1), q = 1.1^(64/128)).
+ds0 = 20 (q - 1)/(q^128 - 1), q = 1.1^(64/128); aero2d_amd.workloads.config2).
 
 A step is one complete batched solve of all cases (warmup steps untimed).
 The line also carries the mixed-precision form on the same batch, per-case
@@ -33,20 +33,8 @@ RES_BYTES_PER_CELL = 96
 
 
 def make_cases(n_cases, ni, nc, seed):
-    from aero2d_amd import naca4_params, ogrid, self_similar_spacing
-    from aero2d_amd.batch import Case
-    rng = np.random.default_rng(seed)
-    cases = []
-    ds0 = self_similar_spacing(nc, 20.0)
-    for k in range(n_cases):
-        m = rng.uniform(0.0, 0.04)
-        p = rng.uniform(0.3, 0.5)
-        t = rng.uniform(0.09, 0.15)
-        M = rng.uniform(0.3, 0.7)
-        a = rng.uniform(-4.0, 6.0)
-        g = ogrid(naca4_params(m, p, t, ni // 2 + 1), nj=nc + 1, ds0=ds0, radius=20.0)
-        cases.append(Case(g, M, a, f"case{k}"))
-    return cases
+    from aero2d_amd.workloads import config2
+    return config2(n_cases, ni, nc, seed)
 
 
 def time_kernels(cases, precision, reps=20):
@@ -61,7 +49,8 @@ def time_kernels(cases, precision, reps=20):
         out = np.empty((len(ids), 10))
         cfl = np.full(len(ids), 10.0)
         _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl), _abi.dptr(out))
-        names = ["residual", "jvp", "line_sweep", "cgs_dots", "assemble", "factor", "cgs_update"]
+        names = ["residual", "jvp", "line_sweep", "cgs_dots", "assemble", "factor", "cgs_update",
+                 "copy", "sweep_rhs", "sweep_seq"]
         res = {}
         for w, name in enumerate(names):
             ms = np.zeros(1)
@@ -88,16 +77,115 @@ def cpu_baseline(case, gpu_iters, steps=2):
                        f"{gpu_iters:.1f} Newton steps per case")}
 
 
+def per_case_errors(a, b):
+    """|Cl|, |Cd|, |Cm| differences per case between two batch results."""
+    return {"dcl": np.abs(a.cl - b.cl).tolist(), "dcd": np.abs(a.cd - b.cd).tolist(),
+            "dcm": np.abs(a.cm - b.cm).tolist()}
+
+
+def run_config3(a):
+    """Transonic mixed vs FP64: 32 NACA 0012/2412 cases on 512 x 256."""
+    from aero2d_amd.batch import solve_batch, default_knobs
+    from aero2d_amd.serve import MIXED_SERVING_KNOBS
+    from aero2d_amd.workloads import config3
+    cases = config3(a.cases or 32)
+    n_iter = a.n_iter or 800
+    r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter)
+    print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
+    kn = default_knobs("mixed", restart=MIXED_SERVING_KNOBS["restart"])
+    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, knobs=kn,
+                      cfl_max=MIXED_SERVING_KNOBS["cfl_max"])
+    print(f"# mixed: {rmx.wall:.1f} s, converged {int(rmx.converged.sum())}", file=sys.stderr, flush=True)
+    v64 = r64.converged.sum() / r64.wall
+    vmx = rmx.converged.sum() / rmx.wall
+    err = per_case_errors(rmx, r64)
+    return {"metric": METRIC, "value": float(vmx), "unit": "cases/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": 1e3 * rmx.wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 inner / f64 outer", "data": "synthetic",
+            "config": {"workload": f"{len(cases)} transonic NACA 0012/2412 cases on 512x256 O-grids, "
+                                   f"mixed vs FP64, rn/r0 <= {a.tol:g}"},
+            "fp64": {"value": float(v64), "converged": int(r64.converged.sum()), "wall_s": r64.wall,
+                     "newton_steps_mean": float(r64.iterations.mean())},
+            "mixed": {"value": float(vmx), "converged": int(rmx.converged.sum()), "wall_s": rmx.wall,
+                      "newton_steps_mean": float(rmx.iterations.mean()),
+                      "arnoldi_per_newton_mean": float(rmx.arnoldi.mean()),
+                      "knobs": MIXED_SERVING_KNOBS},
+            "speedup_mixed_vs_fp64": float(vmx / v64) if v64 > 0 else None,
+            "max_abs": {k: float(max(v)) for k, v in err.items()}, "per_case": err,
+            "cases": [(c.name, c.mach, c.alpha) for c in cases]}
+
+
+def run_config4(a):
+    """Uneven persistent batch: 256 random sections, four sizes, mixed form,
+    autotuned tile shapes."""
+    from aero2d_amd.autotune import shapes_for
+    from aero2d_amd.batch import solve_batch, default_knobs
+    from aero2d_amd.serve import MIXED_SERVING_KNOBS
+    from aero2d_amd.workloads import config4
+    cases, redraws = config4(a.cases or 256)
+    shapes = shapes_for(cases, "mixed")
+    kn = default_knobs("mixed", restart=MIXED_SERVING_KNOBS["restart"])
+    r = solve_batch(cases, "mixed", tol=a.tol, n_iter=a.n_iter or 800, knobs=kn, shapes=shapes,
+                    cfl_max=MIXED_SERVING_KNOBS["cfl_max"])
+    sizes = {}
+    for c, conv, it in zip(cases, r.converged, r.iterations):
+        key = f"{c.grid.x.shape[0]}x{c.grid.x.shape[1] - 1}"
+        d = sizes.setdefault(key, {"cases": 0, "converged": 0, "newton_steps": 0})
+        d["cases"] += 1
+        d["converged"] += int(conv)
+        d["newton_steps"] += int(it)
+    lat = np.percentile(r.t_done, [50, 99]) * 1e3
+    return {"metric": METRIC, "value": float(r.converged.sum() / r.wall), "unit": "cases/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": 1e3 * r.wall, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 inner / f64 outer", "data": "synthetic",
+            "config": {"workload": f"{len(cases)} random NACA 4-digit cases on 128x64..512x256, mixed, "
+                                   f"rn/r0 <= {a.tol:g}", "redraws": redraws},
+            "converged": int(r.converged.sum()), "latency_ms": {"p50": float(lat[0]), "p99": float(lat[1])},
+            "tile_shapes": {f"{k[0]}x{k[1]}": v for k, v in shapes.items()}, "by_size": sizes}
+
+
+def run_config5(a):
+    """Serving: open-loop Poisson streams at several rates, mixed (and FP64)."""
+    from aero2d_amd.serve import serve_stream
+    from aero2d_amd.workloads import config5
+    rates = [float(x) for x in a.rates.split(",")]
+    out = {"metric": METRIC, "unit": "requests/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": f"open-loop Poisson single-case solves, NACA 0012/2412 transonic, "
+                                  f"128x64..768x384, {a.horizon:g} s per rate, rn/r0 <= {a.tol:g}"},
+           "rates": {}}
+    for prec in a.precisions.split(","):
+        for rate in rates:
+            reqs = config5(rate, max(8, int(rate * a.horizon)))
+            res, srv, wall = serve_stream(reqs, prec, tol=a.tol, horizon=a.horizon)
+            released = sum(1 for q in reqs if q.t_arrive <= wall)
+            lat = np.array([x.latency for x in res]) if res else np.array([np.nan])
+            entry = {"released": released, "completed": len(res),
+                     "converged": int(sum(x.converged for x in res)),
+                     "throughput": len(res) / wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "newton_steps": srv.steps}
+            out["rates"].setdefault(prec, {})[str(rate)] = entry
+            print(f"# {prec} rate {rate}: {entry}", file=sys.stderr, flush=True)
+    best = max(e["throughput"] for e in out["rates"].get("mixed", out["rates"][a.precisions.split(',')[0]]).values())
+    out["value"] = best
+    out["dtype"] = "f32 inner / f64 outer"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4", "config5"])
+    ap.add_argument("--rates", default="10,25,50,100,200,400")
+    ap.add_argument("--horizon", type=float, default=30.0)
+    ap.add_argument("--precisions", default="mixed,fp64")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cases", type=int, default=64)
+    ap.add_argument("--cases", type=int, default=None)
     ap.add_argument("--ni", type=int, default=256)
     ap.add_argument("--nc", type=int, default=128)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--n-iter", type=int, default=150)
+    ap.add_argument("--n-iter", type=int, default=None)
     ap.add_argument("--seed", type=int, default=20261203 + 2)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,10 +196,19 @@ def main():
               "a multi-GPU run would shard the batch across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload != "config2":
+        fn = {"config3": run_config3, "config4": run_config4, "config5": run_config5}[a.workload]
+        print(json.dumps(fn(a)), flush=True)
+        return
+
     from aero2d_amd.batch import solve_batch
 
+    a.cases = a.cases or 64
+    a.n_iter = a.n_iter or 150
     cases = make_cases(a.cases, a.ni, a.nc, a.seed)
     print(f"# {len(cases)} cases on {a.ni}x{a.nc} grids", file=sys.stderr, flush=True)
+
+    uploads = []
 
     def run(precision):
         for w in range(a.warmup):
@@ -121,6 +218,7 @@ def main():
         walls, convs, res = [], [], None
         for s in range(a.steps):
             res = solve_batch(cases, precision, tol=a.tol, n_iter=a.n_iter)
+            uploads.append(res.upload_s)
             walls.append(res.wall)
             convs.append(int(res.converged.sum()))
             print(f"# {precision} step {s}: {res.wall:.2f} s, converged {convs[-1]}/{len(cases)}, "
@@ -141,6 +239,8 @@ def main():
                    "gmres": "fixed 50 Arnoldi steps (reference semantics)"},
         "converged": conv64 // a.steps, "latency_ms": {"p50": float(lat[0]), "p99": float(lat[1])},
         "newton_steps_mean": float(r64.iterations.mean()),
+        "upload_s_per_step": float(np.mean(uploads)),
+        "value_pcie_inclusive": conv64 / (wall64 + float(np.sum(uploads[:a.steps]))),
     }
     if not a.no_mixed:
         rmx, wallmx, convmx = run("mixed")

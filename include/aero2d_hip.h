@@ -118,7 +118,8 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
  * stream between two HIP events and returns the mean duration in ms:
  *   0 residual (FP64)   1 Jacobian-vector product   2 line sweep with i-coupling
  *   3 Gram-Schmidt dots (k = 25)   4 block assembly   5 line factorisation
- *   6 Gram-Schmidt update (k = 25)
+ *   6 Gram-Schmidt update (k = 25)   7 plain vector copy (4 words/cell each way)
+ *   8 the per-cell pass of a coupled line sweep   9 the line recurrences alone
  * The cases must have taken a Newton step (their blocks and Krylov vectors
  * hold data); the state is not changed. */
 int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms);

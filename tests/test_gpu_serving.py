@@ -1,0 +1,89 @@
+"""Serving loop and batch behaviour on the GPU.
+
+The results of a case must not depend on which other cases share its Newton
+steps: all reductions are per case and in a fixed order, and the host keeps
+per-case CFL and failure state. Tolerance: 1e-12 absolute on Cl, Cd, Cm (the
+kernels give bit-identical per-case arithmetic; the margin only absorbs
+printing)."""
+import numpy as np
+import pytest
+
+from aero2d_amd import _abi, naca4, naca4_params, ogrid
+from aero2d_amd.batch import Case, solve_batch
+from aero2d_amd.serve import SolveServer, serve_stream
+from aero2d_amd.workloads import Request
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    g1 = ogrid(naca4("0012", 33), nj=17, ds0=1e-2, radius=10.0)            # 64 x 16
+    g2 = ogrid(naca4("2412", 49), nj=25, ds0=6e-3, radius=10.0)            # 96 x 24
+    g3 = ogrid(naca4_params(0.03, 0.35, 0.10, 41), nj=21, ds0=8e-3, radius=10.0)  # 80 x 20
+    return [Case(g1, 0.5, 2.0, "a"), Case(g2, 0.6, 1.0, "b"), Case(g3, 0.45, -1.5, "c"),
+            Case(g1, 0.55, 0.5, "d"), Case(g2, 0.4, 3.0, "e")]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_result_is_the_same_alone_and_in_the_mix(precision):
+    cases = _cases()
+    alone = [serve_stream([Request(0.0, c)], precision, tol=1e-9)[0][0] for c in cases[:2]]
+    reqs = [Request(0.02 * k, c) for k, c in enumerate(cases)]
+    mix, srv, wall = serve_stream(reqs, precision, tol=1e-9)
+    by = {r.name: r for r in mix}
+    assert len(mix) == len(cases) and all(r.converged for r in mix)
+    for a in alone:
+        m = by[a.name]
+        assert a.iterations == m.iterations
+        for f in ("cl", "cd", "cm"):
+            assert abs(getattr(a, f) - getattr(m, f)) <= 1e-12, (a.name, f)
+
+
+def test_batch_result_is_independent_of_batch_composition():
+    cases = _cases()
+    full = solve_batch(cases, "fp64", tol=1e-9, n_iter=300)
+    sub = solve_batch([cases[3], cases[1]], "fp64", tol=1e-9, n_iter=300)
+    assert np.array_equal(full.cl[[3, 1]], sub.cl) and np.array_equal(full.cd[[3, 1]], sub.cd)
+    assert np.array_equal(full.iterations[[3, 1]], sub.iterations)
+
+
+def test_admission_waits_for_hbm_and_keeps_arrival_order():
+    g = _cases()[1].grid                                    # all requests 96 x 24
+    cases = [Case(g, 0.4 + 0.05 * k, 0.5 * k, f"r{k}") for k in range(5)]
+    one = _abi.case_bytes(96, 24, _abi.PREC_MIXED, 30)
+    srv = SolveServer("mixed", hbm_budget=int(2.2 * one), tol=1e-8)
+    try:
+        futs = [srv.enqueue(c, 0.0) for c in cases]
+        peak = 0
+        while any(not f.done() for f in futs):
+            peak = max(peak, srv.step())
+            assert srv.pool.used_bytes() <= srv.budget
+        assert 1 <= peak <= 2
+        res = [f.result() for f in futs]
+        assert all(r.converged for r in res)
+        admits = [r.t_admit for r in res]
+        assert admits == sorted(admits)
+    finally:
+        srv.close()
+
+
+def test_request_larger_than_budget_is_refused():
+    c = _cases()[1]
+    srv = SolveServer("fp64", hbm_budget=1 << 20)
+    try:
+        f = srv.enqueue(c)
+        with pytest.raises(_abi.BudgetExceeded):
+            f.result(timeout=1)
+    finally:
+        srv.close()
+
+
+def test_threaded_submit():
+    cases = _cases()[:3]
+    srv = SolveServer("mixed", tol=1e-8).start()
+    try:
+        futs = [srv.submit(c) for c in cases]
+        res = [f.result(timeout=120) for f in futs]
+        assert all(r.converged for r in res)
+    finally:
+        srv.close()
