@@ -20,7 +20,7 @@ from . import _abi
 from .grid import OGrid
 from .implicit import NewtonLoop
 
-__all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs"]
+__all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs", "SOLVER_DEFAULTS"]
 
 
 @dataclass
@@ -48,10 +48,27 @@ class BatchResult:
     U: list = field(default_factory=list)
 
 
-def default_knobs(precision: str, restart: int = 50, early_exit: bool | None = None) -> _abi.Knobs:
+# Solver settings per form. "fp64" is the reference's algorithm (fixed 50
+# Arnoldi steps, 4 line-Jacobi sweeps, CFL cap 300: aero2d/implicit.py:68-78).
+# "mixed" is tuned for throughput on transonic cases: Arnoldi stops once the
+# preconditioned residual estimate meets gmres_tol, the preconditioner runs
+# two red-black sweeps (as strong as four Jacobi sweeps here at half the
+# cost), and the CFL cap is 1000. Both converge to the same discrete solution.
+SOLVER_DEFAULTS = {
+    "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0),
+    "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=1000.0),
+}
+
+
+def default_knobs(precision: str, restart: int | None = None, early_exit: bool | None = None,
+                  precond_order: int | None = None, jacobi_sweeps: int | None = None) -> _abi.Knobs:
+    """Library knobs for the form's defaults, with optional overrides."""
+    d = SOLVER_DEFAULTS[precision]
     k = _abi.default_knobs()
-    k.gmres_restart = restart
-    k.early_exit = int(precision == "mixed") if early_exit is None else int(early_exit)
+    k.gmres_restart = d["restart"] if restart is None else int(restart)
+    k.early_exit = d["early_exit"] if early_exit is None else int(early_exit)
+    k.precond_order = d["precond_order"] if precond_order is None else int(precond_order)
+    k.jacobi_sweeps = d["jacobi_sweeps"] if jacobi_sweeps is None else int(jacobi_sweeps)
     return k
 
 
@@ -85,7 +102,7 @@ class BatchSolve:
         self.upload_s = time.perf_counter() - t0
         return self
 
-    def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float = 300.0,
+    def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float | None = None,
             keep_state: bool = False, on_step=None, cfl_late: float | None = None,
             late_below: float = 0.0, record_history: bool = False) -> BatchResult:
         """Newton loop to rn/r0 <= tol per case (or n_iter steps); the pool is
@@ -94,6 +111,8 @@ class BatchSolve:
             self.upload()
         cases, ids, pool = self.cases, self.ids, self.pool
         nb = len(cases)
+        if cfl_max is None:
+            cfl_max = SOLVER_DEFAULTS[self.precision]["cfl_max"]
         t0 = time.perf_counter()
         try:
             loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below)
@@ -154,12 +173,13 @@ class BatchSolve:
 
 
 def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, n_iter: int = 200,
-                cfl0: float = 10.0, cfl_max: float = 300.0, knobs: _abi.Knobs | None = None,
+                cfl0: float = 10.0, cfl_max: float | None = None, knobs: _abi.Knobs | None = None,
                 shapes: dict | None = None, keep_state: bool = False, pool_budget: int = 0,
                 on_step=None, cfl_late: float | None = None, late_below: float = 0.0,
                 record_history: bool = False) -> BatchResult:
     """Upload every case, then solve each to rn/r0 <= tol (or n_iter Newton
-    steps). precision: "fp64" or "mixed". shapes: optional {(ni, nc): stencil
+    steps). precision: "fp64" or "mixed"; knobs and cfl_max default to
+    SOLVER_DEFAULTS[precision]. shapes: optional {(ni, nc): stencil
     shape index} from the autotuner. on_step(it, n_active) after each step.
     cfl_late / late_below: raise the CFL cap to cfl_late once rn/r0 < late_below.
     BatchResult.wall covers the Newton loop (inputs resident); .upload_s the

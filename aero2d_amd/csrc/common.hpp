@@ -181,6 +181,19 @@ __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
 }
 
 // Positive modulo for the periodic i direction.
+// Offset of entry m (= row * 4 + column) of block `slot` of cell `cell` in the
+// preconditioner's block storage T blk[5][4][pstride][4].
+__host__ __device__ __forceinline__ size_t blk_off(int slot, int m, size_t cell, size_t pstride) {
+  return ((size_t(slot) * 4 + (m >> 2)) * pstride + cell) * 4 + (m & 3);
+}
+
+// Column of line i in the preconditioner's block storage. With the red-black
+// line order the blocks are kept colour-split (even lines, then odd lines) so
+// that a half sweep reads them contiguously; otherwise column = i.
+__host__ __device__ __forceinline__ int bcol(int i, int ni, int split) {
+  return split ? (i & 1) * (ni >> 1) + (i >> 1) : i;
+}
+
 __device__ __forceinline__ int wrap(int i, int ni) {
   i %= ni;
   return i < 0 ? i + ni : i;

@@ -55,6 +55,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
   d.order = k.order;
   d.early_exit = k.early_exit;
   d.restart = k.gmres_restart;
+  d.precond_rb = k.precond_order == 1;
   return d;
 }
 
@@ -119,7 +120,7 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   L.d = take(4 * n * t);
   L.xa = take(4 * n * t);
   L.xb = take(4 * n * t);
-  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), cdiv(ni, LINES_PER_BLOCK), 1});
+  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, LINES_PER_BLOCK), 1});
   L.part = take(size_t(L.npart) * (kmax + 2) * 8);
   L.sc = take(sizeof(CaseScal));
   L.total = o;
@@ -190,6 +191,7 @@ void default_knobs(a2d_knobs* k) {
   k->jacobi_sweeps = 4;
   k->early_exit = 0;
   k->use_graph = 1;
+  k->precond_order = 0;
 }
 
 Pool& get_pool(int64_t id) {
@@ -306,6 +308,16 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
 // ---- launch sequences ---------------------------------------------------------
 void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int gate) {
   const int S = std::max(1, p.knobs.jacobi_sweeps);
+  if (p.knobs.precond_order == 1) {
+    // red-black: odd lines then even lines, S times, in place in dest
+    for (int s = 0; s < S; ++s)
+      for (int color = 1; color >= 0; --color) {
+        const int coupled = !(s == 0 && color == 1);
+        launch_line_sweep_rb(b.pcell, b.ncell, b.pline, b.nline, src, dest, color, coupled,
+                             (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
+      }
+    return;
+  }
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
@@ -475,6 +487,10 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->gmres_restart < 0 || k->gmres_restart > KMAX)
       fail(-6, "gmres_restart must be in [0, " + std::to_string(KMAX) + "]");
     if (k->jacobi_sweeps < 1) fail(-6, "jacobi_sweeps must be >= 1");
+    if (k->precond_order != 0 && k->precond_order != 1) fail(-6, "precond_order must be 0 or 1");
+    for (auto& c : p.cases)
+      if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
+        fail(-6, "red-black line ordering needs an even number of cells around");
     if (k->order != 1 && k->order != 2) fail(-6, "order must be 1 or 2");
     for (auto& c : p.cases)
       if (c.live && c.kmax < k->gmres_restart)
@@ -519,6 +535,8 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     Pool& p = get_pool(pool);
     if (ni < 4 || nc < 2) fail(-6, "grid too small (need ni >= 4, nc >= 2)");
     if (precision != 0 && precision != 1) fail(-6, "precision must be 0 (fp64) or 1 (mixed)");
+    if (p.knobs.precond_order == 1 && (ni % 2) != 0)
+      fail(-6, "red-black line ordering needs an even number of cells around");
     const int kmax = std::max(1, std::min(p.knobs.gmres_restart, KMAX));
     const Layout L = layout(ni, nc, precision, kmax);
     if (p.budget > 0 && p.used + int64_t(L.total) > p.budget)
@@ -827,7 +845,7 @@ int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, 
     HC(hipMemcpyAsync(raw.data(), c.dev.blk, raw.size(), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     auto get = [&](int slot, int m, size_t idx) -> double {
-      const size_t o = (size_t(slot) * 16 + m) * ps + idx;
+      const size_t o = blk_off(slot, m, idx, ps);
       return c.prec == 0 ? reinterpret_cast<double*>(raw.data())[o] : reinterpret_cast<float*>(raw.data())[o];
     };
     double* outs[5] = {D, Bjp, Bjm, Bim, Bip};
@@ -836,7 +854,8 @@ int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, 
       for (int i = 0; i < c.ni; ++i)
         for (int j = 0; j < c.nc; ++j)
           for (int m = 0; m < 16; ++m)
-            outs[slot][(size_t(i) * c.nc + j) * 16 + m] = get(slot, m, size_t(j) * c.ni + i);
+            outs[slot][(size_t(i) * c.nc + j) * 16 + m] =
+                get(slot, m, size_t(j) * c.ni + bcol(i, c.ni, kn.precond_rb));
     }
   });
 }
@@ -927,6 +946,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 0, b.prec, p.stream); break;
         case 9: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 1, b.prec, p.stream); break;
+        case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");
       }
     };

@@ -223,7 +223,7 @@ __global__ void k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
   if (e >= pc.nb) return;
   const CaseDev& cd = pc.cases[pc.slots[e]];
   CaseScal* sc = cd.sc;
-  const int nblk = cd.nlineb;
+  const int nblk = cd.nlineb * (kn.precond_rb ? 2 : 1);
   double s = 0.0;
   for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
   const double nrm = sqrt(s);

@@ -16,7 +16,7 @@ namespace a2d {
 #endif
 constexpr int VEC_CHUNK = A2D_VEC_CHUNK;  // cells per block of the vector kernels
 constexpr int CELL_BLOCK = 256;      // cells per block of the assembly kernel
-constexpr int LINES_PER_BLOCK = 64;  // i-lines per block of the line kernels
+constexpr int LINES_PER_BLOCK = 32;  // i-lines per block of the line kernels
 
 struct KnobsDev {        // solver knobs the kernels read
   double cfl_explicit;   // EulerSolver.cfl (2.5)
@@ -27,6 +27,7 @@ struct KnobsDev {        // solver knobs the kernels read
   int order;             // 1 or 2
   int early_exit;        // stop Arnoldi once the estimate meets gtol
   int restart;           // Arnoldi steps per Newton step
+  int precond_rb;        // 1: red-black line ordering (else line Jacobi)
 };
 
 // ---- stencil.hip ------------------------------------------------------------
@@ -59,6 +60,12 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // and the line recurrences (pl, nblocks).
 void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
                        int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s);
+// One half sweep of the red-black ordering: lines i = color (mod 2) only,
+// in place in xout (neighbours of the other colour are read from xout);
+// coupled = 0 drops the i-coupling (the very first half sweep). Norm partials
+// go to part[(color * nlineb + block)].
+void launch_line_sweep_rb(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xout,
+                          int color, int coupled, int want_norm, int gate, int prec, hipStream_t s);
 
 // Timing hook: part 0 = the per-cell rhs pass of a coupled sweep, part 1 = the
 // line recurrences alone (reading the d buffer left by a previous sweep).

@@ -5,23 +5,29 @@
 //               aero2d/implicit.py:36-59), assembled into the blocks D, Bim,
 //               Bip, Bjm, Bjp of each cell plus vol/dtau on the diagonal
 //               (aero2d/implicit.py:81-110). Thread per cell.
-//   k_factor    block Thomas elimination along j for every i-line
-//               (aero2d/implicit.py:112-128): den = D - A c_prev, inv = den^-1,
-//               c = inv C. Thread per line, 4x4 blocks in registers.
+//   k_factor    block elimination along j for every i-line, two-sided: the
+//               reference's top-down Thomas recurrence (aero2d/implicit.py:112-128)
+//               on the wall half, the mirror recurrence on the far-field half,
+//               joined at the middle row. Two threads per line, 4x4 blocks in
+//               registers.
 //   one line-Jacobi sweep (aero2d/implicit.py:130-157) is two kernels. The
 //   reference's forward step d_j = inv_j (rhs_j - A_j d_{j-1}) is split as
 //   d_j = g_j - P_j d_{j-1} with g_j = inv_j rhs_j and P_j = inv_j A_j:
 //   k_sweep_rhs  fully parallel, thread per cell: rhs = r - Bim x(i-1) - Bip x(i+1)
 //                (r alone in the first sweep), g = inv rhs.
-//   k_sweep_seq  the j recurrences only: forward d_j = g_j - P_j d_{j-1}, backward
-//                x_j = d_j - c_j x_{j+1}. Four lanes (a quad) per line, lane r
-//                owns row r of each 4x4 product, rows exchanged in the quad;
-//                loads for the next PF steps are in flight while the current PF
-//                steps compute (double-buffered registers).
-//   The same arithmetic as the reference in another association order.
+//   k_sweep_seq  the j recurrences only, both halves of each line at once
+//                (eight lanes per line, lane r owns row r of each 4x4 product,
+//                rows exchanged in the quad; loads for the next PF steps are in
+//                flight while the current PF steps compute).
+//   The same direct solve as the reference in another elimination order, so
+//   the same answer up to rounding.
 //
-// Block storage: T blk[5][16][nc][ni], slot 0 = D then inv, 1 = Bjp then c,
-// 2 = A = Bjm then P = inv A, 3 = Bim, 4 = Bip; entry m = row * 4 + col.
+// Block storage: T blk[5][4][pstride][4] = slot, block row, cell, column, so a
+// block row is one 16-byte load in FP32 (two in FP64). Slots after assembly:
+// 0 = D, 1 = Bjp, 2 = Bjm, 3 = Bim, 4 = Bip; after factorisation 0 = inverse
+// of the eliminated diagonal, 1 = backward coupling (c or e), 2 = forward
+// coupling (P or Q). With the red-black line order the cell index is
+// colour-split (column of line i = (i % 2) ni/2 + i/2).
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -135,7 +141,8 @@ __global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
   const size_t sj1 = (size_t)(nc + 1) * ni;
   const gptr<T> blk = gp(tp<T>(cd.blk));
   const size_t plane = (size_t)cd.pstride;
-  auto at = [&](int slot, int m) -> T& { return blk[((size_t)slot * 16 + m) * plane + idx]; };
+  const size_t bidx = (size_t)j * ni + bcol(i, ni, kn.precond_rb);
+  auto at = [&](int slot, int m) -> T& { return blk[blk_off(slot, m, bidx, plane)]; };
 
   T qc[4], qim[4], qip[4];
   load_cons(U, n, idx, qc);
@@ -275,79 +282,169 @@ __device__ __forceinline__ void inv4(const T a_in[16], T out[16]) {
   for (int m = 0; m < 16; ++m) out[m] = b[m];
 }
 
+// Row-wise access to the block storage (layout in the file header).
+template <class T>
+struct Blk {
+  gptr<T> p;
+  size_t ps;
+  __device__ __forceinline__ gptr<T> row(int slot, int r, size_t cell) const {
+    return p + (((size_t)slot * 4 + r) * ps + cell) * 4;
+  }
+};
+
+typedef float a2d_f4 __attribute__((ext_vector_type(4)));
+typedef double a2d_d2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void load_row(gptr<const float> p, float v[4]) {
+  const a2d_f4 t = *(gptr<const a2d_f4>)p;
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+__device__ __forceinline__ void load_row(gptr<const double> p, double v[4]) {
+  const a2d_d2 a = *(gptr<const a2d_d2>)p;
+  const a2d_d2 b = *(gptr<const a2d_d2>)(p + 2);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void store_row(gptr<float> p, const float v[4]) {
+  a2d_f4 t;
+  t.x = v[0]; t.y = v[1]; t.z = v[2]; t.w = v[3];
+  *(gptr<a2d_f4>)p = t;
+}
+__device__ __forceinline__ void store_row(gptr<double> p, const double v[4]) {
+  a2d_d2 a, b;
+  a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3];
+  *(gptr<a2d_d2>)p = a;
+  *(gptr<a2d_d2>)(p + 2) = b;
+}
+
+template <class T>
+__device__ __forceinline__ void load_blk(const Blk<const T>& B, int slot, size_t cell, T a[16]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) load_row(B.row(slot, r, cell), a + 4 * r);
+}
+template <class T>
+__device__ __forceinline__ void store_blk(const Blk<T>& B, int slot, size_t cell, const T a[16]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) store_row(B.row(slot, r, cell), a + 4 * r);
+}
+template <class T>
+__device__ __forceinline__ void mul4(const T a[16], const T b[16], T out[16]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += a[r * 4 + k] * b[k * 4 + c];
+      out[r * 4 + c] = s;
+    }
+}
+
+// Two-sided (twisted) block elimination of every j-line, two threads per line.
+// With m = nc / 2, the top thread eliminates j = 0 .. m-1 downward from the
+// wall exactly as aero2d/implicit.py:112-128 does (den = D - A c_prev,
+// inv = den^-1, c = inv C, P = inv A); the bottom thread eliminates
+// j = nc-1 .. m+1 upward from the far field (den = D - C e_prev, inv = den^-1,
+// e = inv A, Q = inv C); row m closes both: M = D - A c_{m-1} - C e_{m+1},
+// slot 0 = M^-1, slot 2 = M^-1 A, slot 1 = M^-1 C. In every row slot 0 maps
+// the right-hand side to g, slot 2 multiplies the recurrence value arriving
+// from the line end and slot 1 the one arriving from the middle on the way
+// back. The same direct solve as the reference's with half the chain length.
 template <class T>
 __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
-  const int ni = cd.ni, nc = cd.nc, n = cd.n;
-  const int i = (blockIdx.x - plan.blk_prefix[e]) * 64 + threadIdx.x;
-  if (i >= ni) return;
-  const gptr<T> blk = gp(tp<T>(cd.blk));
-  const size_t plane = (size_t)cd.pstride;
+  const int ni = cd.ni, nc = cd.nc;
+  const int half = threadIdx.x & 1;
+  const int q = (blockIdx.x - plan.blk_prefix[e]) * LINES_PER_BLOCK + (threadIdx.x >> 1);
+  const bool live = q < ni;
+  const int col = live ? q : ni - 1;   // dead lanes shadow a real line and never store
+  const int m = nc >> 1;
+  const int steps = half ? nc - 1 - m : m;
+  const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
+  const Blk<T> B{gp(tp<T>(cd.blk)), (size_t)cd.pstride};
+  const Blk<const T> Bc{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
   T cprev[16];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) cprev[m] = T(0);
-  for (int j = 0; j < nc; ++j) {
-    const size_t idx = (size_t)j * ni + i;
-    T den[16], A[16], C[16];
+  for (int k = 0; k < 16; ++k) cprev[k] = T(0);
+  for (int s = 0; s < m; ++s) {
+    if (s < steps) {
+      const size_t cell = (size_t)(j0 + dj * s) * ni + col;
+      T D[16], A[16], C[16];
+      load_blk(Bc, 0, cell, D);
+      load_blk(Bc, 2, cell, A);
+      load_blk(Bc, 1, cell, C);
+      T L[16], F[16], t[16];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      den[m] = blk[(0 * 16 + m) * plane + idx];
-      A[m] = blk[(2 * 16 + m) * plane + idx];
-      C[m] = blk[(1 * 16 + m) * plane + idx];
+      for (int k = 0; k < 16; ++k) {
+        L[k] = half ? C[k] : A[k];    // couples to the row already eliminated
+        F[k] = half ? A[k] : C[k];    // couples to the row eliminated next
+      }
+      mul4<T>(L, cprev, t);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) D[k] -= t[k];
+      T inv[16], P[16];
+      inv4<T>(D, inv);
+      mul4<T>(inv, F, cprev);
+      mul4<T>(inv, L, P);
+      if (live) {
+        store_blk(B, 0, cell, inv);
+        store_blk(B, 1, cell, cprev);
+        store_blk(B, 2, cell, P);
+      }
     }
+  }
+  T other[16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+  for (int k = 0; k < 16; ++k) other[k] = __shfl_xor(cprev[k], 1, WAVE);
+  if (half == 0 && live) {
+    const size_t cell = (size_t)m * ni + col;
+    T D[16], A[16], C[16], t[16];
+    load_blk(Bc, 0, cell, D);
+    load_blk(Bc, 2, cell, A);
+    load_blk(Bc, 1, cell, C);
+    mul4<T>(A, cprev, t);           // A c_{m-1}
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        T s = T(0);
+    for (int k = 0; k < 16; ++k) D[k] -= t[k];
+    mul4<T>(C, other, t);           // C e_{m+1} (zero when the bottom part is empty)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) s += A[r * 4 + k] * cprev[k * 4 + c];
-        den[r * 4 + c] -= s;
-      }
-    T inv[16];
-    inv4<T>(den, inv);
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        T s = T(0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s += inv[r * 4 + k] * C[k * 4 + c];
-        cprev[r * 4 + c] = s;
-      }
-    T P[16];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        T s = T(0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s += inv[r * 4 + k] * A[k * 4 + c];
-        P[r * 4 + c] = s;
-      }
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      blk[(0 * 16 + m) * plane + idx] = inv[m];
-      blk[(1 * 16 + m) * plane + idx] = cprev[m];
-      blk[(2 * 16 + m) * plane + idx] = P[m];
-    }
+    for (int k = 0; k < 16; ++k) D[k] -= t[k];
+    T inv[16], P[16], Q[16];
+    inv4<T>(D, inv);
+    mul4<T>(inv, A, P);
+    mul4<T>(inv, C, Q);
+    store_blk(B, 0, cell, inv);
+    store_blk(B, 1, cell, Q);
+    store_blk(B, 2, cell, P);
   }
 }
 
-// Sweep part 1, thread per cell: g = inv (src - Bim x(i-1) - Bip x(i+1)) into d.
+// Sweep part 1, thread per cell: g = slot0 (src - Bim x(i-1) - Bip x(i+1))
+// into the work buffer d (four components per cell, contiguous).
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev, int sweep,
-                                                   int gate) {
+                                                   int gate, int color) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   if (gate && !cd.sc->active) return;
   const int ni = cd.ni, n = cd.n;
-  const int idx = (blockIdx.x - plan.blk_prefix[e]) * 256 + threadIdx.x;
-  if (idx >= n) return;
-  const int i = idx % ni, j = idx / ni;
-  const gptr<const T> blk = gp(ctp<T>(cd.blk));
-  const size_t plane = (size_t)cd.pstride;
+  const int t = (blockIdx.x - plan.blk_prefix[e]) * 256 + threadIdx.x;
+  int i, j, col;
+  if (color < 0) {            // every cell, natural block layout
+    if (t >= n) return;
+    i = t % ni;
+    j = t / ni;
+    col = i;
+  } else {                    // cells of lines i = color (mod 2), colour-split blocks
+    const int nh = ni >> 1;
+    if (t >= nh * cd.nc) return;
+    const int q = t % nh;
+    j = t / nh;
+    i = 2 * q + color;
+    col = color * nh + q;
+  }
+  const int idx = j * ni + i;                  // natural position (vectors)
+  const size_t bidx = (size_t)j * ni + col;    // block storage and g
+  const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
   T rhs[4];
   if (src == VEC_NEG_R) {
     const gptr<const double> Rg = gp(cd.R);
@@ -368,25 +465,37 @@ __global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev
       xq[c] = xp[(size_t)c * n + j * ni + ip];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) {
+      T bm[4], bp[4];
+      load_row(B.row(3, r, bidx), bm);
+      load_row(B.row(4, r, bidx), bp);
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        rhs[r] -= blk[(size_t)(3 * 16 + r * 4 + c) * plane + idx] * xm[c] +
-                  blk[(size_t)(4 * 16 + r * 4 + c) * plane + idx] * xq[c];
+      for (int c = 0; c < 4; ++c) rhs[r] -= bm[c] * xm[c] + bp[c] * xq[c];
+    }
   }
-  const gptr<T> g = gp(tp<T>(cd.d));
+  T g[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    T iv[4];
+    load_row(B.row(0, r, bidx), iv);
     T s = T(0);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s += blk[(size_t)(r * 4 + c) * plane + idx] * rhs[c];
-    g[(size_t)r * n + idx] = s;
+    for (int c = 0; c < 4; ++c) s += iv[c] * rhs[c];
+    g[r] = s;
   }
+  store_row(gp(tp<T>(cd.d)) + bidx * 4, g);
 }
 
-// Sweep part 2: the two j recurrences of every line. Block = 64 lines x 4 rows.
+// Sweep part 2: the j recurrences of the two-sided solve. Eight lanes per
+// line: lanes 0-3 (one per block row) run j = 0 .. m-1, lanes 4-7 run
+// j = nc-1 .. m+1, each d_j = g_j - slot2_j d_prev in place over g; the two
+// halves meet at row m (x_m = g_m - slot2_m d_{m-1} - slot1_m f_{m+1}), then
+// walk back out, x_j = d_j - slot1_j x_prev. Rows are exchanged inside each
+// quad of lanes; loads for the next PF steps are in flight while the current
+// PF steps compute (double-buffered registers).
 template <class T, int PF>
-__global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want_norm, int gate) {
+__global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want_norm, int gate,
+                                                   int color) {
   __shared__ double red[4];
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
@@ -394,87 +503,115 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   const int r = threadIdx.x & 3;
-  const int i = lb * 64 + (threadIdx.x >> 2);
-  const bool live = i < ni;
-  const int ic = live ? i : ni - 1;     // dead lanes shadow a real line and never store
-  const size_t plane = (size_t)cd.pstride;
-  const gptr<const T> P = gp(ctp<T>(cd.blk) + (size_t)(2 * 16 + r * 4) * plane + ic);
-  const gptr<const T> C = gp(ctp<T>(cd.blk) + (size_t)(1 * 16 + r * 4) * plane + ic);
-  const gptr<T> d = gp(tp<T>(cd.d) + (size_t)r * n + ic);
+  const int half = (threadIdx.x >> 2) & 1;
+  const int q = lb * LINES_PER_BLOCK + (threadIdx.x >> 3);
+  // colour < 0: every line, natural block layout; else line q of colour
+  // `color` (i = 2q + color), whose blocks sit in column color * ni/2 + q
+  const int nq = color < 0 ? ni : (ni >> 1);
+  const bool live = q < nq;
+  const int qc = live ? q : nq - 1;     // dead lanes shadow a real line and never store
+  const int ic = color < 0 ? qc : 2 * qc + color;
+  const int col = color < 0 ? qc : color * (ni >> 1) + qc;
+  const int m = nc >> 1;
+  const int steps = half ? nc - 1 - m : m;
+  const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
+  const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
+  const gptr<T> gd = gp(tp<T>(cd.d));
   const gptr<T> x = gp(vec_ptr<T>(cd, xout) + (size_t)r * n + ic);
 
-  // forward: d_j = g_j - P_j d_{j-1}, in place over g
   T bm[2][PF][4], bv[2][PF];
-  auto fload = [&](int b, int j0) {
+  // toward the middle: d_j = g_j - slot2_j d_prev
+  auto fcell = [&](int s) -> size_t {
+    const int sc = max(0, min(s, steps - 1));
+    return (size_t)(j0 + dj * sc) * ni + col;
+  };
+  auto fload = [&](int b, int s0) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int j = min(j0 + q, nc - 1);
-      const size_t o = (size_t)j * ni;
-      bv[b][q] = d[o];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bm[b][q][c] = P[(size_t)c * plane + o];
+    for (int k = 0; k < PF; ++k) {
+      const size_t cell = fcell(s0 + k);
+      bv[b][k] = gd[cell * 4 + r];
+      load_row(B.row(2, r, cell), bm[b][k]);
     }
   };
   T dr = T(0);
-  auto fcomp = [&](int b, int j0) {
+  auto fcomp = [&](int b, int s0) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int j = j0 + q;
-      if (j < nc) {
-        const T t = bv[b][q] - bm[b][q][0] * quad_bcast<0>(dr) - bm[b][q][1] * quad_bcast<1>(dr) -
-                    bm[b][q][2] * quad_bcast<2>(dr) - bm[b][q][3] * quad_bcast<3>(dr);
+    for (int k = 0; k < PF; ++k) {
+      if (s0 + k < steps) {
+        const T t = bv[b][k] - bm[b][k][0] * quad_bcast<0>(dr) - bm[b][k][1] * quad_bcast<1>(dr) -
+                    bm[b][k][2] * quad_bcast<2>(dr) - bm[b][k][3] * quad_bcast<3>(dr);
         dr = t;
-        if (live) d[(size_t)j * ni] = dr;
+        if (live) gd[fcell(s0 + k) * 4 + r] = dr;
       }
     }
   };
   fload(0, 0);
-  for (int j0 = 0; j0 < nc; j0 += 2 * PF) {
-    fload(1, j0 + PF);
-    fcomp(0, j0);
-    fload(0, j0 + 2 * PF);
-    fcomp(1, j0 + PF);
+  for (int s0 = 0; s0 < m; s0 += 2 * PF) {
+    fload(1, s0 + PF);
+    fcomp(0, s0);
+    fload(0, s0 + 2 * PF);
+    fcomp(1, s0 + PF);
   }
-  // backward: x_j = d_j - c_j x_{j+1}
-  auto bload = [&](int b, int j0) {
+  // row m: both quads combine the top's d_{m-1} with the bottom's f_{m+1}
+  const T other = __shfl_xor(dr, 4, WAVE);
+  const T dtop = half ? other : dr, fbot = half ? dr : other;
+  T xr;
+  {
+    const size_t cell = (size_t)m * ni + col;
+    T pm[4], qm[4];
+    load_row(B.row(2, r, cell), pm);
+    load_row(B.row(1, r, cell), qm);
+    xr = gd[cell * 4 + r] - pm[0] * quad_bcast<0>(dtop) - pm[1] * quad_bcast<1>(dtop) -
+         pm[2] * quad_bcast<2>(dtop) - pm[3] * quad_bcast<3>(dtop) - qm[0] * quad_bcast<0>(fbot) -
+         qm[1] * quad_bcast<1>(fbot) - qm[2] * quad_bcast<2>(fbot) - qm[3] * quad_bcast<3>(fbot);
+  }
+  double nrm = 0.0;
+  if (live && half == 0) {
+    x[(size_t)m * ni] = xr;
+    nrm += double(xr) * double(xr);
+  }
+  // away from the middle: x_j = d_j - slot1_j x_prev
+  auto bj = [&](int s) -> int {
+    const int sc = max(0, min(s, steps - 1));
+    return steps > 0 ? m - dj * (1 + sc) : m;
+  };
+  auto bload = [&](int b, int s0) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int j = max(j0 - q, 0);
-      const size_t o = (size_t)j * ni;
-      bv[b][q] = d[o];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bm[b][q][c] = C[(size_t)c * plane + o];
+    for (int k = 0; k < PF; ++k) {
+      const size_t cell = (size_t)bj(s0 + k) * ni + col;
+      bv[b][k] = gd[cell * 4 + r];
+      load_row(B.row(1, r, cell), bm[b][k]);
     }
   };
-  T xr = T(0);
-  double nrm = 0.0;
-  auto bcomp = [&](int b, int j0) {
+  auto bcomp = [&](int b, int s0) {
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int j = j0 - q;
-      if (j >= 0) {
-        const T t = bv[b][q] - bm[b][q][0] * quad_bcast<0>(xr) - bm[b][q][1] * quad_bcast<1>(xr) -
-                    bm[b][q][2] * quad_bcast<2>(xr) - bm[b][q][3] * quad_bcast<3>(xr);
+    for (int k = 0; k < PF; ++k) {
+      if (s0 + k < steps) {
+        const T t = bv[b][k] - bm[b][k][0] * quad_bcast<0>(xr) - bm[b][k][1] * quad_bcast<1>(xr) -
+                    bm[b][k][2] * quad_bcast<2>(xr) - bm[b][k][3] * quad_bcast<3>(xr);
         xr = t;
         if (live) {
-          x[(size_t)j * ni] = xr;
+          x[(size_t)bj(s0 + k) * ni] = xr;
           nrm += double(xr) * double(xr);
         }
       }
     }
   };
-  bload(0, nc - 1);
-  for (int j0 = nc - 1; j0 >= 0; j0 -= 2 * PF) {
-    bload(1, j0 - PF);
-    bcomp(0, j0);
-    bload(0, j0 - 2 * PF);
-    bcomp(1, j0 - PF);
+  bload(0, 0);
+  for (int s0 = 0; s0 < m; s0 += 2 * PF) {
+    bload(1, s0 + PF);
+    bcomp(0, s0);
+    bload(0, s0 + 2 * PF);
+    bcomp(1, s0 + PF);
   }
   if (want_norm) {
     nrm = wave_sum(nrm);
     if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = nrm;
     __syncthreads();
-    if (threadIdx.x == 0) cd.part[(size_t)lb * (cd.kmax + 2)] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) {
+      const int slot = color < 0 ? lb : color * cd.nlineb + lb;
+      cd.part[(size_t)slot * (cd.kmax + 2)] = red[0] + red[1] + red[2] + red[3];
+    }
   }
 }
 
@@ -498,26 +635,45 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
 void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
                        int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s) {
   if (prec == 0) {
-    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate);
+    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate,
+                       -1);
     hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate);
+                       want_norm, gate, -1);
   } else {
-    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate);
+    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate,
+                       -1);
     hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate);
+                       want_norm, gate, -1);
+  }
+}
+
+void launch_line_sweep_rb(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xout,
+                          int color, int coupled, int want_norm, int gate, int prec, hipStream_t s) {
+  // the plans are sized for every cell / line; blocks past the colour's half return at once
+  const int sweep = coupled ? 1 : 0;
+  if (prec == 0) {
+    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xout, sweep, gate,
+                       color);
+    hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
+                       want_norm, gate, color);
+  } else {
+    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xout, sweep, gate,
+                       color);
+    hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
+                       want_norm, gate, color);
   }
 }
 
 void launch_sweep_part(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int part, int prec,
                        hipStream_t s) {
   if (part == 0) {
-    if (prec == 0) hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0);
-    else hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0);
+    if (prec == 0) hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0, -1);
+    else hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0, -1);
   } else {
     if (prec == 0)
-      hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0);
+      hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0, -1);
     else
-      hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0);
+      hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0, -1);
   }
 }
 

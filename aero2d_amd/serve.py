@@ -24,16 +24,10 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from .batch import Case
+from .batch import SOLVER_DEFAULTS, Case, default_knobs
 from .implicit import NewtonLoop
 
-__all__ = ["ServeResult", "SolveServer", "serve_stream", "MIXED_SERVING_KNOBS"]
-
-# Mixed-form serving defaults: GMRES(30) with early exit at gmres_tol, CFL cap
-# 1000 (measured on the transonic NACA 0012/2412 workload: all cases converge
-# to the FP64 answer within 1e-8 in Cl with these).
-MIXED_SERVING_KNOBS = dict(restart=30, early_exit=1, cfl_max=1000.0)
-FP64_SERVING_KNOBS = dict(restart=50, early_exit=0, cfl_max=300.0)
+__all__ = ["ServeResult", "SolveServer", "serve_stream"]
 
 
 @dataclass
@@ -74,13 +68,11 @@ class SolveServer:
                  tol: float = 1e-10, n_iter: int = 800, cfl0: float = 10.0,
                  restart: int | None = None, early_exit: int | None = None,
                  cfl_max: float | None = None):
-        defaults = MIXED_SERVING_KNOBS if precision == "mixed" else FP64_SERVING_KNOBS
+        defaults = SOLVER_DEFAULTS[precision]
         self.precision = precision
         self.prec = _abi.PREC_MIXED if precision == "mixed" else _abi.PREC_FP64
-        self.restart = defaults["restart"] if restart is None else restart
-        kn = _abi.default_knobs()
-        kn.gmres_restart = self.restart
-        kn.early_exit = defaults["early_exit"] if early_exit is None else int(early_exit)
+        kn = default_knobs(precision, restart=restart, early_exit=early_exit)
+        self.restart = kn.gmres_restart
         self.pool = _abi.Pool(hbm_budget, kn)
         self.budget = hbm_budget
         self.max_batch = max_batch

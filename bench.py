@@ -9,8 +9,9 @@ ds0 = 20 (q - 1)/(q^128 - 1), q = 1.1^(64/128); aero2d_amd.workloads.config2).
 
 A step is one complete batched solve of all cases (warmup steps untimed).
 The line also carries the mixed-precision form on the same batch, per-case
-completion latency (p50/p99), a live HIP-event roofline of the residual
-kernel and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+completion latency (p50/p99), live HIP-event rooflines of the line-recurrence
+kernel (the largest share of device time) and of the residual kernel, and a
+CPU baseline (the FP64 oracle on the host cores, bounded sample).
 """
 from __future__ import annotations
 
@@ -30,6 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell and launch (FP64): residual reads U(4), S_i(2),
 # S_j(2), writes R(4) = 12 words
 RES_BYTES_PER_CELL = 96
+# line recurrences of one preconditioner sweep (the kernel with the most
+# device time): read g(4), P = inv A (16), c (16), write x (4) = 40 words
+SEQ_BYTES_PER_CELL = 320
+# plain vector copy used to calibrate the HBM counters: 4 words read, 4 written
+COPY_BYTES_PER_CELL = 64
 
 
 def make_cases(n_cases, ni, nc, seed):
@@ -37,20 +43,20 @@ def make_cases(n_cases, ni, nc, seed):
     return config2(n_cases, ni, nc, seed)
 
 
-def time_kernels(cases, precision, reps=20):
+def time_kernels(cases, precision, reps=20, knobs=None):
     """HIP-event durations (ms) of the main kernels on the pool stream, after
     one Newton step of the batch so every buffer holds live data."""
     from aero2d_amd import _abi
     from aero2d_amd.batch import default_knobs
     prec = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED
-    pool = _abi.Pool(0, default_knobs(precision))
+    pool = _abi.Pool(0, knobs if knobs is not None else default_knobs(precision))
     try:
         ids = np.array([pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, prec) for c in cases], np.int32)
         out = np.empty((len(ids), 10))
         cfl = np.full(len(ids), 10.0)
         _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl), _abi.dptr(out))
         names = ["residual", "jvp", "line_sweep", "cgs_dots", "assemble", "factor", "cgs_update",
-                 "copy", "sweep_rhs", "sweep_seq"]
+                 "copy", "sweep_rhs", "sweep_seq", "precond"]
         res = {}
         for w, name in enumerate(names):
             ms = np.zeros(1)
@@ -59,6 +65,39 @@ def time_kernels(cases, precision, reps=20):
     finally:
         pool.close()
     return res
+
+
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --workload kernels` (64 cases, 256 x 128, FP64), each scaled by the
+# copy kernel's expected / counted bytes; None until such a profile exists.
+MEASURED_TRAFFIC = None
+
+
+def rooflines(kt, cells):
+    """HBM rooflines from HIP-event kernel times (ms): the line-recurrence
+    kernel (largest share of device time) and the residual kernel."""
+    def one(name, bpc):
+        ach = bpc * cells / (kt[name] * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS,
+                "traffic": (MEASURED_TRAFFIC or {}).get(name),
+                "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name]}
+    seq = one("sweep_seq", SEQ_BYTES_PER_CELL)
+    seq["kernel"] = "k_sweep_seq<double> (block line recurrences, one sweep)"
+    res = one("residual", RES_BYTES_PER_CELL)
+    res["kernel"] = "k_stencil<double, false> (FP64 residual)"
+    return {"roofline": seq, "roofline_residual": res}
+
+
+def run_kernels(a):
+    """Kernel timings alone (no solve) on the default batch: used under
+    rocprofv3 for the per-dispatch counters."""
+    a.cases = a.cases or 64
+    cases = make_cases(a.cases, a.ni, a.nc, a.seed)
+    kt = time_kernels(cases, "fp64", reps=5)
+    out = {"kernel_ms": kt, "cells": a.cases * a.ni * a.nc}
+    out.update(rooflines(kt, out["cells"]))
+    return out
 
 
 def cpu_baseline(case, gpu_iters, steps=2):
@@ -83,18 +122,26 @@ def per_case_errors(a, b):
             "dcm": np.abs(a.cm - b.cm).tolist()}
 
 
+def max_errors(a, b):
+    """Largest |dCl|, |dCd|, |dCm| over the cases both results converged
+    (an unconverged case has no answer to compare), and that case count."""
+    both = a.converged & b.converged
+    out = {"cases_compared": int(both.sum())}
+    for k in ("cl", "cd", "cm"):
+        d = np.abs(getattr(a, k) - getattr(b, k))[both]
+        out["d" + k] = float(d.max()) if d.size else None
+    return out
+
+
 def run_config3(a):
     """Transonic mixed vs FP64: 32 NACA 0012/2412 cases on 512 x 256."""
-    from aero2d_amd.batch import solve_batch, default_knobs
-    from aero2d_amd.serve import MIXED_SERVING_KNOBS
+    from aero2d_amd.batch import SOLVER_DEFAULTS, solve_batch
     from aero2d_amd.workloads import config3
     cases = config3(a.cases or 32)
     n_iter = a.n_iter or 800
     r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter)
     print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
-    kn = default_knobs("mixed", restart=MIXED_SERVING_KNOBS["restart"])
-    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, knobs=kn,
-                      cfl_max=MIXED_SERVING_KNOBS["cfl_max"])
+    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
     print(f"# mixed: {rmx.wall:.1f} s, converged {int(rmx.converged.sum())}", file=sys.stderr, flush=True)
     v64 = r64.converged.sum() / r64.wall
     vmx = rmx.converged.sum() / rmx.wall
@@ -109,9 +156,10 @@ def run_config3(a):
             "mixed": {"value": float(vmx), "converged": int(rmx.converged.sum()), "wall_s": rmx.wall,
                       "newton_steps_mean": float(rmx.iterations.mean()),
                       "arnoldi_per_newton_mean": float(rmx.arnoldi.mean()),
-                      "knobs": MIXED_SERVING_KNOBS},
+                      "knobs": SOLVER_DEFAULTS["mixed"]},
             "speedup_mixed_vs_fp64": float(vmx / v64) if v64 > 0 else None,
-            "max_abs": {k: float(max(v)) for k, v in err.items()}, "per_case": err,
+            "max_abs_converged": max_errors(rmx, r64), "per_case": err,
+            "converged_fp64": r64.converged.tolist(), "converged_mixed": rmx.converged.tolist(),
             "cases": [(c.name, c.mach, c.alpha) for c in cases]}
 
 
@@ -119,14 +167,11 @@ def run_config4(a):
     """Uneven persistent batch: 256 random sections, four sizes, mixed form,
     autotuned tile shapes."""
     from aero2d_amd.autotune import shapes_for
-    from aero2d_amd.batch import solve_batch, default_knobs
-    from aero2d_amd.serve import MIXED_SERVING_KNOBS
+    from aero2d_amd.batch import solve_batch
     from aero2d_amd.workloads import config4
     cases, redraws = config4(a.cases or 256)
     shapes = shapes_for(cases, "mixed")
-    kn = default_knobs("mixed", restart=MIXED_SERVING_KNOBS["restart"])
-    r = solve_batch(cases, "mixed", tol=a.tol, n_iter=a.n_iter or 800, knobs=kn, shapes=shapes,
-                    cfl_max=MIXED_SERVING_KNOBS["cfl_max"])
+    r = solve_batch(cases, "mixed", tol=a.tol, n_iter=a.n_iter or 800, shapes=shapes)
     sizes = {}
     for c, conv, it in zip(cases, r.converged, r.iterations):
         key = f"{c.grid.x.shape[0]}x{c.grid.x.shape[1] - 1}"
@@ -174,7 +219,8 @@ def run_config5(a):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4", "config5"])
+    ap.add_argument("--workload", default="config2",
+                    choices=["config2", "config3", "config4", "config5", "kernels"])
     ap.add_argument("--rates", default="10,25,50,100,200,400")
     ap.add_argument("--horizon", type=float, default=30.0)
     ap.add_argument("--precisions", default="mixed,fp64")
@@ -197,7 +243,8 @@ def main():
         sys.exit(2)
 
     if a.workload != "config2":
-        fn = {"config3": run_config3, "config4": run_config4, "config5": run_config5}[a.workload]
+        fn = {"config3": run_config3, "config4": run_config4, "config5": run_config5,
+              "kernels": run_kernels}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 
@@ -248,19 +295,13 @@ def main():
         line["mixed"] = {
             "value": convmx / wallmx, "unit": "cases/s", "speedup_vs_fp64": (convmx / wallmx) / value,
             "converged": convmx // a.steps, "latency_ms": {"p50": float(lm[0]), "p99": float(lm[1])},
-            "max_abs_dcl": float(np.max(np.abs(rmx.cl - r64.cl))),
-            "max_abs_dcd": float(np.max(np.abs(rmx.cd - r64.cd))),
-            "max_abs_dcm": float(np.max(np.abs(rmx.cm - r64.cm))),
+            "max_abs_converged": max_errors(rmx, r64),
             "newton_steps_mean": float(rmx.iterations.mean()),
             "arnoldi_per_newton_mean": float(rmx.arnoldi.mean()),
             "gmres": "early exit at the preconditioned residual estimate <= 0.01 |M b|"}
     if not a.no_kernel_timing:
         kt = time_kernels(cases, "fp64")
-        cells = a.cases * a.ni * a.nc
-        ach = RES_BYTES_PER_CELL * cells / (kt["residual"] * 1e-3) / 1e9
-        line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "residual (FP64)",
-                            "bytes_per_launch": RES_BYTES_PER_CELL * cells, "ms_per_launch": kt["residual"]}
+        line.update(rooflines(kt, a.cases * a.ni * a.nc))
         line["kernel_ms"] = kt
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cases[0], float(r64.iterations.mean()))

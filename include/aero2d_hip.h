@@ -48,6 +48,9 @@ typedef struct a2d_knobs {
   int jacobi_sweeps;
   int early_exit;
   int use_graph;      /* replay captured HIP graphs for repeated batches */
+  int precond_order;  /* 0: line Jacobi over i (reference, aero2d/implicit.py:150-157);
+                         1: red-black line ordering (odd lines, then even lines
+                         with the updated odd neighbours), same cost per sweep */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */
@@ -120,6 +123,7 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
  *   3 Gram-Schmidt dots (k = 25)   4 block assembly   5 line factorisation
  *   6 Gram-Schmidt update (k = 25)   7 plain vector copy (4 words/cell each way)
  *   8 the per-cell pass of a coupled line sweep   9 the line recurrences alone
+ *   10 one whole preconditioner application (all sweeps, in the pool's order)
  * The cases must have taken a Newton step (their blocks and Krylov vectors
  * hold data); the state is not changed. */
 int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms);

@@ -28,6 +28,8 @@ Pinning (what checks each function, all in tests/test_oracle_*.py):
                              coupling (wall perturbation reaches far-field rows)
   face Jacobians             finite differences
   line solve / precond       dense numpy.linalg.solve of the assembled system
+  precond_rb                 dense block Gauss-Seidel in red-black line order;
+                             many sweeps converge to the dense full solve
   gmres_batched              dense least squares on small systems
   solve                      reference goldens (benchmarks/results.json:83-104):
                              converged FP64 Cl/Cd at M 0.5 a 2 and M 0.8 a 1.25

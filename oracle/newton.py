@@ -144,6 +144,23 @@ def precond(fac, Bim, Bip, r, sweeps=4):
     return x
 
 
+def precond_rb(fac, Bim, Bip, r, sweeps=4):
+    """Red-black line ordering of the same line solves (no counterpart in the
+    reference, which uses line Jacobi): each sweep solves the odd i-lines with
+    the current even neighbours, then the even i-lines with the just-updated
+    odd ones. Starts from x = 0. Needs an even number of lines."""
+    ni = r.shape[1]
+    assert ni % 2 == 0
+    x = torch.zeros_like(r)
+    for _ in range(sweeps):
+        for color in (1, 0):
+            xb = x.movedim(0, -1)
+            off = _bmv(Bim, torch.roll(xb, 1, 0)) + _bmv(Bip, torch.roll(xb, -1, 0))
+            xn = solve_lines(fac, r - off.movedim(-1, 0))
+            x[:, color::2] = xn[:, color::2]
+    return x
+
+
 # ---------------------------------------------------------------------------
 # GMRES, JAX "batched" semantics (jax.scipy.sparse.linalg.gmres)
 # ---------------------------------------------------------------------------

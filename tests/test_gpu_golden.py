@@ -45,3 +45,36 @@ def test_fp64_reference_trajectory():
     assert abs(r.cl[1] - 0.3486395236312213) < 1e-8
     assert abs(r.cl[2] - 0.3399757887379958) < 1e-4
     assert abs(r.cd[2] - 0.05637206522325736) < 1e-4
+
+
+def test_config1_single_case_matches_oracle():
+    """NACA 0012, 128 x 64, M 0.5, alpha 2, FP64 to rn/r0 <= 1e-10: Cl, Cd, Cm
+    within 1e-8 of the oracle's solve (tests/golden/oracle_config1.json)."""
+    import json
+    import os
+    from aero2d_amd.workloads import config1
+    with open(os.path.join(os.path.dirname(__file__), "golden", "oracle_config1.json")) as f:
+        ref = json.load(f)
+    case = config1()[0]
+    s = EulerSolver(case.grid, dtype=np.float64)
+    r = ImplicitEuler(s).solve([case.mach], [case.alpha], n_iter=300, tol=1e-10, verbose=False)
+    print("iterations", r.iterations_per_case, "cl", repr(r.cl[0]), "oracle", ref["cl"], ref["iterations"])
+    assert r.converged_tol[0]
+    assert abs(r.cl[0] - ref["cl"]) < 1e-8
+    assert abs(r.cd[0] - ref["cd"]) < 1e-8
+    assert abs(r.cm[0] - ref["cm"]) < 1e-8
+
+
+def test_reference_integration_assertions():
+    """The reference's tests/test_euler.py:43-65 on the GPU path: a 96 x 40 grid,
+    M 0.5 at alpha 0 and 2 deg, 40 Newton iterations, tol 1e-5: |Cl(0)| < 2e-3,
+    Cl(2) within 4% of the Prandtl-Glauert-corrected panel value (0.279,
+    README.md:309-311 of the reference), |Cd| < 3e-3, residual drop < -3 orders."""
+    g = ogrid(naca4("0012", 49), nj=41, ds0=3e-3)
+    for dtype in (np.float32, np.float64):
+        r = ImplicitEuler(EulerSolver(g, dtype=dtype)).solve([0.5, 0.5], [0.0, 2.0], n_iter=40,
+                                                            verbose=False, tol=1e-5)
+        assert abs(r.cl[0]) < 2e-3
+        assert abs(r.cl[1] / 0.279 - 1) < 0.04
+        assert abs(r.cd[1]) < 3e-3
+        assert np.all(np.log10(r.res_hist[-1] / r.res_hist[0]) < -3)

@@ -105,9 +105,15 @@ def test_jvp_matches_oracle(case, shape):
     assert rel_err(out, diag[None] * v + jo) < 1e-11
 
 
+@pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("case", CASES)
-def test_assembly_matches_oracle(case):
+def test_assembly_matches_oracle(case, order):
+    """order 1 keeps the blocks colour-split on the device; a2d_assemble
+    returns them in the natural layout either way."""
     g, s, U, h, c = _setup(*case)
+    k = s._pool.knobs()
+    k.precond_order = order
+    s._pool.set_knobs(k)
     mach, alpha = case[2], case[3]
     geo = oracle_geo(g)
     w = winf_t(mach, alpha)
@@ -140,6 +146,62 @@ def test_preconditioner_matches_oracle(case):
     D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
     fac = newton.factor_lines(D, Bjm, Bjp)
     xo = newton.precond(fac, Bim, Bip, torch.tensor(r), 4).numpy()
+    assert rel_err(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", CASES)
+def test_red_black_preconditioner_matches_oracle(case, precision):
+    g, s, U, h, c = _setup(*case, precision=precision)
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    k = s._pool.knobs()
+    k.precond_order = 1
+    s._pool.set_knobs(k)
+    _abi.call("a2d_precond_setup", h, c, 25.0)
+    rng = np.random.default_rng(6)
+    r = np.ascontiguousarray(rng.normal(size=U.shape))
+    x = np.empty_like(r)
+    _abi.call("a2d_precond_apply", h, c, _abi.dptr(r), _abi.dptr(x))
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 25.0)
+    D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
+    fac = newton.factor_lines(D, Bjm, Bjp)
+    xo = newton.precond_rb(fac, Bim, Bip, torch.tensor(r), 4).numpy()
+    # FP32 blocks and sweeps in the mixed form
+    assert rel_err(x, xo) < (1e-9 if precision == "fp64" else 2e-4)
+    xj = newton.precond(fac, Bim, Bip, torch.tensor(r), 4).numpy()
+    assert rel_err(x, xj) > 1e-6          # a different ordering really ran
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("nj, ds0, radius", [(3, 0.2, 1.0), (4, 0.2, 1.0), (10, 2e-2, 5.0), (11, 2e-2, 5.0)])
+def test_preconditioner_line_lengths(nj, ds0, radius, order):
+    """Lines of 2, 3, 9 and 10 cells: the two-sided line elimination meets in
+    the middle row whatever the parity of the length (and with an empty far
+    half for 2 cells)."""
+    g = small_grid(17, nj, ds0=ds0, radius=radius)
+    s = EulerSolver(g, dtype=np.float64)
+    U = perturbed_state(s.ni, s.nc, 0.6, 1.0, 9)
+    h, c = s._scratch(s.freestream(0.6, 1.0), U)
+    k = s._pool.knobs()
+    k.precond_order = order
+    s._pool.set_knobs(k)
+    _abi.call("a2d_precond_setup", h, c, 40.0)
+    r = np.ascontiguousarray(np.random.default_rng(11).normal(size=U.shape))
+    x = np.empty_like(r)
+    _abi.call("a2d_precond_apply", h, c, _abi.dptr(r), _abi.dptr(x))
+    geo = oracle_geo(g)
+    w = winf_t(0.6, 1.0)
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 40.0)
+    D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
+    fac = newton.factor_lines(D, Bjm, Bjp)
+    fn = newton.precond_rb if order else newton.precond
+    xo = fn(fac, Bim, Bip, torch.tensor(r), 4).numpy()
     assert rel_err(x, xo) < 1e-9
 
 

@@ -70,3 +70,12 @@ def test_oracle_rerun_reproduces_fixture(run, grid_geo):
     gb = flow.stack_geometry([geo] * 3)
     r = newton.solve(gb, run["machs"], run["alphas"], n_iter=80, tol=1e-6)
     assert np.allclose(r.cl, run["cl"], rtol=0, atol=1e-12)
+
+
+def test_config1_fixture_is_physical():
+    """The single-case oracle solve (tests/golden/make_config1_golden.py): converged
+    and Cl within 4% of the Prandtl-Glauert value 0.279 (reference README.md:309-311)."""
+    with open(os.path.join(HERE, "oracle_config1.json")) as f:
+        c1 = json.load(f)
+    assert c1["rn_over_r0"] <= 1e-10 and c1["grid"] == [128, 65]
+    assert abs(c1["cl"] / 0.279 - 1) < 0.04

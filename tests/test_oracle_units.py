@@ -273,6 +273,19 @@ def test_line_solve_and_preconditioner_match_dense_algebra():
         xv = np.linalg.solve(L, rv - Bm @ xv)
     xp = newton.precond(fac, torch.tensor(Bim), torch.tensor(Bip), torch.tensor(r), 4).numpy()
     assert np.allclose(xp.transpose(1, 2, 0).reshape(-1), xv, rtol=1e-9, atol=1e-11)
+    # red-black: block Gauss-Seidel over lines, odd lines first, from x = 0
+    xv = np.zeros(N)
+    for _ in range(3):
+        for color in (1, 0):
+            rows = np.concatenate([np.arange(idx(i, 0), idx(i, 0) + 4 * nc) for i in range(color, ni, 2)])
+            rhs = rv - Bm @ xv
+            xv[rows] = np.linalg.solve(L[np.ix_(rows, rows)], rhs[rows])
+    xr = newton.precond_rb(fac, torch.tensor(Bim), torch.tensor(Bip), torch.tensor(r), 3).numpy()
+    assert np.allclose(xr.transpose(1, 2, 0).reshape(-1), xv, rtol=1e-9, atol=1e-11)
+    # and both orderings converge to the solution of the full system (L + B) x = r
+    xf = np.linalg.solve(L + Bm, rv)
+    xr = newton.precond_rb(fac, torch.tensor(Bim), torch.tensor(Bip), torch.tensor(r), 60).numpy()
+    assert np.allclose(xr.transpose(1, 2, 0).reshape(-1), xf, rtol=1e-8, atol=1e-10)
 
 
 def test_gmres_batched_is_the_krylov_least_squares_minimiser():
