@@ -78,3 +78,25 @@ def test_reference_integration_assertions():
         assert abs(r.cl[1] / 0.279 - 1) < 0.04
         assert abs(r.cd[1]) < 3e-3
         assert np.all(np.log10(r.res_hist[-1] / r.res_hist[0]) < -3)
+
+
+def test_mixed_form_matches_fp64_on_the_same_solver_path():
+    """Transonic batch (NACA 0012/2412, M 0.72-0.82, 128 x 64): the mixed form
+    with its default settings against the FP64 form run with the same
+    settings. Both converge to rn/r0 <= 1e-10 and agree per case to
+    |dCl|, |dCm| <= 1e-7 and |dCd| <= 1e-8 (FP32 inner iterations, FP64
+    residual and update)."""
+    from aero2d_amd.batch import SOLVER_DEFAULTS, default_knobs, solve_batch
+    from aero2d_amd.workloads import config3
+    cases = config3(6, 128, 64)
+    d = SOLVER_DEFAULTS["mixed"]
+    kn = default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
+                       precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"])
+    r64 = solve_batch(cases, "fp64", tol=1e-10, n_iter=600, knobs=kn, cfl_max=d["cfl_max"])
+    rmx = solve_batch(cases, "mixed", tol=1e-10, n_iter=600)
+    print("iterations", r64.iterations, rmx.iterations)
+    print("dcl", np.abs(rmx.cl - r64.cl), "dcd", np.abs(rmx.cd - r64.cd))
+    assert np.all(r64.converged) and np.all(rmx.converged)
+    assert np.max(np.abs(rmx.cl - r64.cl)) < 1e-7
+    assert np.max(np.abs(rmx.cd - r64.cd)) < 1e-8
+    assert np.max(np.abs(rmx.cm - r64.cm)) < 1e-7
