@@ -20,7 +20,8 @@ from . import _abi
 from .grid import OGrid
 from .implicit import NewtonLoop
 
-__all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs", "SOLVER_DEFAULTS"]
+__all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs", "cfl_policy",
+           "SOLVER_DEFAULTS"]
 
 
 @dataclass
@@ -50,14 +51,29 @@ class BatchResult:
 
 # Solver settings per form. "fp64" is the reference's algorithm (fixed 50
 # Arnoldi steps, 4 line-Jacobi sweeps, CFL cap 300: aero2d/implicit.py:68-78).
-# "mixed" is tuned for throughput on transonic cases: Arnoldi stops once the
-# preconditioned residual estimate meets gmres_tol, the preconditioner runs
-# two red-black sweeps (as strong as four Jacobi sweeps here at half the
-# cost), and the CFL cap is 1000. Both converge to the same discrete solution.
+# "mixed" is tuned for throughput: Arnoldi stops once the preconditioned
+# residual estimate meets gmres_tol (at most 20 steps), the preconditioner
+# runs two red-black sweeps (as strong as four Jacobi sweeps here at half the
+# cost), and the CFL cap is the reference's 300 until rn/r0 < 1e-3, then 1000
+# (a cap of 1000 from the start stalls transonic cases while their shocks
+# form; raised late it shortens the tail). Both forms solve the same discrete
+# equations; where those have more than one steady state (transonic cases with
+# a shock), the solver path decides which one a case reaches.
 SOLVER_DEFAULTS = {
-    "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0),
-    "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=1000.0),
+    "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0,
+                 cfl_late=None, late_below=0.0),
+    "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=300.0,
+                  cfl_late=1000.0, late_below=1e-3),
 }
+
+
+def cfl_policy(precision: str, cfl_max: float | None = None, cfl_late: float | None = None,
+               late_below: float | None = None) -> tuple:
+    """(cfl_max, cfl_late, late_below) of the form, with optional overrides."""
+    d = SOLVER_DEFAULTS[precision]
+    return (d["cfl_max"] if cfl_max is None else float(cfl_max),
+            d["cfl_late"] if cfl_late is None else float(cfl_late),
+            d["late_below"] if late_below is None else float(late_below))
 
 
 def default_knobs(precision: str, restart: int | None = None, early_exit: bool | None = None,
@@ -104,15 +120,14 @@ class BatchSolve:
 
     def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float | None = None,
             keep_state: bool = False, on_step=None, cfl_late: float | None = None,
-            late_below: float = 0.0, record_history: bool = False) -> BatchResult:
+            late_below: float | None = None, record_history: bool = False) -> BatchResult:
         """Newton loop to rn/r0 <= tol per case (or n_iter steps); the pool is
         released at the end."""
         if self.pool is None:
             self.upload()
         cases, ids, pool = self.cases, self.ids, self.pool
         nb = len(cases)
-        if cfl_max is None:
-            cfl_max = SOLVER_DEFAULTS[self.precision]["cfl_max"]
+        cfl_max, cfl_late, late_below = cfl_policy(self.precision, cfl_max, cfl_late, late_below)
         t0 = time.perf_counter()
         try:
             loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below)
@@ -175,13 +190,14 @@ class BatchSolve:
 def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, n_iter: int = 200,
                 cfl0: float = 10.0, cfl_max: float | None = None, knobs: _abi.Knobs | None = None,
                 shapes: dict | None = None, keep_state: bool = False, pool_budget: int = 0,
-                on_step=None, cfl_late: float | None = None, late_below: float = 0.0,
+                on_step=None, cfl_late: float | None = None, late_below: float | None = None,
                 record_history: bool = False) -> BatchResult:
     """Upload every case, then solve each to rn/r0 <= tol (or n_iter Newton
-    steps). precision: "fp64" or "mixed"; knobs and cfl_max default to
-    SOLVER_DEFAULTS[precision]. shapes: optional {(ni, nc): stencil
-    shape index} from the autotuner. on_step(it, n_active) after each step.
-    cfl_late / late_below: raise the CFL cap to cfl_late once rn/r0 < late_below.
+    steps). precision: "fp64" or "mixed"; knobs and the CFL policy
+    (cfl_max, cfl_late, late_below) default to SOLVER_DEFAULTS[precision].
+    shapes: optional {(ni, nc): stencil shape index} from the autotuner.
+    on_step(it, n_active) after each step. cfl_late / late_below: the CFL cap
+    rises to cfl_late once rn/r0 < late_below (late_below = 0 never).
     BatchResult.wall covers the Newton loop (inputs resident); .upload_s the
     host-to-device set-up before it."""
     bs = BatchSolve(cases, precision, knobs, shapes, pool_budget).upload()

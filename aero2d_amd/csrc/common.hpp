@@ -160,6 +160,25 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
+// Reductions over the first nblk partial rows of a case's partial array
+// (column col), by one wavefront: lane l takes rows l, l + 64, ... in order,
+// then a fixed butterfly. Every lane gets the result; it depends on the case
+// alone, never on the batch it runs in.
+__device__ __forceinline__ double case_sum(const CaseDev& cd, int nblk, int col) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const size_t stride = (size_t)cd.kmax + 2;
+  double s = 0.0;
+  for (int b = lane; b < nblk; b += WAVE) s += cd.part[(size_t)b * stride + col];
+  return wave_sum(s);
+}
+__device__ __forceinline__ double case_max(const CaseDev& cd, int nblk, int col) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const size_t stride = (size_t)cd.kmax + 2;
+  double s = 0.0;
+  for (int b = lane; b < nblk; b += WAVE) s = fmax(s, cd.part[(size_t)b * stride + col]);
+  return wave_max(s);
+}
+
 // Named per-case vectors (ids >= 0 select raw Krylov basis vector V_id).
 enum VecId : int { VEC_X0 = -1, VEC_W = -2, VEC_D = -3, VEC_XA = -4, VEC_XB = -5, VEC_NEG_R = -6 };
 

@@ -14,6 +14,29 @@
 
 namespace a2d {
 
+// Reciprocal and square root of the working precision. FP64 keeps IEEE
+// division and square root (the outer residual, update and forces, and the
+// whole FP64 form). FP32 is only the inner precision of the mixed form (the
+// Jacobian-vector product and preconditioner blocks that feed the Krylov
+// solve), where the hardware's single-instruction v_rcp_f32 / v_sqrt_f32
+// (1 ulp) replace the ten-instruction correctly rounded sequences.
+__host__ __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__host__ __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+__host__ __device__ __forceinline__ float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 template <class T>
 struct Dual {
   T v, d;
@@ -39,7 +62,7 @@ template <class T> __host__ __device__ __forceinline__ Dual<T> operator+(Dual<T>
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
-  const T r = T(1) / b.v;
+  const T r = frcp(b.v);
   const T q = a.v * r;
   return {q, (a.d - q * b.d) * r};
 }
@@ -51,27 +74,30 @@ template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(Dual<T>
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator-(T a, Dual<T> b) { return {a - b.v, -b.d}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
-template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
+template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) {
+  const T r = frcp(b);
+  return {a.v * r, a.d * r};
+}
 template <class T> __host__ __device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
-  const T r = T(1) / b.v;
+  const T r = frcp(b.v);
   const T q = a * r;
   return {q, -q * b.d * r};
 }
 
 // Reciprocal 1/x (one division for value and tangent).
 __host__ __device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
-__host__ __device__ __forceinline__ float drcp(float x) { return 1.0f / x; }
+__host__ __device__ __forceinline__ float drcp(float x) { return frcp(x); }
 template <class T> __host__ __device__ __forceinline__ Dual<T> drcp(Dual<T> x) {
-  const T r = T(1) / x.v;
+  const T r = frcp(x.v);
   return {r, -r * r * x.d};
 }
 
 // ---- elementary functions ---------------------------------------------------
 __host__ __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-__host__ __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__host__ __device__ __forceinline__ float dsqrt(float x) { return fsqrt(x); }
 template <class T> __host__ __device__ __forceinline__ Dual<T> dsqrt(Dual<T> x) {
   T s = dsqrt(x.v);
-  return {s, x.d / (T(2) * s)};
+  return {s, T(0.5) * x.d * frcp(s)};
 }
 
 __host__ __device__ __forceinline__ double dpow(double x, double a) { return pow(x, a); }

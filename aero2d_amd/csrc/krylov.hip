@@ -96,17 +96,19 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
   }
 }
 
-// One block per case: H[j][k] = s_j * sum_b part[b][j]; vn0 = |w|.
-__global__ void k_reduce_dots(Plan pc, int k) {
+// One block per case, a wavefront per column: H[j][k] = s_j * sum_b part[b][j];
+// vn0 = |w|.
+__global__ void __launch_bounds__(NT) k_reduce_dots(Plan pc, int k) {
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
   if (!sc->active) return;
   const int nblk = cd.nvecb;
-  for (int j = threadIdx.x; j <= k + 1; j += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2) + j];
-    if (j <= k) sc->H[j][k] = sc->s[j] * s;
-    else sc->vn0 = sqrt(s);
+  for (int j = threadIdx.x / WAVE; j <= k + 1; j += NT / WAVE) {
+    const double s = case_sum(cd, nblk, j);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      if (j <= k) sc->H[j][k] = sc->s[j] * s;
+      else sc->vn0 = sqrt(s);
+    }
   }
 }
 
@@ -176,57 +178,62 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   }
 }
 
-// One thread per case: finish Arnoldi step k (norm, breakdown test with
+// One wavefront per case: finish Arnoldi step k (norm, breakdown test with
 // threshold eps * |w| as in JAX, Givens update of the Hessenberg column and
-// the residual estimate, optional early exit).
+// the residual estimate, optional early exit). The column and the rotations
+// are staged in LDS so the sequential rotation chain never waits on memory.
 template <class T>
-__global__ void k_normalize(Plan pc, int k, KnobsDev kn) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pc.nb) return;
-  const CaseDev& cd = pc.cases[pc.slots[e]];
+__global__ void __launch_bounds__(WAVE) k_normalize(Plan pc, int k, KnobsDev kn) {
+  __shared__ double h[KMAX + 1], gcs[KMAX], gsn[KMAX];
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
   if (!sc->active) return;
-  const int nblk = cd.nvecb;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
-  const double nrm = sqrt(s);
-  const bool brk = !(nrm > eps_of<T>() * sc->vn0);
-  sc->H[k + 1][k] = brk ? 0.0 : nrm;
-  sc->s[k + 1] = brk ? 0.0 : 1.0 / nrm;
-  // apply the previous rotations to column k
-  for (int i = 0; i < k; ++i) {
-    const double a = sc->H[i][k], b = sc->H[i + 1][k];
-    sc->H[i][k] = sc->gc[i] * a + sc->gs[i] * b;
-    sc->H[i + 1][k] = -sc->gs[i] * a + sc->gc[i] * b;
+  const int lane = threadIdx.x;
+  const double nrm = sqrt(case_sum(cd, cd.nvecb, 0));
+  for (int i = lane; i <= k; i += WAVE) h[i] = sc->H[i][k];
+  for (int i = lane; i < k; i += WAVE) {
+    gcs[i] = sc->gc[i];
+    gsn[i] = sc->gs[i];
   }
-  const double a = sc->H[k][k], b = sc->H[k + 1][k];
-  const double r = hypot(a, b);
-  const double c = (r > 0) ? a / r : 1.0, sn = (r > 0) ? b / r : 0.0;
-  sc->gc[k] = c;
-  sc->gs[k] = sn;
-  sc->H[k][k] = r;
-  sc->H[k + 1][k] = 0.0;
-  sc->g[k + 1] = -sn * sc->g[k];
-  sc->g[k] = c * sc->g[k];
-  sc->gres = fabs(sc->g[k + 1]);
-  sc->kdone = k + 1;
-  bool stop = brk || (k + 1 >= kn.restart);
-  if (kn.early_exit && sc->gres <= fmin(1.0, kn.gtol) * sc->mbnorm) stop = true;
-  if (stop) sc->active = 0;
+  __syncthreads();
+  if (lane == 0) {
+    const bool brk = !(nrm > eps_of<T>() * sc->vn0);
+    const double hk1 = brk ? 0.0 : nrm;
+    sc->s[k + 1] = brk ? 0.0 : 1.0 / nrm;
+    // apply the previous rotations to column k; `a` carries H[i][k]
+    double a = h[0];
+    for (int i = 0; i < k; ++i) {
+      const double b = h[i + 1];
+      h[i] = gcs[i] * a + gsn[i] * b;
+      a = -gsn[i] * a + gcs[i] * b;
+    }
+    const double r = hypot(a, hk1);
+    const double c = (r > 0) ? a / r : 1.0, sn = (r > 0) ? hk1 / r : 0.0;
+    sc->gc[k] = c;
+    sc->gs[k] = sn;
+    h[k] = r;
+    sc->H[k + 1][k] = 0.0;
+    sc->g[k + 1] = -sn * sc->g[k];
+    sc->g[k] = c * sc->g[k];
+    sc->gres = fabs(sc->g[k + 1]);
+    sc->kdone = k + 1;
+    bool stop = brk || (k + 1 >= kn.restart);
+    if (kn.early_exit && sc->gres <= fmin(1.0, kn.gtol) * sc->mbnorm) stop = true;
+    if (stop) sc->active = 0;
+  }
+  __syncthreads();
+  for (int i = lane; i <= k; i += WAVE) sc->H[i][k] = h[i];
 }
 
 // stage 0: |M b| from the x0 sweep's partials.
 // stage 1: beta = |M(b - A x0)| from the V_0 sweep's partials; run flag.
 template <class T>
-__global__ void k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pc.nb) return;
-  const CaseDev& cd = pc.cases[pc.slots[e]];
+__global__ void __launch_bounds__(WAVE) k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
   const int nblk = cd.nlineb * (kn.precond_rb ? 2 : 1);
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
-  const double nrm = sqrt(s);
+  const double nrm = sqrt(case_sum(cd, nblk, 0));
+  if (threadIdx.x != 0) return;
   if (stage == 0) {
     sc->mbnorm = nrm;
     return;
@@ -337,17 +344,12 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
 
 // One thread per case: omega = min(1, max_update / (rel + 1e-30)); accept the
 // step when rn, rel and dU are finite (aero2d/implicit.py:190, 218-219).
-__global__ void k_limiter(Plan pc, KnobsDev kn) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pc.nb) return;
-  const CaseDev& cd = pc.cases[pc.slots[e]];
+__global__ void __launch_bounds__(WAVE) k_limiter(Plan pc, KnobsDev kn) {
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
-  const int nblk = cd.nvecb;
-  double mx = 0.0, ct = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    mx = fmax(mx, cd.part[(size_t)b * (cd.kmax + 2)]);
-    ct += cd.part[(size_t)b * (cd.kmax + 2) + 1];
-  }
+  const double mx = case_max(cd, cd.nvecb, 0);
+  const double ct = case_sum(cd, cd.nvecb, 1);
+  if (threadIdx.x != 0) return;
   sc->rel = mx;
   sc->nonfinite = int(ct);
   const double om = fmin(1.0, kn.max_update / (mx + 1e-30));
@@ -429,21 +431,21 @@ void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int pre
   else hipLaunchKernelGGL(k_dots<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
 }
 void launch_reduce_dots(const Plan& pc, int k, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(64), 0, s, pc, k);
+  hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(NT), 0, s, pc, k);
 }
 void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s) {
   if (prec == 0) hipLaunchKernelGGL(k_update<double>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
   else hipLaunchKernelGGL(k_update<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
 }
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_normalize<double>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, k, kn);
-  else hipLaunchKernelGGL(k_normalize<float>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, k, kn);
+  if (prec == 0) hipLaunchKernelGGL(k_normalize<double>, dim3(pc.nb), dim3(WAVE), 0, s, pc, k, kn);
+  else hipLaunchKernelGGL(k_normalize<float>, dim3(pc.nb), dim3(WAVE), 0, s, pc, k, kn);
 }
 void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s) {
   if (prec == 0)
-    hipLaunchKernelGGL(k_krylov_begin<double>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, stage, kn);
+    hipLaunchKernelGGL(k_krylov_begin<double>, dim3(pc.nb), dim3(WAVE), 0, s, pc, stage, kn);
   else
-    hipLaunchKernelGGL(k_krylov_begin<float>, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, stage, kn);
+    hipLaunchKernelGGL(k_krylov_begin<float>, dim3(pc.nb), dim3(WAVE), 0, s, pc, stage, kn);
 }
 void launch_lstsq(const Plan& pc, hipStream_t s) {
   hipLaunchKernelGGL(k_lstsq, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc);
@@ -453,7 +455,7 @@ void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s) {
   else hipLaunchKernelGGL(k_dupdate<float>, dim3(nblocks), dim3(NT), 0, s, pv);
 }
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s) {
-  hipLaunchKernelGGL(k_limiter, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc, kn);
+  hipLaunchKernelGGL(k_limiter, dim3(pc.nb), dim3(WAVE), 0, s, pc, kn);
 }
 void launch_apply(const Plan& pv, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_apply, dim3(nblocks), dim3(NT), 0, s, pv);

@@ -41,8 +41,8 @@ __host__ __device__ __forceinline__ W4<S> cons_to_prim(S q0, S q1, S q2, S q3) {
 //   f E/rho = (S_k - q) E d,   f p/(rho (S_k - q)) = p d.
 template <class S, class T>
 __host__ __device__ __forceinline__ void hllc_flux(const W4<S>& L, const W4<S>& R, T sx, T sy, S F[4]) {
-  const T area = sqrt(sx * sx + sy * sy);
-  const T ia = T(1) / area;
+  const T area = fsqrt(sx * sx + sy * sy);
+  const T ia = frcp(area);
   const T nx = sx * ia, ny = sy * ia;
   const S irL = drcp(L.r), irR = drcp(R.r);
   const S qL = L.u * nx + L.v * ny;

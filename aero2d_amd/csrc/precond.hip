@@ -508,6 +508,15 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
   // colour < 0: every line, natural block layout; else line q of colour
   // `color` (i = 2q + color), whose blocks sit in column color * ni/2 + q
   const int nq = color < 0 ? ni : (ni >> 1);
+  if (lb * LINES_PER_BLOCK >= nq) {
+    // a whole block past the colour's lines: nothing to solve (the plan is
+    // sized for every line); its norm partial reads zero
+    if (want_norm && threadIdx.x == 0) {
+      const int slot = color < 0 ? lb : color * cd.nlineb + lb;
+      cd.part[(size_t)slot * (cd.kmax + 2)] = 0.0;
+    }
+    return;
+  }
   const bool live = q < nq;
   const int qc = live ? q : nq - 1;     // dead lanes shadow a real line and never store
   const int ic = color < 0 ? qc : 2 * qc + color;

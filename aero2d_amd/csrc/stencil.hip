@@ -439,15 +439,11 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate) {
   }
 }
 
-// rn = sqrt(sum of the residual kernel's partials / n), one thread per case.
-__global__ void k_rn_finalize(Plan pc) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pc.nb) return;
-  const CaseDev& cd = pc.cases[pc.slots[e]];
-  const int nblk = cd.ntiles;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += cd.part[(size_t)b * (cd.kmax + 2)];
-  cd.sc->rn = sqrt(s / cd.n);
+// rn = sqrt(sum of the residual kernel's partials / n), one wavefront per case.
+__global__ void __launch_bounds__(WAVE) k_rn_finalize(Plan pc) {
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
+  const double s = case_sum(cd, cd.ntiles, 0);
+  if (threadIdx.x == 0) cd.sc->rn = sqrt(s / cd.n);
 }
 
 // ---- shape table --------------------------------------------------------------
@@ -515,7 +511,7 @@ void launch_dcirc(const Plan& p, int vin, int gate, int prec, hipStream_t s) {
 }
 
 void launch_rn_finalize(const Plan& pc, hipStream_t s) {
-  hipLaunchKernelGGL(k_rn_finalize, dim3((pc.nb + 63) / 64), dim3(64), 0, s, pc);
+  hipLaunchKernelGGL(k_rn_finalize, dim3(pc.nb), dim3(WAVE), 0, s, pc);
 }
 
 // Test hook: HLLC flux on n independent faces, arrays (4, n) / (2, n).

@@ -24,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from .batch import SOLVER_DEFAULTS, Case, default_knobs
+from .batch import SOLVER_DEFAULTS, Case, cfl_policy, default_knobs
 from .implicit import NewtonLoop
 
 __all__ = ["ServeResult", "SolveServer", "serve_stream"]
@@ -68,7 +68,8 @@ class SolveServer:
                  tol: float = 1e-10, n_iter: int = 800, cfl0: float = 10.0,
                  restart: int | None = None, early_exit: int | None = None,
                  cfl_max: float | None = None):
-        defaults = SOLVER_DEFAULTS[precision]
+        if precision not in SOLVER_DEFAULTS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         self.prec = _abi.PREC_MIXED if precision == "mixed" else _abi.PREC_FP64
         kn = default_knobs(precision, restart=restart, early_exit=early_exit)
@@ -77,7 +78,8 @@ class SolveServer:
         self.budget = hbm_budget
         self.max_batch = max_batch
         self.tol, self.n_iter = tol, n_iter
-        self.loop = NewtonLoop(4096, cfl0, defaults["cfl_max"] if cfl_max is None else cfl_max)
+        cmax, clate, below = cfl_policy(precision, cfl_max)
+        self.loop = NewtonLoop(4096, cfl0, cmax, clate, below)
         self.active: dict[int, _Slot] = {}
         self.waiting: list[tuple[Case, float, Future]] = []
         self.t0 = time.perf_counter()

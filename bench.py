@@ -143,14 +143,15 @@ def max_errors(a, b):
     return out
 
 
-def mixed_path_knobs_fp64():
-    """FP64 form run with the mixed form's solver settings (same preconditioner,
-    Krylov size, early exit and CFL cap): differences from the mixed form are
-    then due to precision alone."""
+def mixed_path_fp64():
+    """solve_batch arguments that run the FP64 form with the mixed form's
+    solver settings (same preconditioner, Krylov size, early exit and CFL
+    policy): differences from the mixed form are then due to precision alone."""
     from aero2d_amd.batch import SOLVER_DEFAULTS, default_knobs
     d = SOLVER_DEFAULTS["mixed"]
-    return default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
-                         precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"]), d["cfl_max"]
+    kn = default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
+                       precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"])
+    return dict(knobs=kn, cfl_max=d["cfl_max"], cfl_late=d["cfl_late"], late_below=d["late_below"])
 
 
 def compare_forms(r64, rmx, r64m):
@@ -181,8 +182,7 @@ def run_config3(a):
     print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
     rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
     print(f"# mixed: {rmx.wall:.1f} s, converged {int(rmx.converged.sum())}", file=sys.stderr, flush=True)
-    kn, cmax = mixed_path_knobs_fp64()
-    r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, knobs=kn, cfl_max=cmax)
+    r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, **mixed_path_fp64())
     print(f"# fp64 on the mixed form's path: {r64m.wall:.1f} s, converged {int(r64m.converged.sum())}",
           file=sys.stderr, flush=True)
     v64 = r64.converged.sum() / r64.wall
@@ -335,8 +335,7 @@ def main():
     }
     if not a.no_mixed:
         rmx, wallmx, convmx = run("mixed")
-        kn, cmax = mixed_path_knobs_fp64()
-        r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=a.n_iter, knobs=kn, cfl_max=cmax)
+        r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=a.n_iter, **mixed_path_fp64())
         lm = np.percentile(rmx.t_done, [50, 99]) * 1e3
         line["mixed"] = {
             "value": convmx / wallmx, "unit": "cases/s", "speedup_vs_fp64": (convmx / wallmx) / value,

@@ -240,3 +240,40 @@ def test_newton_steps_from_freestream_match_oracle(case):
         ro = newton.solve(geo, [mach], [alpha], n_iter=steps, tol=0.0)
         assert rel_err(r.U[0], ro.U[..., 0].numpy()) < 5e-6, steps
         assert abs(r.cl[0] - ro.cl[0]) < 5e-5
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_mixed_jvp_and_blocks_match_oracle(case):
+    """The mixed form's inner precision: the FP32 Jacobian-vector product
+    (hardware 1-ulp reciprocal and square root) and the FP32 preconditioner
+    blocks against the FP64 oracle. FP32 rounding of O(1) fluxes and their
+    differences bounds the agreement near 1e-6 of the field's magnitude; the
+    test allows 1e-4 (a dropped term or wrong index shows as O(1))."""
+    g, s, U, h, c = _setup(*case, precision="mixed")
+    mach, alpha = case[2], case[3]
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    rng = np.random.default_rng(case[4] + 200)
+    v = np.ascontiguousarray(rng.normal(size=U.shape) * np.array([1, 0.5, 0.5, 2.0])[:, None, None])
+    out = np.empty_like(U)
+    res = lambda u: flow.residual_with_circ(u, w, geo)[0]
+    jo = jvp(res, (torch.tensor(U),), (torch.tensor(v),))[1].numpy()
+    _abi.call("a2d_jvp", h, c, _abi.dptr(v), 10.0, 0, _abi.dptr(out))
+    err = rel_err(out, jo)
+    print("fp32 jvp rel err", err)
+    assert err < 1e-4
+    _abi.call("a2d_jvp", h, c, _abi.dptr(v), 10.0, 1, _abi.dptr(out))
+    Wt = flow.cons_to_prim(torch.tensor(U))
+    diag = (geo["vol"] / (flow.local_dt(Wt, geo) / 2.5 * 10.0)).numpy()
+    assert rel_err(out, diag[None] * v + jo) < 1e-4
+    ni, nc = s.ni, s.nc
+    blocks = [np.empty((ni, nc, 4, 4)) for _ in range(5)]
+    _abi.call("a2d_assemble", h, c, 10.0, *[_abi.dptr(b) for b in blocks])
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 10.0)
+    ref = newton.assemble(Ut, w, circ, dti, geo)
+    for got, exp in zip(blocks, ref):
+        e = rel_err(got, exp.numpy())
+        print("fp32 block rel err", e)
+        assert e < 1e-4
