@@ -12,11 +12,12 @@
 //   xf, yf, xw, yw        double [ni]            outer / wall face midpoints
 //   Ut, volt, Sit, Sjt... T      copies in the inner precision (alias the
 //                                double arrays when T = double)
-//   blocks                T [5][16][nc][ni]       inv, c, A(=Bjm), Bim, Bip
+//   blocks                T [5][nc][ni][16]       cell-major 4x4 blocks, five slots
 //   V                     T [K+1][4][nc][ni]      raw Krylov basis
-// Consecutive block planes and basis vectors sit pstride / vstride elements
-// apart: the sizes plus a pad, so that the many arrays a kernel walks in step
-// do not all start at the same offset modulo a large power of two.
+// Consecutive block slots sit 16 pstride elements apart and basis vectors
+// vstride elements apart: the sizes plus a pad, so that the many arrays a
+// kernel walks in step do not all start at the same offset modulo a large
+// power of two.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -67,7 +68,7 @@ struct CaseDev {
   int nlineb;           // line-kernel blocks (64 lines each)
   int shape;            // stencil tile shape index
   int vstride;          // elements between consecutive Krylov basis vectors (>= 4n)
-  int pstride;          // elements between consecutive block planes (>= n)
+  int pstride;          // cells per block slot (>= n); a slot is 16 pstride elements
   double* U;
   double* R;
   double* dU;
@@ -85,7 +86,7 @@ struct CaseDev {
   void* xft;
   void* yft;
   void* diag;           // T [nc][ni]: vol / dtau
-  void* blk;            // T [5][16][nc][ni]
+  void* blk;            // T [5][pstride][16] (cell-major blocks)
   void* V;              // T [kmax+1][4][nc][ni]
   void* x0;             // T [4][nc][ni]
   void* w;              // T [4][nc][ni]
@@ -199,11 +200,10 @@ __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
   }
 }
 
-// Positive modulo for the periodic i direction.
 // Offset of entry m (= row * 4 + column) of block `slot` of cell `cell` in the
-// preconditioner's block storage T blk[5][4][pstride][4].
+// preconditioner's cell-major block storage T blk[5][pstride][16].
 __host__ __device__ __forceinline__ size_t blk_off(int slot, int m, size_t cell, size_t pstride) {
-  return ((size_t(slot) * 4 + (m >> 2)) * pstride + cell) * 4 + (m & 3);
+  return (size_t(slot) * pstride + cell) * 16 + m;
 }
 
 // Column of line i in the preconditioner's block storage. With the red-black
@@ -213,6 +213,7 @@ __host__ __device__ __forceinline__ int bcol(int i, int ni, int split) {
   return split ? (i & 1) * (ni >> 1) + (i >> 1) : i;
 }
 
+// Positive modulo for the periodic i direction.
 __device__ __forceinline__ int wrap(int i, int ni) {
   i %= ni;
   return i < 0 ? i + ni : i;

@@ -313,16 +313,16 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
     for (int s = 0; s < S; ++s)
       for (int color = 1; color >= 0; --color) {
         const int coupled = !(s == 0 && color == 1);
-        launch_line_sweep_rb(b.pcell, b.ncell, b.pline, b.nline, src, dest, color, coupled,
-                             (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
+        launch_line_sweep_rb(b.pline, b.nline, src, dest, color, coupled, (s == S - 1) ? want_norm : 0,
+                             gate, b.prec, p.stream);
       }
     return;
   }
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
-    launch_line_sweep(b.pcell, b.ncell, b.pline, b.nline, src, prev, out, s,
-                      (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
+    launch_line_sweep(b.pline, b.nline, src, prev, out, s > 0, (s == S - 1) ? want_norm : 0, gate,
+                      b.prec, p.stream);
   }
 }
 
@@ -838,7 +838,6 @@ int a2d_assemble(int64_t pool, int case_id, double cfl, double* D, double* Bim, 
     set_cfl_one(p, b, cfl);
     enq_prologue(p, b, kn);
     launch_assemble(b.pcell, b.ncell, kn, c.prec, p.stream);
-    const size_t n = size_t(c.ni) * c.nc;
     const size_t t = c.prec == 0 ? 8 : 4;
     const size_t ps = size_t(c.dev.pstride);
     std::vector<char> raw(80 * ps * t);
@@ -935,17 +934,20 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
           for (auto& g : b.sten)
             launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
           break;
-        case 2:
-          launch_line_sweep(b.pcell, b.ncell, b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 0, 0, b.prec,
-                            p.stream);
+        case 2:  // a coupled line-Jacobi sweep over every line
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 0, 0, b.prec, p.stream);
           break;
         case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
         case 5: launch_factor(b.pline, b.nline, b.prec, p.stream); break;
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
-        case 8: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 0, b.prec, p.stream); break;
-        case 9: launch_sweep_part(b.pcell, b.ncell, b.pline, b.nline, 1, b.prec, p.stream); break;
+        case 8:  // the first (uncoupled) sweep of an application
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, b.prec, p.stream);
+          break;
+        case 9:  // a coupled red-black half sweep (odd lines)
+          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 0, 0, b.prec, p.stream);
+          break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");
       }

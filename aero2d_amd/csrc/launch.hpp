@@ -54,23 +54,18 @@ void launch_hllc_test(int n, const double* WL, const double* WR, const double* S
 // ---- precond.hip --------------------------------------------------------------
 void launch_assemble(const Plan& pcell, int nblocks, const KnobsDev& k, int prec, hipStream_t s);
 void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
-// One line-Jacobi sweep: rhs = src - (Bim x(i-1) + Bip x(i+1)) with x = xprev
-// (sweep > 0), xout = L^-1 rhs. src = VEC_NEG_R reads -R. want_norm: per-block
-// partials of |xout|^2. Two launches: a per-cell pass (pcell, ncell blocks)
-// and the line recurrences (pl, nblocks).
-void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
-                       int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s);
+// One line-Jacobi sweep, one kernel: x_out = L^-1 (src - Bim x(i-1) - Bip x(i+1))
+// with x = xprev, the neighbour terms only when coupled. src = VEC_NEG_R reads
+// -R. want_norm: per-block partials of |xout|^2. Plan pl: LINES_PER_BLOCK
+// lines per block.
+void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
+                       int want_norm, int gate, int prec, hipStream_t s);
 // One half sweep of the red-black ordering: lines i = color (mod 2) only,
 // in place in xout (neighbours of the other colour are read from xout);
 // coupled = 0 drops the i-coupling (the very first half sweep). Norm partials
 // go to part[(color * nlineb + block)].
-void launch_line_sweep_rb(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xout,
-                          int color, int coupled, int want_norm, int gate, int prec, hipStream_t s);
-
-// Timing hook: part 0 = the per-cell rhs pass of a coupled sweep, part 1 = the
-// line recurrences alone (reading the d buffer left by a previous sweep).
-void launch_sweep_part(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int part, int prec,
-                       hipStream_t s);
+void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
+                          int want_norm, int gate, int prec, hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);

@@ -10,24 +10,26 @@
 //               on the wall half, the mirror recurrence on the far-field half,
 //               joined at the middle row. Two threads per line, 4x4 blocks in
 //               registers.
-//   one line-Jacobi sweep (aero2d/implicit.py:130-157) is two kernels. The
-//   reference's forward step d_j = inv_j (rhs_j - A_j d_{j-1}) is split as
-//   d_j = g_j - P_j d_{j-1} with g_j = inv_j rhs_j and P_j = inv_j A_j:
-//   k_sweep_rhs  fully parallel, thread per cell: rhs = r - Bim x(i-1) - Bip x(i+1)
-//                (r alone in the first sweep), g = inv rhs.
-//   k_sweep_seq  the j recurrences only, both halves of each line at once
-//                (eight lanes per line, lane r owns row r of each 4x4 product,
-//                rows exchanged in the quad; loads for the next PF steps are in
-//                flight while the current PF steps compute).
-//   The same direct solve as the reference in another elimination order, so
-//   the same answer up to rounding.
+//               The same pass also stores S0 Bim and S0 Bip (S0 = inverse of
+//               the eliminated diagonal) over Bim and Bip, so that a sweep
+//               forms g = S0 rhs without a second dependent product.
+//   k_sweep     one line-Jacobi sweep (aero2d/implicit.py:130-157) or one
+//               red-black half sweep in a single pass: the right-hand side
+//               with the neighbour-line coupling, and the two-sided block
+//               recurrences of every line (eight lanes per line, lane r owns
+//               row r of each 4x4 product, components exchanged in the quad).
+//   The reference's forward step d_j = inv_j (rhs_j - A_j d_{j-1}) is
+//   regrouped as d_j = g_j - P_j d_{j-1} with g_j = inv_j rhs_j and
+//   P_j = inv_j A_j: the same direct solve as the reference in another
+//   elimination order, so the same answer up to rounding.
 //
-// Block storage: T blk[5][4][pstride][4] = slot, block row, cell, column, so a
-// block row is one 16-byte load in FP32 (two in FP64). Slots after assembly:
-// 0 = D, 1 = Bjp, 2 = Bjm, 3 = Bim, 4 = Bip; after factorisation 0 = inverse
-// of the eliminated diagonal, 1 = backward coupling (c or e), 2 = forward
-// coupling (P or Q). With the red-black line order the cell index is
-// colour-split (column of line i = (i % 2) ni/2 + i/2).
+// Block storage: T blk[5][pstride][4][4] = slot, cell, block row, column
+// (cell-major): a block row is one 16-byte load in FP32 (two in FP64) and the
+// four rows a quad of lanes loads for one cell are one contiguous 64-byte
+// run. Slots after assembly: 0 = D, 1 = Bjp, 2 = Bjm, 3 = Bim, 4 = Bip; after
+// factorisation 0 = S0, 1 = backward coupling (c or e), 2 = forward coupling
+// (P or Q), 3 = S0 Bim, 4 = S0 Bip. With the red-black line order the cell
+// index is colour-split (column of line i = (i % 2) ni/2 + i/2).
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -126,109 +128,6 @@ __device__ __forceinline__ void far_col(const T q[4], T sx, T sy, const Inf<T>& 
 
 constexpr int NT = 256;
 
-template <class T>
-__global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
-  const int e = plan_entry(plan, blockIdx.x);
-  const CaseDev& cd = plan.cases[plan.slots[e]];
-  const int ni = cd.ni, nc = cd.nc, n = cd.n;
-  const int idx = (blockIdx.x - plan.blk_prefix[e]) * NT + threadIdx.x;
-  if (idx >= n) return;
-  const int i = idx % ni, j = idx / ni;
-  const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
-  const gptr<const T> U = gp(base_U<T>(cd));
-  const gptr<const T> Si = gp(met_Si<T>(cd));
-  const gptr<const T> Sj = gp(met_Sj<T>(cd));
-  const size_t sj1 = (size_t)(nc + 1) * ni;
-  const gptr<T> blk = gp(tp<T>(cd.blk));
-  const size_t plane = (size_t)cd.pstride;
-  const size_t bidx = (size_t)j * ni + bcol(i, ni, kn.precond_rb);
-  auto at = [&](int slot, int m) -> T& { return blk[blk_off(slot, m, bidx, plane)]; };
-
-  T qc[4], qim[4], qip[4];
-  load_cons(U, n, idx, qc);
-  load_cons(U, n, j * ni + im, qim);
-  load_cons(U, n, j * ni + ip, qip);
-  T Dm[16];
-  const T dg = gp(ctp<T>(cd.diag))[idx];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) Dm[m] = (m % 5 == 0) ? dg : T(0);
-
-  // i-face i (cells im | i) and i-face i+1 (cells i | ip)
-  const T six0 = Si[j * ni + i], siy0 = Si[n + j * ni + i];
-  const T six1 = Si[j * ni + ip], siy1 = Si[n + j * ni + ip];
-  for (int b = 0; b < 4; ++b) {
-    T col[4];
-    face_col<T>(qim, qc, six0, siy0, b, false, col);   // AL_i -> Bim = -AL_i
-#pragma unroll
-    for (int r = 0; r < 4; ++r) at(3, r * 4 + b) = -col[r];
-    face_col<T>(qim, qc, six0, siy0, b, true, col);    // AR_i -> D -= AR_i
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
-    face_col<T>(qc, qip, six1, siy1, b, false, col);   // AL_{i+1} -> D += AL_{i+1}
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
-    face_col<T>(qc, qip, six1, siy1, b, true, col);    // AR_{i+1} -> Bip
-#pragma unroll
-    for (int r = 0; r < 4; ++r) at(4, r * 4 + b) = col[r];
-  }
-  // j-face j (cells j-1 | j) or the wall
-  const T sjx0 = Sj[j * ni + i], sjy0 = Sj[sj1 + j * ni + i];
-  if (j >= 1) {
-    T qjm[4];
-    load_cons(U, n, idx - ni, qjm);
-    for (int b = 0; b < 4; ++b) {
-      T col[4];
-      face_col<T>(qjm, qc, sjx0, sjy0, b, false, col);  // AL_j -> Bjm = -AL_j
-#pragma unroll
-      for (int r = 0; r < 4; ++r) at(2, r * 4 + b) = -col[r];
-      face_col<T>(qjm, qc, sjx0, sjy0, b, true, col);   // AR_j -> D -= AR_j
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
-    }
-  } else {
-    for (int b = 0; b < 4; ++b) {
-      T col[4];
-      wall_col<T>(qc, sjx0, sjy0, b, col);              // D -= Aw
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        Dm[r * 4 + b] -= col[r];
-        at(2, r * 4 + b) = T(0);
-      }
-    }
-  }
-  // j-face j+1 (cells j | j+1) or the far field
-  const T sjx1 = Sj[(j + 1) * ni + i], sjy1 = Sj[sj1 + (j + 1) * ni + i];
-  if (j + 1 < nc) {
-    T qjp[4];
-    load_cons(U, n, idx + ni, qjp);
-    for (int b = 0; b < 4; ++b) {
-      T col[4];
-      face_col<T>(qc, qjp, sjx1, sjy1, b, false, col);  // AL_{j+1} -> D += AL_{j+1}
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
-      face_col<T>(qc, qjp, sjx1, sjy1, b, true, col);   // AR_{j+1} -> Bjp
-#pragma unroll
-      for (int r = 0; r < 4; ++r) at(1, r * 4 + b) = col[r];
-    }
-  } else {
-    const CaseScal* sc = cd.sc;
-    const Inf<T> inf = {T(sc->winf[0]), T(sc->winf[1]), T(sc->winf[2]), T(sc->winf[3])};
-    const T circ = kn.vortex ? T(sc->circ) : T(0);
-    const T xf = met_xf<T>(cd)[i], yf = met_yf<T>(cd)[i];
-    for (int b = 0; b < 4; ++b) {
-      T col[4];
-      far_col<T>(qc, sjx1, sjy1, inf, xf, yf, circ, b, col);  // D += Af
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        Dm[r * 4 + b] += col[r];
-        at(1, r * 4 + b) = T(0);
-      }
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 16; ++m) at(0, m) = Dm[m];
-}
-
 // 4x4 inverse by Gauss-Jordan elimination with partial pivoting; all indices
 // compile-time so the blocks stay in registers.
 template <class T>
@@ -288,7 +187,7 @@ struct Blk {
   gptr<T> p;
   size_t ps;
   __device__ __forceinline__ gptr<T> row(int slot, int r, size_t cell) const {
-    return p + (((size_t)slot * 4 + r) * ps + cell) * 4;
+    return p + ((size_t)slot * ps + cell) * 16 + r * 4;
   }
 };
 
@@ -339,6 +238,115 @@ __device__ __forceinline__ void mul4(const T a[16], const T b[16], T out[16]) {
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int idx = (blockIdx.x - plan.blk_prefix[e]) * NT + threadIdx.x;
+  if (idx >= n) return;
+  const int i = idx % ni, j = idx / ni;
+  const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
+  const gptr<const T> U = gp(base_U<T>(cd));
+  const gptr<const T> Si = gp(met_Si<T>(cd));
+  const gptr<const T> Sj = gp(met_Sj<T>(cd));
+  const size_t sj1 = (size_t)(nc + 1) * ni;
+  const size_t bidx = (size_t)j * ni + bcol(i, ni, kn.precond_rb);
+  const Blk<T> B{gp(tp<T>(cd.blk)), (size_t)cd.pstride};
+
+  T qc[4], qim[4], qip[4];
+  load_cons(U, n, idx, qc);
+  load_cons(U, n, j * ni + im, qim);
+  load_cons(U, n, j * ni + ip, qip);
+  T Dm[16], Ba[16], Bb[16];
+  const T dg = gp(ctp<T>(cd.diag))[idx];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) Dm[m] = (m % 5 == 0) ? dg : T(0);
+
+  // i-face i (cells im | i) and i-face i+1 (cells i | ip)
+  const T six0 = Si[j * ni + i], siy0 = Si[n + j * ni + i];
+  const T six1 = Si[j * ni + ip], siy1 = Si[n + j * ni + ip];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    T col[4];
+    face_col<T>(qim, qc, six0, siy0, b, false, col);   // AL_i -> Bim = -AL_i
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ba[r * 4 + b] = -col[r];
+    face_col<T>(qim, qc, six0, siy0, b, true, col);    // AR_i -> D -= AR_i
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
+    face_col<T>(qc, qip, six1, siy1, b, false, col);   // AL_{i+1} -> D += AL_{i+1}
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
+    face_col<T>(qc, qip, six1, siy1, b, true, col);    // AR_{i+1} -> Bip
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Bb[r * 4 + b] = col[r];
+  }
+  store_blk(B, 3, bidx, Ba);
+  store_blk(B, 4, bidx, Bb);
+  // j-face j (cells j-1 | j) or the wall
+  const T sjx0 = Sj[j * ni + i], sjy0 = Sj[sj1 + j * ni + i];
+  if (j >= 1) {
+    T qjm[4];
+    load_cons(U, n, idx - ni, qjm);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      face_col<T>(qjm, qc, sjx0, sjy0, b, false, col);  // AL_j -> Bjm = -AL_j
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ba[r * 4 + b] = -col[r];
+      face_col<T>(qjm, qc, sjx0, sjy0, b, true, col);   // AR_j -> D -= AR_j
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] -= col[r];
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      wall_col<T>(qc, sjx0, sjy0, b, col);              // D -= Aw
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Dm[r * 4 + b] -= col[r];
+        Ba[r * 4 + b] = T(0);
+      }
+    }
+  }
+  store_blk(B, 2, bidx, Ba);
+  // j-face j+1 (cells j | j+1) or the far field
+  const T sjx1 = Sj[(j + 1) * ni + i], sjy1 = Sj[sj1 + (j + 1) * ni + i];
+  if (j + 1 < nc) {
+    T qjp[4];
+    load_cons(U, n, idx + ni, qjp);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      face_col<T>(qc, qjp, sjx1, sjy1, b, false, col);  // AL_{j+1} -> D += AL_{j+1}
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Dm[r * 4 + b] += col[r];
+      face_col<T>(qc, qjp, sjx1, sjy1, b, true, col);   // AR_{j+1} -> Bjp
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Bb[r * 4 + b] = col[r];
+    }
+  } else {
+    const CaseScal* sc = cd.sc;
+    const Inf<T> inf = {T(sc->winf[0]), T(sc->winf[1]), T(sc->winf[2]), T(sc->winf[3])};
+    const T circ = kn.vortex ? T(sc->circ) : T(0);
+    const T xf = met_xf<T>(cd)[i], yf = met_yf<T>(cd)[i];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      T col[4];
+      far_col<T>(qc, sjx1, sjy1, inf, xf, yf, circ, b, col);  // D += Af
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Dm[r * 4 + b] += col[r];
+        Bb[r * 4 + b] = T(0);
+      }
+    }
+  }
+  store_blk(B, 1, bidx, Bb);
+  store_blk(B, 0, bidx, Dm);
+}
+
 // Two-sided (twisted) block elimination of every j-line, two threads per line.
 // With m = nc / 2, the top thread eliminates j = 0 .. m-1 downward from the
 // wall exactly as aero2d/implicit.py:112-128 does (den = D - A c_prev,
@@ -369,10 +377,12 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   for (int s = 0; s < m; ++s) {
     if (s < steps) {
       const size_t cell = (size_t)(j0 + dj * s) * ni + col;
-      T D[16], A[16], C[16];
+      T D[16], A[16], C[16], Bm[16], Bp[16];
       load_blk(Bc, 0, cell, D);
       load_blk(Bc, 2, cell, A);
       load_blk(Bc, 1, cell, C);
+      load_blk(Bc, 3, cell, Bm);
+      load_blk(Bc, 4, cell, Bp);
       T L[16], F[16], t[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
@@ -390,6 +400,10 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
         store_blk(B, 0, cell, inv);
         store_blk(B, 1, cell, cprev);
         store_blk(B, 2, cell, P);
+        mul4<T>(inv, Bm, t);
+        store_blk(B, 3, cell, t);
+        mul4<T>(inv, Bp, t);
+        store_blk(B, 4, cell, t);
       }
     }
   }
@@ -415,203 +429,184 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
     store_blk(B, 0, cell, inv);
     store_blk(B, 1, cell, Q);
     store_blk(B, 2, cell, P);
+    load_blk(Bc, 3, cell, A);
+    mul4<T>(inv, A, P);
+    store_blk(B, 3, cell, P);
+    load_blk(Bc, 4, cell, C);
+    mul4<T>(inv, C, Q);
+    store_blk(B, 4, cell, Q);
   }
 }
 
-// Sweep part 1, thread per cell: g = slot0 (src - Bim x(i-1) - Bip x(i+1))
-// into the work buffer d (four components per cell, contiguous).
-template <class T>
-__global__ void __launch_bounds__(256) k_sweep_rhs(Plan plan, int src, int xprev, int sweep,
-                                                   int gate, int color) {
-  const int e = plan_entry(plan, blockIdx.x);
-  const CaseDev& cd = plan.cases[plan.slots[e]];
-  if (gate && !cd.sc->active) return;
-  const int ni = cd.ni, n = cd.n;
-  const int t = (blockIdx.x - plan.blk_prefix[e]) * 256 + threadIdx.x;
-  int i, j, col;
-  if (color < 0) {            // every cell, natural block layout
-    if (t >= n) return;
-    i = t % ni;
-    j = t / ni;
-    col = i;
-  } else {                    // cells of lines i = color (mod 2), colour-split blocks
-    const int nh = ni >> 1;
-    if (t >= nh * cd.nc) return;
-    const int q = t % nh;
-    j = t / nh;
-    i = 2 * q + color;
-    col = color * nh + q;
-  }
-  const int idx = j * ni + i;                  // natural position (vectors)
-  const size_t bidx = (size_t)j * ni + col;    // block storage and g
-  const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
-  T rhs[4];
-  if (src == VEC_NEG_R) {
-    const gptr<const double> Rg = gp(cd.R);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) rhs[r] = T(-Rg[(size_t)r * n + idx]);
-  } else {
-    const gptr<const T> b = vec_gptr<T>(cd, src);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) rhs[r] = b[(size_t)r * n + idx];
-  }
-  if (sweep > 0) {
-    const gptr<const T> xp = vec_gptr<T>(cd, xprev);
-    const int im = (i == 0) ? ni - 1 : i - 1, ip = (i + 1 == ni) ? 0 : i + 1;
-    T xm[4], xq[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      xm[c] = xp[(size_t)c * n + j * ni + im];
-      xq[c] = xp[(size_t)c * n + j * ni + ip];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      T bm[4], bp[4];
-      load_row(B.row(3, r, bidx), bm);
-      load_row(B.row(4, r, bidx), bp);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) rhs[r] -= bm[c] * xm[c] + bp[c] * xq[c];
-    }
-  }
-  T g[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    T iv[4];
-    load_row(B.row(0, r, bidx), iv);
-    T s = T(0);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) s += iv[c] * rhs[c];
-    g[r] = s;
-  }
-  store_row(gp(tp<T>(cd.d)) + bidx * 4, g);
-}
-
-// Sweep part 2: the j recurrences of the two-sided solve. Eight lanes per
-// line: lanes 0-3 (one per block row) run j = 0 .. m-1, lanes 4-7 run
-// j = nc-1 .. m+1, each d_j = g_j - slot2_j d_prev in place over g; the two
-// halves meet at row m (x_m = g_m - slot2_m d_{m-1} - slot1_m f_{m+1}), then
-// walk back out, x_j = d_j - slot1_j x_prev. Rows are exchanged inside each
-// quad of lanes; loads for the next PF steps are in flight while the current
-// PF steps compute (double-buffered registers).
-template <class T, int PF>
-__global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want_norm, int gate,
-                                                   int color) {
+// One line sweep in a single pass. For every line of the sweep (every line
+// when color < 0, line Jacobi; lines i = color mod 2 of a red-black half
+// sweep otherwise):
+//   x = L^-1 (src - Bim x(i-1) - Bip x(i+1))      (neighbour terms when COUP)
+// with the factored blocks (slot 0 = S0, the inverse of the eliminated
+// diagonal; slots 3, 4 = S0 Bim, S0 Bip; slot 2 = the forward coupling; slot 1
+// the backward one):
+//   g_j = S0_j src_j - (S0 Bim)_j xm_j - (S0 Bip)_j xp_j
+//   d_j = g_j - S2_j d_prev                      toward the middle row m
+//   x_m = g_m - S2_m d_top - S1_m f_bot
+//   x_j = d_j - S1_j x_prev                      back out from the middle.
+// Eight lanes per line: lanes 0-3 (one per block row) own the wall half,
+// lanes 4-7 the far-field half. Each lane loads row r of each block (one
+// 16-byte load in FP32; a quad's four rows are one contiguous 64-byte run in
+// the cell-major block storage) and component r of each vector, and the quad
+// exchanges components by DPP. g is formed from loads already in flight, off
+// the recurrence chain. The loads of PF steps (PFB on the way back) are in
+// flight at any time: a register ring whose slot is refilled with the step
+// PF further on as soon as it is consumed. Loads are unconditional (rows past
+// the line's end are clamped to its last row, an L2 hit): a load under a
+// branch makes the compiler wait for every load in flight where the branch
+// joins. The d values of the way in are kept in the case's d buffer
+// ([cell][4]) for the way back.
+template <class T, bool COUP, bool NEGR, int PF, int PFB>
+__global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
+                                               int gate, int color) {
   __shared__ double red[4];
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   if (gate && !cd.sc->active) return;
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int lb = blockIdx.x - plan.blk_prefix[e];
-  const int r = threadIdx.x & 3;
-  const int half = (threadIdx.x >> 2) & 1;
-  const int q = lb * LINES_PER_BLOCK + (threadIdx.x >> 3);
-  // colour < 0: every line, natural block layout; else line q of colour
+  // color < 0: every line, natural block layout; else line q of colour
   // `color` (i = 2q + color), whose blocks sit in column color * ni/2 + q
   const int nq = color < 0 ? ni : (ni >> 1);
   if (lb * LINES_PER_BLOCK >= nq) {
-    // a whole block past the colour's lines: nothing to solve (the plan is
-    // sized for every line); its norm partial reads zero
+    // a whole block past the colour's lines (the plan is sized for every
+    // line): nothing to solve; its norm partial reads zero
     if (want_norm && threadIdx.x == 0) {
       const int slot = color < 0 ? lb : color * cd.nlineb + lb;
       cd.part[(size_t)slot * (cd.kmax + 2)] = 0.0;
     }
     return;
   }
+  const int r = threadIdx.x & 3;
+  const int half = (threadIdx.x >> 2) & 1;
+  const int q = lb * LINES_PER_BLOCK + (threadIdx.x >> 3);
   const bool live = q < nq;
   const int qc = live ? q : nq - 1;     // dead lanes shadow a real line and never store
   const int ic = color < 0 ? qc : 2 * qc + color;
   const int col = color < 0 ? qc : color * (ni >> 1) + qc;
+  const int im = (ic == 0) ? ni - 1 : ic - 1, ip = (ic + 1 == ni) ? 0 : ic + 1;
   const int m = nc >> 1;
   const int steps = half ? nc - 1 - m : m;
   const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
   const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
   const gptr<T> gd = gp(tp<T>(cd.d));
   const gptr<T> x = gp(vec_ptr<T>(cd, xout) + (size_t)r * n + ic);
-
-  T bm[2][PF][4], bv[2][PF];
-  // toward the middle: d_j = g_j - slot2_j d_prev
-  auto fcell = [&](int s) -> size_t {
-    const int sc = max(0, min(s, steps - 1));
-    return (size_t)(j0 + dj * sc) * ni + col;
+  // component r of the source (NEGR: -R, read in FP64) and of the neighbour
+  // lines, row j at offset j*ni
+  const gptr<const double> sR = gp(cd.R) + (size_t)r * n + ic;
+  const gptr<const T> sV = gp(vec_ptr<T>(cd, NEGR ? VEC_W : src) + (size_t)r * n + ic);
+  const gptr<const T> xnbp = gp(vec_ptr<T>(cd, COUP ? xnb : VEC_W) + (size_t)r * n);
+  auto load_src = [&](int jj) -> T {
+    if constexpr (NEGR) return T(-sR[(size_t)jj * ni]);
+    else return sV[(size_t)jj * ni];
   };
-  auto fload = [&](int b, int s0) {
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-      const size_t cell = fcell(s0 + k);
-      bv[b][k] = gd[cell * 4 + r];
-      load_row(B.row(2, r, cell), bm[b][k]);
+
+  // g of row jj from component r of src / xm / xp and rows r of S0, S0 Bim, S0 Bip
+  auto gform = [&](T sv, T xmv, T xpv, const T* s0, const T* s3, const T* s4) -> T {
+    T g = s0[0] * quad_bcast<0>(sv) + s0[1] * quad_bcast<1>(sv) + s0[2] * quad_bcast<2>(sv) +
+          s0[3] * quad_bcast<3>(sv);
+    if constexpr (COUP) {
+      g -= s3[0] * quad_bcast<0>(xmv) + s3[1] * quad_bcast<1>(xmv) + s3[2] * quad_bcast<2>(xmv) +
+           s3[3] * quad_bcast<3>(xmv);
+      g -= s4[0] * quad_bcast<0>(xpv) + s4[1] * quad_bcast<1>(xpv) + s4[2] * quad_bcast<2>(xpv) +
+           s4[3] * quad_bcast<3>(xpv);
     }
+    return g;
+  };
+
+  // ---- toward the middle -------------------------------------------------------
+  // ring of PF steps in flight: step s lives in slot s % PF; consuming a slot
+  // issues the loads of the step PF further on into it
+  T bs[PF], bxm[PF], bxp[PF];
+  T b0[PF][4], b3[PF][4], b4[PF][4], b2[PF][4];
+  auto frow = [&](int s) -> int { return j0 + dj * max(0, min(s, steps - 1)); };
+  auto fload = [&](int k, int s) {
+    const int jj = frow(s);
+    const size_t cell = (size_t)jj * ni + col;
+    bs[k] = load_src(jj);
+    load_row(B.row(0, r, cell), b0[k]);
+    if constexpr (COUP) {
+      bxm[k] = xnbp[(size_t)jj * ni + im];
+      bxp[k] = xnbp[(size_t)jj * ni + ip];
+      load_row(B.row(3, r, cell), b3[k]);
+      load_row(B.row(4, r, cell), b4[k]);
+    }
+    load_row(B.row(2, r, cell), b2[k]);
   };
   T dr = T(0);
-  auto fcomp = [&](int b, int s0) {
+#pragma unroll
+  for (int k = 0; k < PF; ++k) fload(k, k);
+  for (int s0 = 0; s0 < m; s0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       if (s0 + k < steps) {
-        const T t = bv[b][k] - bm[b][k][0] * quad_bcast<0>(dr) - bm[b][k][1] * quad_bcast<1>(dr) -
-                    bm[b][k][2] * quad_bcast<2>(dr) - bm[b][k][3] * quad_bcast<3>(dr);
-        dr = t;
-        if (live) gd[fcell(s0 + k) * 4 + r] = dr;
+        const T g = gform(bs[k], bxm[k], bxp[k], b0[k], b3[k], b4[k]);
+        dr = g - b2[k][0] * quad_bcast<0>(dr) - b2[k][1] * quad_bcast<1>(dr) -
+             b2[k][2] * quad_bcast<2>(dr) - b2[k][3] * quad_bcast<3>(dr);
+        if (live) gd[((size_t)frow(s0 + k) * ni + col) * 4 + r] = dr;
       }
+      fload(k, s0 + k + PF);
     }
-  };
-  fload(0, 0);
-  for (int s0 = 0; s0 < m; s0 += 2 * PF) {
-    fload(1, s0 + PF);
-    fcomp(0, s0);
-    fload(0, s0 + 2 * PF);
-    fcomp(1, s0 + PF);
   }
-  // row m: both quads combine the top's d_{m-1} with the bottom's f_{m+1}
+  // ---- the middle row: the top's d_{m-1} and the bottom's f_{m+1} meet ---------
   const T other = __shfl_xor(dr, 4, WAVE);
   const T dtop = half ? other : dr, fbot = half ? dr : other;
   T xr;
   {
     const size_t cell = (size_t)m * ni + col;
-    T pm[4], qm[4];
+    T s0r[4], s3r[4], s4r[4], pm[4], qm[4];
+    T xmv = T(0), xpv = T(0);
+    const T sv = load_src(m);
+    load_row(B.row(0, r, cell), s0r);
+    if constexpr (COUP) {
+      xmv = xnbp[(size_t)m * ni + im];
+      xpv = xnbp[(size_t)m * ni + ip];
+      load_row(B.row(3, r, cell), s3r);
+      load_row(B.row(4, r, cell), s4r);
+    }
     load_row(B.row(2, r, cell), pm);
     load_row(B.row(1, r, cell), qm);
-    xr = gd[cell * 4 + r] - pm[0] * quad_bcast<0>(dtop) - pm[1] * quad_bcast<1>(dtop) -
-         pm[2] * quad_bcast<2>(dtop) - pm[3] * quad_bcast<3>(dtop) - qm[0] * quad_bcast<0>(fbot) -
-         qm[1] * quad_bcast<1>(fbot) - qm[2] * quad_bcast<2>(fbot) - qm[3] * quad_bcast<3>(fbot);
+    const T g = gform(sv, xmv, xpv, s0r, s3r, s4r);
+    xr = g - pm[0] * quad_bcast<0>(dtop) - pm[1] * quad_bcast<1>(dtop) - pm[2] * quad_bcast<2>(dtop) -
+         pm[3] * quad_bcast<3>(dtop) - qm[0] * quad_bcast<0>(fbot) - qm[1] * quad_bcast<1>(fbot) -
+         qm[2] * quad_bcast<2>(fbot) - qm[3] * quad_bcast<3>(fbot);
   }
   double nrm = 0.0;
   if (live && half == 0) {
     x[(size_t)m * ni] = xr;
     nrm += double(xr) * double(xr);
   }
-  // away from the middle: x_j = d_j - slot1_j x_prev
+  // ---- away from the middle: x_j = d_j - S1_j x_prev (ring of PFB steps) -------
+  T cv[PFB], cm[PFB][4];
   auto bj = [&](int s) -> int {
     const int sc = max(0, min(s, steps - 1));
     return steps > 0 ? m - dj * (1 + sc) : m;
   };
-  auto bload = [&](int b, int s0) {
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-      const size_t cell = (size_t)bj(s0 + k) * ni + col;
-      bv[b][k] = gd[cell * 4 + r];
-      load_row(B.row(1, r, cell), bm[b][k]);
-    }
+  auto bload = [&](int k, int s) {
+    const size_t cell = (size_t)bj(s) * ni + col;
+    cv[k] = gd[cell * 4 + r];
+    load_row(B.row(1, r, cell), cm[k]);
   };
-  auto bcomp = [&](int b, int s0) {
 #pragma unroll
-    for (int k = 0; k < PF; ++k) {
+  for (int k = 0; k < PFB; ++k) bload(k, k);
+  for (int s0 = 0; s0 < m; s0 += PFB) {
+#pragma unroll
+    for (int k = 0; k < PFB; ++k) {
       if (s0 + k < steps) {
-        const T t = bv[b][k] - bm[b][k][0] * quad_bcast<0>(xr) - bm[b][k][1] * quad_bcast<1>(xr) -
-                    bm[b][k][2] * quad_bcast<2>(xr) - bm[b][k][3] * quad_bcast<3>(xr);
-        xr = t;
+        xr = cv[k] - cm[k][0] * quad_bcast<0>(xr) - cm[k][1] * quad_bcast<1>(xr) -
+             cm[k][2] * quad_bcast<2>(xr) - cm[k][3] * quad_bcast<3>(xr);
         if (live) {
           x[(size_t)bj(s0 + k) * ni] = xr;
           nrm += double(xr) * double(xr);
         }
       }
+      bload(k, s0 + k + PFB);
     }
-  };
-  bload(0, 0);
-  for (int s0 = 0; s0 < m; s0 += 2 * PF) {
-    bload(1, s0 + PF);
-    bcomp(0, s0);
-    bload(0, s0 + 2 * PF);
-    bcomp(1, s0 + PF);
   }
   if (want_norm) {
     nrm = wave_sum(nrm);
@@ -624,10 +619,30 @@ __global__ void __launch_bounds__(256) k_sweep_seq(Plan plan, int xout, int want
   }
 }
 
-#ifndef A2D_SWEEP_PF
-#define A2D_SWEEP_PF 8
-#endif
-constexpr int SWEEP_PF = A2D_SWEEP_PF;   // j steps loaded ahead by the line recurrences
+// Steps in flight on the way in (PF) and back out (PFB); the way in loads up
+// to seven rows and components per step, the way back two.
+template <class T> struct SweepDepth;
+template <> struct SweepDepth<float> { static constexpr int in = 8, out = 16; };
+template <> struct SweepDepth<double> { static constexpr int in = 4, out = 8; };
+
+template <class T, bool COUP>
+void run_sweep2(const Plan& pl, int nblocks, int src, int xnb, int xout, int want_norm, int gate,
+                int color, hipStream_t s) {
+  constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
+  if (src == VEC_NEG_R)
+    hipLaunchKernelGGL((k_sweep<T, COUP, true, PF, PFB>), dim3(nblocks), dim3(256), 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
+  else
+    hipLaunchKernelGGL((k_sweep<T, COUP, false, PF, PFB>), dim3(nblocks), dim3(256), 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
+}
+
+template <class T>
+void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int want_norm,
+               int gate, int color, hipStream_t s) {
+  if (coupled) run_sweep2<T, true>(pl, nblocks, src, xnb, xout, want_norm, gate, color, s);
+  else run_sweep2<T, false>(pl, nblocks, src, xnb, xout, want_norm, gate, color, s);
+}
 
 }  // namespace
 
@@ -641,49 +656,17 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
   else hipLaunchKernelGGL(k_factor<float>, dim3(nblocks), dim3(64), 0, s, pl);
 }
 
-void launch_line_sweep(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xprev,
-                       int xout, int sweep, int want_norm, int gate, int prec, hipStream_t s) {
-  if (prec == 0) {
-    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate,
-                       -1);
-    hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate, -1);
-  } else {
-    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xprev, sweep, gate,
-                       -1);
-    hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate, -1);
-  }
+void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
+                       int want_norm, int gate, int prec, hipStream_t s) {
+  if (prec == 0) run_sweep<double>(pl, nblocks, src, xprev, xout, coupled, want_norm, gate, -1, s);
+  else run_sweep<float>(pl, nblocks, src, xprev, xout, coupled, want_norm, gate, -1, s);
 }
 
-void launch_line_sweep_rb(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int src, int xout,
-                          int color, int coupled, int want_norm, int gate, int prec, hipStream_t s) {
-  // the plans are sized for every cell / line; blocks past the colour's half return at once
-  const int sweep = coupled ? 1 : 0;
-  if (prec == 0) {
-    hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, src, xout, sweep, gate,
-                       color);
-    hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate, color);
-  } else {
-    hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, src, xout, sweep, gate,
-                       color);
-    hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, xout,
-                       want_norm, gate, color);
-  }
-}
-
-void launch_sweep_part(const Plan& pcell, int ncell, const Plan& pl, int nblocks, int part, int prec,
-                       hipStream_t s) {
-  if (part == 0) {
-    if (prec == 0) hipLaunchKernelGGL(k_sweep_rhs<double>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0, -1);
-    else hipLaunchKernelGGL(k_sweep_rhs<float>, dim3(ncell), dim3(256), 0, s, pcell, VEC_W, VEC_XA, 1, 0, -1);
-  } else {
-    if (prec == 0)
-      hipLaunchKernelGGL((k_sweep_seq<double, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0, -1);
-    else
-      hipLaunchKernelGGL((k_sweep_seq<float, SWEEP_PF>), dim3(nblocks), dim3(256), 0, s, pl, VEC_XB, 0, 0, -1);
-  }
+void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
+                          int want_norm, int gate, int prec, hipStream_t s) {
+  // neighbours of the other colour are read from xout itself
+  if (prec == 0) run_sweep<double>(pl, nblocks, src, xout, xout, coupled, want_norm, gate, color, s);
+  else run_sweep<float>(pl, nblocks, src, xout, xout, coupled, want_norm, gate, color, s);
 }
 
 }  // namespace a2d

@@ -9,8 +9,8 @@ ds0 = 20 (q - 1)/(q^128 - 1), q = 1.1^(64/128); aero2d_amd.workloads.config2).
 
 A step is one complete batched solve of all cases (warmup steps untimed).
 The line also carries the mixed-precision form on the same batch, per-case
-completion latency (p50/p99), live HIP-event rooflines of the two
-preconditioner sweep kernels (the largest shares of device time) and of the
+completion latency (p50/p99), live HIP-event rooflines of the coupled
+preconditioner sweep kernel (the largest share of device time) and of the
 residual kernel, and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
 """
 from __future__ import annotations
@@ -31,13 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell and launch (FP64): residual reads U(4), S_i(2),
 # S_j(2), writes R(4) = 12 words
 RES_BYTES_PER_CELL = 96
-# per-cell pass of a coupled preconditioner sweep (the kernel with the largest
-# share of device time in the FP64 solve): read r (4), x (4), Bim and Bip (32),
-# inv (16), write g (4) = 60 words
-RHS_BYTES_PER_CELL = 480
-# line recurrences of one sweep (second largest share): read g (4),
-# P = inv A (16), c (16), write x (4) = 40 words
-SEQ_BYTES_PER_CELL = 320
+# one coupled line sweep of the preconditioner (the kernel with the largest
+# share of device time in the FP64 solve): read the source (4), the neighbour
+# lines' x (4), the blocks S0, S0 Bim, S0 Bip, P and c (80), write x (4)
+# = 92 words; the d values of the way in make an extra 4-word round trip
+SWEEP_BYTES_PER_CELL = 736
 # plain vector copy used to calibrate the HBM counters: 4 words read, 4 written
 COPY_BYTES_PER_CELL = 64
 
@@ -60,7 +58,7 @@ def time_kernels(cases, precision, reps=20, knobs=None):
         cfl = np.full(len(ids), 10.0)
         _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl), _abi.dptr(out))
         names = ["residual", "jvp", "line_sweep", "cgs_dots", "assemble", "factor", "cgs_update",
-                 "copy", "sweep_rhs", "sweep_seq", "precond"]
+                 "copy", "sweep_first", "sweep_rb_half", "precond"]
         res = {}
         for w, name in enumerate(names):
             ms = np.zeros(1)
@@ -73,30 +71,26 @@ def time_kernels(cases, precision, reps=20, knobs=None):
 
 # HBM bytes per launch of the default batch (64 cases, 256 x 128, FP64) from
 # two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE, then
-# WRITE_SIZE), fetch scaled by 1.9946 and write by 1.0000, the ratios of the
-# copy kernel's known bytes to what the counters read for it
-# (profiles/traffic_config2_fp64_pmc.txt).
+# WRITE_SIZE), fetch scaled by the ratio of the copy kernel's known bytes to
+# what the counter reads for it (profiles/traffic_config2_fp64_pmc.txt).
 TRAFFIC_CELLS = 64 * 256 * 128
-MEASURED_TRAFFIC = {"sweep_rhs": 1004.7e6, "sweep_seq": 801.6e6, "residual": 405.0e6}
+MEASURED_TRAFFIC = {}
 
 
 def rooflines(kt, cells):
-    """HBM rooflines from HIP-event kernel times (ms): the per-cell sweep
-    kernel (largest share of device time), the line-recurrence kernel (next
-    largest) and the residual kernel."""
+    """HBM rooflines from HIP-event kernel times (ms): the coupled line sweep
+    (largest share of device time) and the residual kernel."""
     def one(name, bpc):
         ach = bpc * cells / (kt[name] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS,
                 "traffic": MEASURED_TRAFFIC.get(name) if cells == TRAFFIC_CELLS else None,
                 "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name]}
-    rhs = one("sweep_rhs", RHS_BYTES_PER_CELL)
-    rhs["kernel"] = "k_sweep_rhs<double> (per-cell pass of a coupled line sweep)"
-    seq = one("sweep_seq", SEQ_BYTES_PER_CELL)
-    seq["kernel"] = "k_sweep_seq<double> (block line recurrences, one sweep)"
+    sw = one("line_sweep", SWEEP_BYTES_PER_CELL)
+    sw["kernel"] = "k_sweep<double, true> (one coupled line-Jacobi sweep, every line)"
     res = one("residual", RES_BYTES_PER_CELL)
     res["kernel"] = "k_stencil<double, false> (FP64 residual)"
-    return {"roofline": rhs, "roofline_line_recurrence": seq, "roofline_residual": res}
+    return {"roofline": sw, "roofline_residual": res}
 
 
 def run_kernels(a):
