@@ -90,7 +90,8 @@ struct CaseDev {
   void* V;              // T [kmax+1][4][nc][ni]
   void* x0;             // T [4][nc][ni]
   void* w;              // T [4][nc][ni]
-  void* d;              // T [4][nc][ni] line-solve forward sweep
+  void* d;              // T [cell][4] line-solve values of the way in
+  void* g0;             // T [cell][4] S0 src of the current preconditioner application
   void* xa;             // T [4][nc][ni] sweep ping-pong
   void* xb;
   double* part;         // [npart][kmax + 2] reduction partials

@@ -63,7 +63,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
 struct Layout {
   size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
   size_t Ut, volt, Sit, Sjt, xft, yft;
-  size_t diag, blk, V, x0, w, d, xa, xb, part, sc;
+  size_t diag, blk, V, x0, w, d, g0, xa, xb, part, sc;
   size_t total;
   int npart;
 };
@@ -118,6 +118,7 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   L.x0 = take(4 * n * t);
   L.w = take(4 * n * t);
   L.d = take(4 * n * t);
+  L.g0 = take(4 * n * t);
   L.xa = take(4 * n * t);
   L.xb = take(4 * n * t);
   L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, LINES_PER_BLOCK), 1});
@@ -313,16 +314,16 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
     for (int s = 0; s < S; ++s)
       for (int color = 1; color >= 0; --color) {
         const int coupled = !(s == 0 && color == 1);
-        launch_line_sweep_rb(b.pline, b.nline, src, dest, color, coupled, (s == S - 1) ? want_norm : 0,
-                             gate, b.prec, p.stream);
+        launch_line_sweep_rb(b.pline, b.nline, src, dest, color, coupled, s > 0,
+                             (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
       }
     return;
   }
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
-    launch_line_sweep(b.pline, b.nline, src, prev, out, s > 0, (s == S - 1) ? want_norm : 0, gate,
-                      b.prec, p.stream);
+    launch_line_sweep(b.pline, b.nline, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
+                      gate, b.prec, p.stream);
   }
 }
 
@@ -613,6 +614,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.x0 = b + L.x0;
     d.w = b + L.w;
     d.d = b + L.d;
+    d.g0 = b + L.g0;
     d.xa = b + L.xa;
     d.xb = b + L.xb;
     d.part = reinterpret_cast<double*>(b + L.part);
@@ -934,8 +936,8 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
           for (auto& g : b.sten)
             launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
           break;
-        case 2:  // a coupled line-Jacobi sweep over every line
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 0, 0, b.prec, p.stream);
+        case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, p.stream);
           break;
         case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
@@ -943,10 +945,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, b.prec, p.stream);
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, p.stream);
           break;
-        case 9:  // a coupled red-black half sweep (odd lines)
-          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 0, 0, b.prec, p.stream);
+        case 9:  // a coupled red-black half sweep (odd lines) of a later sweep
+          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, p.stream);
           break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");

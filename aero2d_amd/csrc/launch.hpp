@@ -56,16 +56,19 @@ void launch_assemble(const Plan& pcell, int nblocks, const KnobsDev& k, int prec
 void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // One line-Jacobi sweep, one kernel: x_out = L^-1 (src - Bim x(i-1) - Bip x(i+1))
 // with x = xprev, the neighbour terms only when coupled. src = VEC_NEG_R reads
-// -R. want_norm: per-block partials of |xout|^2. Plan pl: LINES_PER_BLOCK
-// lines per block.
+// -R. The first sweep of an application keeps S0 src in the case's g0 buffer;
+// reuse_g0 = 1 (a later sweep, same src) reads it instead of src and S0.
+// want_norm: per-block partials of |xout|^2. Plan pl: LINES_PER_BLOCK lines
+// per block.
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
-                       int want_norm, int gate, int prec, hipStream_t s);
+                       int reuse_g0, int want_norm, int gate, int prec, hipStream_t s);
 // One half sweep of the red-black ordering: lines i = color (mod 2) only,
 // in place in xout (neighbours of the other colour are read from xout);
-// coupled = 0 drops the i-coupling (the very first half sweep). Norm partials
-// go to part[(color * nlineb + block)].
+// coupled = 0 drops the i-coupling (the very first half sweep); reuse_g0 as
+// above (the sweeps after the first). Norm partials go to
+// part[(color * nlineb + block)].
 void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
-                          int want_norm, int gate, int prec, hipStream_t s);
+                          int reuse_g0, int want_norm, int gate, int prec, hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);

@@ -461,7 +461,13 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
 // branch makes the compiler wait for every load in flight where the branch
 // joins. The d values of the way in are kept in the case's d buffer
 // ([cell][4]) for the way back.
-template <class T, bool COUP, bool NEGR, int PF, int PFB>
+// Source of g: GS_VEC / GS_NEGR form S0 src from a vector / from -R (read in
+// FP64) and keep it in the case's g0 buffer; GS_G0 reads it back. Every sweep
+// of one preconditioner application has the same source, so only the first
+// sweep over a line reads src and S0.
+enum GSrc : int { GS_VEC = 0, GS_NEGR = 1, GS_G0 = 2 };
+
+template <class T, bool COUP, int GS, int PF, int PFB>
 __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
                                                int gate, int color) {
   __shared__ double red[4];
@@ -496,20 +502,24 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
   const gptr<T> gd = gp(tp<T>(cd.d));
   const gptr<T> x = gp(vec_ptr<T>(cd, xout) + (size_t)r * n + ic);
-  // component r of the source (NEGR: -R, read in FP64) and of the neighbour
-  // lines, row j at offset j*ni
+  // component r of the source (-R read in FP64, or a vector) and of the
+  // neighbour lines, row j at offset j*ni; g0 and d are [cell][4]
   const gptr<const double> sR = gp(cd.R) + (size_t)r * n + ic;
-  const gptr<const T> sV = gp(vec_ptr<T>(cd, NEGR ? VEC_W : src) + (size_t)r * n + ic);
+  const gptr<const T> sV = gp(vec_ptr<T>(cd, GS == GS_VEC ? src : VEC_W) + (size_t)r * n + ic);
   const gptr<const T> xnbp = gp(vec_ptr<T>(cd, COUP ? xnb : VEC_W) + (size_t)r * n);
+  const gptr<T> g0p = gp(tp<T>(cd.g0));
   auto load_src = [&](int jj) -> T {
-    if constexpr (NEGR) return T(-sR[(size_t)jj * ni]);
+    if constexpr (GS == GS_NEGR) return T(-sR[(size_t)jj * ni]);
     else return sV[(size_t)jj * ni];
   };
-
-  // g of row jj from component r of src / xm / xp and rows r of S0, S0 Bim, S0 Bip
-  auto gform = [&](T sv, T xmv, T xpv, const T* s0, const T* s3, const T* s4) -> T {
-    T g = s0[0] * quad_bcast<0>(sv) + s0[1] * quad_bcast<1>(sv) + s0[2] * quad_bcast<2>(sv) +
-          s0[3] * quad_bcast<3>(sv);
+  // S0 src (row r) from component r of src and row r of S0
+  auto s0src = [&](T sv, const T* s0) -> T {
+    return s0[0] * quad_bcast<0>(sv) + s0[1] * quad_bcast<1>(sv) + s0[2] * quad_bcast<2>(sv) +
+           s0[3] * quad_bcast<3>(sv);
+  };
+  // g = S0 src - (S0 Bim) xm - (S0 Bip) xp, with gs = S0 src
+  auto gform = [&](T gs, T xmv, T xpv, const T* s3, const T* s4) -> T {
+    T g = gs;
     if constexpr (COUP) {
       g -= s3[0] * quad_bcast<0>(xmv) + s3[1] * quad_bcast<1>(xmv) + s3[2] * quad_bcast<2>(xmv) +
            s3[3] * quad_bcast<3>(xmv);
@@ -528,8 +538,12 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   auto fload = [&](int k, int s) {
     const int jj = frow(s);
     const size_t cell = (size_t)jj * ni + col;
-    bs[k] = load_src(jj);
-    load_row(B.row(0, r, cell), b0[k]);
+    if constexpr (GS == GS_G0) {
+      bs[k] = g0p[cell * 4 + r];
+    } else {
+      bs[k] = load_src(jj);
+      load_row(B.row(0, r, cell), b0[k]);
+    }
     if constexpr (COUP) {
       bxm[k] = xnbp[(size_t)jj * ni + im];
       bxp[k] = xnbp[(size_t)jj * ni + ip];
@@ -545,10 +559,18 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       if (s0 + k < steps) {
-        const T g = gform(bs[k], bxm[k], bxp[k], b0[k], b3[k], b4[k]);
+        const size_t cell = (size_t)frow(s0 + k) * ni + col;
+        T gs;
+        if constexpr (GS == GS_G0) {
+          gs = bs[k];
+        } else {
+          gs = s0src(bs[k], b0[k]);
+          if (live) g0p[cell * 4 + r] = gs;
+        }
+        const T g = gform(gs, bxm[k], bxp[k], b3[k], b4[k]);
         dr = g - b2[k][0] * quad_bcast<0>(dr) - b2[k][1] * quad_bcast<1>(dr) -
              b2[k][2] * quad_bcast<2>(dr) - b2[k][3] * quad_bcast<3>(dr);
-        if (live) gd[((size_t)frow(s0 + k) * ni + col) * 4 + r] = dr;
+        if (live) gd[cell * 4 + r] = dr;
       }
       fload(k, s0 + k + PF);
     }
@@ -560,9 +582,15 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   {
     const size_t cell = (size_t)m * ni + col;
     T s0r[4], s3r[4], s4r[4], pm[4], qm[4];
-    T xmv = T(0), xpv = T(0);
-    const T sv = load_src(m);
-    load_row(B.row(0, r, cell), s0r);
+    T xmv = T(0), xpv = T(0), gs;
+    if constexpr (GS == GS_G0) {
+      gs = g0p[cell * 4 + r];
+    } else {
+      const T sv = load_src(m);
+      load_row(B.row(0, r, cell), s0r);
+      gs = s0src(sv, s0r);
+      if (live && half == 0) g0p[cell * 4 + r] = gs;
+    }
     if constexpr (COUP) {
       xmv = xnbp[(size_t)m * ni + im];
       xpv = xnbp[(size_t)m * ni + ip];
@@ -571,7 +599,7 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     }
     load_row(B.row(2, r, cell), pm);
     load_row(B.row(1, r, cell), qm);
-    const T g = gform(sv, xmv, xpv, s0r, s3r, s4r);
+    const T g = gform(gs, xmv, xpv, s3r, s4r);
     xr = g - pm[0] * quad_bcast<0>(dtop) - pm[1] * quad_bcast<1>(dtop) - pm[2] * quad_bcast<2>(dtop) -
          pm[3] * quad_bcast<3>(dtop) - qm[0] * quad_bcast<0>(fbot) - qm[1] * quad_bcast<1>(fbot) -
          qm[2] * quad_bcast<2>(fbot) - qm[3] * quad_bcast<3>(fbot);
@@ -625,23 +653,28 @@ template <class T> struct SweepDepth;
 template <> struct SweepDepth<float> { static constexpr int in = 8, out = 16; };
 template <> struct SweepDepth<double> { static constexpr int in = 4, out = 8; };
 
-template <class T, bool COUP>
-void run_sweep2(const Plan& pl, int nblocks, int src, int xnb, int xout, int want_norm, int gate,
-                int color, hipStream_t s) {
-  constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
-  if (src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, COUP, true, PF, PFB>), dim3(nblocks), dim3(256), 0, s, pl, src, xnb,
-                       xout, want_norm, gate, color);
-  else
-    hipLaunchKernelGGL((k_sweep<T, COUP, false, PF, PFB>), dim3(nblocks), dim3(256), 0, s, pl, src, xnb,
-                       xout, want_norm, gate, color);
-}
-
+// Kernel variant for a sweep: coupled or not; g from src (a vector or -R,
+// kept in g0) or from g0 (reuse_g0: a later sweep of the same application).
 template <class T>
-void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int want_norm,
-               int gate, int color, hipStream_t s) {
-  if (coupled) run_sweep2<T, true>(pl, nblocks, src, xnb, xout, want_norm, gate, color, s);
-  else run_sweep2<T, false>(pl, nblocks, src, xnb, xout, want_norm, gate, color, s);
+void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+               int want_norm, int gate, int color, hipStream_t s) {
+  constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
+  const dim3 grid(nblocks), block(256);
+  if (coupled && reuse_g0)
+    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
+                       want_norm, gate, color);
+  else if (coupled && src == VEC_NEG_R)
+    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
+                       want_norm, gate, color);
+  else if (coupled)
+    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
+                       want_norm, gate, color);
+  else if (src == VEC_NEG_R)
+    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
+                       want_norm, gate, color);
+  else
+    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
+                       want_norm, gate, color);
 }
 
 }  // namespace
@@ -657,16 +690,16 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
 }
 
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
-                       int want_norm, int gate, int prec, hipStream_t s) {
-  if (prec == 0) run_sweep<double>(pl, nblocks, src, xprev, xout, coupled, want_norm, gate, -1, s);
-  else run_sweep<float>(pl, nblocks, src, xprev, xout, coupled, want_norm, gate, -1, s);
+                       int reuse_g0, int want_norm, int gate, int prec, hipStream_t s) {
+  if (prec == 0) run_sweep<double>(pl, nblocks, src, xprev, xout, coupled, reuse_g0, want_norm, gate, -1, s);
+  else run_sweep<float>(pl, nblocks, src, xprev, xout, coupled, reuse_g0, want_norm, gate, -1, s);
 }
 
 void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
-                          int want_norm, int gate, int prec, hipStream_t s) {
+                          int reuse_g0, int want_norm, int gate, int prec, hipStream_t s) {
   // neighbours of the other colour are read from xout itself
-  if (prec == 0) run_sweep<double>(pl, nblocks, src, xout, xout, coupled, want_norm, gate, color, s);
-  else run_sweep<float>(pl, nblocks, src, xout, xout, coupled, want_norm, gate, color, s);
+  if (prec == 0) run_sweep<double>(pl, nblocks, src, xout, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else run_sweep<float>(pl, nblocks, src, xout, xout, coupled, reuse_g0, want_norm, gate, color, s);
 }
 
 }  // namespace a2d

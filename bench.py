@@ -31,11 +31,12 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell and launch (FP64): residual reads U(4), S_i(2),
 # S_j(2), writes R(4) = 12 words
 RES_BYTES_PER_CELL = 96
-# one coupled line sweep of the preconditioner (the kernel with the largest
-# share of device time in the FP64 solve): read the source (4), the neighbour
-# lines' x (4), the blocks S0, S0 Bim, S0 Bip, P and c (80), write x (4)
-# = 92 words; the d values of the way in make an extra 4-word round trip
-SWEEP_BYTES_PER_CELL = 736
+# one coupled line sweep of the preconditioner (sweeps 2-4 of each
+# application; the kernel with the largest share of device time in the FP64
+# solve): read g0 = S0 src (4), the neighbour lines' x (4), the blocks S0 Bim,
+# S0 Bip, P and c (64), write x (4) = 76 words; the d values of the way in
+# make an extra 4-word round trip
+SWEEP_BYTES_PER_CELL = 608
 # plain vector copy used to calibrate the HBM counters: 4 words read, 4 written
 COPY_BYTES_PER_CELL = 64
 
@@ -87,7 +88,7 @@ def rooflines(kt, cells):
                 "traffic": MEASURED_TRAFFIC.get(name) if cells == TRAFFIC_CELLS else None,
                 "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name]}
     sw = one("line_sweep", SWEEP_BYTES_PER_CELL)
-    sw["kernel"] = "k_sweep<double, true> (one coupled line-Jacobi sweep, every line)"
+    sw["kernel"] = "k_sweep<double, coupled, g0> (one coupled line-Jacobi sweep, every line)"
     res = one("residual", RES_BYTES_PER_CELL)
     res["kernel"] = "k_stencil<double, false> (FP64 residual)"
     return {"roofline": sw, "roofline_residual": res}
