@@ -1,9 +1,10 @@
 """Serving loop: continuous batching of single-case solves on one GPU.
 
 Requests (an O-grid and a freestream) arrive at any time. A request is
-admitted when the device pool's HBM budget has room for its footprint
-(a2d_case_bytes: state, metrics, Krylov basis and preconditioner blocks);
-its state then stays on the device until it finishes. Every Newton step runs
+admitted, in arrival order, when the device pool's HBM budget has room for
+its footprint (a2d_case_bytes: state, metrics, Krylov basis and
+preconditioner blocks) and the in-flight cell count stays under the cap that
+fills the GPU; its state then stays on the device until it finishes. Every Newton step runs
 over all admitted cases at once (one a2d_newton_step call), so requests that
 arrive while others are in flight join the next step and their work overlaps
 on the device; a case leaves as soon as it converges (rn/r0 <= tol), freezes
@@ -27,7 +28,12 @@ from . import _abi
 from .batch import SOLVER_DEFAULTS, Case, cfl_policy, default_knobs
 from .implicit import NewtonLoop
 
-__all__ = ["ServeResult", "SolveServer", "serve_stream"]
+__all__ = ["ServeResult", "SolveServer", "serve_stream", "MAX_CELLS"]
+
+# In-flight cell cap per form: the cell count at which one Newton step over
+# the in-flight set takes about 250 ms on one MI355X (measured cost once the
+# GPU is full: mixed ~8 ns, FP64 ~50 ns per cell and Newton step).
+MAX_CELLS = {"mixed": 32_000_000, "fp64": 5_000_000}
 
 
 @dataclass
@@ -60,14 +66,20 @@ class SolveServer:
     """Continuous-batching solver over one device pool.
 
     precision "mixed" (default) or "fp64"; hbm_budget in bytes bounds the
-    admitted cases' total footprint; max_batch bounds their count; tol and
-    n_iter are per request. Use submit() from any thread with start(), or
-    drive it synchronously with step() (serve_stream does)."""
+    admitted cases' total footprint; max_batch bounds their count; max_cells
+    bounds their total cell count, i.e. the time of one Newton step over the
+    in-flight set (default MAX_CELLS[precision], about 250 ms per step on one
+    MI355X: ~8 ns per cell and Newton step for the mixed form, ~50 ns for
+    FP64, once the batch fills the GPU). Past that bound, admitting more only
+    makes every admitted case progress more slowly, so later requests wait
+    in arrival order instead. tol and n_iter are per request. Use submit()
+    from any thread with start(), or drive it synchronously with step()
+    (serve_stream does)."""
 
     def __init__(self, precision: str = "mixed", hbm_budget: int = 64 << 30, max_batch: int = 1024,
                  tol: float = 1e-10, n_iter: int = 800, cfl0: float = 10.0,
                  restart: int | None = None, early_exit: int | None = None,
-                 cfl_max: float | None = None):
+                 cfl_max: float | None = None, max_cells: int | None = None):
         if precision not in SOLVER_DEFAULTS:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -77,6 +89,8 @@ class SolveServer:
         self.pool = _abi.Pool(hbm_budget, kn)
         self.budget = hbm_budget
         self.max_batch = max_batch
+        self.max_cells = MAX_CELLS[precision] if max_cells is None else int(max_cells)
+        self.active_cells = 0
         self.tol, self.n_iter = tol, n_iter
         cmax, clate, below = cfl_policy(precision, cfl_max)
         self.loop = NewtonLoop(4096, cfl0, cmax, clate, below)
@@ -106,18 +120,27 @@ class SolveServer:
         self.waiting.append((case, self.now() if t_arrive is None else t_arrive, f))
         return f
 
+    @staticmethod
+    def cells(case: Case) -> int:
+        ni, nj = case.grid.x.shape
+        return ni * (nj - 1)
+
     def _admit(self):
         while self.waiting and len(self.active) < self.max_batch:
             case, t_arr, fut = self.waiting[0]
             if self.pool.used_bytes() + self.footprint(case) > self.budget:
                 break                                   # wait for memory, keep arrival order
+            if self.active and self.active_cells + self.cells(case) > self.max_cells:
+                break                                   # the GPU is full: wait, keep arrival order
             self.waiting.pop(0)
             cid = self.pool.add_case(case.grid.x, case.grid.y, case.mach, case.alpha, self.prec)
             self.loop.reset([cid])
             self.active[cid] = _Slot(case, t_arr, self.now(), fut, cid)
+            self.active_cells += self.cells(case)
 
     def _finish(self, cid: int):
         s = self.active.pop(cid)
+        self.active_cells -= self.cells(s.case)
         f3 = np.empty(3)
         cp = np.empty(s.case.grid.x.shape[0])
         circ = np.empty(1)

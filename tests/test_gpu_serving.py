@@ -67,6 +67,29 @@ def test_admission_waits_for_hbm_and_keeps_arrival_order():
         srv.close()
 
 
+def test_admission_caps_cells_in_flight():
+    """With ample HBM, the in-flight cell cap alone bounds the batch; requests
+    wait in arrival order and each result equals its solve alone."""
+    g = _cases()[1].grid                                    # all requests 96 x 24
+    cases = [Case(g, 0.4 + 0.05 * k, 0.5 * k, f"r{k}") for k in range(5)]
+    srv = SolveServer("mixed", tol=1e-8, max_cells=int(2.5 * 96 * 24))
+    try:
+        futs = [srv.enqueue(c, 0.0) for c in cases]
+        peak = 0
+        while any(not f.done() for f in futs):
+            peak = max(peak, srv.step())
+            assert srv.active_cells <= srv.max_cells
+        assert peak == 2
+        res = [f.result() for f in futs]
+        assert all(r.converged for r in res)
+        admits = [r.t_admit for r in res]
+        assert admits == sorted(admits)
+    finally:
+        srv.close()
+    alone = solve_batch([cases[4]], "mixed", tol=1e-8, n_iter=800)
+    assert abs(res[4].cl - alone.cl[0]) <= 1e-12 and res[4].iterations == alone.iterations[0]
+
+
 def test_request_larger_than_budget_is_refused():
     c = _cases()[1]
     srv = SolveServer("fp64", hbm_budget=1 << 20)
