@@ -72,10 +72,10 @@ def time_kernels(cases, precision, reps=20, knobs=None):
 
 # HBM bytes per launch of the default batch (64 cases, 256 x 128, FP64) from
 # two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE, then
-# WRITE_SIZE), fetch scaled by the ratio of the copy kernel's known bytes to
-# what the counter reads for it (profiles/traffic_config2_fp64_pmc.txt).
+# WRITE_SIZE), fetch scaled by 1.9944 (the copy kernel's known bytes over what
+# the counter reads for it) and write by 1.0 (profiles/traffic_config2_fp64_pmc.txt).
 TRAFFIC_CELLS = 64 * 256 * 128
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {"line_sweep": 1491.9e6, "residual": 404.9e6}
 
 
 def rooflines(kt, cells):
@@ -173,6 +173,11 @@ def run_config3(a):
     from aero2d_amd.workloads import config3
     cases = config3(a.cases or 32)
     n_iter = a.n_iter or 800
+    if a.mixed_only:
+        # the mixed solve alone (used under the profiler)
+        rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
+        return {"workload": f"{len(cases)} transonic cases on 512x256, mixed only", "wall_s": rmx.wall,
+                "converged": int(rmx.converged.sum()), "newton_steps_mean": float(rmx.iterations.mean())}
     r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter)
     print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
     rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
@@ -275,6 +280,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true", help="config3: the mixed solve alone")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (cases are independent; "
