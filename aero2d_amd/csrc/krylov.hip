@@ -21,6 +21,12 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int PER = VEC_CHUNK / NT;   // cells per thread in the vector kernels
+#ifndef A2D_UPDATE_JB64
+#define A2D_UPDATE_JB64 2             // basis vectors per pass of the CGS update, FP64
+#endif
+#ifndef A2D_UPDATE_JB32
+#define A2D_UPDATE_JB32 4             // and FP32
+#endif
 
 template <class T> __device__ __forceinline__ double eps_of() {
   return std::is_same<T, double>::value ? DBL_EPSILON : double(FLT_EPSILON);
@@ -130,43 +136,57 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   const gptr<T> w = vec_gptr<T>(cd, wvec);
   const gptr<const T> Vb = gp(ctp<T>(cd.V));
   const size_t vs = (size_t)cd.vstride;
+  // all PER cells of a thread at once, JB basis vectors per pass, so that
+  // JB x 4 x PER loads are in flight; indices are clamped into the case and
+  // only in-range cells are stored
+  int cc[PER];
+  bool in[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int c = c0 + m * NT;
+    in[m] = c < n;
+    cc[m] = in[m] ? c : n - 1;
+  }
+  double acc[4][PER];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < PER; ++m) acc[v][m] = double(w[(size_t)v * n + cc[m]]);
+  constexpr int JB = std::is_same<T, double>::value ? A2D_UPDATE_JB64 : A2D_UPDATE_JB32;
+  int j = 0;
+  for (; j + JB - 1 <= k; j += JB) {
+    T vv[JB][4][PER];
+#pragma unroll
+    for (int q = 0; q < JB; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) vv[q][v][m] = Vb[(size_t)(j + q) * vs + (size_t)v * n + cc[m]];
+#pragma unroll
+    for (int q = 0; q < JB; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) acc[v][m] -= coef[j + q] * double(vv[q][v][m]);
+  }
+  for (; j <= k; ++j) {
+    const double cj = coef[j];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int m = 0; m < PER; ++m) acc[v][m] -= cj * double(Vb[(size_t)j * vs + (size_t)v * n + cc[m]]);
+  }
   double nrm = 0.0;
-  auto body = [&](auto check) {
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-      const int c = c0 + m * NT;
-      if (decltype(check)::value && c >= n) continue;
-      double acc[4];
+  for (int m = 0; m < PER; ++m) {
+    if (!in[m]) continue;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = double(w[(size_t)v * n + c]);
-      // four basis vectors per pass: 16 loads in flight per element
-      int j = 0;
-      for (; j + 3 <= k; j += 4) {
-        T vv[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) vv[q][v] = Vb[(size_t)(j + q) * vs + (size_t)v * n + c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[v] -= coef[j + q] * double(vv[q][v]);
-      }
-      for (; j <= k; ++j) {
-        const double cj = coef[j];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] -= cj * double(Vb[(size_t)j * vs + (size_t)v * n + c]);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const T r = T(acc[v]);
-        w[(size_t)v * n + c] = r;
-        nrm += double(r) * double(r);
-      }
+    for (int v = 0; v < 4; ++v) {
+      const T r = T(acc[v][m]);
+      w[(size_t)v * n + cc[m]] = r;
+      nrm += double(r) * double(r);
     }
-  };
-  if ((lb + 1) * VEC_CHUNK <= n) body(std::false_type{});
-  else body(std::true_type{});
+  }
   nrm = wave_sum(nrm);
   if ((threadIdx.x & (WAVE - 1)) == 0) sred[threadIdx.x / WAVE] = nrm;
   __syncthreads();
