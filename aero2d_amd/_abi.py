@@ -29,7 +29,7 @@ class Knobs(ctypes.Structure):
                 ("vortex", ctypes.c_int), ("order", ctypes.c_int),
                 ("gmres_restart", ctypes.c_int), ("jacobi_sweeps", ctypes.c_int),
                 ("early_exit", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("precond_order", ctypes.c_int)]
+                ("precond_order", ctypes.c_int), ("positivity", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); the exported surface of include/aero2d_hip.h
@@ -48,6 +48,7 @@ SIGNATURES = {
                                     ctypes.c_double, ctypes.c_int, _ip]),
     "a2d_case_remove": (ctypes.c_int, [_i64, ctypes.c_int]),
     "a2d_case_set_freestream": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "a2d_case_set_step_limit": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double]),
     "a2d_case_set_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
     "a2d_case_get_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
     "a2d_case_set_shape": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int]),

@@ -21,7 +21,7 @@ from .grid import OGrid
 from .implicit import NewtonLoop
 
 __all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs", "cfl_policy",
-           "SOLVER_DEFAULTS"]
+           "path_of", "SOLVER_DEFAULTS"]
 
 
 @dataclass
@@ -50,20 +50,33 @@ class BatchResult:
 
 
 # Solver settings per form. "fp64" is the reference's algorithm (fixed 50
-# Arnoldi steps, 4 line-Jacobi sweeps, CFL cap 300: aero2d/implicit.py:68-78).
-# "mixed" is tuned for throughput: Arnoldi stops once the preconditioned
-# residual estimate meets gmres_tol (at most 20 steps), the preconditioner
-# runs two red-black sweeps (as strong as four Jacobi sweeps here at half the
-# cost), and the CFL cap is the reference's 300 until rn/r0 < 1e-3, then 1000
-# (a cap of 1000 from the start stalls transonic cases while their shocks
-# form; raised late it shortens the tail). Both forms solve the same discrete
-# equations; where those have more than one steady state (transonic cases with
-# a shock), the solver path decides which one a case reaches.
+# Arnoldi steps, 4 line-Jacobi sweeps, CFL cap 300, step limit 0.2:
+# aero2d/implicit.py:68-78). "mixed" is tuned for throughput: Arnoldi stops
+# once the preconditioned residual estimate meets gmres_tol (at most 20
+# steps), the preconditioner runs two red-black sweeps (as strong as four
+# Jacobi sweeps here at half the cost), the CFL cap is the reference's 300
+# until rn/r0 < 1e-3, then 1000 (a cap of 1000 from the start stalls transonic
+# cases while their shocks form; raised late it shortens the tail), and the
+# step limit (largest linearised relative change of rho and p per Newton
+# step) is 0.5 instead of 0.2: shocks cross the grid in fewer steps (config
+# 3: 235 instead of 302 Newton steps per case). Measured trade-off on the
+# 256-case uneven batch: 233 cases converge at 0.5, 237 at 0.2 (243 at 0.3
+# with the positivity check), at +17 % converged cases/s. Two optional
+# safeguards are off by default because at 0.5 they changed no outcome:
+# `fallback_max_update` drops a case to a smaller limit after its first
+# rejected step, and the `positivity` knob rejects a step whose new state is
+# not positive everywhere (p is quadratic in the step, so a large step can
+# leave p <= 0; the reference accepts it and fails on every later step until
+# the case freezes). Both forms solve the same discrete equations; where
+# those have more than one steady state (transonic cases with a shock), the
+# solver path decides which one a case reaches.
 SOLVER_DEFAULTS = {
     "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0,
-                 cfl_late=None, late_below=0.0),
+                 cfl_late=None, late_below=0.0, max_update=0.2, fallback_max_update=0.0,
+                 positivity=0),
     "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=300.0,
-                  cfl_late=1000.0, late_below=1e-3),
+                  cfl_late=1000.0, late_below=1e-3, max_update=0.5, fallback_max_update=0.0,
+                  positivity=0),
 }
 
 
@@ -77,7 +90,8 @@ def cfl_policy(precision: str, cfl_max: float | None = None, cfl_late: float | N
 
 
 def default_knobs(precision: str, restart: int | None = None, early_exit: bool | None = None,
-                  precond_order: int | None = None, jacobi_sweeps: int | None = None) -> _abi.Knobs:
+                  precond_order: int | None = None, jacobi_sweeps: int | None = None,
+                  max_update: float | None = None, positivity: int | None = None) -> _abi.Knobs:
     """Library knobs for the form's defaults, with optional overrides."""
     d = SOLVER_DEFAULTS[precision]
     k = _abi.default_knobs()
@@ -85,7 +99,22 @@ def default_knobs(precision: str, restart: int | None = None, early_exit: bool |
     k.early_exit = d["early_exit"] if early_exit is None else int(early_exit)
     k.precond_order = d["precond_order"] if precond_order is None else int(precond_order)
     k.jacobi_sweeps = d["jacobi_sweeps"] if jacobi_sweeps is None else int(jacobi_sweeps)
+    k.max_update = d["max_update"] if max_update is None else float(max_update)
+    k.positivity = d["positivity"] if positivity is None else int(positivity)
     return k
+
+
+def path_of(form: str) -> dict:
+    """solve_batch keyword arguments that put either precision on the solver
+    path of `form` ("fp64" or "mixed"): its knobs, CFL policy and step-limit
+    fallback. Running FP64 on the mixed form's path isolates the effect of
+    precision."""
+    d = SOLVER_DEFAULTS[form]
+    kn = default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
+                       precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"],
+                       max_update=d["max_update"], positivity=d["positivity"])
+    return dict(knobs=kn, cfl_max=d["cfl_max"], cfl_late=d["cfl_late"], late_below=d["late_below"],
+                fallback_max_update=d["fallback_max_update"])
 
 
 class BatchSolve:
@@ -120,17 +149,22 @@ class BatchSolve:
 
     def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float | None = None,
             keep_state: bool = False, on_step=None, cfl_late: float | None = None,
-            late_below: float | None = None, record_history: bool = False) -> BatchResult:
+            late_below: float | None = None, record_history: bool = False,
+            fallback_max_update: float | None = None) -> BatchResult:
         """Newton loop to rn/r0 <= tol per case (or n_iter steps); the pool is
-        released at the end."""
+        released at the end. fallback_max_update > 0: a case's step limit
+        drops to it after the case's first rejected step (default: the form's)."""
         if self.pool is None:
             self.upload()
         cases, ids, pool = self.cases, self.ids, self.pool
         nb = len(cases)
         cfl_max, cfl_late, late_below = cfl_policy(self.precision, cfl_max, cfl_late, late_below)
+        if fallback_max_update is None:
+            fallback_max_update = SOLVER_DEFAULTS[self.precision]["fallback_max_update"]
+        fallback = fallback_max_update > 0 and fallback_max_update != self.knobs.max_update
         t0 = time.perf_counter()
         try:
-            loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below)
+            loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below, fallback)
             hist = []
             done = np.zeros(nb, bool)
             t_done = np.full(nb, np.nan)
@@ -145,6 +179,8 @@ class BatchSolve:
                 _abi.call("a2d_newton_step", pool.handle, _abi.iptr(sub), len(act), _abi.dptr(cfl),
                           _abi.dptr(out))
                 rn, _ = loop.update(act, out)
+                for k in loop.newly_failed:
+                    _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[k]), float(fallback_max_update))
                 arn[act] += out[:, 6]
                 if record_history:
                     h = np.full(nb, np.nan)
@@ -191,14 +227,17 @@ def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, 
                 cfl0: float = 10.0, cfl_max: float | None = None, knobs: _abi.Knobs | None = None,
                 shapes: dict | None = None, keep_state: bool = False, pool_budget: int = 0,
                 on_step=None, cfl_late: float | None = None, late_below: float | None = None,
-                record_history: bool = False) -> BatchResult:
+                record_history: bool = False, fallback_max_update: float | None = None) -> BatchResult:
     """Upload every case, then solve each to rn/r0 <= tol (or n_iter Newton
     steps). precision: "fp64" or "mixed"; knobs and the CFL policy
     (cfl_max, cfl_late, late_below) default to SOLVER_DEFAULTS[precision].
     shapes: optional {(ni, nc): stencil shape index} from the autotuner.
     on_step(it, n_active) after each step. cfl_late / late_below: the CFL cap
     rises to cfl_late once rn/r0 < late_below (late_below = 0 never).
+    fallback_max_update: a case's step limit after its first rejected step
+    (0: keep the knobs' max_update).
     BatchResult.wall covers the Newton loop (inputs resident); .upload_s the
     host-to-device set-up before it."""
     bs = BatchSolve(cases, precision, knobs, shapes, pool_budget).upload()
-    return bs.run(tol, n_iter, cfl0, cfl_max, keep_state, on_step, cfl_late, late_below, record_history)
+    return bs.run(tol, n_iter, cfl0, cfl_max, keep_state, on_step, cfl_late, late_below, record_history,
+                  fallback_max_update)

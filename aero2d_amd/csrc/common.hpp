@@ -39,6 +39,7 @@ struct CaseScal {
   double rn;           // sqrt(mean((R_rho / vol)^2))
   double rel;          // max relative change of rho, p under the full step
   double omega;        // step limiter
+  double max_update;   // this case's bound on the relative change of rho, p (0: the pool's knob)
   double beta;         // |M(b - A x0)|
   double mbnorm;       // |M b|
   double bnorm;        // |b|

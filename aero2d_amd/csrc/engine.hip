@@ -56,6 +56,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
   d.early_exit = k.early_exit;
   d.restart = k.gmres_restart;
   d.precond_rb = k.precond_order == 1;
+  d.positivity = k.positivity;
   return d;
 }
 
@@ -193,6 +194,7 @@ void default_knobs(a2d_knobs* k) {
   k->early_exit = 0;
   k->use_graph = 1;
   k->precond_order = 0;
+  k->positivity = 0;
 }
 
 Pool& get_pool(int64_t id) {
@@ -365,6 +367,7 @@ void enq_newton(Pool& p, BatchPlan& b) {
   launch_lstsq(b.pc, s);
   launch_dupdate(b.pvec, b.nvec, prec, s);
   launch_limiter(b.pc, kn, s);
+  if (kn.positivity) launch_positivity(b.pvec, b.nvec, b.pc, s);
   launch_apply(b.pvec, b.nvec, s);
   launch_forces(b.pc, nullptr, s);
   launch_gather(b.pc, b.dout, s);
@@ -489,6 +492,7 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
       fail(-6, "gmres_restart must be in [0, " + std::to_string(KMAX) + "]");
     if (k->jacobi_sweeps < 1) fail(-6, "jacobi_sweeps must be >= 1");
     if (k->precond_order != 0 && k->precond_order != 1) fail(-6, "precond_order must be 0 or 1");
+    if (k->positivity != 0 && k->positivity != 1) fail(-6, "positivity must be 0 or 1");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -717,6 +721,17 @@ int a2d_case_set_freestream(int64_t pool, int case_id, double mach, double alpha
     c.winf[2] = mach * std::sin(a);
     c.winf[3] = 1.0 / 1.4;
     HC(hipMemcpy(c.dev.sc->winf, c.winf, sizeof(c.winf), hipMemcpyHostToDevice));
+  });
+}
+
+int a2d_case_set_step_limit(int64_t pool, int case_id, double max_update) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    if (!(max_update >= 0.0)) fail(-6, "max_update must be >= 0 (0 = the pool's knob)");
+    HC(hipMemcpyAsync(&c.dev.sc->max_update, &max_update, sizeof(double), hipMemcpyHostToDevice,
+                      p.stream));
+    HC(hipStreamSynchronize(p.stream));
   });
 }
 

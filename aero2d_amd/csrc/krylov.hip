@@ -362,8 +362,10 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   }
 }
 
-// One thread per case: omega = min(1, max_update / (rel + 1e-30)); accept the
-// step when rn, rel and dU are finite (aero2d/implicit.py:190, 218-219).
+// One wavefront per case: omega = min(1, max_update / (rel + 1e-30)); accept
+// the step when rn, rel and dU are finite (aero2d/implicit.py:190, 218-219).
+// max_update is the case's own bound when set (sc->max_update > 0), else the
+// pool's.
 __global__ void __launch_bounds__(WAVE) k_limiter(Plan pc, KnobsDev kn) {
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
@@ -372,9 +374,55 @@ __global__ void __launch_bounds__(WAVE) k_limiter(Plan pc, KnobsDev kn) {
   if (threadIdx.x != 0) return;
   sc->rel = mx;
   sc->nonfinite = int(ct);
-  const double om = fmin(1.0, kn.max_update / (mx + 1e-30));
+  const double mu = sc->max_update > 0.0 ? sc->max_update : kn.max_update;
+  const double om = fmin(1.0, mu / (mx + 1e-30));
   sc->omega = om;
   sc->accept = (ct == 0.0) && isfinite(mx) && isfinite(sc->rn) && !sc->frozen;
+}
+
+// Cells of U + omega dU with rho <= 0 or p <= 0 (or not finite), per block,
+// for accepted cases.
+__global__ void __launch_bounds__(NT) k_poscount(Plan plan) {
+  __shared__ double scnt[NT / WAVE];
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  double bad = 0.0;
+  if (sc->accept) {
+    const double om = sc->omega;
+    const int n = cd.n;
+    const gptr<const double> U = gp(cd.U), dU = gp(cd.dU);
+    for (int m = 0; m < PER; ++m) {
+      const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
+      if (c >= n) continue;
+      double u[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) u[v] = U[(size_t)v * n + c] + om * dU[(size_t)v * n + c];
+      const W4<double> w = cons_to_prim<double>(u[0], u[1], u[2], u[3]);
+      if (!(w.r > 0.0) || !(w.p > 0.0)) bad += 1.0;
+    }
+  }
+  bad = wave_sum(bad);
+  if ((threadIdx.x & (WAVE - 1)) == 0) scnt[threadIdx.x / WAVE] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ct = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / WAVE; ++q) ct += scnt[q];
+    cd.part[(size_t)lb * (cd.kmax + 2)] = ct;
+  }
+}
+
+// One wavefront per case: reject the step when any cell would be non-positive.
+__global__ void __launch_bounds__(WAVE) k_posreject(Plan pc) {
+  const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
+  CaseScal* sc = cd.sc;
+  const double ct = case_sum(cd, cd.nvecb, 0);
+  if (threadIdx.x == 0 && ct > 0.0) {
+    sc->accept = 0;
+    sc->nonfinite += int(ct);
+  }
 }
 
 // U += omega dU for accepted cases.
@@ -476,6 +524,10 @@ void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s) {
 }
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s) {
   hipLaunchKernelGGL(k_limiter, dim3(pc.nb), dim3(WAVE), 0, s, pc, kn);
+}
+void launch_positivity(const Plan& pv, int nblocks, const Plan& pc, hipStream_t s) {
+  hipLaunchKernelGGL(k_poscount, dim3(nblocks), dim3(NT), 0, s, pv);
+  hipLaunchKernelGGL(k_posreject, dim3(pc.nb), dim3(WAVE), 0, s, pc);
 }
 void launch_apply(const Plan& pv, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_apply, dim3(nblocks), dim3(NT), 0, s, pv);

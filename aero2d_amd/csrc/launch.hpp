@@ -28,6 +28,7 @@ struct KnobsDev {        // solver knobs the kernels read
   int early_exit;        // stop Arnoldi once the estimate meets gtol
   int restart;           // Arnoldi steps per Newton step
   int precond_rb;        // 1: red-black line ordering (else line Jacobi)
+  int positivity;        // 1: reject a step that leaves rho <= 0 or p <= 0 somewhere
 };
 
 // ---- stencil.hip ------------------------------------------------------------
@@ -79,6 +80,10 @@ void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec
 void launch_lstsq(const Plan& pc, hipStream_t s);
 void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s);
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s);
+// Positivity check of U + omega dU (knob `positivity`): per-block counts of
+// cells with rho <= 0 or p <= 0, then a per-case pass that rejects the step
+// when any cell would be non-positive.
+void launch_positivity(const Plan& pv, int nblocks, const Plan& pc, hipStream_t s);
 void launch_apply(const Plan& pv, int nblocks, hipStream_t s);
 void launch_set_cfl(const Plan& pc, const double* cfl, hipStream_t s);
 void launch_gather(const Plan& pc, double* out, hipStream_t s);

@@ -25,12 +25,16 @@ class NewtonLoop:
     (aero2d/implicit.py:211-244), one object per solve. `step_out` rows come
     from a2d_newton_step."""
 
-    def __init__(self, nb, cfl0, cfl_max, cfl_late=None, late_below=0.0):
+    def __init__(self, nb, cfl0, cfl_max, cfl_late=None, late_below=0.0, fallback=False):
         """cfl_late / late_below: once a case's rn/r0 falls below late_below its
         CFL cap rises from cfl_max to cfl_late (None keeps the reference's
-        single cap)."""
+        single cap). fallback: record each case's first rejected step in
+        `newly_failed` (the caller then lowers that case's step limit)."""
         self.cfl0, self.cfl_max = cfl0, cfl_max
         self.cfl_late, self.late_below = cfl_late, late_below
+        self.fallback = fallback
+        self.failed_once = np.zeros(nb, bool)
+        self.newly_failed = np.zeros(0, int)
         self.cfl = np.full(nb, float(cfl0))
         self.r0 = np.full(nb, np.nan)
         self.fails = np.zeros(nb, int)
@@ -46,6 +50,7 @@ class NewtonLoop:
         self.frozen[idx] = False
         self.iters[idx] = 0
         self.last_rn[idx] = np.nan
+        self.failed_once[idx] = False
 
     def update(self, idx, out):
         """Apply one step's results for batch entries idx; returns rn (NaN when
@@ -53,6 +58,9 @@ class NewtonLoop:
         rn, om, acc, cl = out[:, 0], out[:, 1], out[:, 2] > 0.5, out[:, 3]
         finite = np.isfinite(rn) & acc
         self.fails[idx] += ~finite
+        first = ~finite & ~self.failed_once[idx]
+        self.newly_failed = np.asarray(idx)[first] if self.fallback else np.zeros(0, int)
+        self.failed_once[idx] |= ~finite
         self.frozen[idx] |= self.fails[idx] >= 5
         rn = np.where(finite, rn, np.nan)
         first = np.isnan(self.r0[idx])

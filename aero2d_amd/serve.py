@@ -93,7 +93,9 @@ class SolveServer:
         self.active_cells = 0
         self.tol, self.n_iter = tol, n_iter
         cmax, clate, below = cfl_policy(precision, cfl_max)
-        self.loop = NewtonLoop(4096, cfl0, cmax, clate, below)
+        self.fallback_max_update = SOLVER_DEFAULTS[precision]["fallback_max_update"]
+        fb = self.fallback_max_update > 0 and self.fallback_max_update != kn.max_update
+        self.loop = NewtonLoop(4096, cfl0, cmax, clate, below, fb)
         self.active: dict[int, _Slot] = {}
         self.waiting: list[tuple[Case, float, Future]] = []
         self.t0 = time.perf_counter()
@@ -165,6 +167,8 @@ class SolveServer:
         _abi.call("a2d_newton_step", self.pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl),
                   _abi.dptr(out))
         rn, _ = self.loop.update(ids, out)
+        for cid in self.loop.newly_failed:
+            _abi.call("a2d_case_set_step_limit", self.pool.handle, int(cid), float(self.fallback_max_update))
         self.steps += 1
         done = (rn / self.loop.r0[ids] <= self.tol) | self.loop.frozen[ids] | (self.loop.iters[ids] >= self.n_iter)
         for cid in ids[done]:

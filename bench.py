@@ -140,13 +140,11 @@ def max_errors(a, b):
 
 def mixed_path_fp64():
     """solve_batch arguments that run the FP64 form with the mixed form's
-    solver settings (same preconditioner, Krylov size, early exit and CFL
-    policy): differences from the mixed form are then due to precision alone."""
-    from aero2d_amd.batch import SOLVER_DEFAULTS, default_knobs
-    d = SOLVER_DEFAULTS["mixed"]
-    kn = default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
-                       precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"])
-    return dict(knobs=kn, cfl_max=d["cfl_max"], cfl_late=d["cfl_late"], late_below=d["late_below"])
+    solver settings (same preconditioner, Krylov size, early exit, CFL policy
+    and step limit): differences from the mixed form are then due to precision
+    alone."""
+    from aero2d_amd.batch import path_of
+    return path_of("mixed")
 
 
 def compare_forms(r64, rmx, r64m):

@@ -51,6 +51,8 @@ typedef struct a2d_knobs {
   int precond_order;  /* 0: line Jacobi over i (reference, aero2d/implicit.py:150-157);
                          1: red-black line ordering (odd lines, then even lines
                          with the updated odd neighbours), same cost per sweep */
+  int positivity;     /* 0: accept every finite step (reference); 1: also reject a
+                         step that would leave a cell with rho <= 0 or p <= 0 */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */
@@ -82,6 +84,14 @@ int a2d_case_remove(int64_t pool, int case_id);
 int a2d_case_set_freestream(int64_t pool, int case_id, double mach, double alpha_deg);
 int a2d_case_set_state(int64_t pool, int case_id, const double* U);  /* (4, ni, nc) */
 int a2d_case_get_state(int64_t pool, int case_id, double* U);
+/* This case's bound on the relative change of density and pressure per
+ * Newton step (the step limiter omega = min(1, max_update / change));
+ * 0 restores the pool's knob. No counterpart in the reference, whose bound is
+ * one constant (ImplicitEuler.max_update, aero2d/implicit.py:68-78, 187-190):
+ * the mixed form starts cases at a larger bound and drops a case to the
+ * reference's after its first rejected step. Takes effect from the next
+ * Newton step, also for replayed graphs. */
+int a2d_case_set_step_limit(int64_t pool, int case_id, double max_update);
 /* Stencil tile shape used by this case's residual/JVP (index into the shape
  * table of a2d_stencil_shapes). */
 int a2d_case_set_shape(int64_t pool, int case_id, int shape);

@@ -5,7 +5,8 @@ clean run). The failure is forced from the host through a NaN initial state."""
 import numpy as np
 import pytest
 
-from aero2d_amd import EulerSolver, ImplicitEuler, naca4, ogrid
+from aero2d_amd import EulerSolver, ImplicitEuler, _abi, naca4, ogrid
+from aero2d_amd.batch import default_knobs
 
 pytestmark = pytest.mark.gpu
 
@@ -34,3 +35,47 @@ def test_callback_stops_the_solve():
     solver.solve([0.5], [2.0], n_iter=50, verbose=False,
                  callback=lambda it, n, lr, cl: seen.append(it) or it >= 4)
     assert seen == [1, 2, 3, 4] and solver.iterations == 4
+
+
+def test_per_case_step_limit():
+    """a2d_case_set_step_limit bounds one case's step without touching the
+    others: two identical cases, one limited to 0.01, take the same Newton
+    direction with omega = min(1, limit / change) each."""
+    g = ogrid(naca4("0012", 33), nj=17, ds0=1e-2, radius=10.0)
+    pool = _abi.Pool(0, default_knobs("mixed"))
+    try:
+        ids = np.array([pool.add_case(g.x, g.y, 0.5, 2.0, _abi.PREC_MIXED) for _ in range(2)], np.int32)
+        _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[1]), 0.01)
+        out = np.empty((2, 10))
+        cfl = np.full(2, 50.0)
+        _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), 2, _abi.dptr(cfl), _abi.dptr(out))
+        rel, om = out[:, 7], out[:, 1]
+        assert rel[0] == rel[1] and rel[0] > 0.01
+        assert np.isclose(om[0], min(1.0, 0.5 / (rel[0] + 1e-30)), rtol=1e-15, atol=0)
+        assert np.isclose(om[1], 0.01 / (rel[1] + 1e-30), rtol=1e-15, atol=0)
+        _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[1]), 0.0)   # back to the knob
+        with pytest.raises(_abi.HipError):
+            _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[1]), -1.0)
+    finally:
+        pool.close()
+
+
+def test_positivity_check_keeps_a_case_alive():
+    """A 4-digit section at M 0.754, alpha 3.6 deg on 128 x 64 (draw 49 of the
+    uneven-batch seed) with a step limit of 0.5: without the positivity check
+    an accepted step leaves a cell with p <= 0 and every later step is
+    rejected until the case freezes; with the check that step is rejected
+    instead, the state stays physical and the case keeps iterating (its
+    residual stays finite and below its start)."""
+    from aero2d_amd.batch import solve_batch
+    from aero2d_amd.workloads import config4
+    case = config4(50)[0][49]
+    assert case.grid.x.shape == (128, 65)
+    off = solve_batch([case], "mixed", tol=1e-10, n_iter=200,
+                      knobs=default_knobs("mixed", max_update=0.5, positivity=0), fallback_max_update=0.2)
+    on = solve_batch([case], "mixed", tol=1e-10, n_iter=200,
+                     knobs=default_knobs("mixed", max_update=0.5, positivity=1), fallback_max_update=0.2)
+    print("off", off.frozen, off.iterations, "on", on.frozen, on.iterations, on.rn_over_r0)
+    assert off.frozen[0] and off.iterations[0] < 200
+    assert not on.frozen[0] and on.iterations[0] == 200
+    assert np.isfinite(on.rn_over_r0[0]) and on.rn_over_r0[0] < 1e-2

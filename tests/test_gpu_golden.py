@@ -88,14 +88,10 @@ def test_mixed_form_matches_fp64_on_the_same_solver_path():
     residual and update). The sixth draw of this seed (NACA 2412, M 0.777,
     alpha 0.2) is left out: its density residual stalls near 1e-3.7 in a
     limit cycle under every solver setting, the reference's included."""
-    from aero2d_amd.batch import SOLVER_DEFAULTS, default_knobs, solve_batch
+    from aero2d_amd.batch import path_of, solve_batch
     from aero2d_amd.workloads import config3
     cases = config3(6, 128, 64)[:5]
-    d = SOLVER_DEFAULTS["mixed"]
-    kn = default_knobs("fp64", restart=d["restart"], early_exit=d["early_exit"],
-                       precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"])
-    r64 = solve_batch(cases, "fp64", tol=1e-10, n_iter=600, knobs=kn, cfl_max=d["cfl_max"],
-                      cfl_late=d["cfl_late"], late_below=d["late_below"])
+    r64 = solve_batch(cases, "fp64", tol=1e-10, n_iter=600, **path_of("mixed"))
     rmx = solve_batch(cases, "mixed", tol=1e-10, n_iter=600)
     print("iterations", r64.iterations, rmx.iterations)
     print("dcl", np.abs(rmx.cl - r64.cl), "dcd", np.abs(rmx.cd - r64.cd))
