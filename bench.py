@@ -147,22 +147,54 @@ def mixed_path_fp64():
     return path_of("mixed")
 
 
-def compare_forms(r64, rmx, r64m):
-    """Per-case accuracy of the mixed form. precision_error: against FP64 on
-    the same solver path. vs_reference_path: against FP64 with the reference's
-    settings; cases above 1e-5 there while the same-path error is small have
-    converged to another discrete steady state (these transonic cases admit
-    more than one; the solver path picks which)."""
+def steady_state_check(cases, ra, rb, k):
+    """Full FP64 residual (all four components, rms over cells) of case k's
+    final states in two results, relative to its freestream residual. Both
+    below 1e-6 in every component (converged states sit at 1e-9..1e-8; the
+    two states differ by ~1e-2 in Cl, so a state that was not steady would
+    show a residual of that order) means both are steady states of the same
+    discrete equations."""
+    from aero2d_amd import EulerSolver
+    c = cases[k]
+    s = EulerSolver(c.grid, dtype=np.float64)
+    try:
+        winf = s.freestream(c.mach, c.alpha)
+
+        def rms(U):
+            R, _ = s.residual(U, winf, float("nan"))
+            return np.sqrt(np.mean(R.reshape(4, -1) ** 2, axis=1))
+        r0 = rms(s.init_state(winf))
+        qa, qb = rms(ra.U[k]) / r0, rms(rb.U[k]) / r0
+    finally:
+        s.close()
+    return {"case": int(k), "cl": [float(ra.cl[k]), float(rb.cl[k])],
+            "max_rel_residual": [float(qa.max()), float(qb.max())],
+            "both_steady": bool(qa.max() < 1e-6 and qb.max() < 1e-6)}
+
+
+def compare_forms(r64, rmx, r64m, cases=None):
+    """Per-case accuracy of the mixed form. precision_error_same_path: against
+    FP64 on the same solver path. vs_reference_path: against FP64 with the
+    reference's settings. A case above 1e-5 against the reference path is
+    checked (when the final states are kept): if the full residual of both
+    final states is at round-off, the two are distinct steady states of the
+    same discrete equations (these transonic cases admit more than one; the
+    solver path, and at large steps even the rounding, picks which)."""
     same = max_errors(rmx, r64m)
     ref = max_errors(rmx, r64)
-    both = rmx.converged & r64.converged & r64m.converged
+    both = rmx.converged & r64.converged
     dref = np.abs(rmx.cl - r64.cl)
     dsame = np.abs(rmx.cl - r64m.cl)
-    branch = [int(i) for i in np.nonzero(both & (dref > 1e-5) & (dsame <= 1e-5))[0]]
-    other = [int(i) for i in np.nonzero(both & (dref > 1e-5) & (dsame > 1e-5))[0]]
-    return {"precision_error_same_path": same, "vs_reference_path": ref,
-            "other_steady_state_cases": branch, "unexplained_cases": other,
-            "per_case_dcl_same_path": dsame.tolist(), "per_case_dcl_reference_path": dref.tolist()}
+    flagged = [int(i) for i in np.nonzero(both & (dref > 1e-5))[0]]
+    out = {"precision_error_same_path": same, "vs_reference_path": ref,
+           "cases_within_1e-5_of_reference_path": int((both & (dref <= 1e-5)).sum()),
+           "other_steady_state_cases": flagged,
+           "per_case_dcl_same_path": dsame.tolist(), "per_case_dcl_reference_path": dref.tolist()}
+    if cases is not None and r64.U and rmx.U:
+        checks = [steady_state_check(cases, r64, rmx, k) for k in flagged]
+        out["steady_state_checks"] = checks
+        out["unexplained_cases"] = [c["case"] for c in checks if not c["both_steady"]]
+    return out
 
 
 def run_config3(a):
@@ -176,9 +208,9 @@ def run_config3(a):
         rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
         return {"workload": f"{len(cases)} transonic cases on 512x256, mixed only", "wall_s": rmx.wall,
                 "converged": int(rmx.converged.sum()), "newton_steps_mean": float(rmx.iterations.mean())}
-    r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter)
+    r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, keep_state=True)
     print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
-    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
+    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, keep_state=True)
     print(f"# mixed: {rmx.wall:.1f} s, converged {int(rmx.converged.sum())}", file=sys.stderr, flush=True)
     r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, **mixed_path_fp64())
     print(f"# fp64 on the mixed form's path: {r64m.wall:.1f} s, converged {int(r64m.converged.sum())}",
@@ -201,7 +233,7 @@ def run_config3(a):
            "speedup_mixed_vs_fp64": float(vmx / v64) if v64 > 0 else None,
            "converged_fp64": r64.converged.tolist(), "converged_mixed": rmx.converged.tolist(),
            "cases": [(c.name, c.mach, c.alpha) for c in cases]}
-    out.update(compare_forms(r64, rmx, r64m))
+    out.update(compare_forms(r64, rmx, r64m, cases))
     return out
 
 
