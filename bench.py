@@ -281,11 +281,13 @@ def run_config5(a):
             lat = np.array([x.latency for x in res]) if res else np.array([np.nan])
             entry = {"released": released, "completed": len(res),
                      "converged": int(sum(x.converged for x in res)),
-                     "throughput": len(res) / wall, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "throughput": len(res) / wall,
+                     "converged_per_s": sum(x.converged for x in res) / wall,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3),
                      "p99_ms": float(np.percentile(lat, 99) * 1e3), "newton_steps": srv.steps}
             out["rates"].setdefault(prec, {})[str(rate)] = entry
             print(f"# {prec} rate {rate}: {entry}", file=sys.stderr, flush=True)
-    best = max(e["throughput"] for e in out["rates"].get("mixed", out["rates"][a.precisions.split(',')[0]]).values())
+    best = max(e["converged_per_s"] for e in out["rates"].get("mixed", out["rates"][a.precisions.split(',')[0]]).values())
     out["value"] = best
     out["dtype"] = "f32 inner / f64 outer"
     return out
