@@ -7,7 +7,7 @@
 //   * per-case wall reductions: forces / circulation (aero2d/euler.py:318-331,
 //     aero2d/implicit.py:160-164) and the circulation tangent.
 //
-// Tiling: a workgroup of 256 threads owns TI x TJ cells. It stages the
+// Tiling: a workgroup of NTS threads owns TI x TJ cells. It stages the
 // primitive state of the (TI+4) x (TJ+4) halo in LDS (MUSCL needs two cells
 // each side; wall mirror ghosts and far-field ghosts are produced while
 // loading), evaluates every i-face and j-face flux of the tile exactly once
@@ -25,7 +25,18 @@ namespace a2d {
 
 namespace {
 
-constexpr int NT = 256;
+constexpr int NT = 256;             // threads of the per-case wall reductions
+#ifndef A2D_STENCIL_THREADS
+#define A2D_STENCIL_THREADS 512
+#endif
+// Threads of a stencil tile's workgroup (a multiple of 256, so that its waves
+// spread evenly over the CU's four SIMDs). The tile's passes (halo, slopes,
+// faces, cells) are strided loops over this count. The LDS tile limits the
+// workgroups per CU (2 for the FP64 JVP, 3 for the residual, 4 for the FP32
+// JVP), so 512 threads doubles the resident waves of 256: measured on the
+// default batch, FP64 JVP 286 -> 242 us, FP32 JVP 129 -> 112 us, residual
+// unchanged (179 -> 184 us); 320, 384, 768 and 1024 threads were slower.
+constexpr int NTS = A2D_STENCIL_THREADS;
 
 template <class T>
 struct CaseView {
@@ -61,7 +72,7 @@ __device__ __forceinline__ W4<S> load_prim(gptr<const T> U, gptr<const T> v, int
 // The stencil kernel. TAN = false: FP64 residual. TAN = true: tangent-linear
 // in precision T.
 template <class T, bool TAN, int TI, int TJ>
-__global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin, int vout,
+__global__ void __launch_bounds__(NTS) k_stencil(Plan plan, KnobsDev kn, int vin, int vout,
                                                 int mode, int kidx, int use_diag, int want_diag,
                                                 double circ_override, int refresh_ut, int gate) {
   using S = typename std::conditional<TAN, Dual<T>, T>::type;
@@ -75,7 +86,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   // rows, j-direction for rows j0-1..j0+TJ of its columns (value, tangent)
   __shared__ T sSi[TAN ? 2 : 1][4][TJ][TI + 2];
   __shared__ T sSj[TAN ? 2 : 1][4][TJ + 2][TI];
-  __shared__ double sred[NT / WAVE];
+  __shared__ double sred[NTS / WAVE];
 
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
@@ -97,7 +108,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   else circ = S(T(circ_d) * vort);
 
   // ---- stage primitives of the halo in LDS --------------------------------
-  for (int c = tid; c < NH; c += NT) {
+  for (int c = tid; c < NH; c += NTS) {
     const int hi = c % HI, hj = c / HI;
     const int i = wrap(i0 - 2 + hi, ni);
     const int j = j0 - 2 + hj;
@@ -150,7 +161,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   // faces on both of its sides.
   if (kn.order == 2) {
     constexpr int NSI = (TI + 2) * TJ, NSJ = TI * (TJ + 2);
-    for (int c = tid; c < NSI + NSJ; c += NT) {
+    for (int c = tid; c < NSI + NSJ; c += NTS) {
       const bool isI = c < NSI;
       int hi, hj, a0, a1;
       if (isI) {
@@ -192,7 +203,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
   }
 
   // ---- face fluxes ----------------------------------------------------------
-  for (int f = tid; f < NFI + NFJ; f += NT) {
+  for (int f = tid; f < NFI + NFJ; f += NTS) {
     const bool isI = f < NFI;
     int li, lj, h0i, h0j, di, dj;
     T sx, sy;
@@ -273,7 +284,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
 
   // ---- cell differences and outputs ----------------------------------------
   double rn_part = 0.0;
-  for (int c = tid; c < TI * TJ; c += NT) {
+  for (int c = tid; c < TI * TJ; c += NTS) {
     const int li = c % TI, lj = c / TI;
     const int i = i0 + li, j = j0 + lj;
     if (i >= ni || j >= nc) continue;
@@ -335,7 +346,7 @@ __global__ void __launch_bounds__(NT) k_stencil(Plan plan, KnobsDev kn, int vin,
     if (tid == 0) {
       double s = 0.0;
 #pragma unroll
-      for (int w = 0; w < NT / WAVE; ++w) s += sred[w];
+      for (int w = 0; w < NTS / WAVE; ++w) s += sred[w];
       cd.part[(size_t)lt * (cd.kmax + 2)] = s;
     }
   }
@@ -458,14 +469,14 @@ constexpr StencilShape kShapes[] = {
 template <int TI, int TJ>
 void run_residual(const Plan& p, int nblocks, const KnobsDev& k, int want_diag, double circ_override,
                   int refresh_ut, hipStream_t s) {
-  hipLaunchKernelGGL((k_stencil<double, false, TI, TJ>), dim3(nblocks), dim3(NT), 0, s, p, k, 0, 0,
+  hipLaunchKernelGGL((k_stencil<double, false, TI, TJ>), dim3(nblocks), dim3(NTS), 0, s, p, k, 0, 0,
                      0, -1, 0, want_diag, circ_override, refresh_ut, 0);
 }
 
 template <class T, int TI, int TJ>
 void run_jvp(const Plan& p, int nblocks, const KnobsDev& k, int vin, int vout, int mode, int kidx,
              int use_diag, int gate, hipStream_t s) {
-  hipLaunchKernelGGL((k_stencil<T, true, TI, TJ>), dim3(nblocks), dim3(NT), 0, s, p, k, vin, vout,
+  hipLaunchKernelGGL((k_stencil<T, true, TI, TJ>), dim3(nblocks), dim3(NTS), 0, s, p, k, vin, vout,
                      mode, kidx, use_diag, 0, double(NAN), 0, gate);
 }
 
