@@ -273,18 +273,28 @@ __global__ void __launch_bounds__(WAVE) k_krylov_begin(Plan pc, int stage, Knobs
   sc->kdone = 0;
 }
 
-// y = R^-1 g by back substitution over the completed Arnoldi steps.
-__global__ void k_lstsq(Plan pc) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pc.nb) return;
-  CaseScal* sc = pc.cases[pc.slots[e]].sc;
+// y = R^-1 g by back substitution over the completed Arnoldi steps, one
+// wavefront per case: the triangle and g are staged in LDS by all lanes, then
+// the lanes form each row's sum together (fixed-order wave reduction).
+__global__ void __launch_bounds__(WAVE) k_lstsq(Plan pc) {
+  __shared__ double sH[KMAX][KMAX + 1];
+  __shared__ double sy[KMAX];
+  CaseScal* sc = pc.cases[pc.slots[blockIdx.x]].sc;
+  const int lane = threadIdx.x;
   const int kd = sc->run ? sc->kdone : 0;
+  for (int i = 0; i < kd; ++i)
+    for (int l = i + lane; l < kd; l += WAVE) sH[i][l] = sc->H[i][l];
+  __syncthreads();
   for (int i = kd - 1; i >= 0; --i) {
-    double acc = sc->g[i];
-    for (int l = i + 1; l < kd; ++l) acc -= sc->H[i][l] * sc->y[l];
-    sc->y[i] = acc / sc->H[i][i];
+    double part = 0.0;
+    for (int l = i + 1 + lane; l < kd; l += WAVE) part += sH[i][l] * sy[l];
+    const double acc = sc->g[i] - wave_sum(part);
+    const double yi = acc / sH[i][i];
+    if (lane == 0) sy[i] = yi;
+    __syncthreads();
   }
-  sc->kdone = kd;
+  for (int i = lane; i < kd; i += WAVE) sc->y[i] = sy[i];
+  if (lane == 0) sc->kdone = kd;
 }
 
 // dU = x0 + sum_j y_j s_j V_j (stored FP64), with the step-limiter partials:
@@ -516,7 +526,7 @@ void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec
     hipLaunchKernelGGL(k_krylov_begin<float>, dim3(pc.nb), dim3(WAVE), 0, s, pc, stage, kn);
 }
 void launch_lstsq(const Plan& pc, hipStream_t s) {
-  hipLaunchKernelGGL(k_lstsq, dim3(cdiv(pc.nb, 64)), dim3(64), 0, s, pc);
+  hipLaunchKernelGGL(k_lstsq, dim3(pc.nb), dim3(WAVE), 0, s, pc);
 }
 void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s) {
   if (prec == 0) hipLaunchKernelGGL(k_dupdate<double>, dim3(nblocks), dim3(NT), 0, s, pv);

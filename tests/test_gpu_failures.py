@@ -60,22 +60,38 @@ def test_per_case_step_limit():
         pool.close()
 
 
-def test_positivity_check_keeps_a_case_alive():
-    """A 4-digit section at M 0.754, alpha 3.6 deg on 128 x 64 (draw 49 of the
-    uneven-batch seed) with a step limit of 0.5: without the positivity check
-    an accepted step leaves a cell with p <= 0 and every later step is
-    rejected until the case freezes; with the check that step is rejected
-    instead, the state stays physical and the case keeps iterating (its
-    residual stays finite and below its start)."""
-    from aero2d_amd.batch import solve_batch
-    from aero2d_amd.workloads import config4
-    case = config4(50)[0][49]
-    assert case.grid.x.shape == (128, 65)
-    off = solve_batch([case], "mixed", tol=1e-10, n_iter=200,
-                      knobs=default_knobs("mixed", max_update=0.5, positivity=0), fallback_max_update=0.2)
-    on = solve_batch([case], "mixed", tol=1e-10, n_iter=200,
-                     knobs=default_knobs("mixed", max_update=0.5, positivity=1), fallback_max_update=0.2)
-    print("off", off.frozen, off.iterations, "on", on.frozen, on.iterations, on.rn_over_r0)
-    assert off.frozen[0] and off.iterations[0] < 200
-    assert not on.frozen[0] and on.iterations[0] == 200
-    assert np.isfinite(on.rn_over_r0[0]) and on.rn_over_r0[0] < 1e-2
+def test_positivity_check_keeps_states_physical():
+    """With the positivity knob, no accepted step leaves a cell with rho <= 0
+    or p <= 0, however large the steps: four transonic cases (NACA 0012/2412,
+    128 x 64) with a step limit of 2 (ten times the reference's), every state
+    checked after every Newton step; the check must have rejected steps."""
+    from aero2d_amd.batch import BatchSolve
+    from aero2d_amd.euler import cons_to_prim
+    from aero2d_amd.implicit import NewtonLoop
+    from aero2d_amd.workloads import config3
+    cases = config3(4, 128, 64)
+    bs = BatchSolve(cases, "mixed", knobs=default_knobs("mixed", max_update=2.0, positivity=1)).upload()
+    pool, ids = bs.pool, bs.ids
+    loop = NewtonLoop(len(cases), 10.0, 300.0, 1000.0, 1e-3)
+    rejected = 0
+    try:
+        for it in range(60):
+            act = np.nonzero(~loop.frozen)[0]
+            if len(act) == 0:
+                break
+            out = np.empty((len(act), 10))
+            cfl = np.ascontiguousarray(loop.cfl[act])
+            _abi.call("a2d_newton_step", pool.handle, _abi.iptr(np.ascontiguousarray(ids[act])), len(act),
+                      _abi.dptr(cfl), _abi.dptr(out))
+            loop.update(act, out)
+            rejected += int(np.sum(out[:, 2] < 0.5))
+            for k in act:
+                ni, nj = cases[k].grid.x.shape
+                U = np.empty((4, ni, nj - 1))
+                _abi.call("a2d_case_get_state", pool.handle, int(ids[k]), _abi.dptr(U))
+                W = cons_to_prim(U)
+                assert np.all(W[0] > 0) and np.all(W[3] > 0), (it, k)
+    finally:
+        pool.close()
+    print("rejected steps", rejected)
+    assert rejected >= 1
