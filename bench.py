@@ -105,8 +105,9 @@ def run_kernels(a):
     return out
 
 
-def cpu_baseline(case, gpu_iters, steps=2):
-    """FP64 oracle Newton steps of one case on the host cores."""
+def cpu_baseline(case, gpu_iters, steps=2, scale_by="the GPU run's mean"):
+    """FP64 oracle Newton steps of one case on the host cores, scaled to a
+    whole solve by `gpu_iters` Newton steps per case."""
     import torch
     from oracle import flow, newton
     cores = len(os.sched_getaffinity(0))
@@ -117,7 +118,7 @@ def cpu_baseline(case, gpu_iters, steps=2):
     dt = (time.perf_counter() - t) / steps
     return {"value": 1.0 / (dt * gpu_iters), "unit": "cases/s", "cores": cores, "kind": "port",
             "sample": (f"{steps} FP64 Newton steps of one {case.grid.x.shape[0]}x{case.grid.x.shape[1] - 1} "
-                       f"case by the oracle ({dt:.2f} s/step), scaled by the GPU run's mean "
+                       f"case by the oracle ({dt:.2f} s/step), scaled by {scale_by} "
                        f"{gpu_iters:.1f} Newton steps per case")}
 
 
@@ -194,6 +195,40 @@ def compare_forms(r64, rmx, r64m, cases=None):
         checks = [steady_state_check(cases, r64, rmx, k) for k in flagged]
         out["steady_state_checks"] = checks
         out["unexplained_cases"] = [c["case"] for c in checks if not c["both_steady"]]
+    return out
+
+
+def run_config1(a):
+    """Single case: NACA 0012 on 128 x 64 (ratio 1.1 to 20 chords), M 0.5,
+    alpha 2 deg, to rn/r0 <= 1e-10, FP64 and mixed, against the committed
+    oracle solve of the same case (tests/golden/oracle_config1.json)."""
+    from aero2d_amd.batch import solve_batch
+    from aero2d_amd.workloads import config1
+    cases = config1()
+    with open(os.path.join(ROOT, "tests", "golden", "oracle_config1.json")) as f:
+        ref = json.load(f)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        solve_batch(cases, prec, tol=a.tol, n_iter=300)               # warm-up: plans, graphs
+        r = solve_batch(cases, prec, tol=a.tol, n_iter=300)
+        res[prec] = {"value": float(r.converged.sum() / r.wall), "wall_s": r.wall,
+                     "converged": bool(r.converged[0]), "newton_steps": int(r.iterations[0]),
+                     "cl": float(r.cl[0]), "cd": float(r.cd[0]), "cm": float(r.cm[0]),
+                     "vs_oracle": {"dcl": abs(float(r.cl[0]) - ref["cl"]), "dcd": abs(float(r.cd[0]) - ref["cd"]),
+                                   "dcm": abs(float(r.cm[0]) - ref["cm"])}}
+        print(f"# {prec}: {r.wall:.3f} s, {int(r.iterations[0])} Newton steps, |dCl| vs oracle "
+              f"{res[prec]['vs_oracle']['dcl']:.2e}", file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "value": res["fp64"]["value"], "unit": "cases/s", "n_gpus": 1, "steps": 1,
+           "warmup": 1, "ms_per_step": 1e3 * res["fp64"]["wall_s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "NACA 0012, 128x64 O-grid (ratio 1.1 to 20 chords), M 0.5, alpha 2 deg, "
+                                  f"one case, rn/r0 <= {a.tol:g}"},
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "oracle": {"iterations": ref["iterations"], "cl": ref["cl"], "cd": ref["cd"], "cm": ref["cm"],
+                      "full_solve_s_8_cores": ref["seconds"]}}
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cases[0], float(ref["iterations"]), steps=3,
+                                           scale_by="the oracle's full solve of this case,")
     return out
 
 
@@ -296,7 +331,7 @@ def run_config5(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="config2",
-                    choices=["config2", "config3", "config4", "config5", "kernels"])
+                    choices=["config1", "config2", "config3", "config4", "config5", "kernels"])
     ap.add_argument("--rates", default="10,25,50,100,200,400")
     ap.add_argument("--horizon", type=float, default=30.0)
     ap.add_argument("--precisions", default="mixed,fp64")
@@ -320,7 +355,7 @@ def main():
         sys.exit(2)
 
     if a.workload != "config2":
-        fn = {"config3": run_config3, "config4": run_config4, "config5": run_config5,
+        fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,
               "kernels": run_kernels}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return

@@ -82,17 +82,22 @@ def test_reference_integration_assertions():
 
 def test_mixed_form_matches_fp64_on_the_same_solver_path():
     """Transonic batch (NACA 0012/2412, M 0.72-0.82, 128 x 64): the mixed form
-    with its default settings against the FP64 form run with the same
-    settings. Both converge to rn/r0 <= 1e-10 and agree per case to
-    |dCl|, |dCm| <= 1e-7 and |dCd| <= 1e-8 (FP32 inner iterations, FP64
-    residual and update). The sixth draw of this seed (NACA 2412, M 0.777,
-    alpha 0.2) is left out: its density residual stalls near 1e-3.7 in a
-    limit cycle under every solver setting, the reference's included."""
-    from aero2d_amd.batch import path_of, solve_batch
+    against the FP64 form run on the same solver path (the mixed form's
+    Krylov, preconditioner and CFL settings). Both converge to rn/r0 <= 1e-10
+    and agree per case to |dCl|, |dCm| <= 1e-7 and |dCd| <= 1e-8 (FP32 inner
+    iterations, FP64 residual and update). Both run with the reference's step
+    limit 0.2: with the mixed form's larger default limit, FP32 rounding alone
+    can send a transonic case to another of its steady states, which is not a
+    precision error. The sixth draw of this seed (NACA 2412, M 0.777, alpha
+    0.2) is left out: its density residual stalls near 1e-3.7 in a limit cycle
+    under every solver setting, the reference's included."""
+    from aero2d_amd.batch import default_knobs, path_of, solve_batch
     from aero2d_amd.workloads import config3
     cases = config3(6, 128, 64)[:5]
-    r64 = solve_batch(cases, "fp64", tol=1e-10, n_iter=600, **path_of("mixed"))
-    rmx = solve_batch(cases, "mixed", tol=1e-10, n_iter=600)
+    p64 = path_of("mixed")
+    p64["knobs"].max_update = 0.2
+    r64 = solve_batch(cases, "fp64", tol=1e-10, n_iter=600, **p64)
+    rmx = solve_batch(cases, "mixed", tol=1e-10, n_iter=600, knobs=default_knobs("mixed", max_update=0.2))
     print("iterations", r64.iterations, rmx.iterations)
     print("dcl", np.abs(rmx.cl - r64.cl), "dcd", np.abs(rmx.cd - r64.cd))
     assert np.all(r64.converged) and np.all(rmx.converged)
