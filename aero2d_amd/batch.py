@@ -67,16 +67,19 @@ class BatchResult:
 # rejected step, and the `positivity` knob rejects a step whose new state is
 # not positive everywhere (p is quadratic in the step, so a large step can
 # leave p <= 0; the reference accepts it and fails on every later step until
-# the case freezes). Both forms solve the same discrete equations; where
+# the case freezes). The mixed form's line sweeps read the factored couplings
+# from their FP16 copy (`precond_half`): the same Newton steps per case as
+# with FP32 couplings, at three quarters of the preconditioner's cost. Both
+# forms solve the same discrete equations; where
 # those have more than one steady state (transonic cases with a shock), the
 # solver path decides which one a case reaches.
 SOLVER_DEFAULTS = {
     "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0,
                  cfl_late=None, late_below=0.0, max_update=0.2, fallback_max_update=0.0,
-                 positivity=0),
+                 positivity=0, precond_half=0),
     "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=300.0,
                   cfl_late=1000.0, late_below=1e-3, max_update=0.5, fallback_max_update=0.0,
-                  positivity=0),
+                  positivity=0, precond_half=1),
 }
 
 
@@ -91,7 +94,8 @@ def cfl_policy(precision: str, cfl_max: float | None = None, cfl_late: float | N
 
 def default_knobs(precision: str, restart: int | None = None, early_exit: bool | None = None,
                   precond_order: int | None = None, jacobi_sweeps: int | None = None,
-                  max_update: float | None = None, positivity: int | None = None) -> _abi.Knobs:
+                  max_update: float | None = None, positivity: int | None = None,
+                  precond_half: int | None = None) -> _abi.Knobs:
     """Library knobs for the form's defaults, with optional overrides."""
     d = SOLVER_DEFAULTS[precision]
     k = _abi.default_knobs()
@@ -101,6 +105,7 @@ def default_knobs(precision: str, restart: int | None = None, early_exit: bool |
     k.jacobi_sweeps = d["jacobi_sweeps"] if jacobi_sweeps is None else int(jacobi_sweeps)
     k.max_update = d["max_update"] if max_update is None else float(max_update)
     k.positivity = d["positivity"] if positivity is None else int(positivity)
+    k.precond_half = d["precond_half"] if precond_half is None else int(precond_half)
     return k
 
 

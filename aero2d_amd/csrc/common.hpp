@@ -93,6 +93,7 @@ struct CaseDev {
   void* w;              // T [4][nc][ni]
   void* d;              // T [cell][4] line-solve values of the way in
   void* g0;             // T [cell][4] S0 src of the current preconditioner application
+  void* hb;             // mixed form: _Float16 [4][pstride][16], slots 1-4 of blk in FP16
   void* xa;             // T [4][nc][ni] sweep ping-pong
   void* xb;
   double* part;         // [npart][kmax + 2] reduction partials

@@ -57,6 +57,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
   d.restart = k.gmres_restart;
   d.precond_rb = k.precond_order == 1;
   d.positivity = k.positivity;
+  d.precond_half = k.precond_half;
   return d;
 }
 
@@ -64,7 +65,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
 struct Layout {
   size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
   size_t Ut, volt, Sit, Sjt, xft, yft;
-  size_t diag, blk, V, x0, w, d, g0, xa, xb, part, sc;
+  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, part, sc;
   size_t total;
   int npart;
 };
@@ -115,6 +116,7 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   }
   L.diag = take(n * t);
   L.blk = take(80 * plane_stride(n, t) * t);
+  if (prec == 1) L.hb = take(64 * plane_stride(n, t) * 2);
   L.V = take(size_t(kmax + 1) * vec_stride(n, t) * t);
   L.x0 = take(4 * n * t);
   L.w = take(4 * n * t);
@@ -195,6 +197,7 @@ void default_knobs(a2d_knobs* k) {
   k->use_graph = 1;
   k->precond_order = 0;
   k->positivity = 0;
+  k->precond_half = 0;
 }
 
 Pool& get_pool(int64_t id) {
@@ -311,13 +314,14 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
 // ---- launch sequences ---------------------------------------------------------
 void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int gate) {
   const int S = std::max(1, p.knobs.jacobi_sweeps);
+  const int hb = b.prec == 1 && p.knobs.precond_half;
   if (p.knobs.precond_order == 1) {
     // red-black: odd lines then even lines, S times, in place in dest
     for (int s = 0; s < S; ++s)
       for (int color = 1; color >= 0; --color) {
         const int coupled = !(s == 0 && color == 1);
         launch_line_sweep_rb(b.pline, b.nline, src, dest, color, coupled, s > 0,
-                             (s == S - 1) ? want_norm : 0, gate, b.prec, p.stream);
+                             (s == S - 1) ? want_norm : 0, gate, b.prec, hb, p.stream);
       }
     return;
   }
@@ -325,7 +329,7 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
     launch_line_sweep(b.pline, b.nline, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
-                      gate, b.prec, p.stream);
+                      gate, b.prec, hb, p.stream);
   }
 }
 
@@ -493,6 +497,7 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->jacobi_sweeps < 1) fail(-6, "jacobi_sweeps must be >= 1");
     if (k->precond_order != 0 && k->precond_order != 1) fail(-6, "precond_order must be 0 or 1");
     if (k->positivity != 0 && k->positivity != 1) fail(-6, "positivity must be 0 or 1");
+    if (k->precond_half != 0 && k->precond_half != 1) fail(-6, "precond_half must be 0 or 1");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -619,6 +624,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.w = b + L.w;
     d.d = b + L.d;
     d.g0 = b + L.g0;
+    d.hb = precision == 1 ? b + L.hb : nullptr;
     d.xa = b + L.xa;
     d.xb = b + L.xb;
     d.part = reinterpret_cast<double*>(b + L.part);
@@ -942,6 +948,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     BatchPlan& b = get_plan(p, slots);
     const KnobsDev kn = to_dev(p.knobs);
     const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));
+    const int hb = b.prec == 1 && p.knobs.precond_half;
     auto launch = [&] {
       switch (which) {
         case 0:  // as launched in a Newton step: with the local time step and U copy
@@ -952,7 +959,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
             launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
           break;
         case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, p.stream);
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);
           break;
         case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
@@ -960,10 +967,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, p.stream);
+          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);
           break;
         case 9:  // a coupled red-black half sweep (odd lines) of a later sweep
-          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, p.stream);
+          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, hb, p.stream);
           break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");

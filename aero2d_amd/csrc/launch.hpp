@@ -29,6 +29,7 @@ struct KnobsDev {        // solver knobs the kernels read
   int restart;           // Arnoldi steps per Newton step
   int precond_rb;        // 1: red-black line ordering (else line Jacobi)
   int positivity;        // 1: reject a step that leaves rho <= 0 or p <= 0 somewhere
+  int precond_half;      // 1: mixed-form sweeps read the FP16 copy of the couplings
 };
 
 // ---- stencil.hip ------------------------------------------------------------
@@ -59,17 +60,17 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // with x = xprev, the neighbour terms only when coupled. src = VEC_NEG_R reads
 // -R. The first sweep of an application keeps S0 src in the case's g0 buffer;
 // reuse_g0 = 1 (a later sweep, same src) reads it instead of src and S0.
-// want_norm: per-block partials of |xout|^2. Plan pl: LINES_PER_BLOCK lines
-// per block.
+// want_norm: per-block partials of |xout|^2. hb (prec 1 only): couplings from
+// the FP16 copy. Plan pl: LINES_PER_BLOCK lines per block.
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
-                       int reuse_g0, int want_norm, int gate, int prec, hipStream_t s);
+                       int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s);
 // One half sweep of the red-black ordering: lines i = color (mod 2) only,
 // in place in xout (neighbours of the other colour are read from xout);
 // coupled = 0 drops the i-coupling (the very first half sweep); reuse_g0 as
 // above (the sweeps after the first). Norm partials go to
 // part[(color * nlineb + block)].
 void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
-                          int reuse_g0, int want_norm, int gate, int prec, hipStream_t s);
+                          int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);

@@ -30,6 +30,13 @@
 // factorisation 0 = S0, 1 = backward coupling (c or e), 2 = forward coupling
 // (P or Q), 3 = S0 Bim, 4 = S0 Bip. With the red-black line order the cell
 // index is colour-split (column of line i = (i % 2) ni/2 + i/2).
+// Mixed form: the factorisation (in FP32) also writes slots 1-4 to an FP16
+// copy, _Float16 hb[4][pstride][4][4] (slot - 1, cell, row, column); with the
+// knob precond_half the sweeps read their couplings from it (an 8-byte load
+// per block row, half the bytes of the sweeps' dominant traffic). The
+// couplings are products with the inverse of the eliminated diagonal, entries
+// of order one; conversion saturates at +-65504 and keeps NaN, so a factor
+// that failed in FP32 still fails. Slot 0 (S0, unbounded) stays FP32.
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -215,6 +222,28 @@ __device__ __forceinline__ void store_row(gptr<double> p, const double v[4]) {
   *(gptr<a2d_d2>)(p + 2) = b;
 }
 
+typedef _Float16 a2d_h4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ _Float16 to_half_sat(float v) {
+  const float c = fminf(fmaxf(v, -65504.f), 65504.f);
+  return (_Float16)(v == v ? c : v);
+}
+// Row r of slot `slot` (1-4) of the FP16 coupling copy.
+__device__ __forceinline__ gptr<const _Float16> hrow(gptr<const _Float16> H, size_t ps, int slot,
+                                                     int r, size_t cell) {
+  return H + ((size_t)(slot - 1) * ps + cell) * 16 + r * 4;
+}
+__device__ __forceinline__ void store_blk_h(gptr<_Float16> H, size_t ps, int slot, size_t cell,
+                                            const float a[16]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    a2d_h4 t;
+    t.x = to_half_sat(a[4 * r]); t.y = to_half_sat(a[4 * r + 1]);
+    t.z = to_half_sat(a[4 * r + 2]); t.w = to_half_sat(a[4 * r + 3]);
+    *(gptr<a2d_h4>)(H + ((size_t)(slot - 1) * ps + cell) * 16 + r * 4) = t;
+  }
+}
+
 template <class T>
 __device__ __forceinline__ void load_blk(const Blk<const T>& B, int slot, size_t cell, T a[16]) {
 #pragma unroll
@@ -371,6 +400,12 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
   const Blk<T> B{gp(tp<T>(cd.blk)), (size_t)cd.pstride};
   const Blk<const T> Bc{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
+  // slots 1-4 also go to the FP16 copy in the mixed form
+  auto keep = [&](int slot, size_t cell, const T a[16]) {
+    store_blk(B, slot, cell, a);
+    if constexpr (std::is_same<T, float>::value)
+      store_blk_h(gp((_Float16*)cd.hb), B.ps, slot, cell, a);
+  };
   T cprev[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) cprev[k] = T(0);
@@ -398,12 +433,12 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
       mul4<T>(inv, L, P);
       if (live) {
         store_blk(B, 0, cell, inv);
-        store_blk(B, 1, cell, cprev);
-        store_blk(B, 2, cell, P);
+        keep(1, cell, cprev);
+        keep(2, cell, P);
         mul4<T>(inv, Bm, t);
-        store_blk(B, 3, cell, t);
+        keep(3, cell, t);
         mul4<T>(inv, Bp, t);
-        store_blk(B, 4, cell, t);
+        keep(4, cell, t);
       }
     }
   }
@@ -427,15 +462,28 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
     mul4<T>(inv, A, P);
     mul4<T>(inv, C, Q);
     store_blk(B, 0, cell, inv);
-    store_blk(B, 1, cell, Q);
-    store_blk(B, 2, cell, P);
+    keep(1, cell, Q);
+    keep(2, cell, P);
     load_blk(Bc, 3, cell, A);
     mul4<T>(inv, A, P);
-    store_blk(B, 3, cell, P);
+    keep(3, cell, P);
     load_blk(Bc, 4, cell, C);
     mul4<T>(inv, C, Q);
-    store_blk(B, 4, cell, Q);
+    keep(4, cell, Q);
   }
+}
+
+// A block row as loaded: four T, or four FP16 kept packed until the step that
+// consumes them (a conversion placed next to its load would wait for the load).
+template <class T> struct RowT { T v[4]; };
+template <class T, bool HB> struct CplRow { using type = RowT<T>; };
+template <> struct CplRow<float, true> { using type = a2d_h4; };
+template <class T>
+__device__ __forceinline__ void unpack(const RowT<T>& a, T v[4]) {
+  v[0] = a.v[0]; v[1] = a.v[1]; v[2] = a.v[2]; v[3] = a.v[3];
+}
+__device__ __forceinline__ void unpack(const a2d_h4& a, float v[4]) {
+  v[0] = float(a.x); v[1] = float(a.y); v[2] = float(a.z); v[3] = float(a.w);
 }
 
 // One line sweep in a single pass. For every line of the sweep (every line
@@ -465,9 +513,10 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
 // FP64) and keep it in the case's g0 buffer; GS_G0 reads it back. Every sweep
 // of one preconditioner application has the same source, so only the first
 // sweep over a line reads src and S0.
+// HB (FP32 only): the couplings (slots 1-4) come from the FP16 copy.
 enum GSrc : int { GS_VEC = 0, GS_NEGR = 1, GS_G0 = 2 };
 
-template <class T, bool COUP, int GS, int PF, int PFB>
+template <class T, bool COUP, int GS, int PF, int PFB, bool HB>
 __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
                                                int gate, int color) {
   __shared__ double red[4];
@@ -508,9 +557,26 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   const gptr<const T> sV = gp(vec_ptr<T>(cd, GS == GS_VEC ? src : VEC_W) + (size_t)r * n + ic);
   const gptr<const T> xnbp = gp(vec_ptr<T>(cd, COUP ? xnb : VEC_W) + (size_t)r * n);
   const gptr<T> g0p = gp(tp<T>(cd.g0));
-  auto load_src = [&](int jj) -> T {
-    if constexpr (GS == GS_NEGR) return T(-sR[(size_t)jj * ni]);
+  // row r of coupling slot `slot` (1-4) of a cell, as loaded (CR) or as T
+  using CR = typename CplRow<T, HB>::type;
+  auto crow = [&](int slot, size_t cell, CR& a) {
+    if constexpr (HB) a = *(gptr<const a2d_h4>)hrow(gp((const _Float16*)cd.hb), B.ps, slot, r, cell);
+    else load_row(B.row(slot, r, cell), a.v);
+  };
+  auto crow_t = [&](int slot, size_t cell, T v[4]) {
+    CR a;
+    crow(slot, cell, a);
+    unpack(a, v);
+  };
+  // the source as loaded (-R stays FP64 until consumed) and as T
+  using SR = typename std::conditional<GS == GS_NEGR, double, T>::type;
+  auto load_src = [&](int jj) -> SR {
+    if constexpr (GS == GS_NEGR) return sR[(size_t)jj * ni];
     else return sV[(size_t)jj * ni];
+  };
+  auto src_val = [&](SR v) -> T {
+    if constexpr (GS == GS_NEGR) return T(-v);
+    else return v;
   };
   // S0 src (row r) from component r of src and row r of S0
   auto s0src = [&](T sv, const T* s0) -> T {
@@ -532,8 +598,9 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   // ---- toward the middle -------------------------------------------------------
   // ring of PF steps in flight: step s lives in slot s % PF; consuming a slot
   // issues the loads of the step PF further on into it
-  T bs[PF], bxm[PF], bxp[PF];
-  T b0[PF][4], b3[PF][4], b4[PF][4], b2[PF][4];
+  SR bs[PF];
+  T bxm[PF], bxp[PF], b0[PF][4];
+  CR b3[PF], b4[PF], b2[PF];
   auto frow = [&](int s) -> int { return j0 + dj * max(0, min(s, steps - 1)); };
   auto fload = [&](int k, int s) {
     const int jj = frow(s);
@@ -547,32 +614,51 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     if constexpr (COUP) {
       bxm[k] = xnbp[(size_t)jj * ni + im];
       bxp[k] = xnbp[(size_t)jj * ni + ip];
-      load_row(B.row(3, r, cell), b3[k]);
-      load_row(B.row(4, r, cell), b4[k]);
+      crow(3, cell, b3[k]);
+      crow(4, cell, b4[k]);
     }
-    load_row(B.row(2, r, cell), b2[k]);
+    crow(2, cell, b2[k]);
   };
   T dr = T(0);
+  // Fill the ring in slot order (the scheduling barriers keep slot 0's loads
+  // oldest, so that the first wait in the loop can leave the later slots in
+  // flight). Every pass then consumes every slot, and the loop has no per-lane
+  // branch: a step past the line's half computes a value that the select
+  // drops and stores, to the half's last row (where the clamped loads read),
+  // the value that row already holds; dead lanes store what the line they
+  // shadow stores. (Around a per-lane branch the compiler reuses consumed
+  // slots as temporaries and then waits for nearly every load in flight.)
 #pragma unroll
-  for (int k = 0; k < PF; ++k) fload(k, k);
+  for (int k = 0; k < PF; ++k) {
+    fload(k, k);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   for (int s0 = 0; s0 < m; s0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-      if (s0 + k < steps) {
-        const size_t cell = (size_t)frow(s0 + k) * ni + col;
-        T gs;
-        if constexpr (GS == GS_G0) {
-          gs = bs[k];
-        } else {
-          gs = s0src(bs[k], b0[k]);
-          if (live) g0p[cell * 4 + r] = gs;
-        }
-        const T g = gform(gs, bxm[k], bxp[k], b3[k], b4[k]);
-        dr = g - b2[k][0] * quad_bcast<0>(dr) - b2[k][1] * quad_bcast<1>(dr) -
-             b2[k][2] * quad_bcast<2>(dr) - b2[k][3] * quad_bcast<3>(dr);
-        if (live) gd[cell * 4 + r] = dr;
+      const int st = s0 + k;
+      const bool valid = st < steps;
+      const size_t cell = (size_t)frow(st) * ni + col;
+      T gs;
+      if constexpr (GS == GS_G0) {
+        gs = bs[k];
+      } else {
+        gs = s0src(src_val(bs[k]), b0[k]);
+        g0p[cell * 4 + r] = gs;
       }
-      fload(k, s0 + k + PF);
+      T p2[4], p3[4], p4[4];
+      if constexpr (COUP) {
+        unpack(b3[k], p3);
+        unpack(b4[k], p4);
+      }
+      unpack(b2[k], p2);
+      const T g = gform(gs, bxm[k], bxp[k], p3, p4);
+      const T dn = g - p2[0] * quad_bcast<0>(dr) - p2[1] * quad_bcast<1>(dr) -
+                   p2[2] * quad_bcast<2>(dr) - p2[3] * quad_bcast<3>(dr);
+      dr = valid ? dn : dr;
+      gd[cell * 4 + r] = dr;
+      fload(k, st + PF);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   // ---- the middle row: the top's d_{m-1} and the bottom's f_{m+1} meet ---------
@@ -586,7 +672,7 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     if constexpr (GS == GS_G0) {
       gs = g0p[cell * 4 + r];
     } else {
-      const T sv = load_src(m);
+      const T sv = src_val(load_src(m));
       load_row(B.row(0, r, cell), s0r);
       gs = s0src(sv, s0r);
       if (live && half == 0) g0p[cell * 4 + r] = gs;
@@ -594,11 +680,11 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     if constexpr (COUP) {
       xmv = xnbp[(size_t)m * ni + im];
       xpv = xnbp[(size_t)m * ni + ip];
-      load_row(B.row(3, r, cell), s3r);
-      load_row(B.row(4, r, cell), s4r);
+      crow_t(3, cell, s3r);
+      crow_t(4, cell, s4r);
     }
-    load_row(B.row(2, r, cell), pm);
-    load_row(B.row(1, r, cell), qm);
+    crow_t(2, cell, pm);
+    crow_t(1, cell, qm);
     const T g = gform(gs, xmv, xpv, s3r, s4r);
     xr = g - pm[0] * quad_bcast<0>(dtop) - pm[1] * quad_bcast<1>(dtop) - pm[2] * quad_bcast<2>(dtop) -
          pm[3] * quad_bcast<3>(dtop) - qm[0] * quad_bcast<0>(fbot) - qm[1] * quad_bcast<1>(fbot) -
@@ -610,7 +696,8 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     nrm += double(xr) * double(xr);
   }
   // ---- away from the middle: x_j = d_j - S1_j x_prev (ring of PFB steps) -------
-  T cv[PFB], cm[PFB][4];
+  T cv[PFB];
+  CR cm[PFB];
   auto bj = [&](int s) -> int {
     const int sc = max(0, min(s, steps - 1));
     return steps > 0 ? m - dj * (1 + sc) : m;
@@ -618,7 +705,7 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   auto bload = [&](int k, int s) {
     const size_t cell = (size_t)bj(s) * ni + col;
     cv[k] = gd[cell * 4 + r];
-    load_row(B.row(1, r, cell), cm[k]);
+    crow(1, cell, cm[k]);
   };
 #pragma unroll
   for (int k = 0; k < PFB; ++k) bload(k, k);
@@ -626,8 +713,10 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
 #pragma unroll
     for (int k = 0; k < PFB; ++k) {
       if (s0 + k < steps) {
-        xr = cv[k] - cm[k][0] * quad_bcast<0>(xr) - cm[k][1] * quad_bcast<1>(xr) -
-             cm[k][2] * quad_bcast<2>(xr) - cm[k][3] * quad_bcast<3>(xr);
+        T c1[4];
+        unpack(cm[k], c1);
+        xr = cv[k] - c1[0] * quad_bcast<0>(xr) - c1[1] * quad_bcast<1>(xr) -
+             c1[2] * quad_bcast<2>(xr) - c1[3] * quad_bcast<3>(xr);
         if (live) {
           x[(size_t)bj(s0 + k) * ni] = xr;
           nrm += double(xr) * double(xr);
@@ -654,27 +743,38 @@ template <> struct SweepDepth<float> { static constexpr int in = 8, out = 16; };
 template <> struct SweepDepth<double> { static constexpr int in = 4, out = 8; };
 
 // Kernel variant for a sweep: coupled or not; g from src (a vector or -R,
-// kept in g0) or from g0 (reuse_g0: a later sweep of the same application).
-template <class T>
-void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
-               int want_norm, int gate, int color, hipStream_t s) {
+// kept in g0) or from g0 (reuse_g0: a later sweep of the same application);
+// couplings from the FP16 copy (hb).
+template <class T, bool HB>
+void run_sweep_v(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+                 int want_norm, int gate, int color, hipStream_t s) {
   constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
   const dim3 grid(nblocks), block(256);
   if (coupled && reuse_g0)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
-                       want_norm, gate, color);
+    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
   else if (coupled && src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
-                       want_norm, gate, color);
+    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
   else if (coupled)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
-                       want_norm, gate, color);
+    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
   else if (src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
-                       want_norm, gate, color);
+    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
   else
-    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB>), grid, block, 0, s, pl, src, xnb, xout,
-                       want_norm, gate, color);
+    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+                       xout, want_norm, gate, color);
+}
+
+void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+               int want_norm, int gate, int color, int prec, int hb, hipStream_t s) {
+  if (prec == 0)
+    run_sweep_v<double, false>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else if (hb)
+    run_sweep_v<float, true>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else
+    run_sweep_v<float, false>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
 }
 
 }  // namespace
@@ -690,16 +790,14 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
 }
 
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
-                       int reuse_g0, int want_norm, int gate, int prec, hipStream_t s) {
-  if (prec == 0) run_sweep<double>(pl, nblocks, src, xprev, xout, coupled, reuse_g0, want_norm, gate, -1, s);
-  else run_sweep<float>(pl, nblocks, src, xprev, xout, coupled, reuse_g0, want_norm, gate, -1, s);
+                       int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s) {
+  run_sweep(pl, nblocks, src, xprev, xout, coupled, reuse_g0, want_norm, gate, -1, prec, hb, s);
 }
 
 void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
-                          int reuse_g0, int want_norm, int gate, int prec, hipStream_t s) {
+                          int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s) {
   // neighbours of the other colour are read from xout itself
-  if (prec == 0) run_sweep<double>(pl, nblocks, src, xout, xout, coupled, reuse_g0, want_norm, gate, color, s);
-  else run_sweep<float>(pl, nblocks, src, xout, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  run_sweep(pl, nblocks, src, xout, xout, coupled, reuse_g0, want_norm, gate, color, prec, hb, s);
 }
 
 }  // namespace a2d

@@ -53,6 +53,11 @@ typedef struct a2d_knobs {
                          with the updated odd neighbours), same cost per sweep */
   int positivity;     /* 0: accept every finite step (reference); 1: also reject a
                          step that would leave a cell with rho <= 0 or p <= 0 */
+  int precond_half;   /* mixed form only: 1 = the line sweeps read the factored
+                         couplings (computed in FP32) from an FP16 copy, half the
+                         bytes of the preconditioner's dominant traffic; 0 = FP32.
+                         The factorisation writes both copies, so the knob may
+                         change between factorisation and application. */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */

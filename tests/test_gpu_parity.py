@@ -149,15 +149,19 @@ def test_preconditioner_matches_oracle(case):
     assert rel_err(x, xo) < 1e-9
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision, half", [("fp64", 0), ("mixed", 0), ("mixed", 1)])
 @pytest.mark.parametrize("case", CASES)
-def test_red_black_preconditioner_matches_oracle(case, precision):
+def test_red_black_preconditioner_matches_oracle(case, precision, half):
+    """half = 1: the sweeps read the couplings from their FP16 copy (11-bit
+    significand, relative rounding 2^-11 per entry): agreement to 5e-3 of the
+    FP64 oracle; a wrong slot, row or index shows as O(1)."""
     g, s, U, h, c = _setup(*case, precision=precision)
     mach, alpha = case[2], case[3]
     geo = oracle_geo(g)
     w = winf_t(mach, alpha)
     k = s._pool.knobs()
     k.precond_order = 1
+    k.precond_half = half
     s._pool.set_knobs(k)
     _abi.call("a2d_precond_setup", h, c, 25.0)
     rng = np.random.default_rng(6)
@@ -171,23 +175,27 @@ def test_red_black_preconditioner_matches_oracle(case, precision):
     fac = newton.factor_lines(D, Bjm, Bjp)
     xo = newton.precond_rb(fac, Bim, Bip, torch.tensor(r), 4).numpy()
     # FP32 blocks and sweeps in the mixed form
-    assert rel_err(x, xo) < (1e-9 if precision == "fp64" else 2e-4)
+    err = rel_err(x, xo)
+    print(precision, half, "rb precond rel err", err)
+    assert err < (1e-9 if precision == "fp64" else (5e-3 if half else 2e-4))
     xj = newton.precond(fac, Bim, Bip, torch.tensor(r), 4).numpy()
     assert rel_err(x, xj) > 1e-6          # a different ordering really ran
 
 
 @pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("nj, ds0, radius", [(3, 0.2, 1.0), (4, 0.2, 1.0), (10, 2e-2, 5.0), (11, 2e-2, 5.0)])
-def test_preconditioner_line_lengths(nj, ds0, radius, order):
+@pytest.mark.parametrize("precision, half", [("fp64", 0), ("mixed", 1)])
+def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half):
     """Lines of 2, 3, 9 and 10 cells: the two-sided line elimination meets in
     the middle row whatever the parity of the length (and with an empty far
-    half for 2 cells)."""
+    half for 2 cells). Mixed form with FP16 couplings: 5e-3."""
     g = small_grid(17, nj, ds0=ds0, radius=radius)
-    s = EulerSolver(g, dtype=np.float64)
+    s = EulerSolver(g, dtype=np.float64, precision=precision)
     U = perturbed_state(s.ni, s.nc, 0.6, 1.0, 9)
     h, c = s._scratch(s.freestream(0.6, 1.0), U)
     k = s._pool.knobs()
     k.precond_order = order
+    k.precond_half = half
     s._pool.set_knobs(k)
     _abi.call("a2d_precond_setup", h, c, 40.0)
     r = np.ascontiguousarray(np.random.default_rng(11).normal(size=U.shape))
@@ -202,7 +210,7 @@ def test_preconditioner_line_lengths(nj, ds0, radius, order):
     fac = newton.factor_lines(D, Bjm, Bjp)
     fn = newton.precond_rb if order else newton.precond
     xo = fn(fac, Bim, Bip, torch.tensor(r), 4).numpy()
-    assert rel_err(x, xo) < 1e-9
+    assert rel_err(x, xo) < (1e-9 if precision == "fp64" else 5e-3)
 
 
 @pytest.mark.parametrize("case", CASES[:2])
