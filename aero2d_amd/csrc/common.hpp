@@ -66,7 +66,9 @@ struct CaseDev {
   int npart;            // partial-sum slots available per reduction
   int ntiles;           // stencil tiles of this case's tile shape
   int nvecb;            // vector-kernel blocks (VEC_CHUNK cells each)
-  int nlineb;           // line-kernel blocks (64 lines each)
+  int nlineb;           // line-factorisation blocks (LINES_PER_BLOCK lines each)
+  int nlinew;           // 8-line groups (one wavefront of a sweep each): norm partials
+  int nlinerw;          // 8-line groups of one colour (red-black half sweeps)
   int shape;            // stencil tile shape index
   int vstride;          // elements between consecutive Krylov basis vectors (>= 4n)
   int pstride;          // cells per block slot (>= n); a slot is 16 pstride elements
@@ -107,6 +109,7 @@ struct Plan {
   const int* slots;       // nb slot indices
   const int* blk_prefix;  // nb + 1
   int nb;
+  int lines = 0;          // line sweeps: lines per block (8 or 32)
 };
 
 // Map a flat block index to (batch entry, local block) by binary search over

@@ -23,6 +23,7 @@ namespace {
 
 thread_local std::string g_err;
 int g_device = -1;
+int g_num_cu = 256;   // compute units of the device (sweep block sizing)
 std::recursive_mutex g_mu;
 
 struct Fail {
@@ -124,7 +125,7 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   L.g0 = take(4 * n * t);
   L.xa = take(4 * n * t);
   L.xb = take(4 * n * t);
-  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, LINES_PER_BLOCK), 1});
+  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, 8) + 2, 1});
   L.part = take(size_t(L.npart) * (kmax + 2) * 8);
   L.sc = take(sizeof(CaseScal));
   L.total = o;
@@ -148,8 +149,8 @@ struct BatchPlan {
   double* dout = nullptr;
   double* hcfl = nullptr;
   double* hout = nullptr;
-  Plan pc{}, pcell{}, pvec{}, pline{};
-  int ncell = 0, nvec = 0, nline = 0;
+  Plan pc{}, pcell{}, pvec{}, pline{}, psw{}, prb{};
+  int ncell = 0, nvec = 0, nline = 0, nsw = 0, nrb = 0;
   struct Group {
     Plan p;
     int shape, nblocks;
@@ -252,6 +253,16 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   auto pcell = add_prefix([](const CaseHost& c) { return cdiv(c.dev.n, CELL_BLOCK); });
   auto pvec = add_prefix([](const CaseHost& c) { return c.dev.nvecb; });
   auto pline = add_prefix([](const CaseHost& c) { return c.dev.nlineb; });
+  // sweep block size: 32 lines when such blocks fill the device, else 8
+  int big_sw = 0, big_rb = 0;
+  for (int s : slots) {
+    big_sw += cdiv(p.cases[s].dev.ni, SWEEP_LINES_BIG);
+    big_rb += cdiv(p.cases[s].dev.ni / 2, SWEEP_LINES_BIG);
+  }
+  const int lsw = big_sw >= g_num_cu ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
+  const int lrb = big_rb >= g_num_cu ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
+  auto psw = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni, lsw); });
+  auto prb = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni / 2, lrb); });
   std::map<int, std::vector<int>> groups;
   for (int s : slots) groups[p.cases[s].dev.shape].push_back(s);
   struct G {
@@ -296,6 +307,12 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   bp->nvec = pvec.second;
   bp->pline = mk(pline.first);
   bp->nline = pline.second;
+  bp->psw = mk(psw.first);
+  bp->psw.lines = lsw;
+  bp->nsw = psw.second;
+  bp->prb = mk(prb.first);
+  bp->prb.lines = lrb;
+  bp->nrb = prb.second;
   for (auto& g : gs) {
     BatchPlan::Group gr;
     gr.p.cases = p.dcases;
@@ -320,7 +337,7 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
     for (int s = 0; s < S; ++s)
       for (int color = 1; color >= 0; --color) {
         const int coupled = !(s == 0 && color == 1);
-        launch_line_sweep_rb(b.pline, b.nline, src, dest, color, coupled, s > 0,
+        launch_line_sweep_rb(b.prb, b.nrb, src, dest, color, coupled, s > 0,
                              (s == S - 1) ? want_norm : 0, gate, b.prec, hb, p.stream);
       }
     return;
@@ -328,7 +345,7 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
-    launch_line_sweep(b.pline, b.nline, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
+    launch_line_sweep(b.psw, b.nsw, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
                       gate, b.prec, hb, p.stream);
   }
 }
@@ -458,6 +475,7 @@ int a2d_init(int device) {
     HC(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) fail(-1, "no such HIP device");
     HC(hipSetDevice(device));
+    HC(hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, device));
     g_device = device;
   });
 }
@@ -583,6 +601,8 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.ntiles = cdiv(ni, stencil_shape(0).ti) * cdiv(nc, stencil_shape(0).tj);
     d.nvecb = cdiv(int(n), VEC_CHUNK);
     d.nlineb = cdiv(ni, LINES_PER_BLOCK);
+    d.nlinew = cdiv(ni, 8);
+    d.nlinerw = cdiv(ni / 2, 8);
     d.vstride = int(vec_stride(n, precision == 0 ? 8 : 4));
     d.pstride = int(plane_stride(n, precision == 0 ? 8 : 4));
     d.U = reinterpret_cast<double*>(b + L.U);
@@ -959,7 +979,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
             launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
           break;
         case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);
           break;
         case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
@@ -967,10 +987,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
-          launch_line_sweep(b.pline, b.nline, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);
           break;
         case 9:  // a coupled red-black half sweep (odd lines) of a later sweep
-          launch_line_sweep_rb(b.pline, b.nline, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep_rb(b.prb, b.nrb, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, hb, p.stream);
           break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");

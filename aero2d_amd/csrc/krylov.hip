@@ -251,7 +251,7 @@ template <class T>
 __global__ void __launch_bounds__(WAVE) k_krylov_begin(Plan pc, int stage, KnobsDev kn) {
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
-  const int nblk = cd.nlineb * (kn.precond_rb ? 2 : 1);
+  const int nblk = kn.precond_rb ? 2 * cd.nlinerw : cd.nlinew;
   const double nrm = sqrt(case_sum(cd, nblk, 0));
   if (threadIdx.x != 0) return;
   if (stage == 0) {

@@ -16,7 +16,11 @@ namespace a2d {
 #endif
 constexpr int VEC_CHUNK = A2D_VEC_CHUNK;  // cells per block of the vector kernels
 constexpr int CELL_BLOCK = 256;      // cells per block of the assembly kernel
-constexpr int LINES_PER_BLOCK = 32;  // i-lines per block of the line kernels
+constexpr int LINES_PER_BLOCK = 32;  // i-lines per block of the line factorisation
+// Line sweeps run 8 lanes per line, so 8 lines per wavefront, with blocks of
+// 32 lines (4 wavefronts) when the batch fills the device and of 8 lines
+// otherwise (a single 512-line case then spreads over 32 CUs instead of 8).
+constexpr int SWEEP_LINES_BIG = 32, SWEEP_LINES_SMALL = 8;
 
 struct KnobsDev {        // solver knobs the kernels read
   double cfl_explicit;   // EulerSolver.cfl (2.5)
@@ -61,14 +65,15 @@ void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s);
 // -R. The first sweep of an application keeps S0 src in the case's g0 buffer;
 // reuse_g0 = 1 (a later sweep, same src) reads it instead of src and S0.
 // want_norm: per-block partials of |xout|^2. hb (prec 1 only): couplings from
-// the FP16 copy. Plan pl: LINES_PER_BLOCK lines per block.
+// the FP16 copy. Plan pl: pl.lines lines per block. Norm partials: one per
+// 8-line group (wavefront), part[group].
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
                        int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s);
 // One half sweep of the red-black ordering: lines i = color (mod 2) only,
 // in place in xout (neighbours of the other colour are read from xout);
 // coupled = 0 drops the i-coupling (the very first half sweep); reuse_g0 as
 // above (the sweeps after the first). Norm partials go to
-// part[(color * nlineb + block)].
+// part[(color * nlinerw + group)]. Plan pl: the colour's lines only.
 void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int color, int coupled,
                           int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s);
 

@@ -516,10 +516,9 @@ __device__ __forceinline__ void unpack(const a2d_h4& a, float v[4]) {
 // HB (FP32 only): the couplings (slots 1-4) come from the FP16 copy.
 enum GSrc : int { GS_VEC = 0, GS_NEGR = 1, GS_G0 = 2 };
 
-template <class T, bool COUP, int GS, int PF, int PFB, bool HB>
-__global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
+template <class T, bool COUP, int GS, int PF, int PFB, bool HB, int SL>
+__global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
                                                int gate, int color) {
-  __shared__ double red[4];
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   if (gate && !cd.sc->active) return;
@@ -527,19 +526,12 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
   const int lb = blockIdx.x - plan.blk_prefix[e];
   // color < 0: every line, natural block layout; else line q of colour
   // `color` (i = 2q + color), whose blocks sit in column color * ni/2 + q
+  // (the plan then holds nlinerb blocks per case: the colour's lines only)
   const int nq = color < 0 ? ni : (ni >> 1);
-  if (lb * LINES_PER_BLOCK >= nq) {
-    // a whole block past the colour's lines (the plan is sized for every
-    // line): nothing to solve; its norm partial reads zero
-    if (want_norm && threadIdx.x == 0) {
-      const int slot = color < 0 ? lb : color * cd.nlineb + lb;
-      cd.part[(size_t)slot * (cd.kmax + 2)] = 0.0;
-    }
-    return;
-  }
+  if (lb * SL >= nq) return;   // guard: plans hold no such block
   const int r = threadIdx.x & 3;
   const int half = (threadIdx.x >> 2) & 1;
-  const int q = lb * LINES_PER_BLOCK + (threadIdx.x >> 3);
+  const int q = lb * SL + (threadIdx.x >> 3);
   const bool live = q < nq;
   const int qc = live ? q : nq - 1;     // dead lanes shadow a real line and never store
   const int ic = color < 0 ? qc : 2 * qc + color;
@@ -726,12 +718,12 @@ __global__ void __launch_bounds__(256) k_sweep(Plan plan, int src, int xnb, int 
     }
   }
   if (want_norm) {
+    // one partial per wavefront (8 lines), whatever the block size
     nrm = wave_sum(nrm);
-    if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = nrm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int slot = color < 0 ? lb : color * cd.nlineb + lb;
-      cd.part[(size_t)slot * (cd.kmax + 2)] = red[0] + red[1] + red[2] + red[3];
+    const int grp = lb * (SL / 8) + (int)(threadIdx.x / WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && grp * 8 < nq) {
+      const int slot = color < 0 ? grp : color * cd.nlinerw + grp;
+      cd.part[(size_t)slot * (cd.kmax + 2)] = nrm;
     }
   }
 }
@@ -745,36 +737,47 @@ template <> struct SweepDepth<double> { static constexpr int in = 4, out = 8; };
 // Kernel variant for a sweep: coupled or not; g from src (a vector or -R,
 // kept in g0) or from g0 (reuse_g0: a later sweep of the same application);
 // couplings from the FP16 copy (hb).
-template <class T, bool HB>
+template <class T, bool HB, int SL>
 void run_sweep_v(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
                  int want_norm, int gate, int color, hipStream_t s) {
   constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
-  const dim3 grid(nblocks), block(256);
+  const dim3 grid(nblocks), block(8 * SL);
   if (coupled && reuse_g0)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (coupled && src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (coupled)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else
-    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB, HB>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
                        xout, want_norm, gate, color);
+}
+
+template <int SL>
+void run_sweep_sl(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+                  int want_norm, int gate, int color, int prec, int hb, hipStream_t s) {
+  if (prec == 0)
+    run_sweep_v<double, false, SL>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else if (hb)
+    run_sweep_v<float, true, SL>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else
+    run_sweep_v<float, false, SL>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
 }
 
 void run_sweep(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
                int want_norm, int gate, int color, int prec, int hb, hipStream_t s) {
-  if (prec == 0)
-    run_sweep_v<double, false>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
-  else if (hb)
-    run_sweep_v<float, true>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  if (pl.lines == SWEEP_LINES_SMALL)
+    run_sweep_sl<SWEEP_LINES_SMALL>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color,
+                                    prec, hb, s);
   else
-    run_sweep_v<float, false>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+    run_sweep_sl<SWEEP_LINES_BIG>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color,
+                                  prec, hb, s);
 }
 
 }  // namespace
