@@ -409,37 +409,58 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   T cprev[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) cprev[k] = T(0);
-  for (int s = 0; s < m; ++s) {
-    if (s < steps) {
-      const size_t cell = (size_t)(j0 + dj * s) * ni + col;
+  // The five blocks of the next FPF steps are loaded while a step computes
+  // (rows past the half's end are clamped to its last row and not used);
+  // one step ahead in FP64, where two would spill registers.
+  constexpr int FPF = std::is_same<T, float>::value ? 2 : 1;
+  T rb[FPF][5][16];
+  auto fload = [&](int k, int s) {
+    const size_t cell = (size_t)(j0 + dj * max(0, min(s, steps - 1))) * ni + col;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) load_blk(Bc, b, cell, rb[k][b]);
+  };
+#pragma unroll
+  for (int k = 0; k < FPF; ++k) fload(k, k);
+  for (int s0 = 0; s0 < m; s0 += FPF) {
+#pragma unroll
+    for (int k = 0; k < FPF; ++k) {
+      const int s = s0 + k;
+      const bool valid = s < steps;
+      const size_t cell = (size_t)(j0 + dj * min(s, steps - 1)) * ni + col;
       T D[16], A[16], C[16], Bm[16], Bp[16];
-      load_blk(Bc, 0, cell, D);
-      load_blk(Bc, 2, cell, A);
-      load_blk(Bc, 1, cell, C);
-      load_blk(Bc, 3, cell, Bm);
-      load_blk(Bc, 4, cell, Bp);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        D[i] = rb[k][0][i];
+        C[i] = rb[k][1][i];
+        A[i] = rb[k][2][i];
+        Bm[i] = rb[k][3][i];
+        Bp[i] = rb[k][4][i];
+      }
+      fload(k, s + FPF);
       T L[16], F[16], t[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        L[k] = half ? C[k] : A[k];    // couples to the row already eliminated
-        F[k] = half ? A[k] : C[k];    // couples to the row eliminated next
+      for (int i = 0; i < 16; ++i) {
+        L[i] = half ? C[i] : A[i];    // couples to the row already eliminated
+        F[i] = half ? A[i] : C[i];    // couples to the row eliminated next
       }
       mul4<T>(L, cprev, t);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) D[k] -= t[k];
-      T inv[16], P[16];
+      for (int i = 0; i < 16; ++i) D[i] -= t[i];
+      T inv[16], P[16], cn[16];
       inv4<T>(D, inv);
-      mul4<T>(inv, F, cprev);
+      mul4<T>(inv, F, cn);
       mul4<T>(inv, L, P);
-      if (live) {
+      if (live && valid) {
         store_blk(B, 0, cell, inv);
-        keep(1, cell, cprev);
+        keep(1, cell, cn);
         keep(2, cell, P);
         mul4<T>(inv, Bm, t);
         keep(3, cell, t);
         mul4<T>(inv, Bp, t);
         keep(4, cell, t);
       }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) cprev[i] = valid ? cn[i] : cprev[i];
     }
   }
   T other[16];
