@@ -45,6 +45,7 @@ struct CaseScal {
   double bnorm;        // |b|
   double gres;         // GMRES residual estimate of the current iterate
   double vn0;          // |w| before orthogonalisation
+  double vsig;         // FP16 basis: power-of-two storage scale of V_0
   int run;             // the Krylov cycle runs (|M(b - A x0)| > ptol)
   int active;          // still taking Arnoldi steps
   int kdone;           // Arnoldi steps completed
@@ -98,6 +99,8 @@ struct CaseDev {
   void* hb;             // mixed form: _Float16 [4][pstride][16], slots 1-4 of blk in FP16
   void* xa;             // T [4][nc][ni] sweep ping-pong
   void* xb;
+  void* p;              // mixed form, FP16 basis: FP32 preconditioner output [4][nc][ni]
+  void* vc;             // and the FP32 copy of the newest (rounded) basis vector
   double* part;         // [npart][kmax + 2] reduction partials
   CaseScal* sc;
 };
@@ -187,7 +190,9 @@ __device__ __forceinline__ double case_max(const CaseDev& cd, int nblk, int col)
 }
 
 // Named per-case vectors (ids >= 0 select raw Krylov basis vector V_id).
-enum VecId : int { VEC_X0 = -1, VEC_W = -2, VEC_D = -3, VEC_XA = -4, VEC_XB = -5, VEC_NEG_R = -6 };
+enum VecId : int {
+  VEC_X0 = -1, VEC_W = -2, VEC_D = -3, VEC_XA = -4, VEC_XB = -5, VEC_NEG_R = -6, VEC_P = -7, VEC_VC = -8
+};
 
 template <class T>
 __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id);
@@ -202,6 +207,8 @@ __device__ __forceinline__ T* vec_ptr(const CaseDev& cd, int id) {
     case VEC_D: return tp<T>(cd.d);
     case VEC_XA: return tp<T>(cd.xa);
     case VEC_XB: return tp<T>(cd.xb);
+    case VEC_P: return tp<T>(cd.p);
+    case VEC_VC: return tp<T>(cd.vc);
     default: return tp<T>(cd.V) + (size_t)id * cd.vstride;
   }
 }

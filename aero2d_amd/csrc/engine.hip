@@ -59,6 +59,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
   d.precond_rb = k.precond_order == 1;
   d.positivity = k.positivity;
   d.precond_half = k.precond_half;
+  d.basis_half = k.basis_half;
   return d;
 }
 
@@ -66,7 +67,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
 struct Layout {
   size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
   size_t Ut, volt, Sit, Sjt, xft, yft;
-  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, part, sc;
+  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, pv, vc, part, sc;
   size_t total;
   int npart;
 };
@@ -125,6 +126,10 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   L.g0 = take(4 * n * t);
   L.xa = take(4 * n * t);
   L.xb = take(4 * n * t);
+  if (prec == 1) {
+    L.pv = take(4 * n * 4);
+    L.vc = take(4 * n * 4);
+  }
   L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, 8) + 2, 1});
   L.part = take(size_t(L.npart) * (kmax + 2) * 8);
   L.sc = take(sizeof(CaseScal));
@@ -199,6 +204,7 @@ void default_knobs(a2d_knobs* k) {
   k->precond_order = 0;
   k->positivity = 0;
   k->precond_half = 0;
+  k->basis_half = 0;
 }
 
 Pool& get_pool(int64_t id) {
@@ -373,20 +379,26 @@ void enq_newton(Pool& p, BatchPlan& b) {
   launch_dcirc(b.pc, VEC_X0, 0, prec, s);
   for (auto& g : b.sten)
     launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 1, -1, 1, 0, prec, s);
-  enq_precond(p, b, VEC_W, 0, 1, 0);
+  // FP16 basis (mixed form): the preconditioner writes the FP32 vector P, the
+  // basis vectors are stored from it in FP16, and the operator reads their
+  // rounded values in FP32 from vc
+  const int bh = prec == 1 && p.knobs.basis_half;
+  enq_precond(p, b, VEC_W, bh ? VEC_P : 0, 1, 0);
   launch_krylov_begin(b.pc, 1, kn, prec, s);
+  if (bh) launch_store_v0(b.pvec, b.nvec, s);
   const int K = std::min(p.knobs.gmres_restart, KMAX);
   for (int k = 0; k < K; ++k) {
-    launch_dcirc(b.pc, k, 1, prec, s);
-    for (auto& g : b.sten) launch_jvp(g.p, g.shape, g.nblocks, kn, k, VEC_W, 0, k, 1, 1, prec, s);
-    enq_precond(p, b, VEC_W, k + 1, 0, 1);
-    launch_dots(b.pvec, b.nvec, k, k + 1, 1, prec, s);
-    launch_reduce_dots(b.pc, k, s);
-    launch_update(b.pvec, b.nvec, k, k + 1, 1, prec, s);
+    const int vin = bh ? VEC_VC : k, wout = bh ? VEC_P : k + 1;
+    launch_dcirc(b.pc, vin, 1, prec, s);
+    for (auto& g : b.sten) launch_jvp(g.p, g.shape, g.nblocks, kn, vin, VEC_W, 0, k, 1, 1, prec, s);
+    enq_precond(p, b, VEC_W, wout, 0, 1);
+    launch_dots(b.pvec, b.nvec, k, wout, 1, prec, bh, s);
+    launch_reduce_dots(b.pc, k, bh, s);
+    launch_update(b.pvec, b.nvec, k, wout, 1, prec, bh, s);
     launch_normalize(b.pc, k, kn, prec, s);
   }
   launch_lstsq(b.pc, s);
-  launch_dupdate(b.pvec, b.nvec, prec, s);
+  launch_dupdate(b.pvec, b.nvec, prec, bh, s);
   launch_limiter(b.pc, kn, s);
   if (kn.positivity) launch_positivity(b.pvec, b.nvec, b.pc, s);
   launch_apply(b.pvec, b.nvec, s);
@@ -516,6 +528,7 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->precond_order != 0 && k->precond_order != 1) fail(-6, "precond_order must be 0 or 1");
     if (k->positivity != 0 && k->positivity != 1) fail(-6, "positivity must be 0 or 1");
     if (k->precond_half != 0 && k->precond_half != 1) fail(-6, "precond_half must be 0 or 1");
+    if (k->basis_half != 0 && k->basis_half != 1) fail(-6, "basis_half must be 0 or 1");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -647,6 +660,8 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.hb = precision == 1 ? b + L.hb : nullptr;
     d.xa = b + L.xa;
     d.xb = b + L.xb;
+    d.p = precision == 1 ? b + L.pv : nullptr;
+    d.vc = precision == 1 ? b + L.vc : nullptr;
     d.part = reinterpret_cast<double*>(b + L.part);
     d.sc = reinterpret_cast<CaseScal*>(b + L.sc);
 
@@ -969,6 +984,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     const KnobsDev kn = to_dev(p.knobs);
     const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));
     const int hb = b.prec == 1 && p.knobs.precond_half;
+    const int bh = b.prec == 1 && p.knobs.basis_half;
     auto launch = [&] {
       switch (which) {
         case 0:  // as launched in a Newton step: with the local time step and U copy
@@ -981,10 +997,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
           launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);
           break;
-        case 3: launch_dots(b.pvec, b.nvec, k, k + 1, 0, b.prec, p.stream); break;
+        case 3: launch_dots(b.pvec, b.nvec, k, bh ? VEC_P : k + 1, 0, b.prec, bh, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
         case 5: launch_factor(b.pline, b.nline, b.prec, p.stream); break;
-        case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, p.stream); break;
+        case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, bh, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
           launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);

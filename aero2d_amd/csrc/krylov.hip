@@ -5,6 +5,15 @@
 // kept in the case scalars. Normalising is then a scalar update, the JVP
 // applies s_j to its output, and no separate scaling pass touches HBM.
 //
+// FP16 basis (mixed form, knob basis_half; template flag BH): the
+// preconditioner writes its output to an FP32 work vector P; V_raw_j is stored
+// in FP16 as sigma_j times the orthogonalised vector, sigma_j a power of two
+// that brings the entries to order one (norm sqrt(4n)), and s_j = 1/|V_raw_j|
+// is taken from the rounded values themselves. The update also writes the
+// rounded vector in FP32 (vc), which the next Jacobian-vector product reads,
+// so the Arnoldi relation holds for the stored basis. Dot products and the
+// update accumulate in FP64 as before.
+//
 // Reductions are two-level and case-local: every block writes its partial
 // sums to the case's partial array, then one small kernel per case adds them
 // in block order. Results never depend on which other cases share a launch.
@@ -32,9 +41,27 @@ template <class T> __device__ __forceinline__ double eps_of() {
   return std::is_same<T, double>::value ? DBL_EPSILON : double(FLT_EPSILON);
 }
 
+// Power-of-two storage scale for an FP16 basis vector of (estimated) norm nrm
+// over 4n entries: the scaled norm is ~sqrt(4n) / 64, so typical entries are
+// ~1/64 (FP16 normals reach down to 6.1e-5) and an entry overflows only if the
+// true norm exceeds the estimate ~5000-fold for n = 2^17. Conversion also
+// saturates at +-65504 (NaN kept).
+__device__ __forceinline__ double basis_sig(double nrm, int n) {
+  if (!(nrm > 0.0) || !isfinite(nrm)) return 1.0;
+  return exp2(rint(log2(sqrt(4.0 * n) / (64.0 * nrm))));
+}
+__device__ __forceinline__ _Float16 half_sat(double v) {
+  const double c = fmin(fmax(v, -65504.0), 65504.0);
+  return (_Float16)float(v == v ? c : v);
+}
+
+// Element type of the stored basis.
+template <class T, bool BH> using BasisT = typename std::conditional<BH, _Float16, T>::type;
+
 // h_j = <V_j, w> partials for j = 0..k (times s_j at reduction), and <w, w>.
-template <class T>
+template <class T, bool BH>
 __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gate) {
+  using VT = BasisT<T, BH>;
   __shared__ double sred[KMAX + 2][NT / WAVE];
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
@@ -43,7 +70,7 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
   const int lb = blockIdx.x - plan.blk_prefix[e];
   const int c0 = lb * VEC_CHUNK + threadIdx.x;
   const gptr<const T> w = vec_gptr<T>(cd, wvec);
-  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const gptr<const VT> Vb = gp((const VT*)cd.V);
   const size_t vs = (size_t)cd.vstride;
   // indices are clamped into the case and out-of-range cells contribute zero
   // through selects, so every load is unconditional (no per-element branches)
@@ -72,11 +99,11 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
     for (int q = 0; q < JB; ++q) {
       acc[q] = 0.0;
       const int j = min(j0 + q, k);
-      const gptr<const T> V = Vb + (size_t)j * vs;
+      const gptr<const VT> V = Vb + (size_t)j * vs;
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int m = 0; m < PER; ++m) acc[q] += double(V[(size_t)v * n + cc[m]]) * double(wv[v][m]);
+        for (int m = 0; m < PER; ++m) acc[q] += double(T(V[(size_t)v * n + cc[m]])) * double(wv[v][m]);
     }
 #pragma unroll
     for (int q = 0; q < JB; ++q) {
@@ -103,8 +130,12 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
 }
 
 // One block per case, a wavefront per column: H[j][k] = s_j * sum_b part[b][j];
-// vn0 = |w|.
-__global__ void __launch_bounds__(NT) k_reduce_dots(Plan pc, int k) {
+// vn0 = |w|. FP16 basis (bh): also the storage scale of V_{k+1} from the
+// norm the orthogonalised vector will have, |w|^2 - sum_j H[j][k]^2
+// (clamped below at (1e-3 |w|)^2: the estimate's error from the basis's
+// departure from orthonormality is far smaller than that, so the true norm
+// is at most about twice the estimate and no stored entry can overflow).
+__global__ void __launch_bounds__(NT) k_reduce_dots(Plan pc, int k, int bh) {
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
   if (!sc->active) return;
@@ -116,12 +147,22 @@ __global__ void __launch_bounds__(NT) k_reduce_dots(Plan pc, int k) {
       else sc->vn0 = sqrt(s);
     }
   }
+  if (!bh) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double w2 = sc->vn0 * sc->vn0;
+    double h2 = 0.0;
+    for (int j = 0; j <= k; ++j) h2 += sc->H[j][k] * sc->H[j][k];
+    sc->vsig = basis_sig(sqrt(fmax(w2 - h2, 1e-6 * w2)), cd.n);
+  }
 }
 
 // w <- w - sum_j H[j][k] s_j V_j (classical Gram-Schmidt), in place in V_{k+1};
-// partial |w|^2 per block.
-template <class T>
+// partial |w|^2 per block. BH: w is read from wvec, V_raw_{k+1} = sigma w is
+// stored in FP16 and in FP32 (vc), and the partials are of the stored values.
+template <class T, bool BH>
 __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int gate) {
+  using VT = BasisT<T, BH>;
   __shared__ double sred[NT / WAVE];
   __shared__ double coef[KMAX + 1];
   const int e = plan_entry(plan, blockIdx.x);
@@ -134,7 +175,7 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   for (int j = threadIdx.x; j <= k; j += NT) coef[j] = sc->H[j][k] * sc->s[j];
   __syncthreads();
   const gptr<T> w = vec_gptr<T>(cd, wvec);
-  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const gptr<const VT> Vb = gp((const VT*)cd.V);
   const size_t vs = (size_t)cd.vstride;
   // all PER cells of a thread at once, JB basis vectors per pass, so that
   // JB x 4 x PER loads are in flight; indices are clamped into the case and
@@ -155,7 +196,7 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   constexpr int JB = std::is_same<T, double>::value ? A2D_UPDATE_JB64 : A2D_UPDATE_JB32;
   int j = 0;
   for (; j + JB - 1 <= k; j += JB) {
-    T vv[JB][4][PER];
+    VT vv[JB][4][PER];
 #pragma unroll
     for (int q = 0; q < JB; ++q)
 #pragma unroll
@@ -167,24 +208,42 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int m = 0; m < PER; ++m) acc[v][m] -= coef[j + q] * double(vv[q][v][m]);
+        for (int m = 0; m < PER; ++m) acc[v][m] -= coef[j + q] * double(T(vv[q][v][m]));
   }
   for (; j <= k; ++j) {
     const double cj = coef[j];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int m = 0; m < PER; ++m) acc[v][m] -= cj * double(Vb[(size_t)j * vs + (size_t)v * n + cc[m]]);
+      for (int m = 0; m < PER; ++m) acc[v][m] -= cj * double(T(Vb[(size_t)j * vs + (size_t)v * n + cc[m]]));
   }
   double nrm = 0.0;
+  if constexpr (BH) {
+    const double sig = sc->vsig;
+    const gptr<_Float16> Vo = gp((_Float16*)cd.V + (size_t)(k + 1) * vs);
+    const gptr<T> vc = vec_gptr<T>(cd, VEC_VC);
 #pragma unroll
-  for (int m = 0; m < PER; ++m) {
-    if (!in[m]) continue;
+    for (int m = 0; m < PER; ++m) {
+      if (!in[m]) continue;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const T r = T(acc[v][m]);
-      w[(size_t)v * n + cc[m]] = r;
-      nrm += double(r) * double(r);
+      for (int v = 0; v < 4; ++v) {
+        const _Float16 h = half_sat(sig * acc[v][m]);
+        const T r = T(h);
+        Vo[(size_t)v * n + cc[m]] = h;
+        vc[(size_t)v * n + cc[m]] = r;
+        nrm += double(r) * double(r);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      if (!in[m]) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T r = T(acc[v][m]);
+        w[(size_t)v * n + cc[m]] = r;
+        nrm += double(r) * double(r);
+      }
     }
   }
   nrm = wave_sum(nrm);
@@ -195,6 +254,31 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
 #pragma unroll
     for (int q = 0; q < NT / WAVE; ++q) s += sred[q];
     cd.part[(size_t)lb * (cd.kmax + 2)] = s;
+  }
+}
+
+// FP16 basis: V_raw_0 = vsig * P in FP16, and its rounded values in FP32 (vc).
+__global__ void __launch_bounds__(NT) k_store_v0(Plan plan) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const CaseScal* sc = cd.sc;
+  if (!sc->run) return;
+  const int n = cd.n;
+  const int lb = blockIdx.x - plan.blk_prefix[e];
+  const double sig = sc->vsig;
+  const gptr<const float> pv = vec_gptr<float>(cd, VEC_P);
+  const gptr<float> vc = vec_gptr<float>(cd, VEC_VC);
+  const gptr<_Float16> V0 = gp((_Float16*)cd.V);
+  for (int m = 0; m < PER; ++m) {
+    const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
+    if (c >= n) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const size_t o = (size_t)v * n + c;
+      const _Float16 h = half_sat(sig * double(pv[o]));
+      V0[o] = h;
+      vc[o] = float(h);
+    }
   }
 }
 
@@ -217,8 +301,11 @@ __global__ void __launch_bounds__(WAVE) k_normalize(Plan pc, int k, KnobsDev kn)
   }
   __syncthreads();
   if (lane == 0) {
-    const bool brk = !(nrm > eps_of<T>() * sc->vn0);
-    const double hk1 = brk ? 0.0 : nrm;
+    // FP16 basis: nrm is the norm of the stored sigma-scaled vector
+    const double sig = (std::is_same<T, float>::value && kn.basis_half) ? sc->vsig : 1.0;
+    const double hn = nrm / sig;
+    const bool brk = !(hn > eps_of<T>() * sc->vn0);
+    const double hk1 = brk ? 0.0 : hn;
     sc->s[k + 1] = brk ? 0.0 : 1.0 / nrm;
     // apply the previous rotations to column k; `a` carries H[i][k]
     double a = h[0];
@@ -265,7 +352,10 @@ __global__ void __launch_bounds__(WAVE) k_krylov_begin(Plan pc, int stage, Knobs
   const double beta = (nrm > thresh) ? nrm : 0.0;
   const double ptol = sc->mbnorm * fmin(1.0, kn.gtol);
   sc->beta = beta;
-  sc->s[0] = beta > 0 ? 1.0 / beta : 0.0;
+  // FP16 basis: V_raw_0 = vsig * P is stored by k_store_v0
+  const double vsig = (std::is_same<T, float>::value && kn.basis_half) ? basis_sig(beta, cd.n) : 1.0;
+  sc->vsig = vsig;
+  sc->s[0] = beta > 0 ? 1.0 / (vsig * beta) : 0.0;
   sc->g[0] = beta;
   sc->gres = beta;
   sc->run = beta > ptol;
@@ -300,8 +390,9 @@ __global__ void __launch_bounds__(WAVE) k_lstsq(Plan pc) {
 // dU = x0 + sum_j y_j s_j V_j (stored FP64), with the step-limiter partials:
 // max over cells of |rho_new - rho|/rho and |p_new - p|/p for U + dU, and the
 // count of non-finite entries of dU (aero2d/implicit.py:186-189).
-template <class T>
+template <class T, bool BH>
 __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
+  using VT = BasisT<T, BH>;
   __shared__ double coef[KMAX];
   __shared__ double smax[NT / WAVE], scnt[NT / WAVE];
   const int e = plan_entry(plan, blockIdx.x);
@@ -312,7 +403,7 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   for (int j = threadIdx.x; j < kd; j += NT) coef[j] = sc->y[j] * sc->s[j];
   __syncthreads();
   const gptr<const T> x0 = gp(ctp<T>(cd.x0));
-  const gptr<const T> Vb = gp(ctp<T>(cd.V));
+  const gptr<const VT> Vb = gp((const VT*)cd.V);
   const size_t vs = (size_t)cd.vstride;
   const gptr<const double> U = gp(cd.U);
   const gptr<double> dU = gp(cd.dU);
@@ -325,7 +416,7 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
     for (int v = 0; v < 4; ++v) du[v] = double(x0[(size_t)v * n + c]);
     int j = 0;
     for (; j + 4 <= kd; j += 4) {
-      T vv[4][4];
+      VT vv[4][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -333,12 +424,12 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) du[v] += coef[j + q] * double(vv[q][v]);
+        for (int v = 0; v < 4; ++v) du[v] += coef[j + q] * double(T(vv[q][v]));
     }
     for (; j < kd; ++j) {
       const double cj = coef[j];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) du[v] += cj * double(Vb[(size_t)j * vs + (size_t)v * n + c]);
+      for (int v = 0; v < 4; ++v) du[v] += cj * double(T(Vb[(size_t)j * vs + (size_t)v * n + c]));
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -504,16 +595,18 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_dots<double>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
-  else hipLaunchKernelGGL(k_dots<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL((k_dots<double, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else if (bh) hipLaunchKernelGGL((k_dots<float, true>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else hipLaunchKernelGGL((k_dots<float, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
 }
-void launch_reduce_dots(const Plan& pc, int k, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(NT), 0, s, pc, k);
+void launch_reduce_dots(const Plan& pc, int k, int bh, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(NT), 0, s, pc, k, bh);
 }
-void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_update<double>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
-  else hipLaunchKernelGGL(k_update<float>, dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL((k_update<double, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else if (bh) hipLaunchKernelGGL((k_update<float, true>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+  else hipLaunchKernelGGL((k_update<float, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
 }
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s) {
   if (prec == 0) hipLaunchKernelGGL(k_normalize<double>, dim3(pc.nb), dim3(WAVE), 0, s, pc, k, kn);
@@ -528,9 +621,13 @@ void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec
 void launch_lstsq(const Plan& pc, hipStream_t s) {
   hipLaunchKernelGGL(k_lstsq, dim3(pc.nb), dim3(WAVE), 0, s, pc);
 }
-void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_dupdate<double>, dim3(nblocks), dim3(NT), 0, s, pv);
-  else hipLaunchKernelGGL(k_dupdate<float>, dim3(nblocks), dim3(NT), 0, s, pv);
+void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL((k_dupdate<double, false>), dim3(nblocks), dim3(NT), 0, s, pv);
+  else if (bh) hipLaunchKernelGGL((k_dupdate<float, true>), dim3(nblocks), dim3(NT), 0, s, pv);
+  else hipLaunchKernelGGL((k_dupdate<float, false>), dim3(nblocks), dim3(NT), 0, s, pv);
+}
+void launch_store_v0(const Plan& pv, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_store_v0, dim3(nblocks), dim3(NT), 0, s, pv);
 }
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s) {
   hipLaunchKernelGGL(k_limiter, dim3(pc.nb), dim3(WAVE), 0, s, pc, kn);

@@ -34,6 +34,7 @@ struct KnobsDev {        // solver knobs the kernels read
   int precond_rb;        // 1: red-black line ordering (else line Jacobi)
   int positivity;        // 1: reject a step that leaves rho <= 0 or p <= 0 somewhere
   int precond_half;      // 1: mixed-form sweeps read the FP16 copy of the couplings
+  int basis_half;        // 1: mixed-form Krylov basis stored in FP16
 };
 
 // ---- stencil.hip ------------------------------------------------------------
@@ -78,13 +79,17 @@ void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int co
                           int reuse_g0, int want_norm, int gate, int prec, int hb, hipStream_t s);
 
 // ---- krylov.hip -----------------------------------------------------------------
-void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);
-void launch_reduce_dots(const Plan& pc, int k, hipStream_t s);
-void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, hipStream_t s);
+// bh (prec 1 only): the basis is stored in FP16 (wvec is then an FP32 work
+// vector that the update reads; it writes V_{k+1} and vc).
+void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s);
+void launch_reduce_dots(const Plan& pc, int k, int bh, hipStream_t s);
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s);
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_lstsq(const Plan& pc, hipStream_t s);
-void launch_dupdate(const Plan& pv, int nblocks, int prec, hipStream_t s);
+void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, hipStream_t s);
+// FP16 basis: V_raw_0 = vsig * P (and vc) after the V_0 sweep.
+void launch_store_v0(const Plan& pv, int nblocks, hipStream_t s);
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s);
 // Positivity check of U + omega dU (knob `positivity`): per-block counts of
 // cells with rho <= 0 or p <= 0, then a per-case pass that rejects the step

@@ -58,6 +58,10 @@ typedef struct a2d_knobs {
                          bytes of the preconditioner's dominant traffic; 0 = FP32.
                          The factorisation writes both copies, so the knob may
                          change between factorisation and application. */
+  int basis_half;     /* mixed form only: 1 = the Krylov basis vectors are stored
+                         in FP16 (each scaled by a power of two to entries of
+                         order one; Gram-Schmidt arithmetic stays FP32/FP64),
+                         half the bytes of the Gram-Schmidt passes; 0 = FP32 */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */

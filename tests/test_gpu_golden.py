@@ -104,3 +104,26 @@ def test_mixed_form_matches_fp64_on_the_same_solver_path():
     assert np.max(np.abs(rmx.cl - r64.cl)) < 1e-7
     assert np.max(np.abs(rmx.cd - r64.cd)) < 1e-8
     assert np.max(np.abs(rmx.cm - r64.cm)) < 1e-7
+
+
+def test_fp16_storage_options_reach_the_same_steady_states():
+    """Subsonic cases (one steady state each): the mixed form with the FP16
+    coupling copy and the FP16 Krylov basis against FP32 for both. The
+    storage only changes the inner iterations, so both converge to
+    rn/r0 <= 1e-10 and agree to |dCl|, |dCm| <= 1e-8, |dCd| <= 1e-9, in about
+    the same number of Newton steps."""
+    from aero2d_amd.batch import default_knobs, solve_batch
+    from aero2d_amd.workloads import config2
+    cases = [c for c in config2(16, 128, 64) if c.mach < 0.6][:6]
+    r16 = solve_batch(cases, "mixed", tol=1e-10, n_iter=300,
+                      knobs=default_knobs("mixed", precond_half=1, basis_half=1))
+    r32 = solve_batch(cases, "mixed", tol=1e-10, n_iter=300,
+                      knobs=default_knobs("mixed", precond_half=0, basis_half=0))
+    print("iterations", r16.iterations, r32.iterations, "arnoldi", r16.arnoldi.mean(), r32.arnoldi.mean())
+    print("dcl", np.abs(r16.cl - r32.cl), "dcd", np.abs(r16.cd - r32.cd))
+    assert len(cases) >= 4
+    assert np.all(r16.converged) and np.all(r32.converged)
+    assert np.max(np.abs(r16.cl - r32.cl)) < 1e-8
+    assert np.max(np.abs(r16.cd - r32.cd)) < 1e-9
+    assert np.max(np.abs(r16.cm - r32.cm)) < 1e-8
+    assert r16.iterations.sum() <= 1.15 * r32.iterations.sum()
