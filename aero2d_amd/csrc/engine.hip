@@ -24,6 +24,13 @@ namespace {
 thread_local std::string g_err;
 int g_device = -1;
 int g_num_cu = 256;   // compute units of the device (sweep block sizing)
+// 32-line sweep blocks from this many per 100 CUs, else 8-line blocks.
+// Measured crossover (preconditioner application, 512x256 mixed / 256x128
+// FP64): 8-line blocks win at 128 32-line blocks (16 cases, -15..-22 %),
+// 32-line blocks from 192 on (24 cases and more, -3..-12 %).
+#ifndef A2D_SWEEP_BIG_PCT
+#define A2D_SWEEP_BIG_PCT 60
+#endif
 std::recursive_mutex g_mu;
 
 struct Fail {
@@ -265,8 +272,9 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
     big_sw += cdiv(p.cases[s].dev.ni, SWEEP_LINES_BIG);
     big_rb += cdiv(p.cases[s].dev.ni / 2, SWEEP_LINES_BIG);
   }
-  const int lsw = big_sw >= g_num_cu ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
-  const int lrb = big_rb >= g_num_cu ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
+  const int big_min = g_num_cu * A2D_SWEEP_BIG_PCT / 100;
+  const int lsw = big_sw >= big_min ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
+  const int lrb = big_rb >= big_min ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
   auto psw = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni, lsw); });
   auto prb = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni / 2, lrb); });
   std::map<int, std::vector<int>> groups;

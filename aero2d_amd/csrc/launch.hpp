@@ -18,8 +18,9 @@ constexpr int VEC_CHUNK = A2D_VEC_CHUNK;  // cells per block of the vector kerne
 constexpr int CELL_BLOCK = 256;      // cells per block of the assembly kernel
 constexpr int LINES_PER_BLOCK = 32;  // i-lines per block of the line factorisation
 // Line sweeps run 8 lanes per line, so 8 lines per wavefront, with blocks of
-// 32 lines (4 wavefronts) when the batch fills the device and of 8 lines
-// otherwise (a single 512-line case then spreads over 32 CUs instead of 8).
+// 32 lines (4 wavefronts) when the batch nearly fills the device (60 % of
+// the CUs) and of 8 lines otherwise (a single 512-line case then spreads
+// over 32 CUs instead of 8).
 constexpr int SWEEP_LINES_BIG = 32, SWEEP_LINES_SMALL = 8;
 
 struct KnobsDev {        // solver knobs the kernels read
