@@ -95,3 +95,19 @@ def test_positivity_check_keeps_states_physical():
         pool.close()
     print("rejected steps", rejected)
     assert rejected >= 1
+
+
+@pytest.mark.parametrize("name", ["precond_order", "positivity", "precond_half", "basis_half"])
+def test_invalid_switch_knob_is_refused(name):
+    """Switch knobs take 0 or 1; any other value is refused with an error and
+    the pool keeps its knobs."""
+    pool = _abi.Pool(0, default_knobs("mixed"))
+    try:
+        k = pool.knobs()
+        before = getattr(k, name)
+        setattr(k, name, 2)
+        with pytest.raises(_abi.HipError):
+            pool.set_knobs(k)
+        assert getattr(pool.knobs(), name) == before
+    finally:
+        pool.close()

@@ -66,6 +66,18 @@ def test_library_exports_every_declared_symbol():
     _abi.load()
 
 
+def test_knobs_struct_matches_the_header():
+    """The ctypes mirror of a2d_knobs has the header's fields, types and order
+    (a field added on one side only would shift every later knob)."""
+    hdr = open(os.path.join(ROOT, "include", "aero2d_hip.h")).read()
+    body = re.search(r"typedef struct a2d_knobs \{(.*?)\} a2d_knobs;", hdr, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"^\s*(double|int)\s+(\w+);", body, re.M)
+    ctypes_of = {"double": "c_double", "int": "c_int"}
+    mirror = [(n, t.__name__) for n, t in _abi.Knobs._fields_]
+    assert mirror == [(n, ctypes_of[t]) for t, n in fields]
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "aero2d_amd")
     for f in os.listdir(pkg):
