@@ -8,8 +8,10 @@
 //   k_factor    block elimination along j for every i-line, two-sided: the
 //               reference's top-down Thomas recurrence (aero2d/implicit.py:112-128)
 //               on the wall half, the mirror recurrence on the far-field half,
-//               joined at the middle row. Two threads per line, 4x4 blocks in
-//               registers.
+//               joined at the middle row. Eight lanes per line (a quad per
+//               half, lane r = row r of every 4x4 block): products by DPP
+//               broadcasts in the quad, the inverse by Gauss-Jordan with
+//               partial pivoting across the quad.
 //               The same pass also stores S0 Bim and S0 Bip (S0 = inverse of
 //               the eliminated diagonal) over Bim and Bip, so that a sweep
 //               forms g = S0 rhs without a second dependent product.
@@ -135,59 +137,6 @@ __device__ __forceinline__ void far_col(const T q[4], T sx, T sy, const Inf<T>& 
 
 constexpr int NT = 256;
 
-// 4x4 inverse by Gauss-Jordan elimination with partial pivoting; all indices
-// compile-time so the blocks stay in registers.
-template <class T>
-__device__ __forceinline__ void inv4(const T a_in[16], T out[16]) {
-  T a[16], b[16];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    a[m] = a_in[m];
-    b[m] = (m % 5 == 0) ? T(1) : T(0);
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    // pivot search in column c among rows c..3
-    int p = c;
-    T best = fabs(a[c * 4 + c]);
-#pragma unroll
-    for (int r = c + 1; r < 4; ++r) {
-      const T v = fabs(a[r * 4 + c]);
-      if (v > best) { best = v; p = r; }
-    }
-#pragma unroll
-    for (int r = c + 1; r < 4; ++r) {
-      const bool sw = (p == r);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const T ta = a[c * 4 + k], tb = b[c * 4 + k];
-        a[c * 4 + k] = sw ? a[r * 4 + k] : ta;
-        a[r * 4 + k] = sw ? ta : a[r * 4 + k];
-        b[c * 4 + k] = sw ? b[r * 4 + k] : tb;
-        b[r * 4 + k] = sw ? tb : b[r * 4 + k];
-      }
-    }
-    const T piv = T(1) / a[c * 4 + c];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      a[c * 4 + k] *= piv;
-      b[c * 4 + k] *= piv;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r == c) continue;
-      const T f = a[r * 4 + c];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        a[r * 4 + k] -= f * a[c * 4 + k];
-        b[r * 4 + k] -= f * b[c * 4 + k];
-      }
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 16; ++m) out[m] = b[m];
-}
-
 // Row-wise access to the block storage (layout in the file header).
 template <class T>
 struct Blk {
@@ -233,38 +182,11 @@ __device__ __forceinline__ gptr<const _Float16> hrow(gptr<const _Float16> H, siz
                                                      int r, size_t cell) {
   return H + ((size_t)(slot - 1) * ps + cell) * 16 + r * 4;
 }
-__device__ __forceinline__ void store_blk_h(gptr<_Float16> H, size_t ps, int slot, size_t cell,
-                                            const float a[16]) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    a2d_h4 t;
-    t.x = to_half_sat(a[4 * r]); t.y = to_half_sat(a[4 * r + 1]);
-    t.z = to_half_sat(a[4 * r + 2]); t.w = to_half_sat(a[4 * r + 3]);
-    *(gptr<a2d_h4>)(H + ((size_t)(slot - 1) * ps + cell) * 16 + r * 4) = t;
-  }
-}
 
-template <class T>
-__device__ __forceinline__ void load_blk(const Blk<const T>& B, int slot, size_t cell, T a[16]) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r) load_row(B.row(slot, r, cell), a + 4 * r);
-}
 template <class T>
 __device__ __forceinline__ void store_blk(const Blk<T>& B, int slot, size_t cell, const T a[16]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) store_row(B.row(slot, r, cell), a + 4 * r);
-}
-template <class T>
-__device__ __forceinline__ void mul4(const T a[16], const T b[16], T out[16]) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      T s = T(0);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) s += a[r * 4 + k] * b[k * 4 + c];
-      out[r * 4 + c] = s;
-    }
 }
 
 template <class T>
@@ -376,23 +298,104 @@ __global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
   store_blk(B, 0, bidx, Dm);
 }
 
-// Two-sided (twisted) block elimination of every j-line, two threads per line.
-// With m = nc / 2, the top thread eliminates j = 0 .. m-1 downward from the
-// wall exactly as aero2d/implicit.py:112-128 does (den = D - A c_prev,
-// inv = den^-1, c = inv C, P = inv A); the bottom thread eliminates
-// j = nc-1 .. m+1 upward from the far field (den = D - C e_prev, inv = den^-1,
-// e = inv A, Q = inv C); row m closes both: M = D - A c_{m-1} - C e_{m+1},
-// slot 0 = M^-1, slot 2 = M^-1 A, slot 1 = M^-1 C. In every row slot 0 maps
-// the right-hand side to g, slot 2 multiplies the recurrence value arriving
-// from the line end and slot 1 the one arriving from the middle on the way
-// back. The same direct solve as the reference's with half the chain length.
+// Two-sided (twisted) block elimination of every j-line. With m = nc / 2, the
+// top half eliminates j = 0 .. m-1 downward from the wall exactly as
+// aero2d/implicit.py:112-128 does (den = D - A c_prev, inv = den^-1,
+// c = inv C, P = inv A); the bottom half eliminates j = nc-1 .. m+1 upward
+// from the far field (den = D - C e_prev, inv = den^-1, e = inv A, Q = inv C);
+// row m closes both: M = D - A c_{m-1} - C e_{m+1}, slot 0 = M^-1,
+// slot 2 = M^-1 A, slot 1 = M^-1 C. In every row slot 0 maps the right-hand
+// side to g, slot 2 multiplies the recurrence value arriving from the line end
+// and slot 1 the one arriving from the middle on the way back. The same direct
+// solve as the reference's with half the chain length.
+//
+// Eight lanes per line, as in the sweeps: lanes 0-3 the top half, 4-7 the
+// bottom, lane r owning row r of every 4x4 block. A product's row r takes the
+// other rows by DPP broadcast inside the quad, and the inverse is Gauss-Jordan
+// with partial pivoting across the quad (the pivot row is fetched from its lane
+// by a lane shuffle), the same operations in the same order as a one-thread
+// inversion. The rows of the next steps' five blocks load while a step computes.
+
+// quad_bcast with the source lane as a run-time value that unrolling makes a
+// constant.
 template <class T>
-__global__ void __launch_bounds__(64) k_factor(Plan plan) {
+__device__ __forceinline__ T qbc(int c, T x) {
+  switch (c) {
+    case 0: return quad_bcast<0>(x);
+    case 1: return quad_bcast<1>(x);
+    case 2: return quad_bcast<2>(x);
+    default: return quad_bcast<3>(x);
+  }
+}
+// Row r (this lane's) of X Y, Y's rows held by the quad's lanes.
+template <class T>
+__device__ __forceinline__ void qmul(const T x[4], const T y[4], T out[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    out[c] = x[0] * quad_bcast<0>(y[c]) + x[1] * quad_bcast<1>(y[c]) + x[2] * quad_bcast<2>(y[c]) +
+             x[3] * quad_bcast<3>(y[c]);
+}
+// Row r of A^-1 (A's rows held by the quad's lanes): Gauss-Jordan, pivot =
+// first row of largest magnitude in the column among the rows not yet used.
+template <class T>
+__device__ __forceinline__ void qinv(const T a_in[4], int r, T out[4]) {
+  T a[4], b[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = a_in[k];
+    b[k] = (k == r) ? T(1) : T(0);
+  }
+  const int base = (int)(threadIdx.x & ~3u);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const T v0 = quad_bcast<0>(fabs(a[c])), v1 = quad_bcast<1>(fabs(a[c])),
+            v2 = quad_bcast<2>(fabs(a[c])), v3 = quad_bcast<3>(fabs(a[c]));
+    const T vv[4] = {v0, v1, v2, v3};
+    int p = c;
+    T best = vv[c];
+#pragma unroll
+    for (int q = c + 1; q < 4; ++q)
+      if (vv[q] > best) { best = vv[q]; p = q; }
+    // swap rows c and p: lane c takes row p, lane p takes row c
+    T ra[4], rb[4], ca[4], cb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ra[k] = __shfl(a[k], base + p, WAVE);
+      rb[k] = __shfl(b[k], base + p, WAVE);
+      ca[k] = qbc<T>(c, a[k]);
+      cb[k] = qbc<T>(c, b[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = (r == c) ? ra[k] : ((r == p) ? ca[k] : a[k]);
+      b[k] = (r == c) ? rb[k] : ((r == p) ? cb[k] : b[k]);
+    }
+    const T piv = T(1) / qbc<T>(c, a[c]);
+    T pa[4], pb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      pa[k] = qbc<T>(c, a[k]) * piv;
+      pb[k] = qbc<T>(c, b[k]) * piv;
+    }
+    const T f = a[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = (r == c) ? pa[k] : a[k] - f * pa[k];
+      b[k] = (r == c) ? pb[k] : b[k] - f * pb[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[k] = b[k];
+}
+
+template <class T>
+__global__ void __launch_bounds__(8 * LINES_PER_BLOCK) k_factor(Plan plan) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   const int ni = cd.ni, nc = cd.nc;
-  const int half = threadIdx.x & 1;
-  const int q = (blockIdx.x - plan.blk_prefix[e]) * LINES_PER_BLOCK + (threadIdx.x >> 1);
+  const int r = threadIdx.x & 3;
+  const int half = (threadIdx.x >> 2) & 1;
+  const int q = (blockIdx.x - plan.blk_prefix[e]) * LINES_PER_BLOCK + (threadIdx.x >> 3);
   const bool live = q < ni;
   const int col = live ? q : ni - 1;   // dead lanes shadow a real line and never store
   const int m = nc >> 1;
@@ -400,24 +403,25 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
   const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
   const Blk<T> B{gp(tp<T>(cd.blk)), (size_t)cd.pstride};
   const Blk<const T> Bc{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
-  // slots 1-4 also go to the FP16 copy in the mixed form
-  auto keep = [&](int slot, size_t cell, const T a[16]) {
-    store_blk(B, slot, cell, a);
-    if constexpr (std::is_same<T, float>::value)
-      store_blk_h(gp((_Float16*)cd.hb), B.ps, slot, cell, a);
+  // row r of a block: T storage, and for slots 1-4 the FP16 copy (mixed form)
+  auto keep = [&](int slot, size_t cell, const T v[4]) {
+    store_row(B.row(slot, r, cell), v);
+    if constexpr (std::is_same<T, float>::value) {
+      if (slot > 0) {
+        a2d_h4 t;
+        t.x = to_half_sat(v[0]); t.y = to_half_sat(v[1]);
+        t.z = to_half_sat(v[2]); t.w = to_half_sat(v[3]);
+        *(gptr<a2d_h4>)(gp((_Float16*)cd.hb) + ((size_t)(slot - 1) * B.ps + cell) * 16 + r * 4) = t;
+      }
+    }
   };
-  T cprev[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) cprev[k] = T(0);
-  // The five blocks of the next FPF steps are loaded while a step computes
-  // (rows past the half's end are clamped to its last row and not used);
-  // one step ahead in FP64, where two would spill registers.
-  constexpr int FPF = std::is_same<T, float>::value ? 2 : 1;
-  T rb[FPF][5][16];
+  T cprev[4] = {T(0), T(0), T(0), T(0)};
+  constexpr int FPF = 2;
+  T rb[FPF][5][4];
   auto fload = [&](int k, int s) {
     const size_t cell = (size_t)(j0 + dj * max(0, min(s, steps - 1))) * ni + col;
 #pragma unroll
-    for (int b = 0; b < 5; ++b) load_blk(Bc, b, cell, rb[k][b]);
+    for (int b = 0; b < 5; ++b) load_row(Bc.row(b, r, cell), rb[k][b]);
   };
 #pragma unroll
   for (int k = 0; k < FPF; ++k) fload(k, k);
@@ -426,10 +430,10 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
     for (int k = 0; k < FPF; ++k) {
       const int s = s0 + k;
       const bool valid = s < steps;
-      const size_t cell = (size_t)(j0 + dj * min(s, steps - 1)) * ni + col;
-      T D[16], A[16], C[16], Bm[16], Bp[16];
+      const size_t cell = (size_t)(j0 + dj * max(0, min(s, steps - 1))) * ni + col;
+      T D[4], C[4], A[4], Bm[4], Bp[4];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < 4; ++i) {
         D[i] = rb[k][0][i];
         C[i] = rb[k][1][i];
         A[i] = rb[k][2][i];
@@ -437,60 +441,66 @@ __global__ void __launch_bounds__(64) k_factor(Plan plan) {
         Bp[i] = rb[k][4][i];
       }
       fload(k, s + FPF);
-      T L[16], F[16], t[16];
+      T L[4], F[4], t[4];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < 4; ++i) {
         L[i] = half ? C[i] : A[i];    // couples to the row already eliminated
         F[i] = half ? A[i] : C[i];    // couples to the row eliminated next
       }
-      mul4<T>(L, cprev, t);
+      qmul<T>(L, cprev, t);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) D[i] -= t[i];
-      T inv[16], P[16], cn[16];
-      inv4<T>(D, inv);
-      mul4<T>(inv, F, cn);
-      mul4<T>(inv, L, P);
+      for (int i = 0; i < 4; ++i) D[i] -= t[i];
+      T inv[4], P[4], cn[4];
+      qinv<T>(D, r, inv);
+      qmul<T>(inv, F, cn);
+      qmul<T>(inv, L, P);
+      T s3[4], s4[4];
+      qmul<T>(inv, Bm, s3);
+      qmul<T>(inv, Bp, s4);
       if (live && valid) {
-        store_blk(B, 0, cell, inv);
+        keep(0, cell, inv);
         keep(1, cell, cn);
         keep(2, cell, P);
-        mul4<T>(inv, Bm, t);
-        keep(3, cell, t);
-        mul4<T>(inv, Bp, t);
-        keep(4, cell, t);
+        keep(3, cell, s3);
+        keep(4, cell, s4);
       }
 #pragma unroll
-      for (int i = 0; i < 16; ++i) cprev[i] = valid ? cn[i] : cprev[i];
+      for (int i = 0; i < 4; ++i) cprev[i] = valid ? cn[i] : cprev[i];
     }
   }
-  T other[16];
+  // ---- row m: the top's c_{m-1} and the bottom's e_{m+1} --------------------
+  T other[4];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) other[k] = __shfl_xor(cprev[k], 1, WAVE);
-  if (half == 0 && live) {
+  for (int i = 0; i < 4; ++i) other[i] = __shfl_xor(cprev[i], 4, WAVE);
+  {
     const size_t cell = (size_t)m * ni + col;
-    T D[16], A[16], C[16], t[16];
-    load_blk(Bc, 0, cell, D);
-    load_blk(Bc, 2, cell, A);
-    load_blk(Bc, 1, cell, C);
-    mul4<T>(A, cprev, t);           // A c_{m-1}
+    T D[4], A[4], C[4], Bm[4], Bp[4], t[4];
+    load_row(Bc.row(0, r, cell), D);
+    load_row(Bc.row(2, r, cell), A);
+    load_row(Bc.row(1, r, cell), C);
+    load_row(Bc.row(3, r, cell), Bm);
+    load_row(Bc.row(4, r, cell), Bp);
+    const T* ctop = half ? other : cprev;   // c_{m-1}
+    const T* ebot = half ? cprev : other;   // e_{m+1} (zero when the bottom part is empty)
+    qmul<T>(A, ctop, t);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) D[k] -= t[k];
-    mul4<T>(C, other, t);           // C e_{m+1} (zero when the bottom part is empty)
+    for (int i = 0; i < 4; ++i) D[i] -= t[i];
+    qmul<T>(C, ebot, t);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) D[k] -= t[k];
-    T inv[16], P[16], Q[16];
-    inv4<T>(D, inv);
-    mul4<T>(inv, A, P);
-    mul4<T>(inv, C, Q);
-    store_blk(B, 0, cell, inv);
-    keep(1, cell, Q);
-    keep(2, cell, P);
-    load_blk(Bc, 3, cell, A);
-    mul4<T>(inv, A, P);
-    keep(3, cell, P);
-    load_blk(Bc, 4, cell, C);
-    mul4<T>(inv, C, Q);
-    keep(4, cell, Q);
+    for (int i = 0; i < 4; ++i) D[i] -= t[i];
+    T inv[4], P[4], Q[4], s3[4], s4[4];
+    qinv<T>(D, r, inv);
+    qmul<T>(inv, A, P);
+    qmul<T>(inv, C, Q);
+    qmul<T>(inv, Bm, s3);
+    qmul<T>(inv, Bp, s4);
+    if (half == 0 && live) {
+      keep(0, cell, inv);
+      keep(1, cell, Q);
+      keep(2, cell, P);
+      keep(3, cell, s3);
+      keep(4, cell, s4);
+    }
   }
 }
 
@@ -809,8 +819,8 @@ void launch_assemble(const Plan& pv, int nblocks, const KnobsDev& k, int prec, h
 }
 
 void launch_factor(const Plan& pl, int nblocks, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_factor<double>, dim3(nblocks), dim3(64), 0, s, pl);
-  else hipLaunchKernelGGL(k_factor<float>, dim3(nblocks), dim3(64), 0, s, pl);
+  if (prec == 0) hipLaunchKernelGGL(k_factor<double>, dim3(nblocks), dim3(8 * LINES_PER_BLOCK), 0, s, pl);
+  else hipLaunchKernelGGL(k_factor<float>, dim3(nblocks), dim3(8 * LINES_PER_BLOCK), 0, s, pl);
 }
 
 void launch_line_sweep(const Plan& pl, int nblocks, int src, int xprev, int xout, int coupled,
