@@ -144,18 +144,20 @@ template <class T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>
 template <class T> __device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
 
 // Value of x held by lane C of this lane's quad (DPP quad permutation: a VALU
-// operation, no LDS traffic).
+// operation, no LDS traffic). A quad permutation always reads a lane of the
+// same quad, so no "old" value is needed (mov_dpp, bound_ctrl set): this
+// saves the zeroing move that update_dpp(0, ...) puts before every broadcast.
 template <int C>
 __device__ __forceinline__ double quad_bcast(double x) {
   constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 template <int C>
 __device__ __forceinline__ float quad_bcast(float x) {
   constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xF, 0xF, true));
 }
 
 // Wavefront sum with cross-lane exchanges (fixed order, deterministic).
