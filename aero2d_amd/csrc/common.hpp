@@ -140,7 +140,23 @@ template <class T> __device__ __forceinline__ const T* ctp(const void* p) { retu
 #define A2D_GLOBAL
 #endif
 template <class T> using gptr = A2D_GLOBAL T*;
+// A pointer every lane of the wavefront holds, moved to scalar registers, so
+// that loads can take it as their scalar base with a 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
 template <class T> __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+// Element idx of base with the byte offset formed in 32 bits (a case's arrays
+// are far below 4 GB): with a wave-uniform base the load then takes the base
+// from scalar registers and a single 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ gptr<T> at(gptr<T> base, unsigned idx) {
+  return (gptr<T>)((A2D_GLOBAL char*)base + idx * (unsigned)sizeof(T));
+}
 template <class T> __device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
 
 // Value of x held by lane C of this lane's quad (DPP quad permutation: a VALU

@@ -142,8 +142,10 @@ template <class T>
 struct Blk {
   gptr<T> p;
   size_t ps;
-  __device__ __forceinline__ gptr<T> row(int slot, int r, size_t cell) const {
-    return p + ((size_t)slot * ps + cell) * 16 + r * 4;
+  // 32-bit element index from the (wave-uniform) base: the compiler can then
+  // address with a scalar base and one 32-bit vector offset
+  __device__ __forceinline__ gptr<T> row(int slot, int r, unsigned cell) const {
+    return at(p, ((unsigned)slot * (unsigned)ps + cell) * 16u + (unsigned)r * 4u);
   }
 };
 
@@ -179,8 +181,8 @@ __device__ __forceinline__ _Float16 to_half_sat(float v) {
 }
 // Row r of slot `slot` (1-4) of the FP16 coupling copy.
 __device__ __forceinline__ gptr<const _Float16> hrow(gptr<const _Float16> H, size_t ps, int slot,
-                                                     int r, size_t cell) {
-  return H + ((size_t)(slot - 1) * ps + cell) * 16 + r * 4;
+                                                     int r, unsigned cell) {
+  return at(H, ((unsigned)(slot - 1) * (unsigned)ps + cell) * 16u + (unsigned)r * 4u);
 }
 
 template <class T>
@@ -401,8 +403,9 @@ __global__ void __launch_bounds__(8 * LINES_PER_BLOCK) k_factor(Plan plan) {
   const int m = nc >> 1;
   const int steps = half ? nc - 1 - m : m;
   const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
-  const Blk<T> B{gp(tp<T>(cd.blk)), (size_t)cd.pstride};
-  const Blk<const T> Bc{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
+  const Blk<T> B{gp(uniform_ptr(tp<T>(cd.blk))), (size_t)cd.pstride};
+  const Blk<const T> Bc{gp(uniform_ptr(ctp<T>(cd.blk))), (size_t)cd.pstride};
+  const gptr<_Float16> hbw = gp(uniform_ptr((_Float16*)cd.hb));
   // row r of a block: T storage, and for slots 1-4 the FP16 copy (mixed form)
   auto keep = [&](int slot, size_t cell, const T v[4]) {
     store_row(B.row(slot, r, cell), v);
@@ -411,7 +414,7 @@ __global__ void __launch_bounds__(8 * LINES_PER_BLOCK) k_factor(Plan plan) {
         a2d_h4 t;
         t.x = to_half_sat(v[0]); t.y = to_half_sat(v[1]);
         t.z = to_half_sat(v[2]); t.w = to_half_sat(v[3]);
-        *(gptr<a2d_h4>)(gp((_Float16*)cd.hb) + ((size_t)(slot - 1) * B.ps + cell) * 16 + r * 4) = t;
+        *(gptr<a2d_h4>)at(hbw, ((unsigned)(slot - 1) * (unsigned)B.ps + (unsigned)cell) * 16u + (unsigned)r * 4u) = t;
       }
     }
   };
@@ -571,22 +574,26 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   const int m = nc >> 1;
   const int steps = half ? nc - 1 - m : m;
   const int j0 = half ? nc - 1 : 0, dj = half ? -1 : 1;
-  const Blk<const T> B{gp(ctp<T>(cd.blk)), (size_t)cd.pstride};
-  const gptr<T> gd = gp(tp<T>(cd.d));
-  const gptr<T> x = gp(vec_ptr<T>(cd, xout) + (size_t)r * n + ic);
-  // component r of the source (-R read in FP64, or a vector) and of the
-  // neighbour lines, row j at offset j*ni; g0 and d are [cell][4]
-  const gptr<const double> sR = gp(cd.R) + (size_t)r * n + ic;
-  const gptr<const T> sV = gp(vec_ptr<T>(cd, GS == GS_VEC ? src : VEC_W) + (size_t)r * n + ic);
-  const gptr<const T> xnbp = gp(vec_ptr<T>(cd, COUP ? xnb : VEC_W) + (size_t)r * n);
-  const gptr<T> g0p = gp(tp<T>(cd.g0));
+  const Blk<const T> B{gp(uniform_ptr(ctp<T>(cd.blk))), (size_t)cd.pstride};
+  // Every array is addressed as its case's (wave-uniform) base plus a 32-bit
+  // element index, so that an access costs one 32-bit offset rather than
+  // 64-bit address arithmetic. Component r of a vector at row j and line i:
+  // r n + j ni + i; g0 and d are [cell][4].
+  const unsigned rn = (unsigned)r * (unsigned)n, niu = (unsigned)ni, colu = (unsigned)col;
+  const gptr<T> gd = gp(uniform_ptr(tp<T>(cd.d)));
+  const gptr<T> xv = gp(uniform_ptr(vec_ptr<T>(cd, xout)));
+  const gptr<const double> sR = gp(uniform_ptr((const double*)cd.R));
+  const gptr<const T> sV = gp(uniform_ptr((const T*)vec_ptr<T>(cd, GS == GS_VEC ? src : VEC_W)));
+  const gptr<const T> xnbp = gp(uniform_ptr((const T*)vec_ptr<T>(cd, COUP ? xnb : VEC_W)));
+  const gptr<T> g0p = gp(uniform_ptr(tp<T>(cd.g0)));
+  const gptr<const _Float16> hbp = gp(uniform_ptr((const _Float16*)cd.hb));
   // row r of coupling slot `slot` (1-4) of a cell, as loaded (CR) or as T
   using CR = typename CplRow<T, HB>::type;
-  auto crow = [&](int slot, size_t cell, CR& a) {
-    if constexpr (HB) a = *(gptr<const a2d_h4>)hrow(gp((const _Float16*)cd.hb), B.ps, slot, r, cell);
+  auto crow = [&](int slot, unsigned cell, CR& a) {
+    if constexpr (HB) a = *(gptr<const a2d_h4>)hrow(hbp, B.ps, slot, r, cell);
     else load_row(B.row(slot, r, cell), a.v);
   };
-  auto crow_t = [&](int slot, size_t cell, T v[4]) {
+  auto crow_t = [&](int slot, unsigned cell, T v[4]) {
     CR a;
     crow(slot, cell, a);
     unpack(a, v);
@@ -594,8 +601,9 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   // the source as loaded (-R stays FP64 until consumed) and as T
   using SR = typename std::conditional<GS == GS_NEGR, double, T>::type;
   auto load_src = [&](int jj) -> SR {
-    if constexpr (GS == GS_NEGR) return sR[(size_t)jj * ni];
-    else return sV[(size_t)jj * ni];
+    const unsigned idx = rn + (unsigned)jj * niu + (unsigned)ic;
+    if constexpr (GS == GS_NEGR) return *at(sR, idx);
+    else return *at(sV, idx);
   };
   auto src_val = [&](SR v) -> T {
     if constexpr (GS == GS_NEGR) return T(-v);
@@ -627,16 +635,16 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   auto frow = [&](int s) -> int { return j0 + dj * max(0, min(s, steps - 1)); };
   auto fload = [&](int k, int s) {
     const int jj = frow(s);
-    const size_t cell = (size_t)jj * ni + col;
+    const unsigned rowc = (unsigned)jj * niu, cell = rowc + colu;
     if constexpr (GS == GS_G0) {
-      bs[k] = g0p[cell * 4 + r];
+      bs[k] = *at(g0p, cell * 4u + (unsigned)r);
     } else {
       bs[k] = load_src(jj);
       load_row(B.row(0, r, cell), b0[k]);
     }
     if constexpr (COUP) {
-      bxm[k] = xnbp[(size_t)jj * ni + im];
-      bxp[k] = xnbp[(size_t)jj * ni + ip];
+      bxm[k] = *at(xnbp, rn + rowc + (unsigned)im);
+      bxp[k] = *at(xnbp, rn + rowc + (unsigned)ip);
       crow(3, cell, b3[k]);
       crow(4, cell, b4[k]);
     }
@@ -661,13 +669,13 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     for (int k = 0; k < PF; ++k) {
       const int st = s0 + k;
       const bool valid = st < steps;
-      const size_t cell = (size_t)frow(st) * ni + col;
+      const unsigned cell = (unsigned)frow(st) * niu + colu;
       T gs;
       if constexpr (GS == GS_G0) {
         gs = bs[k];
       } else {
         gs = s0src(src_val(bs[k]), b0[k]);
-        g0p[cell * 4 + r] = gs;
+        *at(g0p, cell * 4u + (unsigned)r) = gs;
       }
       T p2[4], p3[4], p4[4];
       if constexpr (COUP) {
@@ -679,7 +687,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
       const T dn = g - p2[0] * quad_bcast<0>(dr) - p2[1] * quad_bcast<1>(dr) -
                    p2[2] * quad_bcast<2>(dr) - p2[3] * quad_bcast<3>(dr);
       dr = valid ? dn : dr;
-      gd[cell * 4 + r] = dr;
+      *at(gd, cell * 4u + (unsigned)r) = dr;
       fload(k, st + PF);
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -689,20 +697,20 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   const T dtop = half ? other : dr, fbot = half ? dr : other;
   T xr;
   {
-    const size_t cell = (size_t)m * ni + col;
+    const unsigned cell = (unsigned)m * niu + colu;
     T s0r[4], s3r[4], s4r[4], pm[4], qm[4];
     T xmv = T(0), xpv = T(0), gs;
     if constexpr (GS == GS_G0) {
-      gs = g0p[cell * 4 + r];
+      gs = *at(g0p, cell * 4u + (unsigned)r);
     } else {
       const T sv = src_val(load_src(m));
       load_row(B.row(0, r, cell), s0r);
       gs = s0src(sv, s0r);
-      if (live && half == 0) g0p[cell * 4 + r] = gs;
+      if (live && half == 0) *at(g0p, cell * 4u + (unsigned)r) = gs;
     }
     if constexpr (COUP) {
-      xmv = xnbp[(size_t)m * ni + im];
-      xpv = xnbp[(size_t)m * ni + ip];
+      xmv = *at(xnbp, rn + (unsigned)m * niu + (unsigned)im);
+      xpv = *at(xnbp, rn + (unsigned)m * niu + (unsigned)ip);
       crow_t(3, cell, s3r);
       crow_t(4, cell, s4r);
     }
@@ -715,7 +723,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   }
   double nrm = 0.0;
   if (live && half == 0) {
-    x[(size_t)m * ni] = xr;
+    *at(xv, rn + (unsigned)m * niu + (unsigned)ic) = xr;
     nrm += double(xr) * double(xr);
   }
   // ---- away from the middle: x_j = d_j - S1_j x_prev (ring of PFB steps) -------
@@ -726,8 +734,8 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     return steps > 0 ? m - dj * (1 + sc) : m;
   };
   auto bload = [&](int k, int s) {
-    const size_t cell = (size_t)bj(s) * ni + col;
-    cv[k] = gd[cell * 4 + r];
+    const unsigned cell = (unsigned)bj(s) * niu + colu;
+    cv[k] = *at(gd, cell * 4u + (unsigned)r);
     crow(1, cell, cm[k]);
   };
 #pragma unroll
@@ -741,7 +749,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
         xr = cv[k] - c1[0] * quad_bcast<0>(xr) - c1[1] * quad_bcast<1>(xr) -
              c1[2] * quad_bcast<2>(xr) - c1[3] * quad_bcast<3>(xr);
         if (live) {
-          x[(size_t)bj(s0 + k) * ni] = xr;
+          *at(xv, rn + (unsigned)bj(s0 + k) * niu + (unsigned)ic) = xr;
           nrm += double(xr) * double(xr);
         }
       }
