@@ -587,6 +587,18 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   const gptr<const T> xnbp = gp(uniform_ptr((const T*)vec_ptr<T>(cd, COUP ? xnb : VEC_W)));
   const gptr<T> g0p = gp(uniform_ptr(tp<T>(cd.g0)));
   const gptr<const _Float16> hbp = gp(uniform_ptr((const _Float16*)cd.hb));
+  // Per-lane constant parts of the element indices; a step adds its row's
+  // first cell (rowc = j ni) to each: g0 and d rowc 4 + cgd, neighbours
+  // rowc + cxm / cxp, the source rowc + csrc, block row (slot, r) of the
+  // step's cell rowc 16 + cblk[slot] (T storage) or + chb[slot] (FP16 copy).
+  const unsigned psu = (unsigned)B.ps, ru = (unsigned)r;
+  const unsigned cgd = colu * 4u + ru;
+  const unsigned cxm = rn + (unsigned)im, cxp = rn + (unsigned)ip, csrc = rn + (unsigned)ic;
+  auto cblk = [&](int slot) -> unsigned { return ((unsigned)slot * psu + colu) * 16u + ru * 4u; };
+  auto chb = [&](int slot) -> unsigned { return ((unsigned)(slot - 1) * psu + colu) * 16u + ru * 4u; };
+  // row r of block `slot` of the cell in row rowc, from the T storage or
+  // (slots 1-4, HB) the FP16 copy
+  auto brow = [&](int slot, unsigned rowc) { return at(B.p, rowc * 16u + cblk(slot)); };
   // row r of coupling slot `slot` (1-4) of a cell, as loaded (CR) or as T
   using CR = typename CplRow<T, HB>::type;
   auto crow = [&](int slot, unsigned cell, CR& a) {
@@ -633,22 +645,28 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   T bxm[PF], bxp[PF], b0[PF][4];
   CR b3[PF], b4[PF], b2[PF];
   auto frow = [&](int s) -> int { return j0 + dj * max(0, min(s, steps - 1)); };
+  unsigned rowr[PF];   // each slot's row offset, for the stores of its step
+  auto cload = [&](int slot, unsigned rowc, CR& a) {
+    if constexpr (HB) a = *(gptr<const a2d_h4>)at(hbp, rowc * 16u + chb(slot));
+    else load_row(brow(slot, rowc), a.v);
+  };
   auto fload = [&](int k, int s) {
-    const int jj = frow(s);
-    const unsigned rowc = (unsigned)jj * niu, cell = rowc + colu;
+    const unsigned rowc = (unsigned)frow(s) * niu;
+    rowr[k] = rowc;
     if constexpr (GS == GS_G0) {
-      bs[k] = *at(g0p, cell * 4u + (unsigned)r);
+      bs[k] = *at(g0p, rowc * 4u + cgd);
     } else {
-      bs[k] = load_src(jj);
-      load_row(B.row(0, r, cell), b0[k]);
+      if constexpr (GS == GS_NEGR) bs[k] = *at(sR, rowc + csrc);
+      else bs[k] = *at(sV, rowc + csrc);
+      load_row(brow(0, rowc), b0[k]);
     }
     if constexpr (COUP) {
-      bxm[k] = *at(xnbp, rn + rowc + (unsigned)im);
-      bxp[k] = *at(xnbp, rn + rowc + (unsigned)ip);
-      crow(3, cell, b3[k]);
-      crow(4, cell, b4[k]);
+      bxm[k] = *at(xnbp, rowc + cxm);
+      bxp[k] = *at(xnbp, rowc + cxp);
+      cload(3, rowc, b3[k]);
+      cload(4, rowc, b4[k]);
     }
-    crow(2, cell, b2[k]);
+    cload(2, rowc, b2[k]);
   };
   T dr = T(0);
   // Fill the ring in slot order (the scheduling barriers keep slot 0's loads
@@ -669,13 +687,13 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     for (int k = 0; k < PF; ++k) {
       const int st = s0 + k;
       const bool valid = st < steps;
-      const unsigned cell = (unsigned)frow(st) * niu + colu;
+      const unsigned rowc = rowr[k];
       T gs;
       if constexpr (GS == GS_G0) {
         gs = bs[k];
       } else {
         gs = s0src(src_val(bs[k]), b0[k]);
-        *at(g0p, cell * 4u + (unsigned)r) = gs;
+        *at(g0p, rowc * 4u + cgd) = gs;
       }
       T p2[4], p3[4], p4[4];
       if constexpr (COUP) {
@@ -687,7 +705,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
       const T dn = g - p2[0] * quad_bcast<0>(dr) - p2[1] * quad_bcast<1>(dr) -
                    p2[2] * quad_bcast<2>(dr) - p2[3] * quad_bcast<3>(dr);
       dr = valid ? dn : dr;
-      *at(gd, cell * 4u + (unsigned)r) = dr;
+      *at(gd, rowc * 4u + cgd) = dr;
       fload(k, st + PF);
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -733,10 +751,12 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     const int sc = max(0, min(s, steps - 1));
     return steps > 0 ? m - dj * (1 + sc) : m;
   };
+  unsigned rowb[PFB];   // each slot's row offset, for the store of its step
   auto bload = [&](int k, int s) {
-    const unsigned cell = (unsigned)bj(s) * niu + colu;
-    cv[k] = *at(gd, cell * 4u + (unsigned)r);
-    crow(1, cell, cm[k]);
+    const unsigned rowc = (unsigned)bj(s) * niu;
+    rowb[k] = rowc;
+    cv[k] = *at(gd, rowc * 4u + cgd);
+    cload(1, rowc, cm[k]);
   };
 #pragma unroll
   for (int k = 0; k < PFB; ++k) bload(k, k);
@@ -749,7 +769,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
         xr = cv[k] - c1[0] * quad_bcast<0>(xr) - c1[1] * quad_bcast<1>(xr) -
              c1[2] * quad_bcast<2>(xr) - c1[3] * quad_bcast<3>(xr);
         if (live) {
-          *at(xv, rn + (unsigned)bj(s0 + k) * niu + (unsigned)ic) = xr;
+          *at(xv, rowb[k] + csrc) = xr;
           nrm += double(xr) * double(xr);
         }
       }
