@@ -758,21 +758,22 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     cv[k] = *at(gd, rowc * 4u + cgd);
     cload(1, rowc, cm[k]);
   };
+  // no per-lane branch, as on the way in: a step past the half's end keeps xr
+  // and stores to the half's last row the value it already holds; dead lanes
+  // store what the line they shadow stores
 #pragma unroll
   for (int k = 0; k < PFB; ++k) bload(k, k);
   for (int s0 = 0; s0 < m; s0 += PFB) {
 #pragma unroll
     for (int k = 0; k < PFB; ++k) {
-      if (s0 + k < steps) {
-        T c1[4];
-        unpack(cm[k], c1);
-        xr = cv[k] - c1[0] * quad_bcast<0>(xr) - c1[1] * quad_bcast<1>(xr) -
-             c1[2] * quad_bcast<2>(xr) - c1[3] * quad_bcast<3>(xr);
-        if (live) {
-          *at(xv, rowb[k] + csrc) = xr;
-          nrm += double(xr) * double(xr);
-        }
-      }
+      const bool valid = s0 + k < steps;
+      T c1[4];
+      unpack(cm[k], c1);
+      const T xn = cv[k] - c1[0] * quad_bcast<0>(xr) - c1[1] * quad_bcast<1>(xr) -
+                   c1[2] * quad_bcast<2>(xr) - c1[3] * quad_bcast<3>(xr);
+      xr = valid ? xn : xr;
+      *at(xv, rowb[k] + csrc) = xr;
+      nrm += (live && valid) ? double(xr) * double(xr) : 0.0;
       bload(k, s0 + k + PFB);
     }
   }
