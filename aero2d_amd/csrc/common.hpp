@@ -101,7 +101,7 @@ struct CaseDev {
   void* xb;
   void* p;              // mixed form, FP16 basis: FP32 preconditioner output [4][nc][ni]
   void* vc;             // and the FP32 copy of the newest (rounded) basis vector
-  double* part;         // [npart][kmax + 2] reduction partials
+  double* part;         // [kmax + 2][npart] reduction partials (column-major)
   CaseScal* sc;
 };
 
@@ -192,18 +192,22 @@ __device__ __forceinline__ double wave_max(double x) {
 // (column col), by one wavefront: lane l takes rows l, l + 64, ... in order,
 // then a fixed butterfly. Every lane gets the result; it depends on the case
 // alone, never on the batch it runs in.
+// Partial col of block blk. Columns are contiguous, so the lanes of a case sum
+// read consecutive blocks' partials of one column (one or two cache lines per
+// load instead of one line per lane).
+__device__ __forceinline__ double& part_at(const CaseDev& cd, int blk, int col) {
+  return cd.part[(size_t)col * cd.npart + blk];
+}
 __device__ __forceinline__ double case_sum(const CaseDev& cd, int nblk, int col) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const size_t stride = (size_t)cd.kmax + 2;
   double s = 0.0;
-  for (int b = lane; b < nblk; b += WAVE) s += cd.part[(size_t)b * stride + col];
+  for (int b = lane; b < nblk; b += WAVE) s += part_at(cd, b, col);
   return wave_sum(s);
 }
 __device__ __forceinline__ double case_max(const CaseDev& cd, int nblk, int col) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const size_t stride = (size_t)cd.kmax + 2;
   double s = 0.0;
-  for (int b = lane; b < nblk; b += WAVE) s = fmax(s, cd.part[(size_t)b * stride + col]);
+  for (int b = lane; b < nblk; b += WAVE) s = fmax(s, part_at(cd, b, col));
   return wave_max(s);
 }
 

@@ -31,6 +31,12 @@ int g_num_cu = 256;   // compute units of the device (sweep block sizing)
 #ifndef A2D_SWEEP_BIG_PCT
 #define A2D_SWEEP_BIG_PCT 60
 #endif
+// The CGS update forms the dot products' sums itself (no separate reduction
+// launch) up to this many of its blocks per CU: 2 = all resident at once at
+// its register use. 0 keeps the separate launch always.
+#ifndef A2D_UPDATE_FUSE_PER_CU
+#define A2D_UPDATE_FUSE_PER_CU 2
+#endif
 std::recursive_mutex g_mu;
 
 struct Fail {
@@ -384,7 +390,7 @@ void enq_newton(Pool& p, BatchPlan& b) {
   enq_precond(p, b, VEC_NEG_R, VEC_X0, 1, 0);
   launch_krylov_begin(b.pc, 0, kn, prec, s);
   // V_0 = M(b - A x0)
-  launch_dcirc(b.pc, VEC_X0, 0, prec, s);
+  launch_dcirc(b.pc, VEC_X0, 0, -1, kn, prec, s);
   for (auto& g : b.sten)
     launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 1, -1, 1, 0, prec, s);
   // FP16 basis (mixed form): the preconditioner writes the FP32 vector P, the
@@ -395,16 +401,21 @@ void enq_newton(Pool& p, BatchPlan& b) {
   launch_krylov_begin(b.pc, 1, kn, prec, s);
   if (bh) launch_store_v0(b.pvec, b.nvec, s);
   const int K = std::min(p.knobs.gmres_restart, KMAX);
+  // the update forms the dot products' sums itself when all its blocks are
+  // resident at once (two per compute unit at its register use)
+  const int red = b.nvec <= A2D_UPDATE_FUSE_PER_CU * g_num_cu;
   for (int k = 0; k < K; ++k) {
     const int vin = bh ? VEC_VC : k, wout = bh ? VEC_P : k + 1;
-    launch_dcirc(b.pc, vin, 1, prec, s);
+    // the circulation kernel first finishes step k - 1 (a launch fewer per
+    // step: a small batch's step is latency bound)
+    launch_dcirc(b.pc, vin, 1, k - 1, kn, prec, s);
     for (auto& g : b.sten) launch_jvp(g.p, g.shape, g.nblocks, kn, vin, VEC_W, 0, k, 1, 1, prec, s);
     enq_precond(p, b, VEC_W, wout, 0, 1);
     launch_dots(b.pvec, b.nvec, k, wout, 1, prec, bh, s);
-    launch_reduce_dots(b.pc, k, bh, s);
-    launch_update(b.pvec, b.nvec, k, wout, 1, prec, bh, s);
-    launch_normalize(b.pc, k, kn, prec, s);
+    if (!red) launch_reduce_dots(b.pc, k, bh, s);
+    launch_update(b.pvec, b.nvec, k, wout, 1, red, prec, bh, s);
   }
+  if (K > 0) launch_normalize(b.pc, K - 1, kn, prec, s);
   launch_lstsq(b.pc, s);
   launch_dupdate(b.pvec, b.nvec, prec, bh, s);
   launch_limiter(b.pc, kn, s);
@@ -887,7 +898,7 @@ int a2d_jvp(int64_t pool, int case_id, const double* v, double cfl, int with_dia
     set_cfl_one(p, b, cfl);
     enq_prologue(p, b, kn);
     upload_field(p, c, v, c.dev.xa, c.prec, 4);
-    launch_dcirc(b.pc, VEC_XA, 0, c.prec, p.stream);
+    launch_dcirc(b.pc, VEC_XA, 0, -1, kn, c.prec, p.stream);
     for (auto& g : b.sten)
       launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_XA, VEC_W, 0, -1, with_diag, 0, c.prec, p.stream);
     download_field(p, c, c.dev.w, out, c.prec, 4);
@@ -1008,7 +1019,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 3: launch_dots(b.pvec, b.nvec, k, bh ? VEC_P : k + 1, 0, b.prec, bh, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
         case 5: launch_factor(b.pline, b.nline, b.prec, p.stream); break;
-        case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, b.prec, bh, p.stream); break;
+        case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, 0, b.prec, bh, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
           launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);

@@ -15,8 +15,11 @@
 // update accumulate in FP64 as before.
 //
 // Reductions are two-level and case-local: every block writes its partial
-// sums to the case's partial array, then one small kernel per case adds them
-// in block order. Results never depend on which other cases share a launch.
+// sums to the case's partial array, then the case's sums are formed from them
+// in a fixed order (by a small kernel per case, or by every block of the next
+// kernel of the chain that needs them). Results never depend on which other
+// cases share a launch.
+#include "arnoldi.hpp"
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -36,10 +39,6 @@ constexpr int PER = VEC_CHUNK / NT;   // cells per thread in the vector kernels
 #ifndef A2D_UPDATE_JB32
 #define A2D_UPDATE_JB32 4             // and FP32
 #endif
-
-template <class T> __device__ __forceinline__ double eps_of() {
-  return std::is_same<T, double>::value ? DBL_EPSILON : double(FLT_EPSILON);
-}
 
 // Power-of-two storage scale for an FP16 basis vector of (estimated) norm nrm
 // over 4n entries: the scaled norm is ~sqrt(4n) / 64, so typical entries are
@@ -125,54 +124,90 @@ __global__ void __launch_bounds__(NT) k_dots(Plan plan, int k, int wvec, int gat
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < NT / WAVE; ++q) s += sred[j][q];
-    cd.part[(size_t)lb * (cd.kmax + 2) + j] = s;
+    part_at(cd, lb, j) = s;
   }
 }
 
-// One block per case, a wavefront per column: H[j][k] = s_j * sum_b part[b][j];
-// vn0 = |w|. FP16 basis (bh): also the storage scale of V_{k+1} from the
-// norm the orthogonalised vector will have, |w|^2 - sum_j H[j][k]^2
-// (clamped below at (1e-3 |w|)^2: the estimate's error from the basis's
-// departure from orthonormality is far smaller than that, so the true norm
-// is at most about twice the estimate and no stored entry can overflow).
+// The case's sums of the dot products' partials, a wavefront per column in
+// block order: hcol[j] = H[j][k] = s_j * sum_b part[b][j] for j <= k, and
+// hcol[k + 1] = |w|. Every block that forms them gets the same values.
+__device__ __forceinline__ void dots_sums(const CaseDev& cd, int k, double* hcol) {
+  for (int j = threadIdx.x / WAVE; j <= k + 1; j += NT / WAVE) {
+    const double s = case_sum(cd, cd.nvecb, j);
+    if ((threadIdx.x & (WAVE - 1)) == 0) hcol[j] = j <= k ? cd.sc->s[j] * s : sqrt(s);
+  }
+}
+
+// FP16 basis: the storage scale of V_{k+1} from the norm the orthogonalised
+// vector will have, |w|^2 - sum_j H[j][k]^2 (clamped below at (1e-3 |w|)^2:
+// the estimate's error from the basis's departure from orthonormality is far
+// smaller than that, so the true norm is at most about twice the estimate and
+// no stored entry can overflow).
+__device__ __forceinline__ double next_sig(const double* hcol, int k, int n) {
+  const double vn0 = hcol[k + 1], w2 = vn0 * vn0;
+  double h2 = 0.0;
+  for (int j = 0; j <= k; ++j) h2 += hcol[j] * hcol[j];
+  return basis_sig(sqrt(fmax(w2 - h2, 1e-6 * w2)), n);
+}
+
+// One block per case: H[:, k], |w| and (bh) the next storage scale into the
+// case scalars, for the update to read.
 __global__ void __launch_bounds__(NT) k_reduce_dots(Plan pc, int k, int bh) {
+  __shared__ double hcol[KMAX + 2];
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
   CaseScal* sc = cd.sc;
   if (!sc->active) return;
-  const int nblk = cd.nvecb;
-  for (int j = threadIdx.x / WAVE; j <= k + 1; j += NT / WAVE) {
-    const double s = case_sum(cd, nblk, j);
-    if ((threadIdx.x & (WAVE - 1)) == 0) {
-      if (j <= k) sc->H[j][k] = sc->s[j] * s;
-      else sc->vn0 = sqrt(s);
-    }
-  }
-  if (!bh) return;
+  dots_sums(cd, k, hcol);
   __syncthreads();
+  for (int j = threadIdx.x; j <= k; j += NT) sc->H[j][k] = hcol[j];
   if (threadIdx.x == 0) {
-    const double w2 = sc->vn0 * sc->vn0;
-    double h2 = 0.0;
-    for (int j = 0; j <= k; ++j) h2 += sc->H[j][k] * sc->H[j][k];
-    sc->vsig = basis_sig(sqrt(fmax(w2 - h2, 1e-6 * w2)), cd.n);
+    sc->vn0 = hcol[k + 1];
+    if (bh) sc->vsig = next_sig(hcol, k, cd.n);
   }
 }
 
 // w <- w - sum_j H[j][k] s_j V_j (classical Gram-Schmidt), in place in V_{k+1};
 // partial |w|^2 per block. BH: w is read from wvec, V_raw_{k+1} = sigma w is
 // stored in FP16 and in FP32 (vc), and the partials are of the stored values.
+//
+// red: every block first forms the case's sums of the dot products' partials
+// itself (the case's first block stores them), in place of k_reduce_dots: one
+// launch fewer, for batches whose blocks are all resident at once; a larger
+// launch would pay the serial reduction once per round of blocks.
 template <class T, bool BH>
-__global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int gate) {
+__global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int gate, int red) {
   using VT = BasisT<T, BH>;
   __shared__ double sred[NT / WAVE];
+  __shared__ double hcol[KMAX + 2];
   __shared__ double coef[KMAX + 1];
+  __shared__ double ssig;
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
-  const CaseScal* sc = cd.sc;
+  CaseScal* sc = cd.sc;
   if (gate && !sc->active) return;
   const int n = cd.n;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   const int c0 = lb * VEC_CHUNK + threadIdx.x;
-  for (int j = threadIdx.x; j <= k; j += NT) coef[j] = sc->H[j][k] * sc->s[j];
+  if (red) {
+    dots_sums(cd, k, hcol);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sig = 1.0;
+      if constexpr (BH) sig = next_sig(hcol, k, n);
+      ssig = sig;
+      if (lb == 0) {
+        sc->vn0 = hcol[k + 1];
+        if constexpr (BH) sc->vsig = sig;
+      }
+    }
+    for (int j = threadIdx.x; j <= k; j += NT) {
+      coef[j] = hcol[j] * sc->s[j];
+      if (lb == 0) sc->H[j][k] = hcol[j];
+    }
+  } else {
+    for (int j = threadIdx.x; j <= k; j += NT) coef[j] = sc->H[j][k] * sc->s[j];
+    if (threadIdx.x == 0) ssig = BH ? sc->vsig : 1.0;
+  }
   __syncthreads();
   const gptr<T> w = vec_gptr<T>(cd, wvec);
   const gptr<const VT> Vb = gp((const VT*)cd.V);
@@ -219,7 +254,7 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
   }
   double nrm = 0.0;
   if constexpr (BH) {
-    const double sig = sc->vsig;
+    const double sig = ssig;
     const gptr<_Float16> Vo = gp((_Float16*)cd.V + (size_t)(k + 1) * vs);
     const gptr<T> vc = vec_gptr<T>(cd, VEC_VC);
 #pragma unroll
@@ -253,7 +288,7 @@ __global__ void __launch_bounds__(NT) k_update(Plan plan, int k, int wvec, int g
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < NT / WAVE; ++q) s += sred[q];
-    cd.part[(size_t)lb * (cd.kmax + 2)] = s;
+    part_at(cd, lb, update_norm_col(cd)) = s;
   }
 }
 
@@ -282,54 +317,14 @@ __global__ void __launch_bounds__(NT) k_store_v0(Plan plan) {
   }
 }
 
-// One wavefront per case: finish Arnoldi step k (norm, breakdown test with
-// threshold eps * |w| as in JAX, Givens update of the Hessenberg column and
-// the residual estimate, optional early exit). The column and the rotations
-// are staged in LDS so the sequential rotation chain never waits on memory.
+// One wavefront per case: finish Arnoldi step k (the last one of a cycle; the
+// others are finished by the kernel that opens the next step).
 template <class T>
 __global__ void __launch_bounds__(WAVE) k_normalize(Plan pc, int k, KnobsDev kn) {
-  __shared__ double h[KMAX + 1], gcs[KMAX], gsn[KMAX];
+  __shared__ ArnoldiLds lds;
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
-  CaseScal* sc = cd.sc;
-  if (!sc->active) return;
-  const int lane = threadIdx.x;
-  const double nrm = sqrt(case_sum(cd, cd.nvecb, 0));
-  for (int i = lane; i <= k; i += WAVE) h[i] = sc->H[i][k];
-  for (int i = lane; i < k; i += WAVE) {
-    gcs[i] = sc->gc[i];
-    gsn[i] = sc->gs[i];
-  }
-  __syncthreads();
-  if (lane == 0) {
-    // FP16 basis: nrm is the norm of the stored sigma-scaled vector
-    const double sig = (std::is_same<T, float>::value && kn.basis_half) ? sc->vsig : 1.0;
-    const double hn = nrm / sig;
-    const bool brk = !(hn > eps_of<T>() * sc->vn0);
-    const double hk1 = brk ? 0.0 : hn;
-    sc->s[k + 1] = brk ? 0.0 : 1.0 / nrm;
-    // apply the previous rotations to column k; `a` carries H[i][k]
-    double a = h[0];
-    for (int i = 0; i < k; ++i) {
-      const double b = h[i + 1];
-      h[i] = gcs[i] * a + gsn[i] * b;
-      a = -gsn[i] * a + gcs[i] * b;
-    }
-    const double r = hypot(a, hk1);
-    const double c = (r > 0) ? a / r : 1.0, sn = (r > 0) ? hk1 / r : 0.0;
-    sc->gc[k] = c;
-    sc->gs[k] = sn;
-    h[k] = r;
-    sc->H[k + 1][k] = 0.0;
-    sc->g[k + 1] = -sn * sc->g[k];
-    sc->g[k] = c * sc->g[k];
-    sc->gres = fabs(sc->g[k + 1]);
-    sc->kdone = k + 1;
-    bool stop = brk || (k + 1 >= kn.restart);
-    if (kn.early_exit && sc->gres <= fmin(1.0, kn.gtol) * sc->mbnorm) stop = true;
-    if (stop) sc->active = 0;
-  }
-  __syncthreads();
-  for (int i = lane; i <= k; i += WAVE) sc->H[i][k] = h[i];
+  if (!cd.sc->active) return;
+  arnoldi_finish<T>(cd, k, kn, lds);
 }
 
 // stage 0: |M b| from the x0 sweep's partials.
@@ -458,8 +453,8 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
       mx = fmax(mx, smax[q]);
       ct += scnt[q];
     }
-    cd.part[(size_t)lb * (cd.kmax + 2)] = mx;
-    cd.part[(size_t)lb * (cd.kmax + 2) + 1] = ct;
+    part_at(cd, lb, 0) = mx;
+    part_at(cd, lb, 1) = ct;
   }
 }
 
@@ -511,7 +506,7 @@ __global__ void __launch_bounds__(NT) k_poscount(Plan plan) {
     double ct = 0.0;
 #pragma unroll
     for (int q = 0; q < NT / WAVE; ++q) ct += scnt[q];
-    cd.part[(size_t)lb * (cd.kmax + 2)] = ct;
+    part_at(cd, lb, 0) = ct;
   }
 }
 
@@ -603,10 +598,14 @@ void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int pre
 void launch_reduce_dots(const Plan& pc, int k, int bh, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_dots, dim3(pc.nb), dim3(NT), 0, s, pc, k, bh);
 }
-void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL((k_update<double, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
-  else if (bh) hipLaunchKernelGGL((k_update<float, true>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
-  else hipLaunchKernelGGL((k_update<float, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate);
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int red, int prec, int bh,
+                   hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL((k_update<double, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate, red);
+  else if (bh)
+    hipLaunchKernelGGL((k_update<float, true>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate, red);
+  else
+    hipLaunchKernelGGL((k_update<float, false>), dim3(nblocks), dim3(NT), 0, s, pv, k, wvec, gate, red);
 }
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s) {
   if (prec == 0) hipLaunchKernelGGL(k_normalize<double>, dim3(pc.nb), dim3(WAVE), 0, s, pc, k, kn);

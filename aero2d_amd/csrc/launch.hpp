@@ -54,7 +54,9 @@ void launch_residual(const Plan& ps, int shape, int nblocks, const KnobsDev& k, 
 void launch_jvp(const Plan& ps, int shape, int nblocks, const KnobsDev& k, int vin, int vout,
                 int mode, int kidx, int use_diag, int gate, int prec, hipStream_t s);
 void launch_forces(const Plan& pc, double* cp_out, hipStream_t s);
-void launch_dcirc(const Plan& pc, int vin, int gate, int prec, hipStream_t s);
+// kprev >= 0: finish Arnoldi step kprev first (as launch_normalize would).
+void launch_dcirc(const Plan& pc, int vin, int gate, int kprev, const KnobsDev& kn, int prec,
+                  hipStream_t s);
 void launch_rn_finalize(const Plan& pc, hipStream_t s);
 void launch_hllc_test(int n, const double* WL, const double* WR, const double* S, double* F,
                       hipStream_t s);
@@ -83,8 +85,11 @@ void launch_line_sweep_rb(const Plan& pl, int nblocks, int src, int xout, int co
 // bh (prec 1 only): the basis is stored in FP16 (wvec is then an FP32 work
 // vector that the update reads; it writes V_{k+1} and vc).
 void launch_dots(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s);
+// H[:, k] from the dot products' partials into the case scalars.
 void launch_reduce_dots(const Plan& pc, int k, int bh, hipStream_t s);
-void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int prec, int bh, hipStream_t s);
+// red: the update forms H[:, k] itself (no launch_reduce_dots before it).
+void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int red, int prec, int bh,
+                   hipStream_t s);
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_lstsq(const Plan& pc, hipStream_t s);

@@ -14,6 +14,7 @@
 // into LDS, then differences them. HBM traffic is one read of U (and v),
 // the face metrics and one write of the result; neighbouring tiles re-read
 // their 2-cell halos from L2.
+#include "arnoldi.hpp"
 #include "common.hpp"
 #include "launch.hpp"
 #include "physics.hpp"
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(NTS) k_stencil(Plan plan, KnobsDev kn, int vin
       double s = 0.0;
 #pragma unroll
       for (int w = 0; w < NTS / WAVE; ++w) s += sred[w];
-      cd.part[(size_t)lt * (cd.kmax + 2)] = s;
+      part_at(cd, lt, 0) = s;
     }
   }
 }
@@ -408,13 +409,20 @@ __global__ void __launch_bounds__(NT) k_forces(Plan plan, double* cp_out) {
 // Tangent of Gamma along vector v: dGamma = |V_inf|/2 * dCl, dCl from the wall
 // pressure extrapolation 1.5 p_0 - 0.5 p_1 (the circulation coupling of the
 // implicit residual, aero2d/implicit.py:160-167).
+// kprev >= 0: the block first finishes Arnoldi step kprev (whose vector v is),
+// and goes on only if the case proceeds to the next step.
 template <class T>
-__global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate) {
+__global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate, int kprev, KnobsDev kn) {
   __shared__ double red[2][NT / WAVE];
+  __shared__ ArnoldiLds lds;
   const int e = blockIdx.x;
   const CaseDev& cd = plan.cases[plan.slots[e]];
   CaseScal* sc = cd.sc;
   if (gate && !sc->active) return;
+  if (kprev >= 0) {
+    arnoldi_finish<T>(cd, kprev, kn, lds);
+    if (!lds.live) return;
+  }
   const int ni = cd.ni, n = cd.n, nc = cd.nc;
   const CaseView<T> cv = view<T>(cd);
   const gptr<const T> v = vec_gptr<T>(cd, vin);
@@ -516,9 +524,10 @@ void launch_forces(const Plan& p, double* cp_out, hipStream_t s) {
   hipLaunchKernelGGL(k_forces, dim3(p.nb), dim3(NT), 0, s, p, cp_out);
 }
 
-void launch_dcirc(const Plan& p, int vin, int gate, int prec, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL(k_dcirc<double>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate);
-  else hipLaunchKernelGGL(k_dcirc<float>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate);
+void launch_dcirc(const Plan& p, int vin, int gate, int kprev, const KnobsDev& kn, int prec,
+                  hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL(k_dcirc<double>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate, kprev, kn);
+  else hipLaunchKernelGGL(k_dcirc<float>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate, kprev, kn);
 }
 
 void launch_rn_finalize(const Plan& pc, hipStream_t s) {
