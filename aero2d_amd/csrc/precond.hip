@@ -783,7 +783,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     const int grp = lb * (SL / 8) + (int)(threadIdx.x / WAVE);
     if ((threadIdx.x & (WAVE - 1)) == 0 && grp * 8 < nq) {
       const int slot = color < 0 ? grp : color * cd.nlinerw + grp;
-      cd.part[(size_t)slot * (cd.kmax + 2)] = nrm;
+      part_at(cd, slot, 0) = nrm;
     }
   }
 }
