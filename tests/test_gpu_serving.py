@@ -47,6 +47,21 @@ def test_batch_result_is_independent_of_batch_composition():
     assert np.array_equal(full.iterations[[3, 1]], sub.iterations)
 
 
+def test_large_batch_reduction_path_gives_the_same_bits():
+    """A 512 x 256 case alone (128 update blocks) has its dot-product sums
+    formed inside the CGS update; in a batch of six (768 update blocks, more
+    than two per compute unit of a 256-CU device) they come from a separate
+    per-case kernel. Both paths sum the same partials in the same order, so the
+    states after three Newton steps must agree bit for bit."""
+    from aero2d_amd.workloads import config3
+    cases = config3(6, 512, 256)
+    full = solve_batch(cases, "mixed", tol=1e-14, n_iter=3, keep_state=True)
+    one = solve_batch([cases[4]], "mixed", tol=1e-14, n_iter=3, keep_state=True)
+    assert full.iterations[4] == one.iterations[0] == 3
+    assert np.array_equal(np.asarray(full.U[4]), np.asarray(one.U[0]))
+    assert full.cl[4] == one.cl[0] and full.cd[4] == one.cd[0]
+
+
 def test_admission_waits_for_hbm_and_keeps_arrival_order():
     g = _cases()[1].grid                                    # all requests 96 x 24
     cases = [Case(g, 0.4 + 0.05 * k, 0.5 * k, f"r{k}") for k in range(5)]
