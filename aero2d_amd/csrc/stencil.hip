@@ -418,7 +418,8 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate, int 
   const int e = blockIdx.x;
   const CaseDev& cd = plan.cases[plan.slots[e]];
   CaseScal* sc = cd.sc;
-  if (gate && !sc->active) return;
+  // a step is finished only for a case still in its Arnoldi loop
+  if ((gate || kprev >= 0) && !sc->active) return;
   if (kprev >= 0) {
     arnoldi_finish<T>(cd, kprev, kn, lds);
     if (!lds.live) return;
