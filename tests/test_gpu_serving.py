@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from aero2d_amd import _abi, naca4, naca4_params, ogrid
-from aero2d_amd.batch import Case, solve_batch
+from aero2d_amd.batch import Case, default_knobs, solve_batch
 from aero2d_amd.serve import SolveServer, serve_stream
 from aero2d_amd.workloads import Request
 
@@ -60,6 +60,41 @@ def test_large_batch_reduction_path_gives_the_same_bits():
     assert full.iterations[4] == one.iterations[0] == 3
     assert np.array_equal(np.asarray(full.U[4]), np.asarray(one.U[0]))
     assert full.cl[4] == one.cl[0] and full.cd[4] == one.cd[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_eager_launches(precision):
+    """From a batch's second Newton step on, the step is replayed from a
+    captured HIP graph. Four steps with graphs (one eager, one capture and
+    launch, two replays), each at a new CFL uploaded by the host and followed
+    by state downloads and a synchronisation, give the same bits after every
+    step as the same four steps launched eagerly."""
+    cases = _cases()[:3]
+    prec = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED
+    hist = {}
+    for use_graph in (1, 0):
+        kn = default_knobs(precision)
+        kn.use_graph = use_graph
+        pool = _abi.Pool(0, kn)
+        try:
+            ids = np.array([pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, prec) for c in cases], np.int32)
+            out = np.empty((len(ids), 10))
+            h = []
+            for step in range(4):
+                cfl = np.full(len(ids), 10.0 * 2.0 ** step)
+                _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl), _abi.dptr(out))
+                h.append(out.copy())
+                for k, c in enumerate(cases):
+                    ni, nj = c.grid.x.shape
+                    U = np.empty((4, ni, nj - 1))
+                    _abi.call("a2d_case_get_state", pool.handle, int(ids[k]), _abi.dptr(U))
+                    h.append(U)
+            hist[use_graph] = h
+        finally:
+            pool.close()
+    assert len(hist[1]) == len(hist[0]) == 4 * (1 + len(cases))
+    for i, (a, b) in enumerate(zip(hist[1], hist[0])):
+        assert np.array_equal(a, b, equal_nan=True), i
 
 
 def test_admission_waits_for_hbm_and_keeps_arrival_order():
