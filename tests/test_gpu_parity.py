@@ -213,6 +213,55 @@ def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half):
     assert rel_err(x, xo) < (1e-9 if precision == "fp64" else 5e-3)
 
 
+def test_largest_served_grid_matches_oracle():
+    """768 x 384 cells (294,912), the largest size the serving workload draws
+    (NACA 2412, its stretched far-field grid): residual, Jacobian-vector
+    product and both preconditioner orderings vs the oracle at the small-grid
+    tolerances, then one full FP64 Newton step with GMRES(10) from a perturbed
+    state, where the oracle's own response to a 1e-15 perturbation is ~1e-15,
+    to 1e-10."""
+    from aero2d_amd import ImplicitEuler, naca4
+    from aero2d_amd.workloads import grid_for
+    g = grid_for(naca4("2412", 385), 384)[0]
+    mach, alpha = 0.7, 1.5
+    s = EulerSolver(g, dtype=np.float64)
+    assert (s.ni, s.nc) == (768, 384)
+    U = perturbed_state(s.ni, s.nc, mach, alpha, 4, amp=0.02)
+    h, c = s._scratch(s.freestream(mach, alpha), U)
+    geo = oracle_geo(g)
+    w = winf_t(mach, alpha)
+    Ut = torch.tensor(U)
+    circ = flow.circulation(Ut, w, geo)
+    R = np.empty((4, s.ni, s.nc))
+    rn = np.zeros(1)
+    _abi.call("a2d_residual", h, c, float("nan"), _abi.dptr(R), _abi.dptr(rn))
+    Ro, _ = flow.residual(Ut, w, circ, geo)
+    assert rel_err(R, Ro.numpy()) < 1e-11
+    v = np.ascontiguousarray(np.random.default_rng(3).normal(size=U.shape))
+    out = np.empty_like(U)
+    _abi.call("a2d_jvp", h, c, _abi.dptr(v), 10.0, 0, _abi.dptr(out))
+    jo = jvp(lambda u: flow.residual_with_circ(u, w, geo)[0], (Ut,), (torch.tensor(v),))[1].numpy()
+    assert rel_err(out, jo) < 1e-11
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 25.0)
+    D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
+    fac = newton.factor_lines(D, Bjm, Bjp)
+    r = np.ascontiguousarray(np.random.default_rng(5).normal(size=U.shape))
+    for order, fn in ((0, newton.precond), (1, newton.precond_rb)):
+        k = s._pool.knobs()
+        k.precond_order = order
+        s._pool.set_knobs(k)
+        _abi.call("a2d_precond_setup", h, c, 25.0)
+        x = np.empty_like(r)
+        _abi.call("a2d_precond_apply", h, c, _abi.dptr(r), _abi.dptr(x))
+        assert rel_err(x, fn(fac, Bim, Bip, torch.tensor(r), 4).numpy()) < 1e-9, order
+    r1 = ImplicitEuler(s, gmres_restart=10).solve([mach], [alpha], n_iter=1, verbose=False, tol=0.0,
+                                                 U0=U[None])
+    ro = newton.solve(flow.stack_geometry([geo]), [mach], [alpha], n_iter=1, tol=0.0,
+                      p=newton.Params(restart=10), U0=Ut[..., None])
+    assert rel_err(r1.U[0], ro.U[..., 0].numpy()) < 1e-10
+    assert abs(r1.cl[0] - ro.cl[0]) < 1e-10
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_newton_steps_match_oracle(case):
     """One and three full Newton steps (50 Arnoldi steps each) vs the oracle,
