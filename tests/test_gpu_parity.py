@@ -213,25 +213,30 @@ def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half):
     assert rel_err(x, xo) < (1e-9 if precision == "fp64" else 5e-3)
 
 
-def test_largest_served_grid_matches_oracle():
+def _largest_served(precision):
     """768 x 384 cells (294,912), the largest size the serving workload draws
-    (NACA 2412, its stretched far-field grid): residual, Jacobian-vector
-    product and both preconditioner orderings vs the oracle at the small-grid
-    tolerances, then one full FP64 Newton step with GMRES(10) from a perturbed
-    state, where the oracle's own response to a 1e-15 perturbation is ~1e-15,
-    to 1e-10."""
-    from aero2d_amd import ImplicitEuler, naca4
+    (NACA 2412 on its stretched far-field grid), at a perturbed state."""
+    from aero2d_amd import naca4
     from aero2d_amd.workloads import grid_for
     g = grid_for(naca4("2412", 385), 384)[0]
     mach, alpha = 0.7, 1.5
-    s = EulerSolver(g, dtype=np.float64)
+    s = EulerSolver(g, dtype=np.float64, precision=precision)
     assert (s.ni, s.nc) == (768, 384)
     U = perturbed_state(s.ni, s.nc, mach, alpha, 4, amp=0.02)
     h, c = s._scratch(s.freestream(mach, alpha), U)
     geo = oracle_geo(g)
     w = winf_t(mach, alpha)
     Ut = torch.tensor(U)
-    circ = flow.circulation(Ut, w, geo)
+    return g, s, U, h, c, geo, w, Ut, flow.circulation(Ut, w, geo), mach, alpha
+
+
+def test_largest_served_grid_matches_oracle():
+    """At 768 x 384: residual, Jacobian-vector product and both preconditioner
+    orderings vs the oracle at the small-grid tolerances, then one full FP64
+    Newton step with GMRES(10) from the perturbed state, where the oracle's own
+    response to a 1e-15 perturbation is ~1e-15, to 1e-10."""
+    from aero2d_amd import ImplicitEuler
+    g, s, U, h, c, geo, w, Ut, circ, mach, alpha = _largest_served("fp64")
     R = np.empty((4, s.ni, s.nc))
     rn = np.zeros(1)
     _abi.call("a2d_residual", h, c, float("nan"), _abi.dptr(R), _abi.dptr(rn))
@@ -260,6 +265,33 @@ def test_largest_served_grid_matches_oracle():
                       p=newton.Params(restart=10), U0=Ut[..., None])
     assert rel_err(r1.U[0], ro.U[..., 0].numpy()) < 1e-10
     assert abs(r1.cl[0] - ro.cl[0]) < 1e-10
+
+
+def test_largest_served_grid_mixed_form_matches_oracle():
+    """The mixed form's FP32 kernels at 768 x 384 against the FP64 oracle, at
+    the small-grid bars: Jacobian-vector product 1e-4, red-black preconditioner
+    2e-4 with FP32 couplings and 5e-3 reading their FP16 copy (a wrong index
+    or dropped term shows as O(1))."""
+    g, s, U, h, c, geo, w, Ut, circ, mach, alpha = _largest_served("mixed")
+    v = np.ascontiguousarray(np.random.default_rng(3).normal(size=U.shape))
+    out = np.empty_like(U)
+    _abi.call("a2d_jvp", h, c, _abi.dptr(v), 10.0, 0, _abi.dptr(out))
+    jo = jvp(lambda u: flow.residual_with_circ(u, w, geo)[0], (Ut,), (torch.tensor(v),))[1].numpy()
+    assert rel_err(out, jo) < 1e-4
+    dti = 1.0 / (flow.local_dt(flow.cons_to_prim(Ut), geo) / 2.5 * 25.0)
+    D, Bim, Bip, Bjm, Bjp = newton.assemble(Ut, w, circ, dti, geo)
+    fac = newton.factor_lines(D, Bjm, Bjp)
+    r = np.ascontiguousarray(np.random.default_rng(5).normal(size=U.shape))
+    xo = newton.precond_rb(fac, Bim, Bip, torch.tensor(r), 4).numpy()
+    for half, tol in ((0, 2e-4), (1, 5e-3)):
+        k = s._pool.knobs()
+        k.precond_order = 1
+        k.precond_half = half
+        s._pool.set_knobs(k)
+        _abi.call("a2d_precond_setup", h, c, 25.0)
+        x = np.empty_like(r)
+        _abi.call("a2d_precond_apply", h, c, _abi.dptr(r), _abi.dptr(x))
+        assert rel_err(x, xo) < tol, half
 
 
 @pytest.mark.parametrize("case", CASES[:2])
