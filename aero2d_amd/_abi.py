@@ -143,7 +143,12 @@ def default_knobs() -> Knobs:
 
 
 def case_bytes(ni: int, nc: int, precision: int, restart: int = 50) -> int:
-    return int(load().a2d_case_bytes(ni, nc, precision, restart))
+    """Device footprint of one case (a2d_case_bytes). Raises HipError (-6)
+    for a size the library refuses."""
+    b = int(load().a2d_case_bytes(ni, nc, precision, restart))
+    if b < 0:
+        raise HipError("a2d_case_bytes", b, f"a {ni}x{nc} case is outside the library's limits")
+    return b
 
 
 def stencil_shapes():

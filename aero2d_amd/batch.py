@@ -80,11 +80,11 @@ class BatchResult:
 # solver path decides which one a case reaches.
 SOLVER_DEFAULTS = {
     "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0,
-                 cfl_late=None, late_below=0.0, max_update=0.2, fallback_max_update=0.0,
-                 positivity=0, precond_half=0, basis_half=0),
+                 cfl_late=None, late_below=0.0, max_update=0.2, max_update_late=None,
+                 fallback_max_update=0.0, positivity=0, precond_half=0, basis_half=0),
     "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=300.0,
-                  cfl_late=1000.0, late_below=1e-3, max_update=0.5, fallback_max_update=0.0,
-                  positivity=0, precond_half=1, basis_half=1),
+                  cfl_late=1000.0, late_below=1e-3, max_update=0.5, max_update_late=None,
+                  fallback_max_update=0.0, positivity=0, precond_half=1, basis_half=1),
 }
 
 
@@ -125,7 +125,7 @@ def path_of(form: str) -> dict:
                        precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"],
                        max_update=d["max_update"], positivity=d["positivity"])
     return dict(knobs=kn, cfl_max=d["cfl_max"], cfl_late=d["cfl_late"], late_below=d["late_below"],
-                fallback_max_update=d["fallback_max_update"])
+                fallback_max_update=d["fallback_max_update"], max_update_late=d["max_update_late"] or 0.0)
 
 
 class BatchSolve:
@@ -134,8 +134,9 @@ class BatchSolve:
     Newton loop, timing only device-resident work plus the host control."""
 
     def __init__(self, cases: list[Case], precision: str = "fp64", knobs: _abi.Knobs | None = None,
-                 shapes: dict | None = None, pool_budget: int = 0):
+                 shapes: dict | None = None, pool_budget: int = 0, states: list | None = None):
         self.cases = cases
+        self.states = states
         self.precision = precision
         self.prec = _abi.PREC_FP64 if precision == "fp64" else _abi.PREC_MIXED
         self.knobs = knobs if knobs is not None else default_knobs(precision)
@@ -155,16 +156,26 @@ class BatchSolve:
                 sh = self.shapes.get((c.grid.x.shape[0], c.grid.x.shape[1] - 1))
                 if sh is not None:
                     _abi.call("a2d_case_set_shape", self.pool.handle, int(self.ids[k]), int(sh))
+            if self.states is not None and self.states[k] is not None:
+                Uk = np.ascontiguousarray(self.states[k], np.float64)
+                _abi.call("a2d_case_set_state", self.pool.handle, int(self.ids[k]), _abi.dptr(Uk))
         self.upload_s = time.perf_counter() - t0
         return self
 
     def run(self, tol: float = 1e-10, n_iter: int = 200, cfl0: float = 10.0, cfl_max: float | None = None,
             keep_state: bool = False, on_step=None, cfl_late: float | None = None,
             late_below: float | None = None, record_history: bool = False,
-            fallback_max_update: float | None = None) -> BatchResult:
+            fallback_max_update: float | None = None, max_update_late: float | None = None,
+            stall: tuple | None = None) -> BatchResult:
         """Newton loop to rn/r0 <= tol per case (or n_iter steps); the pool is
         released at the end. fallback_max_update > 0: a case's step limit
-        drops to it after the case's first rejected step (default: the form's)."""
+        drops to it after the case's first rejected step (default: the form's).
+        max_update_late > 0: a case's step limit rises to it once its rn/r0
+        falls below late_below (default: the form's; 0 = never). stall =
+        (after, window, drop): a case that has taken `after` Newton steps and
+        whose best rn/r0 over the last `window` steps is not `drop` times below
+        its best before them is retired as stalled (not converged); None keeps
+        every case until tol or n_iter."""
         if self.pool is None:
             self.upload()
         cases, ids, pool = self.cases, self.ids, self.pool
@@ -172,11 +183,16 @@ class BatchSolve:
         cfl_max, cfl_late, late_below = cfl_policy(self.precision, cfl_max, cfl_late, late_below)
         if fallback_max_update is None:
             fallback_max_update = SOLVER_DEFAULTS[self.precision]["fallback_max_update"]
+        if max_update_late is None:
+            max_update_late = SOLVER_DEFAULTS[self.precision]["max_update_late"] or 0.0
         fallback = fallback_max_update > 0 and fallback_max_update != self.knobs.max_update
+        staged = max_update_late > 0 and max_update_late != self.knobs.max_update
         t0 = time.perf_counter()
         try:
             loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below, fallback)
             hist = []
+            best = []                      # best rn/r0 so far, per step (stall rule)
+            stalled = np.zeros(nb, bool)
             done = np.zeros(nb, bool)
             t_done = np.full(nb, np.nan)
             arn = np.zeros(nb)
@@ -192,12 +208,28 @@ class BatchSolve:
                 rn, _ = loop.update(act, out)
                 for k in loop.newly_failed:
                     _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[k]), float(fallback_max_update))
+                if staged:
+                    for k in loop.newly_late:
+                        if not (fallback and loop.failed_once[k]):
+                            _abi.call("a2d_case_set_step_limit", pool.handle, int(ids[k]),
+                                      float(max_update_late))
                 arn[act] += out[:, 6]
+                rel = rn / loop.r0[act]
                 if record_history:
                     h = np.full(nb, np.nan)
-                    h[act] = rn / loop.r0[act]
+                    h[act] = rel
                     hist.append(h)
-                fin = (rn / loop.r0[act] <= tol) | loop.frozen[act]
+                fin = (rel <= tol) | loop.frozen[act]
+                if stall is not None:
+                    b = best[-1].copy() if best else np.full(nb, np.inf)
+                    b[act] = np.fmin(b[act], rel)
+                    best.append(b)
+                    after, window, drop = stall
+                    if it >= window:
+                        old = best[it - window]
+                        st = (loop.iters[act] >= after) & ~(b[act] * drop <= old[act]) & ~fin
+                        stalled[act[st]] = True
+                        fin = fin | st
                 now = time.perf_counter() - t0
                 t_done[act[fin]] = now
                 done[act[fin]] = True
@@ -232,6 +264,7 @@ class BatchSolve:
                           t_done, wall, arn / np.maximum(loop.iters, 1), states)
         res.history = np.array(hist) if record_history else None
         res.upload_s = self.upload_s
+        res.stalled = stalled
         return res
 
 
@@ -239,7 +272,9 @@ def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, 
                 cfl0: float = 10.0, cfl_max: float | None = None, knobs: _abi.Knobs | None = None,
                 shapes: dict | None = None, keep_state: bool = False, pool_budget: int = 0,
                 on_step=None, cfl_late: float | None = None, late_below: float | None = None,
-                record_history: bool = False, fallback_max_update: float | None = None) -> BatchResult:
+                record_history: bool = False, fallback_max_update: float | None = None,
+                max_update_late: float | None = None, stall: tuple | None = None,
+                states: list | None = None) -> BatchResult:
     """Upload every case, then solve each to rn/r0 <= tol (or n_iter Newton
     steps). precision: "fp64" or "mixed"; knobs and the CFL policy
     (cfl_max, cfl_late, late_below) default to SOLVER_DEFAULTS[precision].
@@ -247,9 +282,51 @@ def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, 
     on_step(it, n_active) after each step. cfl_late / late_below: the CFL cap
     rises to cfl_late once rn/r0 < late_below (late_below = 0 never).
     fallback_max_update: a case's step limit after its first rejected step
-    (0: keep the knobs' max_update).
+    (0: keep the knobs' max_update). max_update_late: a case's step limit
+    once its rn/r0 < late_below (0: keep it). stall: the stall rule of
+    BatchSolve.run (None: off). states: optional initial states (4, ni, nc)
+    per case (None entries start from the freestream).
     BatchResult.wall covers the Newton loop (inputs resident); .upload_s the
     host-to-device set-up before it."""
-    bs = BatchSolve(cases, precision, knobs, shapes, pool_budget).upload()
+    bs = BatchSolve(cases, precision, knobs, shapes, pool_budget, states).upload()
     return bs.run(tol, n_iter, cfl0, cfl_max, keep_state, on_step, cfl_late, late_below, record_history,
-                  fallback_max_update)
+                  fallback_max_update, max_update_late, stall)
+
+
+def residual_components(case: Case, U: np.ndarray) -> np.ndarray:
+    """Full FP64 residual of state U for `case`, per component: rms over cells
+    of R_k / vol (the reference's rn for k = rho, aero2d/implicit.py:191),
+    divided by the same quantity at the freestream state (the first Newton
+    step's residual). Returns four ratios (rho, rho u, rho v, E)."""
+    from .euler import EulerSolver
+    s = EulerSolver(case.grid, dtype=np.float64)
+    try:
+        winf = s.freestream(case.mach, case.alpha)
+
+        def rms(Uk):
+            R, _ = s.residual(Uk, winf, float("nan"))
+            return np.sqrt(np.mean((R / s.vol[None]) ** 2, axis=(1, 2)))
+        return rms(np.asarray(U, np.float64)) / rms(s.init_state(winf))
+    finally:
+        s.close()
+
+
+def cross_restart(case: Case, U: np.ndarray, precision: str = "fp64", n_iter: int = 40, **path) -> dict:
+    """Restart a solve from state U (another solve's final state) and report
+    whether U is a steady state of this form's discrete equations: the
+    restarted solve (precision and solver path `path`, default the form's own)
+    takes n_iter Newton steps; a steady U stays put. The restart begins at the
+    path's largest CFL (where the solve that produced U ended), not at cfl0.
+    Returns Cl/Cd/Cm of U and of the restarted state, their differences, and
+    the restarted state's residual components (residual_components)."""
+    if "cfl0" not in path:
+        cmax, clate, _ = cfl_policy(precision, path.get("cfl_max"), path.get("cfl_late"), path.get("late_below"))
+        path["cfl0"] = max(cmax, clate or 0.0)
+    r0 = solve_batch([case], precision, tol=0.0, n_iter=0, states=[U])
+    r1 = solve_batch([case], precision, tol=0.0, n_iter=n_iter, states=[U], keep_state=True, **path)
+    comp = residual_components(case, r1.U[0])
+    return {"cl": [float(r0.cl[0]), float(r1.cl[0])], "cd": [float(r0.cd[0]), float(r1.cd[0])],
+            "cm": [float(r0.cm[0]), float(r1.cm[0])],
+            "dcl": abs(float(r1.cl[0] - r0.cl[0])), "dcd": abs(float(r1.cd[0] - r0.cd[0])),
+            "dcm": abs(float(r1.cm[0] - r0.cm[0])), "residual": comp.tolist(),
+            "newton_steps": int(r1.iterations[0])}

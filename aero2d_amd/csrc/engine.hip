@@ -101,6 +101,18 @@ int max_tiles(int ni, int nc) {
 size_t plane_stride(size_t n, size_t t) { return n + A2D_PAD_BYTES / t; }
 size_t vec_stride(size_t n, size_t t) { return 4 * n + A2D_PAD_BYTES / t; }
 
+// Kernels form element offsets inside a case's arrays in 32-bit unsigned
+// arithmetic (byte offsets from a wave-uniform base). The largest such array is
+// the preconditioner's block storage, 80 * pstride elements of the inner type:
+// a case whose block storage reaches 4 GiB is refused (FP64: about 6.7 M
+// cells, mixed: about 13.4 M cells).
+constexpr uint64_t ADDR32_LIMIT = uint64_t(1) << 32;
+uint64_t addr32_bytes(int ni, int nc, int prec) {
+  const size_t t = prec == 0 ? 8 : 4;
+  return uint64_t(80) * plane_stride(size_t(ni) * size_t(nc), t) * t;
+}
+bool addr32_ok(int ni, int nc, int prec) { return addr32_bytes(ni, nc, prec) < ADDR32_LIMIT; }
+
 Layout layout(int ni, int nc, int prec, int kmax) {
   Layout L{};
   const size_t n = size_t(ni) * nc;
@@ -234,6 +246,17 @@ CaseHost& get_case(Pool& p, int id) {
 void ensure_init() {
   if (g_device < 0) fail(-1, "a2d_init has not been called");
   HC(hipSetDevice(g_device));
+}
+
+// Every id names a live case, and no id appears twice: kernels that update a
+// case in place (state update, Krylov vectors) would otherwise run twice over
+// the same arrays within one launch.
+void check_batch(Pool& p, const std::vector<int>& slots) {
+  for (int s : slots) get_case(p, s);
+  std::vector<int> sorted(slots);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    fail(-6, "a batch must not name a case twice");
 }
 
 // ---- plans --------------------------------------------------------------------
@@ -576,6 +599,8 @@ int64_t a2d_pool_used_bytes(int64_t pool) {
 }
 
 int64_t a2d_case_bytes(int ni, int nc, int precision, int restart) {
+  if (ni < 4 || nc < 2 || (precision != 0 && precision != 1)) return -6;
+  if (!addr32_ok(ni, nc, precision)) return -6;
   return int64_t(layout(ni, nc, precision, std::max(restart, 1)).total);
 }
 
@@ -595,6 +620,9 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     Pool& p = get_pool(pool);
     if (ni < 4 || nc < 2) fail(-6, "grid too small (need ni >= 4, nc >= 2)");
     if (precision != 0 && precision != 1) fail(-6, "precision must be 0 (fp64) or 1 (mixed)");
+    if (!addr32_ok(ni, nc, precision))
+      fail(-6, "grid too large: the case's block storage (" + std::to_string(addr32_bytes(ni, nc, precision)) +
+                   " bytes) must stay below 4 GiB for 32-bit in-case addressing");
     if (p.knobs.precond_order == 1 && (ni % 2) != 0)
       fail(-6, "red-black line ordering needs an even number of cells around");
     const int kmax = std::max(1, std::min(p.knobs.gmres_restart, KMAX));
@@ -966,7 +994,7 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
     Pool& p = get_pool(pool);
     if (n <= 0) fail(-6, "empty batch");
     std::vector<int> slots(case_ids, case_ids + n);
-    for (int s : slots) get_case(p, s);
+    check_batch(p, slots);
     BatchPlan& b = get_plan(p, slots);
     std::memcpy(b.hcfl, cfl, n * sizeof(double));
     HC(hipMemcpyAsync(b.dcfl, b.hcfl, n * sizeof(double), hipMemcpyHostToDevice, p.stream));
@@ -998,7 +1026,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     Pool& p = get_pool(pool);
     if (n <= 0 || reps <= 0) fail(-6, "empty batch or no repetitions");
     std::vector<int> slots(case_ids, case_ids + n);
-    for (int s : slots) get_case(p, s);
+    check_batch(p, slots);
     BatchPlan& b = get_plan(p, slots);
     const KnobsDev kn = to_dev(p.knobs);
     const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));

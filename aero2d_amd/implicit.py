@@ -35,6 +35,8 @@ class NewtonLoop:
         self.fallback = fallback
         self.failed_once = np.zeros(nb, bool)
         self.newly_failed = np.zeros(0, int)
+        self.late = np.zeros(nb, bool)          # rn/r0 has fallen below late_below
+        self.newly_late = np.zeros(0, int)
         self.cfl = np.full(nb, float(cfl0))
         self.r0 = np.full(nb, np.nan)
         self.fails = np.zeros(nb, int)
@@ -51,6 +53,7 @@ class NewtonLoop:
         self.iters[idx] = 0
         self.last_rn[idx] = np.nan
         self.failed_once[idx] = False
+        self.late[idx] = False
 
     def update(self, idx, out):
         """Apply one step's results for batch entries idx; returns rn (NaN when
@@ -69,8 +72,12 @@ class NewtonLoop:
         r0 = self.r0[idx]
         ser = self.cfl0 * (r0 / np.where(np.isfinite(rn), np.maximum(rn, 1e-30), r0))
         cap = np.full(len(idx), float(self.cfl_max))
+        below = np.isfinite(rn) & (rn / r0 < self.late_below)
         if self.cfl_late is not None:
-            cap = np.where(np.isfinite(rn) & (rn / r0 < self.late_below), float(self.cfl_late), cap)
+            cap = np.where(below, float(self.cfl_late), cap)
+        cross = below & ~self.late[idx]
+        self.newly_late = np.asarray(idx)[cross]
+        self.late[idx] |= below
         cfl = np.clip(ser, self.cfl0, cap)
         cfl = np.where(om < 1.0, np.maximum(cfl * 0.5, 1.0), cfl)
         cfl = np.where(~finite, 1.0, cfl)

@@ -28,7 +28,7 @@ from . import _abi
 from .batch import SOLVER_DEFAULTS, Case, cfl_policy, default_knobs
 from .implicit import NewtonLoop
 
-__all__ = ["ServeResult", "SolveServer", "serve_stream", "MAX_CELLS"]
+__all__ = ["ServeResult", "SolveServer", "serve_stream", "censored_latency", "MAX_CELLS"]
 
 # In-flight cell cap per form: the cell count at which one Newton step over
 # the in-flight set takes about 250 ms on one MI355X (measured cost once the
@@ -86,6 +86,7 @@ class SolveServer:
         self.prec = _abi.PREC_MIXED if precision == "mixed" else _abi.PREC_FP64
         kn = default_knobs(precision, restart=restart, early_exit=early_exit)
         self.restart = kn.gmres_restart
+        self.order_rb = kn.precond_order == 1
         self.pool = _abi.Pool(hbm_budget, kn)
         self.budget = hbm_budget
         self.max_batch = max_batch
@@ -102,6 +103,7 @@ class SolveServer:
         self._inbox: queue.Queue = queue.Queue()
         self._thread = None
         self._stop = threading.Event()
+        self.error: BaseException | None = None     # set when the background loop stopped on one
         self.steps = 0
 
     # ---- clock -------------------------------------------------------------------
@@ -113,11 +115,26 @@ class SolveServer:
         ni, nj = case.grid.x.shape
         return _abi.case_bytes(ni, nj - 1, self.prec, self.restart)
 
-    def enqueue(self, case: Case, t_arrive: float | None = None, future: Future | None = None) -> Future:
-        """Queue a request (not thread-safe; use submit() across threads)."""
-        f = future if future is not None else Future()
+    def check(self, case: Case):
+        """Raise when the request can never be admitted: a size the library
+        refuses, a footprint above the whole HBM budget, or an odd cell count
+        around the contour under the red-black line ordering (the mixed form's
+        default), which pairs lines of two colours."""
+        ni, nj = case.grid.x.shape
+        if self.order_rb and ni % 2:
+            raise ValueError(f"{ni} cells around: the red-black line ordering of the {self.precision} "
+                             "form needs an even count")
         if self.footprint(case) > self.budget:
-            f.set_exception(_abi.BudgetExceeded("a2d_case_add", -2, "request larger than the HBM budget"))
+            raise _abi.BudgetExceeded("a2d_case_add", -2, "request larger than the HBM budget")
+
+    def enqueue(self, case: Case, t_arrive: float | None = None, future: Future | None = None) -> Future:
+        """Queue a request (not thread-safe; use submit() across threads). A
+        request that can never be admitted fails at once on its own future."""
+        f = future if future is not None else Future()
+        try:
+            self.check(case)
+        except Exception as e:          # noqa: BLE001 - reported on the request's future
+            f.set_exception(e)
             return f
         self.waiting.append((case, self.now() if t_arrive is None else t_arrive, f))
         return f
@@ -135,7 +152,11 @@ class SolveServer:
             if self.active and self.active_cells + self.cells(case) > self.max_cells:
                 break                                   # the GPU is full: wait, keep arrival order
             self.waiting.pop(0)
-            cid = self.pool.add_case(case.grid.x, case.grid.y, case.mach, case.alpha, self.prec)
+            try:
+                cid = self.pool.add_case(case.grid.x, case.grid.y, case.mach, case.alpha, self.prec)
+            except _abi.HipError as e:
+                fut.set_exception(e)                    # this request fails; admission goes on
+                continue
             self.loop.reset([cid])
             self.active[cid] = _Slot(case, t_arr, self.now(), fut, cid)
             self.active_cells += self.cells(case)
@@ -177,31 +198,61 @@ class SolveServer:
 
     # ---- threaded API ----------------------------------------------------------------------------
     def submit(self, case: Case) -> Future:
-        """Thread-safe: queue a request for the background loop (start() first)."""
+        """Thread-safe: queue a request for the background loop (start() first).
+        After the loop has stopped on an error, the request fails at once."""
         f = Future()
         self._inbox.put((case, self.now(), f))
+        if self.error is not None:      # the loop has stopped: nothing will take it
+            self._drain_inbox(RuntimeError(f"solve server stopped: {self.error!r}"))
         return f
+
+    def _drain_inbox(self, exc: BaseException):
+        while True:
+            try:
+                f = self._inbox.get_nowait()[2]
+            except queue.Empty:
+                break
+            if not f.done():
+                f.set_exception(exc)
+
+    def fail_all(self, exc: BaseException):
+        """Fail every request still waiting, in flight or in the inbox."""
+        pending = [f for _, _, f in self.waiting] + [s.future for s in self.active.values()]
+        self.waiting.clear()
+        self.active.clear()
+        self.active_cells = 0
+        for f in pending:
+            if not f.done():
+                f.set_exception(exc)
+        self._drain_inbox(exc)
 
     def start(self):
         def run():
-            while not self._stop.is_set():
-                while True:
-                    try:
-                        case, t, f = self._inbox.get_nowait()
-                    except queue.Empty:
-                        break
-                    self.enqueue(case, t, f)
-                if self.step() == 0:
-                    time.sleep(0.001)
+            try:
+                while not self._stop.is_set():
+                    while True:
+                        try:
+                            case, t, f = self._inbox.get_nowait()
+                        except queue.Empty:
+                            break
+                        self.enqueue(case, t, f)
+                    if self.step() == 0:
+                        time.sleep(0.001)
+            except BaseException as e:          # noqa: BLE001 - no request may hang
+                self.error = e
+                self.fail_all(RuntimeError(f"solve server stopped: {e!r}"))
         self._thread = threading.Thread(target=run, daemon=True)
         self._thread.start()
         return self
 
     def close(self):
+        """Stop the background loop and release the pool; requests not
+        finished by then fail with a RuntimeError ("closed")."""
         if self._thread is not None:
             self._stop.set()
             self._thread.join()
             self._thread = None
+        self.fail_all(RuntimeError("solve server closed before the request finished"))
         self.pool.close()
 
 
@@ -210,8 +261,11 @@ def serve_stream(requests, precision: str = "mixed", hbm_budget: int = 64 << 30,
     """Replay an open-loop request stream (objects with .t_arrive and .case, t
     in seconds from the start) against a SolveServer in real time. Requests are
     released when the wall clock reaches their arrival time; the run ends when
-    all released requests finish or, with `horizon`, at that many seconds
-    (unfinished requests are reported). Returns (results, server, wall)."""
+    all released requests finish or, with `horizon`, at that many seconds.
+    Returns (results, server, wall): the finished requests' results, and the
+    server, whose `unfinished` lists the arrival times of the requests released
+    but not finished by the end (waiting or in flight; their latency is at
+    least wall - t_arrive, see censored_latency)."""
     srv = SolveServer(precision, hbm_budget, tol=tol, **kw)
     futures = []
     k = 0
@@ -234,4 +288,32 @@ def serve_stream(requests, precision: str = "mixed", hbm_budget: int = 64 << 30,
     finally:
         srv.close()
     results = [f.result() for f in futures if f.done() and f.exception() is None]
+    srv.unfinished = [reqs[i].t_arrive for i, f in enumerate(futures)
+                      if not (f.done() and f.exception() is None)]
     return results, srv, wall
+
+
+def censored_latency(results, unfinished, wall, q=(50, 99)) -> dict:
+    """Latency percentiles over EVERY released request: a finished request
+    counts its latency, an unfinished one wall - t_arrive (a lower bound: it is
+    censored at the end of the run). Also the completed fraction. Raising
+    censored values can only raise a percentile, so each percentile is a lower
+    bound; it is exact unless a censored value is among the order statistics
+    it interpolates (then it is listed in `censored_at`)."""
+    lat = [r.latency for r in results] + [wall - t for t in unfinished]
+    flag = [False] * len(results) + [True] * len(unfinished)
+    out = {"released": len(lat), "completed": len(results),
+           "completed_fraction": len(results) / len(lat) if lat else float("nan")}
+    if not lat:
+        return out
+    order = np.argsort(lat, kind="stable")
+    lat_s = np.asarray(lat)[order]
+    flag_s = np.asarray(flag)[order]
+    cens = []
+    for p in q:
+        out[f"p{p}_ms"] = float(np.percentile(lat_s, p)) * 1e3
+        k_hi = int(np.ceil(p / 100 * (len(lat_s) - 1)))     # numpy's linear rule reads ranks <= k_hi
+        if flag_s[:k_hi + 1].any():
+            cens.append(p)
+    out["censored_at"] = cens
+    return out

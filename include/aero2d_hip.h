@@ -77,7 +77,8 @@ int a2d_pool_destroy(int64_t pool);
 int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k);
 int a2d_pool_get_knobs(int64_t pool, a2d_knobs* k);
 int64_t a2d_pool_used_bytes(int64_t pool);
-/* Device bytes one case of this size occupies (the admission footprint). */
+/* Device bytes one case of this size occupies (the admission footprint), or
+ * -6 for a size a2d_case_add refuses (see there). */
 int64_t a2d_case_bytes(int ni, int nc, int precision, int restart);
 
 /* Cases. Replaces EulerSolver(grid, dtype) (aero2d/euler.py:219-250) for one
@@ -85,7 +86,10 @@ int64_t a2d_case_bytes(int ni, int nc, int precision, int restart);
  * aero2d/euler.py:44-60), freestream winf = (1, M cos a, M sin a, 1/gamma)
  * (aero2d/euler.py:253-256) and U = freestream (aero2d/euler.py:383-384).
  * x, y: node coordinates (ni, nc + 1). Fails with -2 when the pool's HBM
- * budget would be exceeded. */
+ * budget would be exceeded, and with -6 (before any device work) for a grid
+ * whose preconditioner block storage, 80 * ni * nc values of the inner type,
+ * would reach 4 GiB: kernels address inside a case with 32-bit offsets. That
+ * caps a case at about 6.7 M cells in FP64 and 13.4 M cells in the mixed form. */
 int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y, double mach,
                  double alpha_deg, int precision, int* case_id);
 int a2d_case_remove(int64_t pool, int case_id);
@@ -128,7 +132,8 @@ int a2d_precond_setup(int64_t pool, int case_id, double cfl);
 int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x);
 
 /* One pseudo-transient Newton step (ImplicitEuler.step, aero2d/implicit.py:169-192)
- * for n cases of one precision, each at its own CFL. The update is applied
+ * for n cases of one precision, each at its own CFL. The ids must be distinct
+ * (-6 otherwise: a case's in-place updates would run twice). The update is applied
  * on the device only when rn, the step and its limiter are finite
  * (aero2d/implicit.py:218-225). out: 10 doubles per case:
  *   rn (before the update), omega, accepted (0/1), cl, cd, cm (after),
@@ -144,7 +149,7 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
  *   8 the per-cell pass of a coupled line sweep   9 the line recurrences alone
  *   10 one whole preconditioner application (all sweeps, in the pool's order)
  * The cases must have taken a Newton step (their blocks and Krylov vectors
- * hold data); the state is not changed. */
+ * hold data); the state is not changed. The ids must be distinct. */
 int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms);
 
 /* Test hook: the device HLLC flux on n faces. WL, WR (4, n), S (2, n), F (4, n). */

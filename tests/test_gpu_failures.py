@@ -111,3 +111,52 @@ def test_invalid_switch_knob_is_refused(name):
         assert getattr(pool.knobs(), name) == before
     finally:
         pool.close()
+
+
+def test_oversized_case_is_refused_before_any_device_work():
+    """Kernels address inside a case with 32-bit byte offsets; the largest
+    array is the block storage, 80 * cells values of the inner type. A case
+    whose block storage reaches 4 GiB is refused with -6 by a2d_case_add (and
+    a2d_case_bytes reports -6 for it); nothing is allocated."""
+    pool = _abi.Pool(0, default_knobs("fp64"))
+    try:
+        ni, nc = 4096, 1640            # 6.72 M cells: 80 * 8 B * n = 4.30 GB >= 4 GiB in FP64
+        assert 80 * 8 * ni * nc >= 1 << 32
+        assert _abi.load().a2d_case_bytes(ni, nc, _abi.PREC_FP64, 50) == -6
+        assert _abi.load().a2d_case_bytes(ni, nc, _abi.PREC_MIXED, 20) > 0      # half the bytes in FP32
+        x = np.zeros((ni, nc + 1))
+        y = np.zeros((ni, nc + 1))
+        with pytest.raises(_abi.HipError) as e:
+            pool.add_case(x, y, 0.5, 2.0, _abi.PREC_FP64)
+        assert e.value.code == -6 and "4 GiB" in str(e.value)
+        assert pool.used_bytes() == 0
+    finally:
+        pool.close()
+
+
+def test_repeated_case_id_is_refused():
+    """A batch that names a case twice would run that case's in-place kernels
+    twice per launch: a2d_newton_step and a2d_time_kernel refuse it with -6
+    and leave the state untouched."""
+    g = ogrid(naca4("0012", 17), nj=9, ds0=2e-2, radius=5.0)
+    pool = _abi.Pool(0, default_knobs("fp64"))
+    try:
+        a = pool.add_case(g.x, g.y, 0.5, 2.0, _abi.PREC_FP64)
+        b = pool.add_case(g.x, g.y, 0.6, 1.0, _abi.PREC_FP64)
+        U0 = np.empty((4, 32, 8))
+        _abi.call("a2d_case_get_state", pool.handle, a, _abi.dptr(U0))
+        ids = np.array([a, b, a], np.int32)
+        out = np.empty((3, 10))
+        cfl = np.full(3, 10.0)
+        with pytest.raises(_abi.HipError) as e:
+            _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), 3, _abi.dptr(cfl), _abi.dptr(out))
+        assert e.value.code == -6
+        ms = np.zeros(1)
+        with pytest.raises(_abi.HipError) as e:
+            _abi.call("a2d_time_kernel", pool.handle, _abi.iptr(ids), 3, 0, 1, _abi.dptr(ms))
+        assert e.value.code == -6
+        U1 = np.empty((4, 32, 8))
+        _abi.call("a2d_case_get_state", pool.handle, a, _abi.dptr(U1))
+        assert np.array_equal(U0, U1)
+    finally:
+        pool.close()

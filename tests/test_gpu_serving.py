@@ -160,3 +160,51 @@ def test_threaded_submit():
         assert all(r.converged for r in res)
     finally:
         srv.close()
+
+
+def test_bad_request_fails_on_its_own_future():
+    """A request the server can never solve (an odd cell count around the
+    contour under the mixed form's red-black line ordering) fails on its own
+    future; a valid request submitted beside it completes, and the background
+    loop keeps serving."""
+    good = ogrid(naca4("0012", 33), nj=17, ds0=1e-2, radius=10.0)          # 64 x 16
+    x, y = good.x[:-1].copy(), good.y[:-1].copy()                          # 63 cells around
+    from aero2d_amd.grid import OGrid
+    odd = OGrid(x, y)
+    srv = SolveServer("mixed", hbm_budget=8 << 30, tol=1e-9).start()
+    try:
+        f_bad = srv.submit(Case(odd, 0.5, 2.0, "odd"))
+        f_good = srv.submit(Case(good, 0.5, 2.0, "good"))
+        with pytest.raises(ValueError):
+            f_bad.result(timeout=60)
+        r = f_good.result(timeout=120)
+        assert r.converged
+        f_again = srv.submit(Case(good, 0.55, 1.0, "again"))
+        assert f_again.result(timeout=120).converged
+        assert srv.error is None
+    finally:
+        srv.close()
+
+
+def test_server_failure_fails_every_pending_request():
+    """When the background loop stops on an unexpected error, every request
+    waiting, in flight or submitted later fails instead of hanging."""
+    g = ogrid(naca4("0012", 33), nj=17, ds0=1e-2, radius=10.0)
+    srv = SolveServer("mixed", hbm_budget=8 << 30, tol=1e-9)
+
+    def broken_step():
+        srv._admit()
+        raise RuntimeError("forced from the host")
+    srv.step = broken_step
+    srv.start()
+    try:
+        futs = [srv.submit(Case(g, 0.5, 2.0, f"r{k}")) for k in range(3)]
+        for f in futs:
+            with pytest.raises(RuntimeError):
+                f.result(timeout=60)
+        late = srv.submit(Case(g, 0.5, 2.0, "late"))
+        with pytest.raises(RuntimeError):
+            late.result(timeout=10)
+        assert isinstance(srv.error, RuntimeError)
+    finally:
+        srv.close()

@@ -112,3 +112,20 @@ def test_newton_loop_reports_each_case_first_failure_once():
     plain = NewtonLoop(3, 10.0, 300.0)
     plain.update(np.arange(3), out)
     assert plain.newly_failed.tolist() == []
+
+
+def test_censored_latency_counts_unfinished_requests():
+    """Latency percentiles include unfinished requests at wall - t_arrive and
+    flag a percentile that a censored value reaches."""
+    from types import SimpleNamespace
+    from aero2d_amd.serve import censored_latency
+    done = [SimpleNamespace(latency=v) for v in (1.0, 2.0, 3.0)]
+    out = censored_latency(done, [5.0, 9.0], wall=10.0)       # censored at 5 s and 1 s
+    assert out["released"] == 5 and out["completed"] == 3
+    assert out["completed_fraction"] == 0.6
+    assert np.isclose(out["p50_ms"], 2000.0)                  # sorted 1, 1c, 2, 3, 5c
+    assert out["censored_at"] == [50, 99]
+    out = censored_latency(done, [], wall=10.0)
+    assert out["censored_at"] == [] and np.isclose(out["p50_ms"], 2000.0)
+    out = censored_latency(done, [0.0], wall=10.0)            # censored at 10 s: above p50
+    assert out["censored_at"] == [99]
