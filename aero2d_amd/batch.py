@@ -78,13 +78,24 @@ class BatchResult:
 # forms solve the same discrete equations; where
 # those have more than one steady state (transonic cases with a shock), the
 # solver path decides which one a case reaches.
+#
+# Stall rule (both forms alike): a case that has taken 250 Newton steps and
+# whose best rn/r0 over the last 200 steps is not 10x below its best before
+# them is retired as not converged. Transonic cases on a limit cycle sit at
+# rn/r0 ~ 1e-3 from step ~50 on; the slowest case that does converge (32
+# transonic cases of 512 x 256) sat on a plateau for ~200 steps before its
+# shock settled, which the window keeps.
+STALL_RULE = (250, 200, 10.0)
+
 SOLVER_DEFAULTS = {
     "fp64": dict(restart=50, early_exit=0, precond_order=0, jacobi_sweeps=4, cfl_max=300.0,
                  cfl_late=None, late_below=0.0, max_update=0.2, max_update_late=None,
-                 fallback_max_update=0.0, positivity=0, precond_half=0, basis_half=0),
+                 fallback_max_update=0.0, positivity=0, precond_half=0, basis_half=0,
+                 stall=STALL_RULE),
     "mixed": dict(restart=20, early_exit=1, precond_order=1, jacobi_sweeps=2, cfl_max=300.0,
                   cfl_late=1000.0, late_below=1e-3, max_update=0.5, max_update_late=None,
-                  fallback_max_update=0.0, positivity=0, precond_half=1, basis_half=1),
+                  fallback_max_update=0.0, positivity=0, precond_half=1, basis_half=1,
+                  stall=STALL_RULE),
 }
 
 
@@ -125,7 +136,8 @@ def path_of(form: str) -> dict:
                        precond_order=d["precond_order"], jacobi_sweeps=d["jacobi_sweeps"],
                        max_update=d["max_update"], positivity=d["positivity"])
     return dict(knobs=kn, cfl_max=d["cfl_max"], cfl_late=d["cfl_late"], late_below=d["late_below"],
-                fallback_max_update=d["fallback_max_update"], max_update_late=d["max_update_late"] or 0.0)
+                fallback_max_update=d["fallback_max_update"], max_update_late=d["max_update_late"] or 0.0,
+                stall=d["stall"])
 
 
 class BatchSolve:
@@ -174,8 +186,9 @@ class BatchSolve:
         falls below late_below (default: the form's; 0 = never). stall =
         (after, window, drop): a case that has taken `after` Newton steps and
         whose best rn/r0 over the last `window` steps is not `drop` times below
-        its best before them is retired as stalled (not converged); None keeps
-        every case until tol or n_iter."""
+        its best before them is retired as stalled (not converged); None
+        takes the form's rule (STALL_RULE), () keeps every case until tol or
+        n_iter."""
         if self.pool is None:
             self.upload()
         cases, ids, pool = self.cases, self.ids, self.pool
@@ -185,6 +198,9 @@ class BatchSolve:
             fallback_max_update = SOLVER_DEFAULTS[self.precision]["fallback_max_update"]
         if max_update_late is None:
             max_update_late = SOLVER_DEFAULTS[self.precision]["max_update_late"] or 0.0
+        if stall is None:
+            stall = SOLVER_DEFAULTS[self.precision]["stall"]
+        stall = tuple(stall) if stall else None
         fallback = fallback_max_update > 0 and fallback_max_update != self.knobs.max_update
         staged = max_update_late > 0 and max_update_late != self.knobs.max_update
         t0 = time.perf_counter()
@@ -284,7 +300,7 @@ def solve_batch(cases: list[Case], precision: str = "fp64", tol: float = 1e-10, 
     fallback_max_update: a case's step limit after its first rejected step
     (0: keep the knobs' max_update). max_update_late: a case's step limit
     once its rn/r0 < late_below (0: keep it). stall: the stall rule of
-    BatchSolve.run (None: off). states: optional initial states (4, ni, nc)
+    BatchSolve.run (None: the form's, (): off). states: optional initial states (4, ni, nc)
     per case (None entries start from the freestream).
     BatchResult.wall covers the Newton loop (inputs resident); .upload_s the
     host-to-device set-up before it."""

@@ -1,17 +1,25 @@
 """Benchmark: converged cases/s of the batched implicit Euler solve on one MI355X.
 
-Workload (default): 64 NACA 4-digit cases on 256 x 128 O-grids, FP64, each
-solved to a relative density residual rn/r0 <= 1e-10 — the uniform FP64 batch
-whose throughput is the headline metric. Airfoils, Mach numbers and angles
-are drawn from a fixed seed: m ~ U[0, 4]%, p ~ U[0.3, 0.5], t ~ U[9, 15]%,
-M ~ U[0.3, 0.7], alpha ~ U[-4, 6] deg; grids ogrid(nj = 129, radius 20,
-ds0 = 20 (q - 1)/(q^128 - 1), q = 1.1^(64/128); aero2d_amd.workloads.config2).
+Workload (default, `--workload config3`): 32 transonic cases on 512 x 256
+O-grids (NACA 0012 or 2412 by a seeded coin, M ~ U[0.72, 0.82], alpha ~
+U[0, 2.5] deg; aero2d_amd.workloads.config3), each solved to a relative
+density residual rn/r0 <= 1e-10 by the mixed-precision form (FP32
+Jacobian-vector products, line preconditioner and Krylov arithmetic, FP64
+residual, update and forces). A step is one complete batched solve of all
+cases; W untimed solves, then K timed ones give `value`.
 
-A step is one complete batched solve of all cases (warmup steps untimed).
-The line also carries the mixed-precision form on the same batch, per-case
-completion latency (p50/p99), live HIP-event rooflines of the coupled
-preconditioner sweep kernel (the largest share of device time) and of the
-residual kernel, and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+Beside it, once each: the plain FP64 form with the reference's algorithm
+(speedup_mixed_vs_fp64), FP64 on the mixed form's solver path (splits the
+speedup into precision and solver path), the mixed form with FP32 storage,
+and the mixed form without the stall rule; per-case |dCl|, |dCd|, |dCm|
+against the FP64 solve with cross-restart checks of the cases outside the
+bar; live HIP-event rooflines (the mixed form's dominant kernel, the FP32
+Jacobian-vector product; the FP64 residual; the FP64 coupled line sweep); a
+CPU baseline (the FP64 oracle on the host cores, bounded sample).
+
+Other workloads: config1 (one case), config2 (64 cases of 256 x 128, FP64
+vs mixed), config4 (256 cases of uneven sizes), config5 (serving), kernels /
+kernels3 (kernel timings alone, for counter runs).
 """
 from __future__ import annotations
 
@@ -76,6 +84,10 @@ def time_kernels(cases, precision, reps=20, knobs=None):
 # the counter reads for it) and write by 1.0 (profiles/traffic_config2_fp64_pmc.txt).
 TRAFFIC_CELLS = 64 * 256 * 128
 MEASURED_TRAFFIC = {"line_sweep": 1492.1e6, "residual": 404.9e6}
+# HBM bytes per launch at the transonic batch (32 x 512 x 256 cells), from the
+# same two counter runs of `bench.py --workload kernels3` (none measured yet:
+# null in the bench line)
+MEASURED_TRAFFIC_C3 = {}
 
 
 def rooflines(kt, cells):
@@ -103,6 +115,18 @@ def run_kernels(a):
     out = {"kernel_ms": kt, "cells": a.cases * a.ni * a.nc}
     out.update(rooflines(kt, out["cells"]))
     return out
+
+
+def run_kernels3(a):
+    """Kernel timings alone on the transonic batch (32 x 512 x 256 cells): the
+    mixed form's kernels, then the FP64 residual and Jacobian-vector product on
+    16 of the cases (used under rocprofv3 for the per-dispatch counters)."""
+    from aero2d_amd.workloads import config3
+    cases = config3(a.cases or 32)
+    kt = time_kernels(cases, "mixed", reps=5)
+    k64 = time_kernels(cases[:16], "fp64", reps=5)
+    return {"kernel_ms_mixed": kt, "kernel_ms_fp64_16cases": k64,
+            "cells": sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases)}
 
 
 def cpu_baseline(case, gpu_iters, steps=2, scale_by="the GPU run's mean"):
@@ -232,9 +256,86 @@ def run_config1(a):
     return out
 
 
+def cross_restart_checks(cases, ra, rb, flagged, n_iter=80):
+    """For each flagged case (converged in both results, |dCl| > 1e-5), restart
+    the FP64 reference-path solve from the mixed final state and the mixed
+    solve from the FP64 final state. A state that the other form's Newton
+    iteration keeps (Cl moves <= 1e-8, every residual component <= 1e-10 of
+    its freestream value) is a steady state of the same discrete equations:
+    the two forms then reached two distinct steady states, not two
+    precisions of one."""
+    from aero2d_amd.batch import cross_restart
+    out = []
+    for k in flagged:
+        a = cross_restart(cases[k], ra.U[k], "fp64", n_iter=n_iter)
+        b = cross_restart(cases[k], rb.U[k], "mixed", n_iter=n_iter)
+        ok_a = a["dcl"] <= 1e-8 and max(a["residual"]) <= 1e-10
+        ok_b = b["dcl"] <= 1e-8 and max(b["residual"]) <= 1e-10
+        out.append({"case": int(k), "cl_mixed": float(ra.cl[k]), "cl_fp64": float(rb.cl[k]),
+                    "fp64_from_mixed_state": a, "mixed_from_fp64_state": b,
+                    "distinct_steady_states": bool(ok_a and ok_b)})
+        print(f"# cross restart case {k}: fp64 from mixed dCl {a['dcl']:.1e} res {max(a['residual']):.1e}; "
+              f"mixed from fp64 dCl {b['dcl']:.1e} res {max(b['residual']):.1e}", file=sys.stderr, flush=True)
+    return out
+
+
+def accuracy(rmx, r64, cases, restart_checks=True):
+    """Per-case |dCl|, |dCd|, |dCm| of the mixed form against the FP64
+    reference-path solve, the count within the bar (|dCl| <= 1e-5 and
+    |dCd| <= 1e-6) over the cases both converge, and cross-restart checks of
+    the others."""
+    both = rmx.converged & r64.converged
+    dcl, dcd, dcm = (np.abs(getattr(rmx, k) - getattr(r64, k)) for k in ("cl", "cd", "cm"))
+    within = both & (dcl <= 1e-5) & (dcd <= 1e-6)
+    flagged = [int(k) for k in np.nonzero(both & ~within)[0]]
+    out = {"cases_both_converged": int(both.sum()), "cases_within_bar": int(within.sum()),
+           "bar": {"dcl": 1e-5, "dcd": 1e-6},
+           "max_within_bar": {"dcl": float(dcl[within].max()) if within.any() else None,
+                              "dcd": float(dcd[within].max()) if within.any() else None,
+                              "dcm": float(dcm[within].max()) if within.any() else None},
+           "per_case": {"dcl": dcl.tolist(), "dcd": dcd.tolist(), "dcm": dcm.tolist()},
+           "outside_bar": flagged}
+    if restart_checks and flagged:
+        chk = cross_restart_checks(cases, rmx, r64, flagged)
+        out["cross_restart"] = chk
+        out["distinct_steady_state_cases"] = [c["case"] for c in chk if c["distinct_steady_states"]]
+        out["unexplained_cases"] = [c["case"] for c in chk if not c["distinct_steady_states"]]
+    else:
+        out["distinct_steady_state_cases"] = []
+        out["unexplained_cases"] = []
+    return out
+
+
+def form_entry(r, walls=None, convs=None):
+    walls = [r.wall] if walls is None else walls
+    convs = [int(r.converged.sum())] if convs is None else convs
+    lat = np.percentile(r.t_done, [50, 99]) * 1e3
+    stalled = getattr(r, "stalled", np.zeros(len(r.cl), bool))
+    return {"value": float(np.sum(convs) / np.sum(walls)), "unit": "cases/s",
+            "converged": int(r.converged.sum()), "cases": len(r.cl),
+            "not_converged": [int(k) for k in np.nonzero(~r.converged)[0]],
+            "retired_by_stall_rule": [int(k) for k in np.nonzero(stalled)[0]],
+            "wall_s": float(np.mean(walls)), "newton_steps_mean": float(r.iterations.mean()),
+            "arnoldi_per_newton_mean": float(r.arnoldi.mean()),
+            "latency_ms": {"p50": float(lat[0]), "p99": float(lat[1])},
+            "upload_s": float(r.upload_s),
+            "value_pcie_inclusive": float(np.sum(convs) / (np.sum(walls) + len(walls) * r.upload_s))}
+
+
+# Dominant kernel of the mixed transonic solve (largest single share of device
+# time): the FP32 Jacobian-vector product, 17 words per cell: read U 4, v 4,
+# face metrics S_i 2 and S_j 2, diag 1; write 4 (4 B each).
+JVP32_BYTES_PER_CELL = 68
+JVP64_BYTES_PER_CELL = 136
+
+
 def run_config3(a):
-    """Transonic mixed vs FP64: 32 NACA 0012/2412 cases on 512 x 256."""
-    from aero2d_amd.batch import SOLVER_DEFAULTS, solve_batch
+    """Transonic mixed vs FP64: 32 NACA 0012/2412 cases on 512 x 256, the
+    workload the >= 10x mixed-over-FP64 target is stated on. K timed mixed
+    solves after W untimed ones give `value`; the FP64 reference-path solve,
+    FP64 on the mixed form's path, the mixed form with FP32 storage and the
+    mixed form without the stall rule run once each."""
+    from aero2d_amd.batch import SOLVER_DEFAULTS, default_knobs, path_of, solve_batch
     from aero2d_amd.workloads import config3
     cases = config3(a.cases or 32)
     n_iter = a.n_iter or 800
@@ -243,32 +344,88 @@ def run_config3(a):
         rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
         return {"workload": f"{len(cases)} transonic cases on 512x256, mixed only", "wall_s": rmx.wall,
                 "converged": int(rmx.converged.sum()), "newton_steps_mean": float(rmx.iterations.mean())}
+    for w in range(a.warmup):
+        r = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter)
+        print(f"# warmup mixed: {r.wall:.2f} s, converged {int(r.converged.sum())}", file=sys.stderr, flush=True)
+    walls, convs, rmx = [], [], None
+    for s in range(a.steps):
+        # the last timed solve keeps its final states (copied after the timed region)
+        rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, keep_state=s == a.steps - 1)
+        walls.append(rmx.wall)
+        convs.append(int(rmx.converged.sum()))
+        print(f"# mixed step {s}: {rmx.wall:.2f} s, converged {convs[-1]}/{len(cases)}, Newton steps mean "
+              f"{rmx.iterations.mean():.1f}", file=sys.stderr, flush=True)
     r64 = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, keep_state=True)
-    print(f"# fp64: {r64.wall:.1f} s, converged {int(r64.converged.sum())}", file=sys.stderr, flush=True)
-    rmx = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, keep_state=True)
-    print(f"# mixed: {rmx.wall:.1f} s, converged {int(rmx.converged.sum())}", file=sys.stderr, flush=True)
-    r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, **mixed_path_fp64())
+    print(f"# fp64 (reference path): {r64.wall:.1f} s, converged {int(r64.converged.sum())}",
+          file=sys.stderr, flush=True)
+    r64m = solve_batch(cases, "fp64", tol=a.tol, n_iter=n_iter, **path_of("mixed"))
     print(f"# fp64 on the mixed form's path: {r64m.wall:.1f} s, converged {int(r64m.converged.sum())}",
           file=sys.stderr, flush=True)
-    v64 = r64.converged.sum() / r64.wall
-    vmx = rmx.converged.sum() / rmx.wall
-    out = {"metric": METRIC, "value": float(vmx), "unit": "cases/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-           "ms_per_step": 1e3 * rmx.wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 inner / f64 outer", "data": "synthetic",
-           "config": {"workload": f"{len(cases)} transonic NACA 0012/2412 cases on 512x256 O-grids, "
-                                  f"mixed vs FP64, rn/r0 <= {a.tol:g}"},
-           "fp64": {"value": float(v64), "converged": int(r64.converged.sum()), "wall_s": r64.wall,
-                    "newton_steps_mean": float(r64.iterations.mean()), "knobs": SOLVER_DEFAULTS["fp64"]},
-           "mixed": {"value": float(vmx), "converged": int(rmx.converged.sum()), "wall_s": rmx.wall,
-                     "newton_steps_mean": float(rmx.iterations.mean()),
-                     "arnoldi_per_newton_mean": float(rmx.arnoldi.mean()),
-                     "knobs": SOLVER_DEFAULTS["mixed"]},
-           "fp64_mixed_path": {"value": float(r64m.converged.sum() / r64m.wall),
-                               "converged": int(r64m.converged.sum()), "wall_s": r64m.wall},
-           "speedup_mixed_vs_fp64": float(vmx / v64) if v64 > 0 else None,
-           "converged_fp64": r64.converged.tolist(), "converged_mixed": rmx.converged.tolist(),
+    r32 = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter,
+                      knobs=default_knobs("mixed", precond_half=0, basis_half=0))
+    print(f"# mixed, FP32 storage: {r32.wall:.1f} s, converged {int(r32.converged.sum())}",
+          file=sys.stderr, flush=True)
+    rns = solve_batch(cases, "mixed", tol=a.tol, n_iter=n_iter, stall=())
+    print(f"# mixed, no stall rule: {rns.wall:.1f} s, converged {int(rns.converged.sum())}",
+          file=sys.stderr, flush=True)
+    vmx = float(np.sum(convs) / np.sum(walls))
+    v64 = float(r64.converged.sum() / r64.wall)
+    v64m = float(r64m.converged.sum() / r64m.wall)
+    v32 = float(r32.converged.sum() / r32.wall)
+    acc = accuracy(rmx, r64, cases, restart_checks=not a.no_restart_checks)
+    acc32 = accuracy(r32, r64, cases, restart_checks=False)
+    out = {"metric": METRIC, "value": vmx, "unit": "cases/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * float(np.mean(walls)), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 outer", "data": "synthetic",
+           "config": {"workload": f"{len(cases)} transonic NACA 0012/2412 cases on 512x256 O-grids "
+                                  f"(M 0.72-0.82, alpha 0-2.5 deg), mixed precision, rn/r0 <= {a.tol:g}; "
+                                  "a step is one batched solve of all cases",
+                      "cases": len(cases), "ni": 512, "nc": 256, "tol": a.tol, "n_iter": n_iter},
+           "converged": int(rmx.converged.sum()),
+           "speedup_mixed_vs_fp64": vmx / v64 if v64 > 0 else None,
+           "speedup_split": {"precision": v64m and vmx / v64m, "solver_path": v64 and v64m / v64,
+                             "note": "mixed / FP64-on-the-mixed-path is the precision part; "
+                                     "FP64-on-the-mixed-path / FP64 reference path the solver-path part"},
+           "claim": "the >= 10x target is stated for the mixed form's default (FP16 storage of the Krylov "
+                    "basis and the factored couplings) against the FP64 reference algorithm; "
+                    "mixed_fp32_storage is the form with FP32 storage",
+           "mixed": form_entry(rmx, walls, convs) | {"knobs": SOLVER_DEFAULTS["mixed"],
+                                                     "accuracy_vs_fp64": acc},
+           "fp64": form_entry(r64) | {"knobs": SOLVER_DEFAULTS["fp64"]},
+           "fp64_mixed_path": form_entry(r64m),
+           "mixed_fp32_storage": form_entry(r32) | {"speedup_vs_fp64": v32 / v64 if v64 > 0 else None,
+                                                    "accuracy_vs_fp64": {k: acc32[k] for k in (
+                                                        "cases_both_converged", "cases_within_bar",
+                                                        "outside_bar", "max_within_bar")}},
+           "mixed_no_stall_rule": form_entry(rns),
            "cases": [(c.name, c.mach, c.alpha) for c in cases]}
-    out.update(compare_forms(r64, rmx, r64m, cases))
+    if not a.no_kernel_timing:
+        kt = time_kernels(cases, "mixed", reps=20)
+        cells = sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases)
+        rl = {}
+        for name, bpc, kname in (("jvp", JVP32_BYTES_PER_CELL, "k_stencil<float, true> (FP32 Jacobian-vector product)"),
+                                 ("residual", RES_BYTES_PER_CELL, "k_stencil<double, false> (FP64 residual)")):
+            ach = bpc * cells / (kt[name] * 1e-3) / 1e9
+            rl[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_C3.get(name),
+                        "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name], "kernel": kname}
+        out["roofline"] = rl["jvp"]
+        out["roofline_residual"] = rl["residual"]
+        out["kernel_ms_mixed"] = kt
+        k64 = time_kernels(cases[:16], "fp64", reps=10)
+        c64 = sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases[:16])
+        sw = rooflines(k64, c64)["roofline"]
+        sw["cells"] = c64
+        out["roofline_fp64_sweep"] = sw
+        out["kernel_ms_fp64_16cases"] = k64
+    if not a.no_cpu_baseline:
+        conv = r64.converged
+        steps = float(r64.iterations[conv].mean()) if conv.any() else float(r64.iterations.mean())
+        cb = cpu_baseline(cases[0], steps, scale_by="the GPU FP64 run's mean over converged cases,")
+        # the oracle runs the FP64 reference algorithm: it converges the same cases
+        cb["value"] *= float(conv.mean())
+        cb["sample"] += f"; times the FP64 converged fraction {conv.mean():.3f}"
+        out["cpu_baseline"] = cb
     return out
 
 
@@ -330,8 +487,8 @@ def run_config5(a):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="config2",
-                    choices=["config1", "config2", "config3", "config4", "config5", "kernels"])
+    ap.add_argument("--workload", default="config3",
+                    choices=["config1", "config2", "config3", "config4", "config5", "kernels", "kernels3"])
     ap.add_argument("--rates", default="10,25,50,100,200,400")
     ap.add_argument("--horizon", type=float, default=30.0)
     ap.add_argument("--precisions", default="mixed,fp64")
@@ -348,6 +505,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="config3: the mixed solve alone")
+    ap.add_argument("--no-restart-checks", action="store_true",
+                    help="config3: skip the cross-restart checks of cases outside the accuracy bar")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (cases are independent; "
@@ -356,7 +515,7 @@ def main():
 
     if a.workload != "config2":
         fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,
-              "kernels": run_kernels}[a.workload]
+              "kernels": run_kernels, "kernels3": run_kernels3}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 
