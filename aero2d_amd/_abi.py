@@ -30,7 +30,8 @@ class Knobs(ctypes.Structure):
                 ("gmres_restart", ctypes.c_int), ("jacobi_sweeps", ctypes.c_int),
                 ("early_exit", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("precond_order", ctypes.c_int), ("positivity", ctypes.c_int),
-                ("precond_half", ctypes.c_int), ("basis_half", ctypes.c_int)]
+                ("precond_half", ctypes.c_int), ("basis_half", ctypes.c_int),
+                ("stencil_march", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); the exported surface of include/aero2d_hip.h

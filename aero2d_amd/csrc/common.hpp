@@ -65,7 +65,7 @@ struct CaseDev {
   int prec;             // 0 = fp64 inner, 1 = fp32 inner
   int kmax;             // Krylov dimension allocated
   int npart;            // partial-sum slots available per reduction
-  int ntiles;           // stencil tiles of this case's tile shape
+  int ntiles;           // stencil workgroups (= residual-norm partials) of this case's tile shape
   int nvecb;            // vector-kernel blocks (VEC_CHUNK cells each)
   int nlineb;           // line-factorisation blocks (LINES_PER_BLOCK lines each)
   int nlinew;           // 8-line groups (one wavefront of a sweep each): norm partials

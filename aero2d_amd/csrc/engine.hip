@@ -85,12 +85,11 @@ struct Layout {
   int npart;
 };
 
+// Residual-norm partials of the largest stencil launch (any tile shape, or
+// the column march).
 int max_tiles(int ni, int nc) {
-  int m = 0;
-  for (int s = 0; s < stencil_num_shapes(); ++s) {
-    StencilShape sh = stencil_shape(s);
-    m = std::max(m, cdiv(ni, sh.ti) * cdiv(nc, sh.tj));
-  }
+  int m = march_units(ni, nc, MARCH_ROWS_RES);
+  for (int s = 0; s < stencil_num_shapes(); ++s) m = std::max(m, stencil_blocks(s, ni, nc));
   return m;
 }
 
@@ -181,6 +180,10 @@ struct BatchPlan {
   double* hout = nullptr;
   Plan pc{}, pcell{}, pvec{}, pline{}, psw{}, prb{};
   int ncell = 0, nvec = 0, nline = 0, nsw = 0, nrb = 0;
+  // column-march stencil: residual (pmr) and tangent (pmj) block ranges, and
+  // the batch's unit counts that decide whether it is used
+  Plan pmr{}, pmj{};
+  int nmr = 0, nmj = 0;
   struct Group {
     Plan p;
     int shape, nblocks;
@@ -230,6 +233,7 @@ void default_knobs(a2d_knobs* k) {
   k->positivity = 0;
   k->precond_half = 0;
   k->basis_half = 0;
+  k->stencil_march = 1;
 }
 
 Pool& get_pool(int64_t id) {
@@ -306,6 +310,9 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   const int lrb = big_rb >= big_min ? SWEEP_LINES_BIG : SWEEP_LINES_SMALL;
   auto psw = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni, lsw); });
   auto prb = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni / 2, lrb); });
+  const int rows_jvp = bp->prec == 0 ? MARCH_ROWS_JVP64 : MARCH_ROWS_JVP32;
+  auto pmr = add_prefix([](const CaseHost& c) { return march_blocks(c.ni, c.nc, MARCH_ROWS_RES); });
+  auto pmj = add_prefix([&](const CaseHost& c) { return march_blocks(c.ni, c.nc, rows_jvp); });
   std::map<int, std::vector<int>> groups;
   for (int s : slots) groups[p.cases[s].dev.shape].push_back(s);
   struct G {
@@ -356,6 +363,10 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   bp->prb = mk(prb.first);
   bp->prb.lines = lrb;
   bp->nrb = prb.second;
+  bp->pmr = mk(pmr.first);
+  bp->nmr = pmr.second;
+  bp->pmj = mk(pmj.first);
+  bp->nmj = pmj.second;
   for (auto& g : gs) {
     BatchPlan::Group gr;
     gr.p.cases = p.dcases;
@@ -393,12 +404,43 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
   }
 }
 
+// Stencil kernel choice for a batch (knob stencil_march: 0 the LDS tiles, 2
+// the column march, 1 the march when the batch has enough of its units to
+// fill the device: it trades the tiles' halo work for a serial walk up each
+// chunk, which wins only with several wavefronts per SIMD. Measured, 512 x 256
+// cases: FP64 residual 359 -> 298 us at 32 cases (march units: 4.5 per
+// SIMD), 165 -> 142 us at 16, but 51 -> 70 us at 4; the FP32 tangent (8-row
+// chunks) 205 -> 169 us at 32 cases, slower at 4; the FP64 tangent is slower
+// as a march at every size (its registers allow one wavefront per SIMD).
+bool march_res(const Pool& p, const BatchPlan& b) {
+  const int k = p.knobs.stencil_march;
+  return k == 2 || (k == 1 && b.nmr * MARCH_WAVES >= 2 * 4 * g_num_cu);
+}
+bool march_jvp(const Pool& p, const BatchPlan& b) {
+  const int k = p.knobs.stencil_march;
+  return k == 2 || (k == 1 && b.prec == 1 && b.nmj * MARCH_WAVES >= 4 * 4 * g_num_cu);
+}
+
+void enq_residual(Pool& p, BatchPlan& b, const KnobsDev& kn, int want_diag, double circ, int refresh_ut) {
+  const bool m = march_res(p, b);
+  if (m) launch_march_residual(b.pmr, b.nmr, kn, want_diag, circ, refresh_ut, p.stream);
+  else
+    for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, want_diag, circ, refresh_ut, p.stream);
+  launch_rn_finalize(b.pc, m, p.stream);
+}
+
+void enq_jvp(Pool& p, BatchPlan& b, const KnobsDev& kn, int vin, int vout, int mode, int kidx, int use_diag,
+             int gate) {
+  if (march_jvp(p, b)) launch_march_jvp(b.pmj, b.nmj, kn, vin, vout, mode, kidx, use_diag, gate, b.prec, p.stream);
+  else
+    for (auto& g : b.sten)
+      launch_jvp(g.p, g.shape, g.nblocks, kn, vin, vout, mode, kidx, use_diag, gate, b.prec, p.stream);
+}
+
 // Residual with the local time step at each case's CFL (sc->cfl), rn.
 void enq_prologue(Pool& p, BatchPlan& b, const KnobsDev& kn) {
   launch_forces(b.pc, nullptr, p.stream);
-  for (auto& g : b.sten)
-    launch_residual(g.p, g.shape, g.nblocks, kn, 1, NAN, 1, p.stream);
-  launch_rn_finalize(b.pc, p.stream);
+  enq_residual(p, b, kn, 1, NAN, 1);
 }
 
 void enq_newton(Pool& p, BatchPlan& b) {
@@ -414,8 +456,7 @@ void enq_newton(Pool& p, BatchPlan& b) {
   launch_krylov_begin(b.pc, 0, kn, prec, s);
   // V_0 = M(b - A x0)
   launch_dcirc(b.pc, VEC_X0, 0, -1, kn, prec, s);
-  for (auto& g : b.sten)
-    launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 1, -1, 1, 0, prec, s);
+  enq_jvp(p, b, kn, VEC_X0, VEC_W, 1, -1, 1, 0);
   // FP16 basis (mixed form): the preconditioner writes the FP32 vector P, the
   // basis vectors are stored from it in FP16, and the operator reads their
   // rounded values in FP32 from vc
@@ -432,7 +473,7 @@ void enq_newton(Pool& p, BatchPlan& b) {
     // the circulation kernel first finishes step k - 1 (a launch fewer per
     // step: a small batch's step is latency bound)
     launch_dcirc(b.pc, vin, 1, k - 1, kn, prec, s);
-    for (auto& g : b.sten) launch_jvp(g.p, g.shape, g.nblocks, kn, vin, VEC_W, 0, k, 1, 1, prec, s);
+    enq_jvp(p, b, kn, vin, VEC_W, 0, k, 1, 1);
     enq_precond(p, b, VEC_W, wout, 0, 1);
     launch_dots(b.pvec, b.nvec, k, wout, 1, prec, bh, s);
     if (!red) launch_reduce_dots(b.pc, k, bh, s);
@@ -571,6 +612,7 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->positivity != 0 && k->positivity != 1) fail(-6, "positivity must be 0 or 1");
     if (k->precond_half != 0 && k->precond_half != 1) fail(-6, "precond_half must be 0 or 1");
     if (k->basis_half != 0 && k->basis_half != 1) fail(-6, "basis_half must be 0 or 1");
+    if (k->stencil_march < 0 || k->stencil_march > 2) fail(-6, "stencil_march must be 0, 1 or 2");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -658,7 +700,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.kmax = kmax;
     d.npart = L.npart;
     d.shape = 0;
-    d.ntiles = cdiv(ni, stencil_shape(0).ti) * cdiv(nc, stencil_shape(0).tj);
+    d.ntiles = stencil_blocks(0, ni, nc);
     d.nvecb = cdiv(int(n), VEC_CHUNK);
     d.nlineb = cdiv(ni, LINES_PER_BLOCK);
     d.nlinew = cdiv(ni, 8);
@@ -829,7 +871,7 @@ int a2d_case_set_shape(int64_t pool, int case_id, int shape) {
     CaseHost& c = get_case(p, case_id);
     if (shape < 0 || shape >= stencil_num_shapes()) fail(-6, "no such stencil shape");
     c.dev.shape = shape;
-    c.dev.ntiles = cdiv(c.ni, stencil_shape(shape).ti) * cdiv(c.nc, stencil_shape(shape).tj);
+    c.dev.ntiles = stencil_blocks(shape, c.ni, c.nc);
     HC(hipMemcpy(p.dcases + case_id, &c.dev, sizeof(CaseDev), hipMemcpyHostToDevice));
     drop_plans_with(p, case_id);
   });
@@ -859,8 +901,7 @@ int a2d_residual(int64_t pool, int case_id, double circ, double* R, double* rn) 
     BatchPlan& b = get_plan(p, {case_id});
     const KnobsDev kn = to_dev(p.knobs);
     if (std::isnan(circ)) launch_forces(b.pc, nullptr, p.stream);
-    for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 0, circ, 0, p.stream);
-    launch_rn_finalize(b.pc, p.stream);
+    enq_residual(p, b, kn, 0, circ, 0);
     if (R) download_field(p, c, c.dev.R, R, 0, 4);
     if (rn) {
       HC(hipMemcpyAsync(rn, &c.dev.sc->rn, sizeof(double), hipMemcpyDeviceToHost, p.stream));
@@ -927,8 +968,7 @@ int a2d_jvp(int64_t pool, int case_id, const double* v, double cfl, int with_dia
     enq_prologue(p, b, kn);
     upload_field(p, c, v, c.dev.xa, c.prec, 4);
     launch_dcirc(b.pc, VEC_XA, 0, -1, kn, c.prec, p.stream);
-    for (auto& g : b.sten)
-      launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_XA, VEC_W, 0, -1, with_diag, 0, c.prec, p.stream);
+    enq_jvp(p, b, kn, VEC_XA, VEC_W, 0, -1, with_diag, 0);
     download_field(p, c, c.dev.w, out, c.prec, 4);
   });
 }
@@ -1035,11 +1075,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     auto launch = [&] {
       switch (which) {
         case 0:  // as launched in a Newton step: with the local time step and U copy
-          for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, 1, NAN, 1, p.stream);
+          enq_residual(p, b, kn, 1, NAN, 1);
           break;
         case 1:
-          for (auto& g : b.sten)
-            launch_jvp(g.p, g.shape, g.nblocks, kn, VEC_X0, VEC_W, 0, -1, 1, 0, b.prec, p.stream);
+          enq_jvp(p, b, kn, VEC_X0, VEC_W, 0, -1, 1, 0);
           break;
         case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
           launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);

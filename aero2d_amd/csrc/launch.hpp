@@ -42,6 +42,27 @@ struct KnobsDev {        // solver knobs the kernels read
 struct StencilShape { int ti, tj; };
 int stencil_num_shapes();
 StencilShape stencil_shape(int idx);
+// Workgroups of one case's stencil launch (= its residual-norm partials).
+int stencil_blocks(int shape, int ni, int nc);
+
+// ---- march.hip ----------------------------------------------------------------
+// The column-march stencil: a unit is one wavefront walking a strip of 60
+// columns up a chunk of rows; four units per workgroup. Chunk heights: the
+// FP64 residual and tangent 16 rows, the FP32 tangent 8 (twice the
+// wavefronts, for latency hiding at its higher occupancy). The residual writes
+// one norm partial per unit.
+constexpr int MARCH_OWN = 60, MARCH_WAVES = 4;
+constexpr int MARCH_ROWS_RES = 16, MARCH_ROWS_JVP64 = 16, MARCH_ROWS_JVP32 = 8;
+__host__ __device__ inline int march_units(int ni, int nc, int rows) {
+  return ((ni + MARCH_OWN - 1) / MARCH_OWN) * ((nc + rows - 1) / rows);
+}
+__host__ __device__ inline int march_blocks(int ni, int nc, int rows) {
+  return (march_units(ni, nc, rows) + MARCH_WAVES - 1) / MARCH_WAVES;
+}
+void launch_march_residual(const Plan& p, int nblocks, const KnobsDev& k, int want_diag,
+                           double circ_override, int refresh_ut, hipStream_t s);
+void launch_march_jvp(const Plan& p, int nblocks, const KnobsDev& k, int vin, int vout, int mode,
+                      int kidx, int use_diag, int gate, int prec, hipStream_t s);
 
 // FP64 residual R(U); want_diag: also diag = vol/dtau at the case CFL;
 // refresh_ut: also the FP32 copy of U for mixed-precision cases; rn partials
@@ -57,7 +78,8 @@ void launch_forces(const Plan& pc, double* cp_out, hipStream_t s);
 // kprev >= 0: finish Arnoldi step kprev first (as launch_normalize would).
 void launch_dcirc(const Plan& pc, int vin, int gate, int kprev, const KnobsDev& kn, int prec,
                   hipStream_t s);
-void launch_rn_finalize(const Plan& pc, hipStream_t s);
+// march: the residual ran as the column march (its partials are per unit).
+void launch_rn_finalize(const Plan& pc, int march, hipStream_t s);
 void launch_hllc_test(int n, const double* WL, const double* WR, const double* S, double* F,
                       hipStream_t s);
 

@@ -460,9 +460,9 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate, int 
 }
 
 // rn = sqrt(sum of the residual kernel's partials / n), one wavefront per case.
-__global__ void __launch_bounds__(WAVE) k_rn_finalize(Plan pc) {
+__global__ void __launch_bounds__(WAVE) k_rn_finalize(Plan pc, int march) {
   const CaseDev& cd = pc.cases[pc.slots[blockIdx.x]];
-  const double s = case_sum(cd, cd.ntiles, 0);
+  const double s = case_sum(cd, march ? march_units(cd.ni, cd.nc, MARCH_ROWS_RES) : cd.ntiles, 0);
   if (threadIdx.x == 0) cd.sc->rn = sqrt(s / cd.n);
 }
 
@@ -493,6 +493,9 @@ void run_jvp(const Plan& p, int nblocks, const KnobsDev& k, int vin, int vout, i
 
 int stencil_num_shapes() { return int(sizeof(kShapes) / sizeof(kShapes[0])); }
 StencilShape stencil_shape(int idx) { return kShapes[idx]; }
+int stencil_blocks(int shape, int ni, int nc) {
+  return ((ni + kShapes[shape].ti - 1) / kShapes[shape].ti) * ((nc + kShapes[shape].tj - 1) / kShapes[shape].tj);
+}
 
 void launch_residual(const Plan& p, int shape, int nblocks, const KnobsDev& k, int want_diag,
                      double circ_override, int refresh_ut, hipStream_t s) {
@@ -531,8 +534,8 @@ void launch_dcirc(const Plan& p, int vin, int gate, int kprev, const KnobsDev& k
   else hipLaunchKernelGGL(k_dcirc<float>, dim3(p.nb), dim3(NT), 0, s, p, vin, gate, kprev, kn);
 }
 
-void launch_rn_finalize(const Plan& pc, hipStream_t s) {
-  hipLaunchKernelGGL(k_rn_finalize, dim3(pc.nb), dim3(WAVE), 0, s, pc);
+void launch_rn_finalize(const Plan& pc, int march, hipStream_t s) {
+  hipLaunchKernelGGL(k_rn_finalize, dim3(pc.nb), dim3(WAVE), 0, s, pc, march);
 }
 
 // Test hook: HLLC flux on n independent faces, arrays (4, n) / (2, n).

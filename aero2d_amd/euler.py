@@ -102,11 +102,13 @@ class EulerSolver:
     the Jacobian-vector product, preconditioner and Krylov vectors are FP32.
     `precision` ("fp64" / "mixed") overrides dtype. irs_eps and mg_levels
     belong to the reference's explicit scheme and are stored only.
+    stencil_march: the residual/tangent kernel (a2d_knobs.stencil_march: 0 LDS
+    tiles, 2 column march, 1 the march for batches that fill the device).
     """
 
     def __init__(self, grid: OGrid, dtype=np.float32, cfl: float = 2.5, irs_eps: float = 0.6,
                  order: int = 2, vortex_correction: bool = True, mg_levels: int = 1,
-                 lim_k: float = 0.02, precision: str | None = None):
+                 lim_k: float = 0.02, precision: str | None = None, stencil_march: int = 1):
         if order not in (1, 2):
             raise ValueError("order must be 1 or 2")
         self.dtype = np.dtype(dtype)
@@ -120,6 +122,7 @@ class EulerSolver:
         self.xw, self.yw = m.xw, m.yw
         self.cfl, self.irs_eps, self.order, self.lim_k = cfl, irs_eps, order, lim_k
         self.vortex = vortex_correction
+        self.stencil_march = stencil_march
         self.mg_levels = mg_levels
         self.ni, self.nc = m.vol.shape
         self._pool = None
@@ -132,6 +135,7 @@ class EulerSolver:
         k.lim_k = self.lim_k
         k.vortex = int(bool(self.vortex))
         k.order = int(self.order)
+        k.stencil_march = int(self.stencil_march)
         for name, v in over.items():
             setattr(k, name, v)
         return k

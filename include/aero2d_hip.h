@@ -62,6 +62,11 @@ typedef struct a2d_knobs {
                          in FP16 (each scaled by a power of two to entries of
                          order one; Gram-Schmidt arithmetic stays FP32/FP64),
                          half the bytes of the Gram-Schmidt passes; 0 = FP32 */
+  int stencil_march;  /* residual / Jacobian-vector product kernel: 0 = LDS tiles
+                         (the case's tile shape), 2 = column march (a wavefront
+                         walks a strip of columns up the rows), 1 = the march
+                         for batches large enough to fill the device (default);
+                         same results to round-off either way */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */

@@ -30,7 +30,11 @@ def _setup(n_side, nj, mach, alpha, seed, precision="fp64", shape=None):
     s = EulerSolver(g, dtype=np.float64, precision=precision)
     U = perturbed_state(s.ni, s.nc, mach, alpha, seed)
     h, c = s._scratch(s.freestream(mach, alpha), U)
-    if shape is not None:
+    if shape == "march":                 # the column-march kernels instead of LDS tiles
+        k = s._pool.knobs()
+        k.stencil_march = 2
+        s._pool.set_knobs(k)
+    elif shape is not None:
         _abi.call("a2d_case_set_shape", h, c, shape)
     return g, s, U, h, c
 
@@ -49,7 +53,7 @@ def test_hllc_device_matches_oracle():
     assert rel_err(F, Fo) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, "march"])
 @pytest.mark.parametrize("case", CASES)
 def test_residual_matches_oracle(case, shape):
     g, s, U, h, c = _setup(*case, shape=shape)
@@ -84,7 +88,7 @@ def test_forces_and_local_dt_match_oracle(case):
     assert rel_err(dt, dto) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [0, 2])
+@pytest.mark.parametrize("shape", [0, 2, "march"])
 @pytest.mark.parametrize("case", CASES)
 def test_jvp_matches_oracle(case, shape):
     g, s, U, h, c = _setup(*case, shape=shape)
@@ -294,15 +298,18 @@ def test_largest_served_grid_mixed_form_matches_oracle():
         assert rel_err(x, xo) < tol, half
 
 
+@pytest.mark.parametrize("march", [1, 2])
 @pytest.mark.parametrize("case", CASES[:2])
-def test_newton_steps_match_oracle(case):
+def test_newton_steps_match_oracle(case, march):
     """One and three full Newton steps (50 Arnoldi steps each) vs the oracle,
     from a non-uniform state: the oracle's own response to a 1e-15 perturbation
-    of such a state is 1e-15, so the GPU must agree to 1e-10."""
+    of such a state is 1e-15, so the GPU must agree to 1e-10. march = 2 runs
+    the residual (its norm partials included) and the tangent as the column
+    march."""
     n_side, nj, mach, alpha, seed = case
     g = small_grid(n_side, nj)
     from aero2d_amd import ImplicitEuler
-    s = EulerSolver(g, dtype=np.float64)
+    s = EulerSolver(g, dtype=np.float64, stencil_march=march)
     geo = flow.stack_geometry([oracle_geo(g)])
     U0 = perturbed_state(s.ni, s.nc, mach, alpha, 7, amp=0.02)
     for steps in (1, 3):
@@ -331,14 +338,16 @@ def test_newton_steps_from_freestream_match_oracle(case):
         assert abs(r.cl[0] - ro.cl[0]) < 5e-5
 
 
+@pytest.mark.parametrize("kernel", [None, "march"])
 @pytest.mark.parametrize("case", CASES)
-def test_mixed_jvp_and_blocks_match_oracle(case):
+def test_mixed_jvp_and_blocks_match_oracle(case, kernel):
     """The mixed form's inner precision: the FP32 Jacobian-vector product
-    (hardware 1-ulp reciprocal and square root) and the FP32 preconditioner
-    blocks against the FP64 oracle. FP32 rounding of O(1) fluxes and their
-    differences bounds the agreement near 1e-6 of the field's magnitude; the
-    test allows 1e-4 (a dropped term or wrong index shows as O(1))."""
-    g, s, U, h, c = _setup(*case, precision="mixed")
+    (hardware 1-ulp reciprocal and square root; LDS tiles or the column march)
+    and the FP32 preconditioner blocks against the FP64 oracle. FP32 rounding
+    of O(1) fluxes and their differences bounds the agreement near 1e-6 of the
+    field's magnitude; the test allows 1e-4 (a dropped term or wrong index
+    shows as O(1))."""
+    g, s, U, h, c = _setup(*case, precision="mixed", shape=kernel)
     mach, alpha = case[2], case[3]
     geo = oracle_geo(g)
     w = winf_t(mach, alpha)
