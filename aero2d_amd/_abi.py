@@ -31,7 +31,7 @@ class Knobs(ctypes.Structure):
                 ("early_exit", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("precond_order", ctypes.c_int), ("positivity", ctypes.c_int),
                 ("precond_half", ctypes.c_int), ("basis_half", ctypes.c_int),
-                ("stencil_march", ctypes.c_int)]
+                ("stencil_march", ctypes.c_int), ("sweep_lds", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); the exported surface of include/aero2d_hip.h

@@ -113,6 +113,7 @@ struct Plan {
   const int* blk_prefix;  // nb + 1
   int nb;
   int lines = 0;          // line sweeps: lines per block (8 or 32)
+  int dsteps = 0;         // line sweeps: d values per lane kept in LDS (0: in HBM)
 };
 
 // Map a flat block index to (batch entry, local block) by binary search over

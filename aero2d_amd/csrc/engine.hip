@@ -24,6 +24,7 @@ namespace {
 thread_local std::string g_err;
 int g_device = -1;
 int g_num_cu = 256;   // compute units of the device (sweep block sizing)
+int g_lds_block = 65536, g_lds_cu = 65536;   // LDS bytes per workgroup / per CU
 // 32-line sweep blocks from this many per 100 CUs, else 8-line blocks.
 // Measured crossover (preconditioner application, 512x256 mixed / 256x128
 // FP64): 8-line blocks win at 128 32-line blocks (16 cases, -15..-22 %),
@@ -234,6 +235,7 @@ void default_knobs(a2d_knobs* k) {
   k->precond_half = 0;
   k->basis_half = 0;
   k->stencil_march = 1;
+  k->sweep_lds = 1;
 }
 
 Pool& get_pool(int64_t id) {
@@ -363,6 +365,23 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   bp->prb = mk(prb.first);
   bp->prb.lines = lrb;
   bp->nrb = prb.second;
+  // Sweeps of 32-line blocks (batches that fill the device) keep their d
+  // values in LDS when every block of the launch can then still be resident at
+  // once (the LDS takes no residency from the launch): the red-black half
+  // sweep of 32 transonic 512 x 256 cases 97 -> 90 us. Small batches' 8-line
+  // blocks are latency bound, and measured slower with LDS (one 512 x 256
+  // case: an application 111 -> 119 us), so they keep the HBM round trip.
+  int mmax = 0;
+  for (int s : slots) mmax = std::max(mmax, p.cases[s].nc >> 1);
+  const size_t tsz = bp->prec == 0 ? 8 : 4;
+  auto lds_steps = [&](int lines, int nblocks) {
+    const size_t bytes = size_t(8 * lines) * size_t(mmax) * tsz;
+    if (bytes == 0 || bytes > size_t(g_lds_block)) return 0;
+    const size_t per_cu = size_t(g_lds_cu) / bytes;
+    return size_t(nblocks) <= per_cu * size_t(g_num_cu) ? mmax : 0;
+  };
+  bp->psw.dsteps = lds_steps(lsw, bp->nsw);
+  bp->prb.dsteps = lds_steps(lrb, bp->nrb);
   bp->pmr = mk(pmr.first);
   bp->nmr = pmr.second;
   bp->pmj = mk(pmj.first);
@@ -382,16 +401,25 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   return ref;
 }
 
+// A sweep plan as the knob sweep_lds has it: 0 the d values through HBM, 2 in
+// LDS wherever the plan found room, 1 (default) that only for 32-line blocks.
+Plan sweep_plan(const Pool& p, Plan q) {
+  const int k = p.knobs.sweep_lds;
+  if (k == 0 || (k == 1 && q.lines != SWEEP_LINES_BIG)) q.dsteps = 0;
+  return q;
+}
+
 // ---- launch sequences ---------------------------------------------------------
 void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int gate) {
   const int S = std::max(1, p.knobs.jacobi_sweeps);
   const int hb = b.prec == 1 && p.knobs.precond_half;
+  const Plan psw = sweep_plan(p, b.psw), prb = sweep_plan(p, b.prb);
   if (p.knobs.precond_order == 1) {
     // red-black: odd lines then even lines, S times, in place in dest
     for (int s = 0; s < S; ++s)
       for (int color = 1; color >= 0; --color) {
         const int coupled = !(s == 0 && color == 1);
-        launch_line_sweep_rb(b.prb, b.nrb, src, dest, color, coupled, s > 0,
+        launch_line_sweep_rb(prb, b.nrb, src, dest, color, coupled, s > 0,
                              (s == S - 1) ? want_norm : 0, gate, b.prec, hb, p.stream);
       }
     return;
@@ -399,7 +427,7 @@ void enq_precond(Pool& p, BatchPlan& b, int src, int dest, int want_norm, int ga
   for (int s = 0; s < S; ++s) {
     const int out = (s == S - 1) ? dest : (s % 2 == 0 ? VEC_XA : VEC_XB);
     const int prev = (s == 0) ? VEC_XB : ((s - 1) % 2 == 0 ? VEC_XA : VEC_XB);
-    launch_line_sweep(b.psw, b.nsw, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
+    launch_line_sweep(psw, b.nsw, src, prev, out, s > 0, s > 0, (s == S - 1) ? want_norm : 0,
                       gate, b.prec, hb, p.stream);
   }
 }
@@ -571,6 +599,8 @@ int a2d_init(int device) {
     if (device < 0 || device >= count) fail(-1, "no such HIP device");
     HC(hipSetDevice(device));
     HC(hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, device));
+    HC(hipDeviceGetAttribute(&g_lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    HC(hipDeviceGetAttribute(&g_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
     g_device = device;
   });
 }
@@ -613,6 +643,7 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->precond_half != 0 && k->precond_half != 1) fail(-6, "precond_half must be 0 or 1");
     if (k->basis_half != 0 && k->basis_half != 1) fail(-6, "basis_half must be 0 or 1");
     if (k->stencil_march < 0 || k->stencil_march > 2) fail(-6, "stencil_march must be 0, 1 or 2");
+    if (k->sweep_lds < 0 || k->sweep_lds > 2) fail(-6, "sweep_lds must be 0, 1 or 2");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -1081,7 +1112,8 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
           enq_jvp(p, b, kn, VEC_X0, VEC_W, 0, -1, 1, 0);
           break;
         case 2:  // a coupled line-Jacobi sweep over every line (sweeps 2.. of an application)
-          launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep(sweep_plan(p, b.psw), b.nsw, VEC_W, VEC_XA, VEC_XB, 1, 1, 0, 0,
+                            b.prec, hb, p.stream);
           break;
         case 3: launch_dots(b.pvec, b.nvec, k, bh ? VEC_P : k + 1, 0, b.prec, bh, p.stream); break;
         case 4: launch_assemble(b.pcell, b.ncell, kn, b.prec, p.stream); break;
@@ -1089,10 +1121,12 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
         case 6: launch_update(b.pvec, b.nvec, k, VEC_XA, 0, 0, b.prec, bh, p.stream); break;
         case 7: launch_copy_vec(b.pvec, b.nvec, VEC_X0, VEC_XB, b.prec, p.stream); break;
         case 8:  // the first (uncoupled) sweep of an application
-          launch_line_sweep(b.psw, b.nsw, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep(sweep_plan(p, b.psw), b.nsw, VEC_W, VEC_XA, VEC_XB, 0, 0, 0, 0,
+                            b.prec, hb, p.stream);
           break;
         case 9:  // a coupled red-black half sweep (odd lines) of a later sweep
-          launch_line_sweep_rb(b.prb, b.nrb, VEC_W, VEC_XB, 1, 1, 1, 0, 0, b.prec, hb, p.stream);
+          launch_line_sweep_rb(sweep_plan(p, b.prb), b.nrb, VEC_W, VEC_XB, 1, 1, 1, 0, 0,
+                               b.prec, hb, p.stream);
           break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
         default: fail(-6, "unknown kernel id");

@@ -541,8 +541,10 @@ __device__ __forceinline__ void unpack(const a2d_h4& a, float v[4]) {
 // PF further on as soon as it is consumed. Loads are unconditional (rows past
 // the line's end are clamped to its last row, an L2 hit): a load under a
 // branch makes the compiler wait for every load in flight where the branch
-// joins. The d values of the way in are kept in the case's d buffer
-// ([cell][4]) for the way back.
+// joins. The d values of the way in are kept for the way back: in LDS when
+// the plan grants it (DL: dsteps slots per lane, [step][thread], each lane
+// reading back only what it wrote), else in the case's d buffer ([cell][4]),
+// an HBM round trip of 8 words per cell.
 // Source of g: GS_VEC / GS_NEGR form S0 src from a vector / from -R (read in
 // FP64) and keep it in the case's g0 buffer; GS_G0 reads it back. Every sweep
 // of one preconditioner application has the same source, so only the first
@@ -550,9 +552,11 @@ __device__ __forceinline__ void unpack(const a2d_h4& a, float v[4]) {
 // HB (FP32 only): the couplings (slots 1-4) come from the FP16 copy.
 enum GSrc : int { GS_VEC = 0, GS_NEGR = 1, GS_G0 = 2 };
 
-template <class T, bool COUP, int GS, int PF, int PFB, bool HB, int SL>
+template <class T, bool COUP, int GS, int PF, int PFB, bool HB, int SL, bool DL>
 __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, int xout, int want_norm,
                                                int gate, int color) {
+  extern __shared__ unsigned char sweep_lds[];
+  T* const sd = reinterpret_cast<T*>(sweep_lds);
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   if (gate && !cd.sc->active) return;
@@ -705,7 +709,8 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
       const T dn = g - p2[0] * quad_bcast<0>(dr) - p2[1] * quad_bcast<1>(dr) -
                    p2[2] * quad_bcast<2>(dr) - p2[3] * quad_bcast<3>(dr);
       dr = valid ? dn : dr;
-      *at(gd, rowc * 4u + cgd) = dr;
+      if constexpr (DL) sd[(unsigned)max(0, min(st, steps - 1)) * (8u * SL) + threadIdx.x] = dr;
+      else *at(gd, rowc * 4u + cgd) = dr;
       fload(k, st + PF);
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -755,7 +760,13 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   auto bload = [&](int k, int s) {
     const unsigned rowc = (unsigned)bj(s) * niu;
     rowb[k] = rowc;
-    cv[k] = *at(gd, rowc * 4u + cgd);
+    if constexpr (DL) {
+      // the way in stored step t of this half in slot t; row bj(s) is step steps - 1 - s
+      const int sc = max(0, min(s, steps - 1));
+      cv[k] = sd[(unsigned)max(0, steps - 1 - sc) * (8u * SL) + threadIdx.x];
+    } else {
+      cv[k] = *at(gd, rowc * 4u + cgd);
+    }
     cload(1, rowc, cm[k]);
   };
   // no per-lane branch, as on the way in: a step past the half's end keeps xr
@@ -797,26 +808,36 @@ template <> struct SweepDepth<double> { static constexpr int in = 4, out = 8; };
 // Kernel variant for a sweep: coupled or not; g from src (a vector or -R,
 // kept in g0) or from g0 (reuse_g0: a later sweep of the same application);
 // couplings from the FP16 copy (hb).
-template <class T, bool HB, int SL>
-void run_sweep_v(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+template <class T, bool HB, int SL, bool DL>
+void run_sweep_d(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
                  int want_norm, int gate, int color, hipStream_t s) {
   constexpr int PF = SweepDepth<T>::in, PFB = SweepDepth<T>::out;
   const dim3 grid(nblocks), block(8 * SL);
+  const size_t lds = DL ? size_t(8 * SL) * size_t(pl.dsteps) * sizeof(T) : 0;
   if (coupled && reuse_g0)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_G0, PF, PFB, HB, SL, DL>), grid, block, lds, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (coupled && src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_NEGR, PF, PFB, HB, SL, DL>), grid, block, lds, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (coupled)
-    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, true, GS_VEC, PF, PFB, HB, SL, DL>), grid, block, lds, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else if (src == VEC_NEG_R)
-    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, false, GS_NEGR, PF, PFB, HB, SL, DL>), grid, block, lds, s, pl, src, xnb,
                        xout, want_norm, gate, color);
   else
-    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB, HB, SL>), grid, block, 0, s, pl, src, xnb,
+    hipLaunchKernelGGL((k_sweep<T, false, GS_VEC, PF, PFB, HB, SL, DL>), grid, block, lds, s, pl, src, xnb,
                        xout, want_norm, gate, color);
+}
+
+template <class T, bool HB, int SL>
+void run_sweep_v(const Plan& pl, int nblocks, int src, int xnb, int xout, int coupled, int reuse_g0,
+                 int want_norm, int gate, int color, hipStream_t s) {
+  if (pl.dsteps > 0)
+    run_sweep_d<T, HB, SL, true>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
+  else
+    run_sweep_d<T, HB, SL, false>(pl, nblocks, src, xnb, xout, coupled, reuse_g0, want_norm, gate, color, s);
 }
 
 template <int SL>

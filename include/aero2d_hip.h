@@ -67,6 +67,11 @@ typedef struct a2d_knobs {
                          walks a strip of columns up the rows), 1 = the march
                          for batches large enough to fill the device (default);
                          same results to round-off either way */
+  int sweep_lds;      /* where a line sweep keeps the intermediate values of its
+                         elimination between its two passes: 0 = HBM; 2 = LDS
+                         when every block of the launch can still be resident;
+                         1 = that only for batches that fill the device (32-line
+                         blocks; default). Same results bit for bit */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */

@@ -223,3 +223,20 @@ def test_mixed_defaults_meet_the_accuracy_bar_on_transonic_draws():
         print("case", k, "fp64 from mixed", a["dcl"], a["residual"], "mixed from fp64", b["dcl"], b["residual"])
         assert a["dcl"] <= 1e-8 and max(a["residual"]) <= 1e-10
         assert b["dcl"] <= 1e-8 and max(b["residual"]) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_sweep_lds_gives_the_same_bits_as_hbm(precision):
+    """The line sweeps' intermediate values kept in LDS or sent through HBM:
+    the same arithmetic, so four Newton steps of a transonic batch give the
+    same bits either way."""
+    from aero2d_amd.workloads import config3
+    cases = config3(6, 256, 128)
+    outs = []
+    for lds in (2, 0):
+        kn = default_knobs(precision)
+        kn.sweep_lds = lds
+        r = solve_batch(cases, precision, tol=0.0, n_iter=4, keep_state=True, knobs=kn, stall=())
+        outs.append(r)
+    for k in range(len(cases)):
+        assert np.array_equal(outs[0].U[k], outs[1].U[k]), k

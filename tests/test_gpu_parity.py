@@ -133,12 +133,18 @@ def test_assembly_matches_oracle(case, order):
         assert rel_err(got, exp.numpy()) < 1e-11
 
 
+@pytest.mark.parametrize("sweep_lds", [2, 0])
 @pytest.mark.parametrize("case", CASES)
-def test_preconditioner_matches_oracle(case):
+def test_preconditioner_matches_oracle(case, sweep_lds):
+    """Line-Jacobi preconditioner, its eliminations' intermediate values in
+    LDS (sweep_lds = 2) or in HBM (0)."""
     g, s, U, h, c = _setup(*case)
     mach, alpha = case[2], case[3]
     geo = oracle_geo(g)
     w = winf_t(mach, alpha)
+    k = s._pool.knobs()
+    k.sweep_lds = sweep_lds
+    s._pool.set_knobs(k)
     _abi.call("a2d_precond_setup", h, c, 25.0)
     rng = np.random.default_rng(5)
     r = np.ascontiguousarray(rng.normal(size=U.shape))
@@ -153,9 +159,10 @@ def test_preconditioner_matches_oracle(case):
     assert rel_err(x, xo) < 1e-9
 
 
+@pytest.mark.parametrize("sweep_lds", [2, 0])
 @pytest.mark.parametrize("precision, half", [("fp64", 0), ("mixed", 0), ("mixed", 1)])
 @pytest.mark.parametrize("case", CASES)
-def test_red_black_preconditioner_matches_oracle(case, precision, half):
+def test_red_black_preconditioner_matches_oracle(case, precision, half, sweep_lds):
     """half = 1: the sweeps read the couplings from their FP16 copy (11-bit
     significand, relative rounding 2^-11 per entry): agreement to 5e-3 of the
     FP64 oracle; a wrong slot, row or index shows as O(1)."""
@@ -166,6 +173,7 @@ def test_red_black_preconditioner_matches_oracle(case, precision, half):
     k = s._pool.knobs()
     k.precond_order = 1
     k.precond_half = half
+    k.sweep_lds = sweep_lds
     s._pool.set_knobs(k)
     _abi.call("a2d_precond_setup", h, c, 25.0)
     rng = np.random.default_rng(6)
@@ -189,7 +197,8 @@ def test_red_black_preconditioner_matches_oracle(case, precision, half):
 @pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("nj, ds0, radius", [(3, 0.2, 1.0), (4, 0.2, 1.0), (10, 2e-2, 5.0), (11, 2e-2, 5.0)])
 @pytest.mark.parametrize("precision, half", [("fp64", 0), ("mixed", 1)])
-def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half):
+@pytest.mark.parametrize("sweep_lds", [2, 0])
+def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half, sweep_lds):
     """Lines of 2, 3, 9 and 10 cells: the two-sided line elimination meets in
     the middle row whatever the parity of the length (and with an empty far
     half for 2 cells). Mixed form with FP16 couplings: 5e-3."""
@@ -200,6 +209,7 @@ def test_preconditioner_line_lengths(nj, ds0, radius, order, precision, half):
     k = s._pool.knobs()
     k.precond_order = order
     k.precond_half = half
+    k.sweep_lds = sweep_lds
     s._pool.set_knobs(k)
     _abi.call("a2d_precond_setup", h, c, 40.0)
     r = np.ascontiguousarray(np.random.default_rng(11).normal(size=U.shape))
