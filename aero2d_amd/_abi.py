@@ -54,6 +54,8 @@ SIGNATURES = {
     "a2d_case_set_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
     "a2d_case_get_state": (ctypes.c_int, [_i64, ctypes.c_int, _dp]),
     "a2d_case_set_shape": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int]),
+    "a2d_case_park": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int]),
+    "a2d_pool_stats": (ctypes.c_int, [_i64, ctypes.POINTER(_i64), ctypes.c_int]),
     "a2d_stencil_shapes": (ctypes.c_int, [_ip, _ip, ctypes.c_int]),
     "a2d_residual": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double, _dp, _dp]),
     "a2d_forces": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp, _dp]),
@@ -180,6 +182,20 @@ class Pool:
 
     def used_bytes(self) -> int:
         return int(load().a2d_pool_used_bytes(self.handle))
+
+    STAT_NAMES = ("steps", "graph_replays", "graph_captures", "plans", "plan_tables_allocated",
+                  "slabs_allocated", "slabs_reused", "bytes_held")
+
+    def stats(self) -> dict:
+        """a2d_pool_stats as a dict (graph_replay_fraction added)."""
+        out = (_i64 * len(self.STAT_NAMES))()
+        call("a2d_pool_stats", self.handle, out, len(self.STAT_NAMES))
+        d = {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
+        d["graph_replay_fraction"] = d["graph_replays"] / d["steps"] if d["steps"] else 0.0
+        return d
+
+    def park(self, cid: int, parked: bool = True):
+        call("a2d_case_park", self.handle, int(cid), int(bool(parked)))
 
     def add_case(self, x: np.ndarray, y: np.ndarray, mach: float, alpha: float, precision: int) -> int:
         x = np.ascontiguousarray(x, np.float64)

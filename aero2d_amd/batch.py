@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _abi
 from .grid import OGrid
-from .implicit import NewtonLoop
+from .implicit import NewtonLoop, StallRule
 
 __all__ = ["Case", "BatchResult", "BatchSolve", "solve_batch", "default_knobs", "cfl_policy",
            "path_of", "SOLVER_DEFAULTS"]
@@ -207,7 +207,7 @@ class BatchSolve:
         try:
             loop = NewtonLoop(nb, cfl0, cfl_max, cfl_late, late_below, fallback)
             hist = []
-            best = []                      # best rn/r0 so far, per step (stall rule)
+            rule = StallRule(nb, *stall) if stall is not None else None
             stalled = np.zeros(nb, bool)
             done = np.zeros(nb, bool)
             t_done = np.full(nb, np.nan)
@@ -236,16 +236,10 @@ class BatchSolve:
                     h[act] = rel
                     hist.append(h)
                 fin = (rel <= tol) | loop.frozen[act]
-                if stall is not None:
-                    b = best[-1].copy() if best else np.full(nb, np.inf)
-                    b[act] = np.fmin(b[act], rel)
-                    best.append(b)
-                    after, window, drop = stall
-                    if it >= window:
-                        old = best[it - window]
-                        st = (loop.iters[act] >= after) & ~(b[act] * drop <= old[act]) & ~fin
-                        stalled[act[st]] = True
-                        fin = fin | st
+                if rule is not None:
+                    st = rule.update(act, rel) & ~fin
+                    stalled[act[st]] = True
+                    fin = fin | st
                 now = time.perf_counter() - t0
                 t_done[act[fin]] = now
                 done[act[fin]] = True

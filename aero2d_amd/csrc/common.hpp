@@ -52,7 +52,8 @@ struct CaseScal {
   int accept;          // the Newton update was applied
   int nonfinite;       // count of non-finite entries in the update
   int frozen;          // host says: do not update this case
-  int pad[2];
+  int parked;          // host says: this case stays in its batch but takes no step
+  int pad;
   double s[KMAX + 1];          // scale of raw basis vector j (V_true = s_j V_raw)
   double H[KMAX + 1][KMAX];    // Hessenberg, H[row][col]
   double gc[KMAX], gs[KMAX];   // Givens rotations

@@ -81,7 +81,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
 struct Layout {
   size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
   size_t Ut, volt, Sit, Sjt, xft, yft;
-  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, pv, vc, part, sc;
+  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, pv, vc, part, cp, sc;
   size_t total;
   int npart;
 };
@@ -157,6 +157,7 @@ Layout layout(int ni, int nc, int prec, int kmax) {
   }
   L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, 8) + 2, 1});
   L.part = take(size_t(L.npart) * (kmax + 2) * 8);
+  L.cp = take(size_t(ni) * 8);
   L.sc = take(sizeof(CaseScal));
   L.total = o;
   return L;
@@ -166,16 +167,22 @@ struct CaseHost {
   bool live = false;
   int ni = 0, nc = 0, prec = 0, kmax = 0;
   char* slab = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;     // the slab's capacity (>= the layout's total)
+  double* cp = nullptr;  // wall Cp buffer (ni) for a2d_forces
   CaseDev dev{};
   double winf[4];
 };
 
+// Device buffers of released plan tables, by capacity in ints, for reuse.
+using TableCache = std::multimap<size_t, int*>;
+
 struct BatchPlan {
   std::vector<int> slots;
   int prec = 0, nb = 0;
-  int* dbuf = nullptr;
-  double* dcfl = nullptr;
+  int* dbuf = nullptr;     // prefix tables (from / back to the pool's table cache)
+  size_t dbuf_cap = 0;
+  TableCache* tables = nullptr;
+  double* dcfl = nullptr;  // the pool's step buffers (shared by its plans)
   double* dout = nullptr;
   double* hcfl = nullptr;
   double* hout = nullptr;
@@ -195,12 +202,14 @@ struct BatchPlan {
   uint64_t knob_version = 0;
   ~BatchPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
-    if (dbuf) (void)hipFree(dbuf);
-    if (dcfl) (void)hipFree(dcfl);
-    if (dout) (void)hipFree(dout);
-    if (hcfl) (void)hipHostFree(hcfl);
-    if (hout) (void)hipHostFree(hout);
+    if (dbuf) tables->emplace(dbuf_cap, dbuf);
   }
+};
+
+// Counters of a pool's work (a2d_pool_stats).
+struct PoolStats {
+  int64_t steps = 0, replays = 0, captures = 0, plans = 0, tables_new = 0;
+  int64_t slabs_new = 0, slabs_reused = 0;
 };
 
 struct Pool {
@@ -211,6 +220,19 @@ struct Pool {
   CaseDev* dcases = nullptr;
   a2d_knobs knobs{};
   uint64_t knob_version = 1;
+  // released case slabs kept for reuse, by capacity (bytes); with a budget,
+  // used + cached stays within it
+  std::multimap<size_t, char*> free_slabs;
+  int64_t cached = 0;
+  TableCache tables;
+  // step buffers shared by every plan of the pool (one step runs at a time):
+  // per-case CFL in, 10 results per case out; pinned host staging
+  double* dcfl = nullptr;
+  double* dout = nullptr;
+  double* hcfl = nullptr;
+  double* hout = nullptr;
+  PoolStats stats;
+  // declared last: its plans return their tables to `tables` when destroyed
   std::map<std::vector<int>, std::unique_ptr<BatchPlan>> plans;
   std::vector<double> stage_d;
   std::vector<float> stage_f;
@@ -252,6 +274,21 @@ CaseHost& get_case(Pool& p, int id) {
 void ensure_init() {
   if (g_device < 0) fail(-1, "a2d_init has not been called");
   HC(hipSetDevice(g_device));
+}
+
+// A removed case's slab goes to the pool's cache for the next admission; with
+// a budget, cached slabs are freed (largest first) so used + cached fits it.
+void release_slab(Pool& p, CaseHost& c) {
+  p.used -= int64_t(c.bytes);
+  p.free_slabs.emplace(c.bytes, c.slab);
+  p.cached += int64_t(c.bytes);
+  while (p.budget > 0 && p.used + p.cached > p.budget && !p.free_slabs.empty()) {
+    auto big = std::prev(p.free_slabs.end());
+    (void)hipFree(big->second);
+    p.cached -= int64_t(big->first);
+    p.free_slabs.erase(big);
+  }
+  c.slab = nullptr;
 }
 
 // Every id names a live case, and no id appears twice: kernels that update a
@@ -338,12 +375,26 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
     g.total = acc;
     gs.push_back(g);
   }
-  HC(hipMalloc(&bp->dbuf, h.size() * sizeof(int)));
+  bp->tables = &p.tables;
+  {
+    // a released table of at least this size (within 2x), else a new one
+    auto it = p.tables.lower_bound(h.size());
+    if (it != p.tables.end() && it->first <= 2 * h.size()) {
+      bp->dbuf = it->second;
+      bp->dbuf_cap = it->first;
+      p.tables.erase(it);
+    } else {
+      HC(hipMalloc(&bp->dbuf, h.size() * sizeof(int)));
+      bp->dbuf_cap = h.size();
+      p.stats.tables_new++;
+    }
+  }
+  p.stats.plans++;
   HC(hipMemcpy(bp->dbuf, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
-  HC(hipMalloc(&bp->dcfl, nb * sizeof(double)));
-  HC(hipMalloc(&bp->dout, nb * 10 * sizeof(double)));
-  HC(hipHostMalloc(&bp->hcfl, nb * sizeof(double), hipHostMallocDefault));
-  HC(hipHostMalloc(&bp->hout, nb * 10 * sizeof(double), hipHostMallocDefault));
+  bp->dcfl = p.dcfl;
+  bp->dout = p.dout;
+  bp->hcfl = p.hcfl;
+  bp->hout = p.hout;
   auto mk = [&](size_t prefix_at) {
     Plan q;
     q.cases = p.dcases;
@@ -614,6 +665,10 @@ int a2d_pool_create(int64_t hbm_budget_bytes, int64_t* pool_out) {
     default_knobs(&p->knobs);
     HC(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     HC(hipMalloc(&p->dcases, MAX_SLOTS * sizeof(CaseDev)));
+    HC(hipMalloc(&p->dcfl, MAX_SLOTS * sizeof(double)));
+    HC(hipMalloc(&p->dout, MAX_SLOTS * 10 * sizeof(double)));
+    HC(hipHostMalloc(&p->hcfl, MAX_SLOTS * sizeof(double), hipHostMallocDefault));
+    HC(hipHostMalloc(&p->hout, MAX_SLOTS * 10 * sizeof(double), hipHostMallocDefault));
     *pool_out = p->id;
     g_pools[p->id] = std::move(p);
   });
@@ -626,7 +681,13 @@ int a2d_pool_destroy(int64_t pool) {
     p.plans.clear();
     for (auto& c : p.cases)
       if (c.live) (void)hipFree(c.slab);
+    for (auto& kv : p.free_slabs) (void)hipFree(kv.second);
+    for (auto& kv : p.tables) (void)hipFree(kv.second);
     (void)hipFree(p.dcases);
+    (void)hipFree(p.dcfl);
+    (void)hipFree(p.dout);
+    (void)hipHostFree(p.hcfl);
+    (void)hipHostFree(p.hout);
     (void)hipStreamDestroy(p.stream);
     g_pools.erase(pool);
   });
@@ -703,6 +764,40 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     if (p.budget > 0 && p.used + int64_t(L.total) > p.budget)
       fail(-2, "HBM budget exceeded: case needs " + std::to_string(L.total) + " bytes, " +
                    std::to_string(p.budget - p.used) + " free");
+    // a released slab of this size (up to 25 % larger), else a new one; cached
+    // slabs are freed first where a new one would pass the budget
+    char* slab = nullptr;
+    size_t cap = L.total;
+    {
+      auto it = p.free_slabs.lower_bound(L.total);
+      if (it != p.free_slabs.end() && it->first <= L.total + L.total / 4 &&
+          (p.budget <= 0 || p.used + int64_t(it->first) <= p.budget)) {
+        slab = it->second;
+        cap = it->first;
+        p.cached -= int64_t(cap);
+        p.free_slabs.erase(it);
+        p.stats.slabs_reused++;
+      } else {
+        while (!p.free_slabs.empty() &&
+               (p.budget > 0 && p.used + p.cached + int64_t(L.total) > p.budget)) {
+          auto big = std::prev(p.free_slabs.end());
+          (void)hipFree(big->second);
+          p.cached -= int64_t(big->first);
+          p.free_slabs.erase(big);
+        }
+        hipError_t e = hipMalloc(&slab, L.total);
+        if (e != hipSuccess && !p.free_slabs.empty()) {
+          // out of device memory: release every cached slab and try once more
+          (void)hipGetLastError();
+          for (auto& kv : p.free_slabs) (void)hipFree(kv.second);
+          p.free_slabs.clear();
+          p.cached = 0;
+          e = hipMalloc(&slab, L.total);
+        }
+        hip_check(e, "hipMalloc(case slab)");
+        p.stats.slabs_new++;
+      }
+    }
     int slot = -1;
     for (int s = 0; s < int(p.cases.size()); ++s)
       if (!p.cases[s].live) { slot = s; break; }
@@ -717,10 +812,10 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     c.nc = nc;
     c.prec = precision;
     c.kmax = kmax;
-    c.bytes = L.total;
-    HC(hipMalloc(&c.slab, L.total));
+    c.bytes = cap;
+    c.slab = slab;
     c.live = true;
-    p.used += int64_t(L.total);
+    p.used += int64_t(cap);
     const size_t n = size_t(ni) * nc;
     char* b = c.slab;
     CaseDev& d = c.dev;
@@ -784,6 +879,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.vc = precision == 1 ? b + L.vc : nullptr;
     d.part = reinterpret_cast<double*>(b + L.part);
     d.sc = reinterpret_cast<CaseScal*>(b + L.sc);
+    c.cp = reinterpret_cast<double*>(b + L.cp);
 
     // metrics from the nodes, FP64 (aero2d/euler.py:44-60); node (i, j) at x[i*(nc+1)+j]
     const int nj = nc + 1;
@@ -810,8 +906,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     }
     for (double v : hvol)
       if (!(v > 0)) {
-        (void)hipFree(c.slab);
-        p.used -= int64_t(c.bytes);
+        release_slab(p, c);
         c = CaseHost{};
         fail(-8, "grid has a cell of non-positive area");
       }
@@ -866,8 +961,7 @@ int a2d_case_remove(int64_t pool, int case_id) {
     CaseHost& c = get_case(p, case_id);
     HC(hipStreamSynchronize(p.stream));
     drop_plans_with(p, case_id);
-    (void)hipFree(c.slab);
-    p.used -= int64_t(c.bytes);
+    release_slab(p, c);
     c = CaseHost{};
   });
 }
@@ -893,6 +987,25 @@ int a2d_case_set_step_limit(int64_t pool, int case_id, double max_update) {
     HC(hipMemcpyAsync(&c.dev.sc->max_update, &max_update, sizeof(double), hipMemcpyHostToDevice,
                       p.stream));
     HC(hipStreamSynchronize(p.stream));
+  });
+}
+
+int a2d_case_park(int64_t pool, int case_id, int parked) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& c = get_case(p, case_id);
+    if (parked != 0 && parked != 1) fail(-6, "parked must be 0 or 1");
+    HC(hipMemcpyAsync(&c.dev.sc->parked, &parked, sizeof(int), hipMemcpyHostToDevice, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+  });
+}
+
+int a2d_pool_stats(int64_t pool, int64_t* out, int n) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    const int64_t v[] = {p.stats.steps,     p.stats.replays,     p.stats.captures,     p.stats.plans,
+                         p.stats.tables_new, p.stats.slabs_new,  p.stats.slabs_reused, p.cached};
+    for (int k = 0; k < n && k < int(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
   });
 }
 
@@ -946,14 +1059,11 @@ int a2d_forces(int64_t pool, int case_id, double* clcdcm, double* cp, double* ci
     Pool& p = get_pool(pool);
     CaseHost& c = get_case(p, case_id);
     BatchPlan& b = get_plan(p, {case_id});
-    double* dcp = nullptr;
-    HC(hipMalloc(&dcp, c.ni * sizeof(double)));
-    launch_forces(b.pc, dcp, p.stream);
+    launch_forces(b.pc, c.cp, p.stream);
     CaseScal sc;
     HC(hipMemcpyAsync(&sc, c.dev.sc, offsetof(CaseScal, s), hipMemcpyDeviceToHost, p.stream));
-    if (cp) HC(hipMemcpyAsync(cp, dcp, c.ni * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    if (cp) HC(hipMemcpyAsync(cp, c.cp, c.ni * sizeof(double), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
-    (void)hipFree(dcp);
     if (clcdcm) {
       clcdcm[0] = sc.cl;
       clcdcm[1] = sc.cd;
@@ -1070,8 +1180,10 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
     std::memcpy(b.hcfl, cfl, n * sizeof(double));
     HC(hipMemcpyAsync(b.dcfl, b.hcfl, n * sizeof(double), hipMemcpyHostToDevice, p.stream));
     b.uses++;
+    p.stats.steps++;
     const bool graph = p.knobs.use_graph && b.uses >= 2;
     if (graph && (!b.exec || b.knob_version != p.knob_version)) {
+      p.stats.captures++;
       if (b.exec) {
         (void)hipGraphExecDestroy(b.exec);
         b.exec = nullptr;
@@ -1084,8 +1196,12 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
       (void)hipGraphDestroy(gr);
       b.knob_version = p.knob_version;
     }
-    if (graph) HC(hipGraphLaunch(b.exec, p.stream));
-    else enq_newton(p, b);
+    if (graph) {
+      HC(hipGraphLaunch(b.exec, p.stream));
+      p.stats.replays++;
+    } else {
+      enq_newton(p, b);
+    }
     HC(hipMemcpyAsync(b.hout, b.dout, n * 10 * sizeof(double), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     std::memcpy(out, b.hout, n * 10 * sizeof(double));

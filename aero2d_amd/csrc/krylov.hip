@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(WAVE) k_krylov_begin(Plan pc, int stage, Knobs
   sc->s[0] = beta > 0 ? 1.0 / (vsig * beta) : 0.0;
   sc->g[0] = beta;
   sc->gres = beta;
-  sc->run = beta > ptol;
+  sc->run = beta > ptol && !sc->parked;
   sc->active = sc->run && kn.restart > 0;
   sc->kdone = 0;
 }
@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   const CaseScal* sc = cd.sc;
+  if (sc->parked) return;
   const int n = cd.n, kd = sc->kdone;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   for (int j = threadIdx.x; j < kd; j += NT) coef[j] = sc->y[j] * sc->s[j];
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(WAVE) k_limiter(Plan pc, KnobsDev kn) {
   const double mu = sc->max_update > 0.0 ? sc->max_update : kn.max_update;
   const double om = fmin(1.0, mu / (mx + 1e-30));
   sc->omega = om;
-  sc->accept = (ct == 0.0) && isfinite(mx) && isfinite(sc->rn) && !sc->frozen;
+  sc->accept = (ct == 0.0) && isfinite(mx) && isfinite(sc->rn) && !sc->frozen && !sc->parked;
 }
 
 // Cells of U + omega dU with rho <= 0 or p <= 0 (or not finite), per block,
@@ -544,7 +545,7 @@ __global__ void k_set_cfl(Plan pc, const double* cfl) {
   if (e >= pc.nb) return;
   CaseScal* sc = pc.cases[pc.slots[e]].sc;
   sc->cfl = cfl[e];
-  sc->active = 1;
+  sc->active = !sc->parked;
   sc->run = 0;
   sc->kdone = 0;
   sc->accept = 0;

@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(MNT, MINW) k_march(Plan plan, KnobsDev kn, int
   const int e = plan_entry(plan, blk);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   const CaseScal* sc = cd.sc;
-  if (gate && !sc->active) return;
+  if (sc->parked || (gate && !sc->active)) return;
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int nstrip = (ni + MOWN - 1) / MOWN;
   const int nunit = nstrip * ((nc + MH - 1) / MH);

@@ -195,6 +195,7 @@ template <class T>
 __global__ void __launch_bounds__(NT) k_assemble(Plan plan, KnobsDev kn) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
+  if (cd.sc->parked) return;
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int idx = (blockIdx.x - plan.blk_prefix[e]) * NT + threadIdx.x;
   if (idx >= n) return;
@@ -394,6 +395,7 @@ template <class T>
 __global__ void __launch_bounds__(8 * LINES_PER_BLOCK) k_factor(Plan plan) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
+  if (cd.sc->parked) return;
   const int ni = cd.ni, nc = cd.nc;
   const int r = threadIdx.x & 3;
   const int half = (threadIdx.x >> 2) & 1;
@@ -559,7 +561,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   T* const sd = reinterpret_cast<T*>(sweep_lds);
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
-  if (gate && !cd.sc->active) return;
+  if (cd.sc->parked || (gate && !cd.sc->active)) return;
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   // color < 0: every line, natural block layout; else line q of colour

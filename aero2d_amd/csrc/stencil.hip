@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(NTS) k_stencil(Plan plan, KnobsDev kn, int vin
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
   const CaseScal* sc = cd.sc;
-  if (gate && !sc->active) return;
+  if (sc->parked || (gate && !sc->active)) return;
   const int ni = cd.ni, nc = cd.nc, n = cd.n;
   const int tiles_i = (ni + TI - 1) / TI;
   const int lt = blockIdx.x - plan.blk_prefix[e];
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate, int 
   const CaseDev& cd = plan.cases[plan.slots[e]];
   CaseScal* sc = cd.sc;
   // a step is finished only for a case still in its Arnoldi loop
-  if ((gate || kprev >= 0) && !sc->active) return;
+  if (sc->parked || ((gate || kprev >= 0) && !sc->active)) return;
   if (kprev >= 0) {
     arnoldi_finish<T>(cd, kprev, kn, lds);
     if (!lds.live) return;

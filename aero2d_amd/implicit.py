@@ -17,7 +17,37 @@ import numpy as np
 from . import _abi
 from .euler import EulerResult, EulerSolver
 
-__all__ = ["ImplicitEuler", "NewtonLoop"]
+__all__ = ["ImplicitEuler", "NewtonLoop", "StallRule"]
+
+
+class StallRule:
+    """Retire cases whose convergence has stalled (limit cycles): a case that
+    has taken `after` Newton steps and whose best rn/r0 over its last `window`
+    steps is not `drop` times below its best before them. Per-case state lives
+    in slots (reset() for a new case in a slot), so a batch and a serving loop
+    apply the same rule case by case."""
+
+    def __init__(self, nb, after, window, drop):
+        self.after, self.window, self.drop = int(after), int(window), float(drop)
+        self.ring = np.full((nb, self.window + 1), np.inf)   # best rn/r0 after each step
+        self.count = np.zeros(nb, int)
+
+    def reset(self, idx):
+        self.ring[idx] = np.inf
+        self.count[idx] = 0
+
+    def update(self, idx, rel):
+        """Record one step's rn/r0 (NaN for a rejected step) of the cases idx;
+        returns which of them have stalled."""
+        idx = np.asarray(idx)
+        c = self.count[idx]
+        prev = self.ring[idx, (c - 1) % (self.window + 1)]
+        best = np.where(c > 0, np.fmin(prev, rel), np.fmin(np.inf, rel))
+        best = np.where(np.isnan(best), np.inf, best)
+        self.ring[idx, c % (self.window + 1)] = best
+        self.count[idx] = c + 1
+        old = self.ring[idx, (c + 1) % (self.window + 1)]   # best `window` steps ago
+        return (c + 1 > self.window) & (c + 1 >= self.after) & ~(best * self.drop <= old)
 
 
 class NewtonLoop:

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _abi
 from .batch import SOLVER_DEFAULTS, Case, cfl_policy, default_knobs
-from .implicit import NewtonLoop
+from .implicit import NewtonLoop, StallRule
 
 __all__ = ["ServeResult", "SolveServer", "serve_stream", "censored_latency", "MAX_CELLS"]
 
@@ -44,6 +44,7 @@ class ServeResult:
     cm: float
     converged: bool
     frozen: bool
+    stalled: bool
     iterations: int
     rn_over_r0: float
     t_arrive: float
@@ -79,7 +80,8 @@ class SolveServer:
     def __init__(self, precision: str = "mixed", hbm_budget: int = 64 << 30, max_batch: int = 1024,
                  tol: float = 1e-10, n_iter: int = 800, cfl0: float = 10.0,
                  restart: int | None = None, early_exit: int | None = None,
-                 cfl_max: float | None = None, max_cells: int | None = None):
+                 cfl_max: float | None = None, max_cells: int | None = None,
+                 stall: tuple | None = None, park: bool = True):
         if precision not in SOLVER_DEFAULTS:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -97,6 +99,10 @@ class SolveServer:
         self.fallback_max_update = SOLVER_DEFAULTS[precision]["fallback_max_update"]
         fb = self.fallback_max_update > 0 and self.fallback_max_update != kn.max_update
         self.loop = NewtonLoop(4096, cfl0, cmax, clate, below, fb)
+        stall = SOLVER_DEFAULTS[precision]["stall"] if stall is None else stall
+        self.stall = StallRule(4096, *stall) if stall else None
+        self.park = park
+        self.parked: set[int] = set()      # finished cases kept in the batch, doing no work
         self.active: dict[int, _Slot] = {}
         self.waiting: list[tuple[Case, float, Future]] = []
         self.t0 = time.perf_counter()
@@ -158,43 +164,72 @@ class SolveServer:
                 fut.set_exception(e)                    # this request fails; admission goes on
                 continue
             self.loop.reset([cid])
+            if self.stall is not None:
+                self.stall.reset([cid])
             self.active[cid] = _Slot(case, t_arr, self.now(), fut, cid)
             self.active_cells += self.cells(case)
 
-    def _finish(self, cid: int):
+    def _finish(self, cid: int, stalled: bool = False):
+        """Report a finished case; it leaves the batch at once when a request
+        is waiting for its room, else it is parked (kept in the batch doing no
+        work), so that the batch's composition, and its captured graph, stay."""
         s = self.active.pop(cid)
         self.active_cells -= self.cells(s.case)
         f3 = np.empty(3)
         cp = np.empty(s.case.grid.x.shape[0])
         circ = np.empty(1)
         _abi.call("a2d_forces", self.pool.handle, cid, _abi.dptr(f3), _abi.dptr(cp), _abi.dptr(circ))
-        self.pool.remove_case(cid)
+        if self.park and not self.waiting:
+            self.pool.park(cid)
+            self.parked.add(cid)
+        else:
+            self.pool.remove_case(cid)
         rel = self.loop.last_rn[cid] / self.loop.r0[cid]
         res = ServeResult(s.case.name, float(f3[0]), float(f3[1]), float(f3[2]),
                           bool(rel <= self.tol and not self.loop.frozen[cid]), bool(self.loop.frozen[cid]),
-                          int(self.loop.iters[cid]), float(rel), s.t_arrive, s.t_admit, self.now())
+                          bool(stalled), int(self.loop.iters[cid]), float(rel), s.t_arrive, s.t_admit,
+                          self.now())
         s.future.set_result(res)
+
+    def _unpark_all(self):
+        for cid in self.parked:
+            self.pool.remove_case(cid)
+        self.parked.clear()
 
     # ---- one Newton step over everything admitted --------------------------------------------
     def step(self) -> int:
         """Admit what fits, take one Newton step over all active cases and
-        retire the finished ones. Returns the number of active cases stepped."""
+        retire the finished ones. Returns the number of active cases stepped.
+        Parked cases leave the batch when a request waits (their memory is
+        needed) or when no active case is left."""
+        if self.parked and (self.waiting or not self.active):
+            self._unpark_all()
         self._admit()
         if not self.active:
             return 0
         ids = np.array(sorted(self.active), np.int32)
-        cfl = np.ascontiguousarray(self.loop.cfl[ids])
-        out = np.empty((len(ids), 10))
-        _abi.call("a2d_newton_step", self.pool.handle, _abi.iptr(ids), len(ids), _abi.dptr(cfl),
+        batch = np.array(sorted(set(self.active) | self.parked), np.int32)
+        cfl = np.ascontiguousarray(self.loop.cfl[batch])
+        out = np.empty((len(batch), 10))
+        _abi.call("a2d_newton_step", self.pool.handle, _abi.iptr(batch), len(batch), _abi.dptr(cfl),
                   _abi.dptr(out))
+        out = out[np.isin(batch, ids)]                  # parked cases' rows are stale
         rn, _ = self.loop.update(ids, out)
         for cid in self.loop.newly_failed:
             _abi.call("a2d_case_set_step_limit", self.pool.handle, int(cid), float(self.fallback_max_update))
         self.steps += 1
-        done = (rn / self.loop.r0[ids] <= self.tol) | self.loop.frozen[ids] | (self.loop.iters[ids] >= self.n_iter)
-        for cid in ids[done]:
-            self._finish(int(cid))
+        rel = rn / self.loop.r0[ids]
+        done = (rel <= self.tol) | self.loop.frozen[ids] | (self.loop.iters[ids] >= self.n_iter)
+        stalled = np.zeros(len(ids), bool)
+        if self.stall is not None:
+            stalled = self.stall.update(ids, rel) & ~done
+        for k in np.nonzero(done | stalled)[0]:
+            self._finish(int(ids[k]), bool(stalled[k]))
         return len(ids)
+
+    def stats(self) -> dict:
+        """The pool's counters (a2d_pool_stats): steps, graph replays, plans built, slabs reused."""
+        return self.pool.stats()
 
     # ---- threaded API ----------------------------------------------------------------------------
     def submit(self, case: Case) -> Future:
@@ -220,6 +255,7 @@ class SolveServer:
         pending = [f for _, _, f in self.waiting] + [s.future for s in self.active.values()]
         self.waiting.clear()
         self.active.clear()
+        self.parked.clear()
         self.active_cells = 0
         for f in pending:
             if not f.done():
@@ -265,7 +301,8 @@ def serve_stream(requests, precision: str = "mixed", hbm_budget: int = 64 << 30,
     Returns (results, server, wall): the finished requests' results, and the
     server, whose `unfinished` lists the arrival times of the requests released
     but not finished by the end (waiting or in flight; their latency is at
-    least wall - t_arrive, see censored_latency)."""
+    least wall - t_arrive, see censored_latency) and `final_stats` its pool's
+    counters at the end."""
     srv = SolveServer(precision, hbm_budget, tol=tol, **kw)
     futures = []
     k = 0
@@ -285,6 +322,7 @@ def serve_stream(requests, precision: str = "mixed", hbm_budget: int = 64 << 30,
                 if k < len(reqs):
                     time.sleep(max(0.0, min(0.01, reqs[k].t_arrive - srv.now())))
         wall = srv.now()
+        srv.final_stats = srv.stats()
     finally:
         srv.close()
     results = [f.result() for f in futures if f.done() and f.exception() is None]

@@ -87,6 +87,14 @@ int a2d_pool_destroy(int64_t pool);
 int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k);
 int a2d_pool_get_knobs(int64_t pool, a2d_knobs* k);
 int64_t a2d_pool_used_bytes(int64_t pool);
+/* Counters of a pool's work since its creation (no counterpart in the
+ * reference), up to n of: Newton steps, steps replayed from a captured HIP
+ * graph, graph captures, batch plans built, plan tables allocated (the rest
+ * reused released ones), case slabs allocated, case slabs reused from removed
+ * cases, bytes of removed cases' slabs held for reuse. A removed case's slab
+ * is kept for the next case of about its size; with a budget, held slabs are
+ * freed as needed to keep live plus held bytes within it. */
+int a2d_pool_stats(int64_t pool, int64_t* out, int n);
 /* Device bytes one case of this size occupies (the admission footprint), or
  * -6 for a size a2d_case_add refuses (see there). */
 int64_t a2d_case_bytes(int ni, int nc, int precision, int restart);
@@ -115,6 +123,13 @@ int a2d_case_get_state(int64_t pool, int case_id, double* U);
  * reference's after its first rejected step. Takes effect from the next
  * Newton step, also for replayed graphs. */
 int a2d_case_set_step_limit(int64_t pool, int case_id, double max_update);
+/* Park (1) or unpark (0) a case: a parked case may stay in the batches it is
+ * named in, takes no Newton step there (no kernel does work for it, its state
+ * is kept, its rows of a2d_newton_step's output are to be ignored). No
+ * counterpart in the reference: it lets a serving loop keep a batch's
+ * composition, and so its captured HIP graph, when a case finishes while no
+ * request waits to take its place. */
+int a2d_case_park(int64_t pool, int case_id, int parked);
 /* Stencil tile shape used by this case's residual/JVP (index into the shape
  * table of a2d_stencil_shapes). */
 int a2d_case_set_shape(int64_t pool, int case_id, int shape);

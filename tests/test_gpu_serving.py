@@ -208,3 +208,109 @@ def test_server_failure_fails_every_pending_request():
         assert isinstance(srv.error, RuntimeError)
     finally:
         srv.close()
+
+
+def test_parked_case_keeps_the_batch_and_its_graph():
+    """A case parked in a batch takes no step: its state keeps its bits while
+    the others advance, and the others' results equal a batch without it. The
+    batch keeps its composition, so from its second step on every step
+    replays the captured graph (a2d_pool_stats)."""
+    cases = _cases()[:3]
+    pool = _abi.Pool(0, default_knobs("mixed"))
+    try:
+        ids = np.array([pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, _abi.PREC_MIXED) for c in cases],
+                       np.int32)
+        out = np.empty((3, 10))
+        cfl = np.full(3, 10.0)
+        _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), 3, _abi.dptr(cfl), _abi.dptr(out))
+        pool.park(int(ids[1]))
+        ni, nj = cases[1].grid.x.shape
+        U0 = np.empty((4, ni, nj - 1))
+        _abi.call("a2d_case_get_state", pool.handle, int(ids[1]), _abi.dptr(U0))
+        for _ in range(4):
+            _abi.call("a2d_newton_step", pool.handle, _abi.iptr(ids), 3, _abi.dptr(cfl), _abi.dptr(out))
+        U1 = np.empty_like(U0)
+        _abi.call("a2d_case_get_state", pool.handle, int(ids[1]), _abi.dptr(U1))
+        assert np.array_equal(U0, U1)
+        st = pool.stats()
+        assert st["steps"] == 5 and st["graph_replays"] == 4 and st["plans"] == 1
+        ni, nj = cases[2].grid.x.shape
+        Ua = np.empty((4, ni, nj - 1))
+        _abi.call("a2d_case_get_state", pool.handle, int(ids[2]), _abi.dptr(Ua))
+        with pytest.raises(_abi.HipError):
+            _abi.call("a2d_case_park", pool.handle, int(ids[1]), 2)
+    finally:
+        pool.close()
+    # the same five steps of cases 0 and 2 alone, with case 1 stepping once
+    pool = _abi.Pool(0, default_knobs("mixed"))
+    try:
+        jd = np.array([pool.add_case(c.grid.x, c.grid.y, c.mach, c.alpha, _abi.PREC_MIXED) for c in cases],
+                      np.int32)
+        out = np.empty((3, 10))
+        _abi.call("a2d_newton_step", pool.handle, _abi.iptr(jd), 3, _abi.dptr(np.full(3, 10.0)), _abi.dptr(out))
+        sub = np.ascontiguousarray(jd[[0, 2]])
+        for _ in range(4):
+            _abi.call("a2d_newton_step", pool.handle, _abi.iptr(sub), 2, _abi.dptr(np.full(2, 10.0)),
+                      _abi.dptr(out[:2].copy()))
+        Ub = np.empty_like(Ua)
+        _abi.call("a2d_case_get_state", pool.handle, int(jd[2]), _abi.dptr(Ub))
+    finally:
+        pool.close()
+    assert np.array_equal(Ua, Ub)
+
+
+def test_removed_case_slabs_are_reused():
+    """Cases removed from a pool leave their slabs for the next cases of their
+    size: admitting, removing and re-admitting allocates no new slab, and
+    a2d_forces allocates nothing per call."""
+    g = _cases()[0].grid
+    pool = _abi.Pool(0, default_knobs("mixed"))
+    try:
+        a = [pool.add_case(g.x, g.y, 0.5, 1.0, _abi.PREC_MIXED) for _ in range(3)]
+        for c in a:
+            pool.remove_case(c)
+        st = pool.stats()
+        assert st["slabs_allocated"] == 3 and st["bytes_held"] > 0
+        b = [pool.add_case(g.x, g.y, 0.6, 2.0, _abi.PREC_MIXED) for _ in range(3)]
+        st = pool.stats()
+        assert st["slabs_allocated"] == 3 and st["slabs_reused"] == 3 and st["bytes_held"] == 0
+        f3, cp, circ = np.empty(3), np.empty(g.x.shape[0]), np.empty(1)
+        for c in b:
+            _abi.call("a2d_forces", pool.handle, c, _abi.dptr(f3), _abi.dptr(cp), _abi.dptr(circ))
+    finally:
+        pool.close()
+
+
+def test_budget_frees_held_slabs():
+    """With an HBM budget, slabs held for reuse are released when a case of
+    another size needs the room: live plus held bytes stay within the budget."""
+    g1, g2 = _cases()[0].grid, _cases()[1].grid
+    one = _abi.case_bytes(g2.x.shape[0], g2.x.shape[1] - 1, _abi.PREC_MIXED, 20)
+    pool = _abi.Pool(int(1.5 * one), default_knobs("mixed"))
+    try:
+        c = pool.add_case(g2.x, g2.y, 0.5, 1.0, _abi.PREC_MIXED)
+        pool.remove_case(c)
+        assert pool.stats()["bytes_held"] > 0
+        d = pool.add_case(g1.x, g1.y, 0.5, 1.0, _abi.PREC_MIXED)      # a smaller size: no reuse
+        st = pool.stats()
+        assert pool.used_bytes() + st["bytes_held"] <= int(1.5 * one)
+        pool.remove_case(d)
+    finally:
+        pool.close()
+
+
+def test_serving_parks_finished_cases_and_replays_graphs():
+    """At low load (no request waiting when a case finishes) the server parks
+    finished cases, so most steps replay a captured graph; results are the
+    same as without parking."""
+    cases = _cases()
+    reqs = [Request(0.0, c) for c in cases[:3]]
+    res_p, srv_p, _ = serve_stream(reqs, "mixed", tol=1e-9)
+    res_n, srv_n, _ = serve_stream(reqs, "mixed", tol=1e-9, park=False)
+    by = {r.name: r for r in res_n}
+    for r in res_p:
+        assert r.iterations == by[r.name].iterations and abs(r.cl - by[r.name].cl) <= 1e-12
+    sp, sn = srv_p.final_stats, srv_n.final_stats
+    print("parking", sp, "no parking", sn)
+    assert sp["graph_replay_fraction"] > sn["graph_replay_fraction"]
+    assert sp["graph_replay_fraction"] > 0.9
