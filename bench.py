@@ -84,10 +84,10 @@ def time_kernels(cases, precision, reps=20, knobs=None):
 # the counter reads for it) and write by 1.0 (profiles/traffic_config2_fp64_pmc.txt).
 TRAFFIC_CELLS = 64 * 256 * 128
 MEASURED_TRAFFIC = {"line_sweep": 1492.1e6, "residual": 404.9e6}
-# HBM bytes per launch at the transonic batch (32 x 512 x 256 cells), from the
-# same two counter runs of `bench.py --workload kernels3` (none measured yet:
-# null in the bench line)
-MEASURED_TRAFFIC_C3 = {}
+# HBM bytes per launch at the transonic batch (32 x 512 x 256 cells): FETCH_SIZE
+# x 1.9967 + WRITE_SIZE of `bench.py --workload kernels3`
+# (profiles/kernels_config3_counters.txt)
+MEASURED_TRAFFIC_C3 = {"sweep_rb_half": 469.8e6, "jvp": 438.0e6, "residual": 674.7e6}
 
 
 def rooflines(kt, cells):
@@ -102,7 +102,7 @@ def rooflines(kt, cells):
     sw = one("line_sweep", SWEEP_BYTES_PER_CELL)
     sw["kernel"] = "k_sweep<double, coupled, g0> (one coupled line-Jacobi sweep, every line)"
     res = one("residual", RES_BYTES_PER_CELL)
-    res["kernel"] = "k_stencil<double, false> (FP64 residual)"
+    res["kernel"] = "FP64 residual (LDS tiles or column march, as chosen for the batch)"
     return {"roofline": sw, "roofline_residual": res}
 
 
@@ -323,10 +323,13 @@ def form_entry(r, walls=None, convs=None):
 
 
 # Dominant kernel of the mixed transonic solve (largest single share of device
-# time): the FP32 Jacobian-vector product, 17 words per cell: read U 4, v 4,
-# face metrics S_i 2 and S_j 2, diag 1; write 4 (4 B each).
+# time, profiles/solve_config3_mixed_trace.txt): the coupled red-black half
+# sweep with FP16 couplings, per cell of its colour: read S0 src (g0, 4 FP32),
+# the two neighbour lines' x (8 FP32), the four coupling blocks (64 FP16);
+# write x (4 FP32) = 192 B. Next: the FP32 Jacobian-vector product, 17 words
+# per cell: read U 4, v 4, face metrics S_i 2 and S_j 2, diag 1; write 4.
+SWEEP_RB_BYTES_PER_COLOUR_CELL = 192
 JVP32_BYTES_PER_CELL = 68
-JVP64_BYTES_PER_CELL = 136
 
 
 def run_config3(a):
@@ -403,13 +406,19 @@ def run_config3(a):
         kt = time_kernels(cases, "mixed", reps=20)
         cells = sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases)
         rl = {}
-        for name, bpc, kname in (("jvp", JVP32_BYTES_PER_CELL, "k_stencil<float, true> (FP32 Jacobian-vector product)"),
-                                 ("residual", RES_BYTES_PER_CELL, "k_stencil<double, false> (FP64 residual)")):
-            ach = bpc * cells / (kt[name] * 1e-3) / 1e9
+        for name, nbytes, kname in (
+                ("sweep_rb_half", SWEEP_RB_BYTES_PER_COLOUR_CELL * cells // 2,
+                 "k_sweep<float, coupled, g0, FP16 couplings, 32 lines, d in LDS> (one red-black half sweep)"),
+                ("jvp", JVP32_BYTES_PER_CELL * cells,
+                 "k_march<float, tangent> (FP32 Jacobian-vector product, column march)"),
+                ("residual", RES_BYTES_PER_CELL * cells, "k_march<double, residual> (FP64 residual, column march)")):
+            ach = nbytes / (kt[name] * 1e-3) / 1e9
             rl[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_C3.get(name),
-                        "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name], "kernel": kname}
-        out["roofline"] = rl["jvp"]
+                        "bytes_per_launch": nbytes, "ms_per_launch": kt[name], "kernel": kname}
+        out["roofline"] = rl["sweep_rb_half"]
+        out["roofline_jvp"] = rl["jvp"]
+        out["roofline_jvp"]["note"] = "instruction bound: VALU busy 0.93 (profiles/kernels_config3_counters.txt)"
         out["roofline_residual"] = rl["residual"]
         out["kernel_ms_mixed"] = kt
         k64 = time_kernels(cases[:16], "fp64", reps=10)
