@@ -9,7 +9,8 @@ hand-written gfx950 HIP kernels behind a C ABI (include/aero2d_hip.h).
     g = ogrid(naca4("0012", 65), nj=49, ds0=2e-3, radius=40.0)
     r = ImplicitEuler(EulerSolver(g, dtype=np.float64)).solve([0.5], [2.0], tol=1e-10)
 """
-from .geometry import Airfoil, naca4, naca4_params, cosine_spacing
+from .geometry import (Airfoil, naca4, naca4_params, cosine_spacing, normalize, repanel, smooth_airfoil,
+                       load_airfoil)
 from .grid import OGrid, ogrid, growth_ratio, self_similar_spacing
 from .euler import (GAMMA, EulerResult, EulerSolver, compute_metrics, cons_to_prim, hllc_flux,
                     prim_to_cons)
@@ -17,6 +18,7 @@ from .implicit import ImplicitEuler
 
 __version__ = "0.1.0"
 
-__all__ = ["Airfoil", "naca4", "naca4_params", "cosine_spacing", "OGrid", "ogrid", "growth_ratio",
+__all__ = ["Airfoil", "naca4", "naca4_params", "cosine_spacing", "normalize", "repanel", "smooth_airfoil",
+           "load_airfoil", "OGrid", "ogrid", "growth_ratio",
            "self_similar_spacing", "GAMMA", "EulerResult", "EulerSolver", "compute_metrics",
            "cons_to_prim", "hllc_flux", "prim_to_cons", "ImplicitEuler"]
