@@ -66,6 +66,7 @@ SIGNATURES = {
     "a2d_precond_apply": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp]),
     "a2d_newton_step": (ctypes.c_int, [_i64, _ip, ctypes.c_int, _dp, _dp]),
     "a2d_hllc_flux": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "a2d_hllc_flux_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
     "a2d_time_kernel": (ctypes.c_int, [_i64, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
 }
 

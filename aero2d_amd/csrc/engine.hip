@@ -241,6 +241,15 @@ struct Pool {
 std::map<int64_t, std::unique_ptr<Pool>> g_pools;
 int64_t g_next_pool = 1;
 
+// Staging memory of the allocation-free HLLC entry (a2d_hllc_flux_noalloc):
+// WL, WR (4 n each), S (2 n), F (4 n) for at most NOALLOC_MAX_FACES faces,
+// 7.3 MB held on every device the library runs on until the process ends.
+// Host code reaches it only through the per-device address from
+// hipGetSymbolAddress (the array's name is not a device address on the host).
+constexpr int NOALLOC_MAX_FACES = 1 << 16;
+static __device__ __align__(256) double scratch[14 * NOALLOC_MAX_FACES];
+std::map<int, double*> g_scratch;   // device -> address of its copy of `scratch`
+
 void default_knobs(a2d_knobs* k) {
   k->cfl_explicit = 2.5;
   k->lim_k = 0.02;
@@ -652,6 +661,11 @@ int a2d_init(int device) {
     HC(hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HC(hipDeviceGetAttribute(&g_lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HC(hipDeviceGetAttribute(&g_lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+    if (!g_scratch.count(device)) {
+      double* ptr = nullptr;
+      HC(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+      g_scratch[device] = ptr;
+    }
     g_device = device;
   });
 }
@@ -1268,6 +1282,25 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
       launch_factor(b.pline, b.nline, b.prec, p.stream);
       HC(hipStreamSynchronize(p.stream));
     }
+  });
+}
+
+int a2d_hllc_flux_noalloc(int n, const double* WL, const double* WR, const double* S, double* F) {
+  return guarded([&] {
+    ensure_init();
+    if (n < 0 || n > NOALLOC_MAX_FACES)
+      fail(-6, "a2d_hllc_flux_noalloc takes at most " + std::to_string(NOALLOC_MAX_FACES) + " faces");
+    if (n == 0) return;
+    auto it = g_scratch.find(g_device);
+    if (it == g_scratch.end()) fail(-1, "no scratch address for this device: call a2d_init");
+    double* d = it->second;
+    const size_t nn = size_t(n);
+    HC(hipMemcpy(d, WL, nn * 4 * 8, hipMemcpyHostToDevice));
+    HC(hipMemcpy(d + 4 * nn, WR, nn * 4 * 8, hipMemcpyHostToDevice));
+    HC(hipMemcpy(d + 8 * nn, S, nn * 2 * 8, hipMemcpyHostToDevice));
+    launch_hllc_test(n, d, d + 4 * nn, d + 8 * nn, d + 10 * nn, nullptr);
+    HC(hipGetLastError());
+    HC(hipMemcpy(F, d + 10 * nn, nn * 4 * 8, hipMemcpyDeviceToHost));
   });
 }
 

@@ -179,6 +179,13 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
 
 /* Test hook: the device HLLC flux on n faces. WL, WR (4, n), S (2, n), F (4, n). */
 int a2d_hllc_flux(int n, const double* WL, const double* WR, const double* S, double* F);
+/* The same without any device allocation: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Stages through a fixed device
+ * array of the library (7.3 MB per device, held until the process ends), so
+ * n is at most 65536 (-6 beyond that). Calls that overlap on one device race
+ * on that array: do not call it from two threads at once. */
+int a2d_hllc_flux_noalloc(int n, const double* WL, const double* WR, const double* S, double* F);
 
 #ifdef __cplusplus
 }
