@@ -12,6 +12,8 @@ Modules:
   newton  face Jacobians, assembly, block line solve, line-Jacobi
           preconditioner, JAX-"batched" GMRES, Newton step and host loop
           (aero2d/implicit.py, jax.scipy.sparse.linalg.gmres)
+  explicit  four-stage RK with local time steps, implicit residual smoothing,
+          FAS V-cycle and the chunked host loop (aero2d/euler.py:219-435)
 
 Pinning (what checks each function, all in tests/test_oracle_*.py):
   cons_to_prim/prim_to_cons  round trip (tests/test_euler.py:27-29 of the reference)
@@ -34,6 +36,13 @@ Pinning (what checks each function, all in tests/test_oracle_*.py):
   solve                      reference goldens (benchmarks/results.json:83-104):
                              converged FP64 Cl/Cd at M 0.5 a 2 and M 0.8 a 1.25
                              on naca4("0012",65), ogrid(nj=49, ds0=2e-3, radius=40)
+  explicit.smooth            the smoothing operators applied to its output
+                             return the input (tests/test_oracle_explicit.py)
+  explicit.sum4/prolong      volume-weighted mean of four cells; constant and
+                             linear corrections reproduced
+  explicit.iterate/cycle/    the solve drives R(U) to zero; that state is a
+  solve                      fixed point of both; Newton steps of `newton`
+                             from it leave it (same zero of the residual)
 Parity unpinned: the per-iteration trajectory of JAX's GMRES (its internal
 re-orthogonalisation test and normal-equation least squares are restated from
 JAX's published source, not run); only converged answers are pinned.
