@@ -31,7 +31,8 @@ class Knobs(ctypes.Structure):
                 ("early_exit", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("precond_order", ctypes.c_int), ("positivity", ctypes.c_int),
                 ("precond_half", ctypes.c_int), ("basis_half", ctypes.c_int),
-                ("stencil_march", ctypes.c_int), ("sweep_lds", ctypes.c_int)]
+                ("stencil_march", ctypes.c_int), ("sweep_lds", ctypes.c_int),
+                ("irs_eps", ctypes.c_double)]
 
 
 # name -> (restype, argtypes); the exported surface of include/aero2d_hip.h
@@ -65,6 +66,8 @@ SIGNATURES = {
     "a2d_precond_setup": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_double]),
     "a2d_precond_apply": (ctypes.c_int, [_i64, ctypes.c_int, _dp, _dp]),
     "a2d_newton_step": (ctypes.c_int, [_i64, _ip, ctypes.c_int, _dp, _dp]),
+    "a2d_case_link_coarse": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int]),
+    "a2d_explicit_cycles": (ctypes.c_int, [_i64, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "a2d_hllc_flux": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
     "a2d_hllc_flux_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
     "a2d_time_kernel": (ctypes.c_int, [_i64, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
@@ -209,6 +212,18 @@ class Pool:
 
     def remove_case(self, cid: int):
         call("a2d_case_remove", self.handle, cid)
+
+    def link_coarse(self, cid: int, coarse: int):
+        """a2d_case_link_coarse: `coarse` becomes the next coarser grid level of `cid`."""
+        call("a2d_case_link_coarse", self.handle, int(cid), int(coarse))
+
+    def explicit_cycles(self, ids, ncycles: int, fresh: bool) -> np.ndarray:
+        """a2d_explicit_cycles: (n, 5) of res, cl, cd, cm, Gamma per case."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        out = np.empty((len(ids), 5))
+        call("a2d_explicit_cycles", self.handle, ids.ctypes.data_as(_ip), len(ids), int(ncycles), int(bool(fresh)),
+             dptr(out))
+        return out
 
     def close(self):
         if getattr(self, "handle", None) is not None:

@@ -14,6 +14,9 @@
 //                                double arrays when T = double)
 //   blocks                T [5][nc][ni][16]       cell-major 4x4 blocks, five slots
 //   V                     T [K+1][4][nc][ni]      raw Krylov basis
+// The explicit scheme (FP64 cases only) reuses the Newton buffers: dU holds
+// the Runge-Kutta stage's starting state, w the j-smoothed residual, xa the
+// FAS forcing and xb the restricted state of a coarse level, diag vol / dt.
 // Consecutive block slots sit 16 pstride elements apart and basis vectors
 // vstride elements apart: the sizes plus a pad, so that the many arrays a
 // kernel walks in step do not all start at the same offset modulo a large
@@ -46,6 +49,7 @@ struct CaseScal {
   double gres;         // GMRES residual estimate of the current iterate
   double vn0;          // |w| before orthogonalisation
   double vsig;         // FP16 basis: power-of-two storage scale of V_0
+  double xres;         // explicit scheme: sqrt(mean(((U - U0)_rho / dt)^2)) of the last cycle
   int run;             // the Krylov cycle runs (|M(b - A x0)| > ptol)
   int active;          // still taking Arnoldi steps
   int kdone;           // Arnoldi steps completed
@@ -103,6 +107,7 @@ struct CaseDev {
   void* p;              // mixed form, FP16 basis: FP32 preconditioner output [4][nc][ni]
   void* vc;             // and the FP32 copy of the newest (rounded) basis vector
   double* part;         // [kmax + 2][npart] reduction partials (column-major)
+  double* irs;          // explicit scheme: [2][nc] elimination coefficients of the j smoothing
   CaseScal* sc;
 };
 

@@ -81,7 +81,7 @@ KnobsDev to_dev(const a2d_knobs& k) {
 struct Layout {
   size_t U, R, dU, vol, Si, Sj, xf, yf, xw, yw;
   size_t Ut, volt, Sit, Sjt, xft, yft;
-  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, pv, vc, part, cp, sc;
+  size_t diag, blk, hb, V, x0, w, d, g0, xa, xb, pv, vc, part, cp, irs, sc;
   size_t total;
   int npart;
 };
@@ -155,9 +155,12 @@ Layout layout(int ni, int nc, int prec, int kmax) {
     L.pv = take(4 * n * 4);
     L.vc = take(4 * n * 4);
   }
-  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, 8) + 2, 1});
+  // partial rows: stencil/march units, vector blocks, sweep groups, and the
+  // explicit scheme's per-row residual partials (nc)
+  L.npart = std::max({max_tiles(ni, nc), cdiv(int(n), VEC_CHUNK), 2 * cdiv(ni, 8) + 2, nc, 1});
   L.part = take(size_t(L.npart) * (kmax + 2) * 8);
   L.cp = take(size_t(ni) * 8);
+  L.irs = take(size_t(2) * nc * 8);
   L.sc = take(sizeof(CaseScal));
   L.total = o;
   return L;
@@ -169,6 +172,9 @@ struct CaseHost {
   char* slab = nullptr;
   size_t bytes = 0;     // the slab's capacity (>= the layout's total)
   double* cp = nullptr;  // wall Cp buffer (ni) for a2d_forces
+  int coarse = -1;       // explicit scheme: the case of the next coarser grid level
+  double irs_eps = -1;   // smoothing coefficient the case's irs table was formed for
+  bool parked = false;
   CaseDev dev{};
   double winf[4];
 };
@@ -192,6 +198,9 @@ struct BatchPlan {
   // the batch's unit counts that decide whether it is used
   Plan pmr{}, pmj{};
   int nmr = 0, nmj = 0;
+  // explicit scheme: rows (four per block) and columns (256 per block)
+  Plan prow{}, pcol{};
+  int nrow = 0, ncol = 0, ni_max = 0;
   struct Group {
     Plan p;
     int shape, nblocks;
@@ -267,6 +276,7 @@ void default_knobs(a2d_knobs* k) {
   k->basis_half = 0;
   k->stencil_march = 1;
   k->sweep_lds = 1;
+  k->irs_eps = 0.6;
 }
 
 Pool& get_pool(int64_t id) {
@@ -359,6 +369,8 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   auto psw = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni, lsw); });
   auto prb = add_prefix([&](const CaseHost& c) { return cdiv(c.dev.ni / 2, lrb); });
   const int rows_jvp = bp->prec == 0 ? MARCH_ROWS_JVP64 : MARCH_ROWS_JVP32;
+  auto prow = add_prefix([](const CaseHost& c) { return c.nc; });
+  auto pcol = add_prefix([](const CaseHost& c) { return cdiv(4 * c.ni, EXPLICIT_BLOCK); });
   auto pmr = add_prefix([](const CaseHost& c) { return march_blocks(c.ni, c.nc, MARCH_ROWS_RES); });
   auto pmj = add_prefix([&](const CaseHost& c) { return march_blocks(c.ni, c.nc, rows_jvp); });
   std::map<int, std::vector<int>> groups;
@@ -442,6 +454,11 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   };
   bp->psw.dsteps = lds_steps(lsw, bp->nsw);
   bp->prb.dsteps = lds_steps(lrb, bp->nrb);
+  bp->prow = mk(prow.first);
+  bp->nrow = prow.second;
+  bp->pcol = mk(pcol.first);
+  bp->ncol = pcol.second;
+  for (int s : slots) bp->ni_max = std::max(bp->ni_max, p.cases[s].ni);
   bp->pmr = mk(pmr.first);
   bp->nmr = pmr.second;
   bp->pmj = mk(pmj.first);
@@ -509,12 +526,13 @@ bool march_jvp(const Pool& p, const BatchPlan& b) {
   return k == 2 || (k == 1 && b.prec == 1 && b.nmj * MARCH_WAVES >= 4 * 4 * g_num_cu);
 }
 
-void enq_residual(Pool& p, BatchPlan& b, const KnobsDev& kn, int want_diag, double circ, int refresh_ut) {
+void enq_residual(Pool& p, BatchPlan& b, const KnobsDev& kn, int want_diag, double circ, int refresh_ut,
+                  bool finalize = true) {
   const bool m = march_res(p, b);
   if (m) launch_march_residual(b.pmr, b.nmr, kn, want_diag, circ, refresh_ut, p.stream);
   else
     for (auto& g : b.sten) launch_residual(g.p, g.shape, g.nblocks, kn, want_diag, circ, refresh_ut, p.stream);
-  launch_rn_finalize(b.pc, m, p.stream);
+  if (finalize) launch_rn_finalize(b.pc, m, p.stream);
 }
 
 void enq_jvp(Pool& p, BatchPlan& b, const KnobsDev& kn, int vin, int vout, int mode, int kidx, int use_diag,
@@ -575,6 +593,65 @@ void enq_newton(Pool& p, BatchPlan& b) {
   launch_apply(b.pvec, b.nvec, s);
   launch_forces(b.pc, nullptr, s);
   launch_gather(b.pc, b.dout, s);
+}
+
+// ---- explicit scheme ------------------------------------------------------------
+constexpr double RK_STAGES[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};   // aero2d/euler.py:340
+
+// Elimination coefficients of the j smoothing, (1 + 2 eps) on the diagonal,
+// 1 + eps on both end rows, -eps off it (aero2d/euler.py:305-311): m_j =
+// 1 / pivot_j and c_j = -eps m_j, formed in FP64 on the host.
+void irs_table(Pool& p, CaseHost& c, double eps) {
+  if (c.irs_eps == eps) return;
+  const int nc = c.nc;
+  std::vector<double> t(2 * size_t(nc));
+  double* m = t.data();
+  double* cc = t.data() + nc;
+  for (int j = 0; j < nc; ++j) {
+    const double b = (j == 0 || j == nc - 1) ? 1.0 + eps : 1.0 + 2.0 * eps;
+    m[j] = 1.0 / (j == 0 ? b : b + eps * cc[j - 1]);
+    cc[j] = -eps * m[j];
+  }
+  HC(hipMemcpyAsync(c.dev.irs, t.data(), t.size() * 8, hipMemcpyHostToDevice, p.stream));
+  HC(hipStreamSynchronize(p.stream));
+  c.irs_eps = eps;
+}
+
+// One Runge-Kutta iteration on level l (aero2d/euler.py:334-345): the local
+// time step from the starting state (the first stage's residual kernel), four
+// stages of residual, smoothing and update; coarse levels add their forcing
+// and run first order. want_res: the res of the iteration per case.
+void enq_iterate(Pool& p, std::vector<BatchPlan*>& L, int l, const KnobsDev& kn, bool want_res) {
+  BatchPlan& b = *L[l];
+  KnobsDev k = kn;
+  if (l > 0) k.order = 1;
+  const double eps = p.knobs.irs_eps;
+  const int forcing = l > 0;
+  const int cm = rk_rows_cm(b.ni_max);
+  for (int st = 0; st < 4; ++st) {
+    enq_residual(p, b, k, st == 0, NAN, 0, false);
+    if (eps > 0) launch_irs_j(b.pcol, b.ncol, eps, forcing, p.stream);
+    launch_rk_rows(b.prow, b.nrow, cm, eps, RK_STAGES[st], st == 0, want_res && st == 3, forcing, p.stream);
+  }
+  if (want_res) launch_xres(b.pc, p.stream);
+}
+
+// FAS V-cycle from level l down (aero2d/euler.py:367-381).
+void enq_cycle(Pool& p, std::vector<BatchPlan*>& L, int l, const KnobsDev& kn) {
+  enq_iterate(p, L, l, kn, l == 0);
+  if (l + 1 >= int(L.size())) return;
+  BatchPlan& f = *L[l];
+  BatchPlan& c = *L[l + 1];
+  KnobsDev kf = kn, kc = kn;
+  if (l > 0) kf.order = 1;
+  kc.order = 1;
+  enq_residual(p, f, kf, 0, NAN, 0, false);
+  launch_restrict(c.pcell, c.ncell, f.pc.slots, l > 0, p.stream);
+  enq_residual(p, c, kc, 0, NAN, 0, false);
+  launch_forcing(c.pcell, c.ncell, p.stream);
+  enq_cycle(p, L, l + 1, kn);
+  launch_prolong(f.pcell, f.ncell, c.pc.slots, p.stream);
+  enq_iterate(p, L, l, kn, false);
 }
 
 // ---- host <-> device field transfer -------------------------------------------
@@ -719,6 +796,8 @@ int a2d_pool_set_knobs(int64_t pool, const a2d_knobs* k) {
     if (k->basis_half != 0 && k->basis_half != 1) fail(-6, "basis_half must be 0 or 1");
     if (k->stencil_march < 0 || k->stencil_march > 2) fail(-6, "stencil_march must be 0, 1 or 2");
     if (k->sweep_lds < 0 || k->sweep_lds > 2) fail(-6, "sweep_lds must be 0, 1 or 2");
+    if (!(k->irs_eps >= 0.0) || !std::isfinite(k->irs_eps)) fail(-6, "irs_eps must be finite and >= 0");
+    if (!(k->cfl_explicit > 0.0)) fail(-6, "cfl_explicit must be > 0");
     for (auto& c : p.cases)
       if (c.live && k->precond_order == 1 && (c.ni % 2) != 0)
         fail(-6, "red-black line ordering needs an even number of cells around");
@@ -894,6 +973,7 @@ int a2d_case_add(int64_t pool, int ni, int nc, const double* x, const double* y,
     d.part = reinterpret_cast<double*>(b + L.part);
     d.sc = reinterpret_cast<CaseScal*>(b + L.sc);
     c.cp = reinterpret_cast<double*>(b + L.cp);
+    d.irs = reinterpret_cast<double*>(b + L.irs);
 
     // metrics from the nodes, FP64 (aero2d/euler.py:44-60); node (i, j) at x[i*(nc+1)+j]
     const int nj = nc + 1;
@@ -977,6 +1057,8 @@ int a2d_case_remove(int64_t pool, int case_id) {
     drop_plans_with(p, case_id);
     release_slab(p, c);
     c = CaseHost{};
+    for (auto& o : p.cases)
+      if (o.coarse == case_id) o.coarse = -1;
   });
 }
 
@@ -1009,6 +1091,7 @@ int a2d_case_park(int64_t pool, int case_id, int parked) {
     Pool& p = get_pool(pool);
     CaseHost& c = get_case(p, case_id);
     if (parked != 0 && parked != 1) fail(-6, "parked must be 0 or 1");
+    c.parked = parked != 0;
     HC(hipMemcpyAsync(&c.dev.sc->parked, &parked, sizeof(int), hipMemcpyHostToDevice, p.stream));
     HC(hipStreamSynchronize(p.stream));
   });
@@ -1219,6 +1302,85 @@ int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl,
     HC(hipMemcpyAsync(b.hout, b.dout, n * 10 * sizeof(double), hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     std::memcpy(out, b.hout, n * 10 * sizeof(double));
+  });
+}
+
+int a2d_case_link_coarse(int64_t pool, int case_id, int coarse_id) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    CaseHost& f = get_case(p, case_id);
+    if (coarse_id < 0) {
+      f.coarse = -1;
+      return;
+    }
+    CaseHost& c = get_case(p, coarse_id);
+    if (coarse_id == case_id) fail(-6, "a case cannot be its own coarse level");
+    if (f.prec != 0 || c.prec != 0) fail(-6, "the explicit scheme's grid levels must be FP64 cases");
+    if ((f.ni % 2) != 0 || (f.nc % 2) != 0 || 2 * c.ni != f.ni || 2 * c.nc != f.nc)
+      fail(-6, "the coarse case must have half the cells of the fine case along i and j (even counts)");
+    f.coarse = coarse_id;
+  });
+}
+
+int a2d_explicit_cycles(int64_t pool, const int* case_ids, int n, int ncycles, int fresh, double* out) {
+  return guarded([&] {
+    Pool& p = get_pool(pool);
+    if (n <= 0) fail(-6, "empty batch");
+    if (ncycles < 0) fail(-6, "ncycles must be >= 0");
+    std::vector<std::vector<int>> levels{std::vector<int>(case_ids, case_ids + n)};
+    check_batch(p, levels[0]);
+    for (;;) {
+      const std::vector<int>& cur = levels.back();
+      int linked = 0;
+      for (int s : cur) linked += p.cases[s].coarse >= 0;
+      if (linked == 0) break;
+      if (linked != int(cur.size())) fail(-6, "every case of a batch needs the same number of grid levels");
+      std::vector<int> nxt;
+      for (int s : cur) {
+        const CaseHost& f = p.cases[s];
+        const CaseHost& c = get_case(p, f.coarse);
+        if (2 * c.ni != f.ni || 2 * c.nc != f.nc || c.prec != 0)
+          fail(-6, "a coarse level no longer matches its fine case");
+        nxt.push_back(f.coarse);
+      }
+      if (levels.size() >= 16) fail(-6, "too many grid levels (do the coarse links form a cycle?)");
+      levels.push_back(nxt);
+    }
+    std::vector<int> all;
+    int ni_max = 0;
+    for (auto& lv : levels)
+      for (int s : lv) {
+        const CaseHost& c = p.cases[s];
+        if (c.prec != 0) fail(-5, "the explicit scheme runs FP64 cases only");
+        if (c.parked) fail(-6, "a parked case takes no steps");
+        ni_max = std::max(ni_max, c.ni);
+        all.push_back(s);
+      }
+    check_batch(p, all);   // no case on two levels or twice on one
+    const double eps = p.knobs.irs_eps;
+    if (eps > 0 && rk_rows_lds(rk_rows_cm(ni_max)) > size_t(g_lds_block))
+      fail(-6, "rows too long for the smoothing kernel (" + std::to_string(ni_max) + " cells around)");
+    if (eps > 0)
+      for (int s : all) irs_table(p, p.cases[s], eps);
+    // fetch every level's plan with no eviction in between
+    if (p.plans.size() + levels.size() >= 64) p.plans.clear();
+    std::vector<BatchPlan*> L;
+    for (auto& lv : levels) L.push_back(&get_plan(p, lv));
+    const KnobsDev kn = to_dev(p.knobs);
+    const double cfl = p.knobs.cfl_explicit;
+    BatchPlan& top = *L[0];
+    for (int it = 0; it < ncycles; ++it) {
+      for (size_t l = 0; l < L.size(); ++l)
+        launch_level_begin(L[l]->pc, top.pc.slots, cfl, fresh && it == 0 && l == 0, p.stream);
+      enq_cycle(p, L, 0, kn);
+      // Gamma = Cl |V_inf| / 2 of the new state (aero2d/euler.py:400-405)
+      launch_forces(top.pc, nullptr, p.stream);
+    }
+    if (ncycles == 0 && fresh) launch_level_begin(top.pc, top.pc.slots, cfl, 1, p.stream);
+    launch_xgather(top.pc, top.dout, p.stream);
+    HC(hipMemcpyAsync(top.hout, top.dout, size_t(n) * 5 * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    HC(hipStreamSynchronize(p.stream));
+    std::memcpy(out, top.hout, size_t(n) * 5 * sizeof(double));
   });
 }
 

@@ -128,4 +128,27 @@ void launch_set_cfl(const Plan& pc, const double* cfl, hipStream_t s);
 void launch_gather(const Plan& pc, double* out, hipStream_t s);
 void launch_copy_vec(const Plan& pv, int nblocks, int src, int dst, int prec, hipStream_t s);
 
+// ---- explicit.hip ---------------------------------------------------------------
+// The explicit scheme's kernels (FP64 cases). Plans: prow, one block per four
+// rows (nc blocks per case); pcol, NT-thread blocks over the 4 ni columns;
+// cell plans CELL_BLOCK cells per block.
+// Elements per lane of the row kernel for rows of up to ni_max cells, and the
+// LDS bytes its workgroup then takes.
+constexpr int EXPLICIT_BLOCK = 256;   // threads per workgroup of the explicit kernels
+int rk_rows_cm(int ni_max);
+size_t rk_rows_lds(int cm);
+// Smoothing along j of R (+ the forcing in xa) into w.
+void launch_irs_j(const Plan& pcol, int nblocks, double eps, int forcing, hipStream_t s);
+// One Runge-Kutta stage, U = U0 - a x / diag: x the smoothed residual (from w,
+// smoothing along i; eps > 0) or R (+ P when forcing; eps = 0). first: U0 =
+// U, kept in dU; want_res: the residual partials of the stage.
+void launch_rk_rows(const Plan& prow, int nblocks, int cm, double eps, double a, int first, int want_res,
+                    int forcing, hipStream_t s);
+void launch_xres(const Plan& pc, hipStream_t s);
+void launch_level_begin(const Plan& pc, const int* top_slots, double cfl, int zero_circ, hipStream_t s);
+void launch_restrict(const Plan& pcc, int nblocks, const int* fine_slots, int fine_forcing, hipStream_t s);
+void launch_forcing(const Plan& pcc, int nblocks, hipStream_t s);
+void launch_prolong(const Plan& pfc, int nblocks, const int* coarse_slots, hipStream_t s);
+void launch_xgather(const Plan& pc, double* out, hipStream_t s);
+
 }  // namespace a2d

@@ -100,8 +100,10 @@ class EulerSolver:
     dtype float64 selects the FP64 form; float32 (the reference default)
     selects the mixed form, whose residual, update and forces stay FP64 while
     the Jacobian-vector product, preconditioner and Krylov vectors are FP32.
-    `precision` ("fp64" / "mixed") overrides dtype. irs_eps and mg_levels
-    belong to the reference's explicit scheme and are stored only.
+    `precision` ("fp64" / "mixed") overrides dtype. cfl, irs_eps and
+    mg_levels are the explicit scheme's (`solve`), which runs in FP64 whatever
+    the dtype; with mg_levels > 1 every level must have even cell counts
+    along i and j (ValueError otherwise, as the reference raises).
     stencil_march: the residual/tangent kernel (a2d_knobs.stencil_march: 0 LDS
     tiles, 2 column march, 1 the march for batches that fill the device).
     """
@@ -125,6 +127,14 @@ class EulerSolver:
         self.stencil_march = stencil_march
         self.mg_levels = mg_levels
         self.ni, self.nc = m.vol.shape
+        self.iterations = 0
+        self.cl_history = np.zeros((0,))
+        self._levels = [(np.asarray(grid.x, np.float64), np.asarray(grid.y, np.float64))]
+        for _ in range(mg_levels - 1):
+            x, y = self._levels[-1]
+            if (x.shape[0] % 2) or ((x.shape[1] - 1) % 2):
+                raise ValueError("multigrid needs an even number of cells along i and j on every level")
+            self._levels.append((x[::2, ::2], y[::2, ::2]))
         self._pool = None
         self._case = None
 
@@ -203,6 +213,93 @@ class EulerSolver:
         circ = np.empty(1)
         _abi.call("a2d_forces", h, c, _abi.dptr(out), _abi.dptr(cp), _abi.dptr(circ))
         return out[0], out[1], out[2], cp
+
+    # ---- the explicit scheme ------------------------------------------------------
+    def explicit_batch(self, machs, alphas, U0=None):
+        """Device cases of the explicit scheme for each (M, alpha): a new pool
+        holding, per case, one FP64 case per grid level, each linked to the
+        next coarser one. U0 (B, 4, ni, nc) or (4, ni, nc): starting state
+        (freestream otherwise). Returns (pool, ids of the finest level); the
+        caller closes the pool."""
+        pool = _abi.Pool(0, self.knobs(irs_eps=float(self.irs_eps)))
+        try:
+            ids = []
+            for b in range(len(machs)):
+                prev = None
+                for x, y in self._levels:
+                    cid = pool.add_case(x, y, machs[b], alphas[b], _abi.PREC_FP64)
+                    if prev is None:
+                        ids.append(cid)
+                    else:
+                        pool.link_coarse(prev, cid)
+                    prev = cid
+            if U0 is not None:
+                U0 = np.asarray(U0, np.float64)
+                for b, cid in enumerate(ids):
+                    Ub = np.ascontiguousarray(U0[b] if U0.ndim == 4 else U0)
+                    _abi.call("a2d_case_set_state", pool.handle, cid, _abi.dptr(Ub))
+        except Exception:
+            pool.close()
+            raise
+        return pool, ids
+
+    def solve(self, machs, alphas, n_iter=4000, chunk=200, tol=1e-7, verbose=True, U0=None,
+              tol_cl=2e-5, tol_cd=2e-6):
+        """Batched explicit solve (aero2d/euler.py:386-435): chunks of `chunk`
+        cycles of four-stage Runge-Kutta with local time steps, residual
+        smoothing and (mg_levels > 1) a FAS V-cycle, Gamma = Cl |V_inf| / 2
+        after every cycle (0 at the start); stop when every case's res fell
+        below tol of its first chunk's, or Cl and Cd of every case moved less
+        than tol_cl, tol_cd over the last chunk. Raises FloatingPointError when
+        the state is not finite at the end. All cases run as one batch on the
+        GPU in FP64. Returns EulerResult with res_hist (n_chunks, B); sets
+        self.iterations and self.cl_history."""
+        machs = np.atleast_1d(np.asarray(machs, np.float64))
+        alphas = np.atleast_1d(np.asarray(alphas, np.float64))
+        if machs.shape != alphas.shape or machs.ndim != 1:
+            raise ValueError("machs and alphas must be 1-D arrays of one length")
+        nb = len(machs)
+        pool, ids = self.explicit_batch(machs, alphas, U0)
+        try:
+            hist, clh, cdh = [], [], []
+            res0 = None
+            self.iterations = 0
+            for it in range(0, n_iter, chunk):
+                out = pool.explicit_cycles(ids, chunk, fresh=(it == 0))
+                self.iterations = it + chunk
+                res, cl, cd = out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy()
+                if res0 is None:
+                    res0 = res
+                hist.append(res)
+                clh.append(cl)
+                cdh.append(cd)
+                if verbose:
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        worst = np.max(np.log10(res / res0))
+                    print(f"  iter {it + chunk:6d}  log10(res/res0) max {worst:+.2f}  Cl {np.array2string(cl, precision=4)}")
+                if not np.all(np.isfinite(res)):
+                    break
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    if np.all(res / res0 < tol):
+                        break
+                if len(clh) > 2 and np.all(np.abs(clh[-1] - clh[-2]) < tol_cl) \
+                        and np.all(np.abs(cdh[-1] - cdh[-2]) < tol_cd):
+                    break
+            self.cl_history = np.array(clh)
+            U = np.empty((nb, 4, self.ni, self.nc))
+            for b, cid in enumerate(ids):
+                _abi.call("a2d_case_get_state", pool.handle, cid, _abi.dptr(U[b]))
+            if not np.all(np.isfinite(U)):
+                raise FloatingPointError("the solution diverged (NaN)")
+            cl, cd, cm = np.empty(nb), np.empty(nb), np.empty(nb)
+            cp = np.empty((nb, self.ni))
+            f3, circ = np.empty(3), np.empty(1)
+            for b, cid in enumerate(ids):
+                _abi.call("a2d_forces", pool.handle, cid, _abi.dptr(f3), _abi.dptr(cp[b]), _abi.dptr(circ))
+                cl[b], cd[b], cm[b] = f3
+        finally:
+            pool.close()
+        return EulerResult(machs, alphas, cl, cd, cm, cp, np.array(hist), U)
 
     def close(self):
         if self._pool is not None:

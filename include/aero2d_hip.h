@@ -72,6 +72,9 @@ typedef struct a2d_knobs {
                          when every block of the launch can still be resident;
                          1 = that only for batches that fill the device (32-line
                          blocks; default). Same results bit for bit */
+  double irs_eps;     /* explicit scheme: residual smoothing coefficient
+                         (EulerSolver.irs_eps, aero2d/euler.py:219-236; 0.6);
+                         0 switches smoothing off */
 } a2d_knobs;
 
 /* Library set-up on one HIP device; idempotent. */
@@ -164,6 +167,27 @@ int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x);
  *   rn (before the update), omega, accepted (0/1), cl, cd, cm (after),
  *   Arnoldi steps taken, max relative change, beta, GMRES residual estimate. */
 int a2d_newton_step(int64_t pool, const int* case_ids, int n, const double* cfl, double* out);
+
+/* The explicit scheme (EulerSolver.solve's cycle, aero2d/euler.py:334-405).
+ * a2d_case_link_coarse: make coarse_id the next coarser grid level of case_id
+ * (the reference's EulerSolver.coarse, built from grid.x[::2, ::2],
+ * aero2d/euler.py:238-250); both FP64 cases, the coarse one with half the cells
+ * of the fine one along i and j (even counts, -6 otherwise); coarse_id -1
+ * unlinks. The coarse case takes its freestream and circulation from the
+ * finest case of the batch at every cycle; its node coordinates are the
+ * caller's (every other node of the fine grid for the reference's scheme).
+ * a2d_explicit_cycles: ncycles cycles on n distinct FP64 cases, each with the
+ * same number of linked levels: a four-stage Runge-Kutta iteration with local
+ * time steps at CFL cfl_explicit and residual smoothing (knob irs_eps), a FAS
+ * V-cycle through the coarse levels (first order there) with a second
+ * iteration after the correction, then Gamma = Cl |V_inf| / 2 of the new state
+ * for the next cycle (aero2d/euler.py:396-405). fresh: Gamma = 0 before the
+ * first cycle (the start of EulerSolver.solve). out: 5 doubles per case, res of
+ * the last cycle (sqrt(mean(((U - U0)_rho / dt)^2)) of its first iteration),
+ * cl, cd, cm, Gamma. -5 for a case of the mixed form, -6 for a parked case or
+ * rows of more than 2048 cells (the smoothing kernel keeps a row in LDS). */
+int a2d_case_link_coarse(int64_t pool, int case_id, int coarse_id);
+int a2d_explicit_cycles(int64_t pool, const int* case_ids, int n, int ncycles, int fresh, double* out);
 
 /* Kernel timing (no counterpart in the reference; for bench.py). Launches
  * kernel family `which` over the batch `reps` times back to back on the pool's
