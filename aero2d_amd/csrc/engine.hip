@@ -198,9 +198,10 @@ struct BatchPlan {
   // the batch's unit counts that decide whether it is used
   Plan pmr{}, pmj{};
   int nmr = 0, nmj = 0;
-  // explicit scheme: rows (four per block) and columns (256 per block)
-  Plan prow{}, pcol{};
-  int nrow = 0, ncol = 0, ni_max = 0;
+  // explicit scheme: rows (four per block) and columns (256 per block, or 32
+  // per block for the segmented j smoothing)
+  Plan prow{}, pcol{}, pseg{};
+  int nrow = 0, ncol = 0, nseg = 0, ni_max = 0, nc_max = 0;
   struct Group {
     Plan p;
     int shape, nblocks;
@@ -371,6 +372,7 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   const int rows_jvp = bp->prec == 0 ? MARCH_ROWS_JVP64 : MARCH_ROWS_JVP32;
   auto prow = add_prefix([](const CaseHost& c) { return c.nc; });
   auto pcol = add_prefix([](const CaseHost& c) { return cdiv(4 * c.ni, EXPLICIT_BLOCK); });
+  auto pseg = add_prefix([](const CaseHost& c) { return cdiv(4 * c.ni, IRS_J_COLS_PER_BLOCK); });
   auto pmr = add_prefix([](const CaseHost& c) { return march_blocks(c.ni, c.nc, MARCH_ROWS_RES); });
   auto pmj = add_prefix([&](const CaseHost& c) { return march_blocks(c.ni, c.nc, rows_jvp); });
   std::map<int, std::vector<int>> groups;
@@ -458,7 +460,12 @@ BatchPlan& get_plan(Pool& p, const std::vector<int>& slots) {
   bp->nrow = prow.second;
   bp->pcol = mk(pcol.first);
   bp->ncol = pcol.second;
-  for (int s : slots) bp->ni_max = std::max(bp->ni_max, p.cases[s].ni);
+  bp->pseg = mk(pseg.first);
+  bp->nseg = pseg.second;
+  for (int s : slots) {
+    bp->ni_max = std::max(bp->ni_max, p.cases[s].ni);
+    bp->nc_max = std::max(bp->nc_max, p.cases[s].nc);
+  }
   bp->pmr = mk(pmr.first);
   bp->nmr = pmr.second;
   bp->pmj = mk(pmj.first);
@@ -617,6 +624,12 @@ void irs_table(Pool& p, CaseHost& c, double eps) {
   c.irs_eps = eps;
 }
 
+void enq_irs_j(Pool& p, BatchPlan& b, double eps, int forcing) {
+  const int sl = irs_j_seg_len(b.nc_max);
+  if (sl > 0) launch_irs_j_seg(b.pseg, b.nseg, sl, eps, forcing, p.stream);
+  else launch_irs_j(b.pcol, b.ncol, eps, forcing, p.stream);
+}
+
 // One Runge-Kutta iteration on level l (aero2d/euler.py:334-345): the local
 // time step from the starting state (the first stage's residual kernel), four
 // stages of residual, smoothing and update; coarse levels add their forcing
@@ -630,7 +643,7 @@ void enq_iterate(Pool& p, std::vector<BatchPlan*>& L, int l, const KnobsDev& kn,
   const int cm = rk_rows_cm(b.ni_max);
   for (int st = 0; st < 4; ++st) {
     enq_residual(p, b, k, st == 0, NAN, 0, false);
-    if (eps > 0) launch_irs_j(b.pcol, b.ncol, eps, forcing, p.stream);
+    if (eps > 0) enq_irs_j(p, b, eps, forcing);
     launch_rk_rows(b.prow, b.nrow, cm, eps, RK_STAGES[st], st == 0, want_res && st == 3, forcing, p.stream);
   }
   if (want_res) launch_xres(b.pc, p.stream);
@@ -1395,6 +1408,18 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     const int k = std::min(25, std::max(0, std::min(p.knobs.gmres_restart, KMAX) - 2));
     const int hb = b.prec == 1 && p.knobs.precond_half;
     const int bh = b.prec == 1 && p.knobs.basis_half;
+    if ((which == 11 || which == 12) && b.prec != 0) fail(-5, "the explicit scheme's kernels run FP64 cases only");
+    const double eps = p.knobs.irs_eps > 0 ? p.knobs.irs_eps : 0.6;
+    if (which == 11 || which == 12)
+      for (int s : slots) irs_table(p, p.cases[s], eps);
+    if (which == 12 && rk_rows_lds(rk_rows_cm(b.ni_max)) > size_t(g_lds_block))
+      fail(-6, "rows too long for the smoothing kernel");
+    // a Runge-Kutta stage overwrites U: keep it in xb (FP64 scratch) meanwhile
+    if (which == 12)
+      for (int s : slots) {
+        const CaseHost& c = p.cases[s];
+        HC(hipMemcpyAsync(c.dev.xb, c.dev.U, size_t(4) * c.ni * c.nc * 8, hipMemcpyDeviceToDevice, p.stream));
+      }
     auto launch = [&] {
       switch (which) {
         case 0:  // as launched in a Newton step: with the local time step and U copy
@@ -1421,6 +1446,10 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
                                b.prec, hb, p.stream);
           break;
         case 10: enq_precond(p, b, VEC_W, VEC_XA, 0, 0); break;
+        case 11: enq_irs_j(p, b, eps, 0); break;
+        case 12:  // a later Runge-Kutta stage: smoothing along i and the update
+          launch_rk_rows(b.prow, b.nrow, rk_rows_cm(b.ni_max), eps, 0.5, 0, 0, 0, p.stream);
+          break;
         default: fail(-6, "unknown kernel id");
       }
     };
@@ -1437,6 +1466,13 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = double(t) / reps;
+    if (which == 12) {
+      for (int s : slots) {
+        const CaseHost& c = p.cases[s];
+        HC(hipMemcpyAsync(c.dev.U, c.dev.xb, size_t(4) * c.ni * c.nc * 8, hipMemcpyDeviceToDevice, p.stream));
+      }
+      HC(hipStreamSynchronize(p.stream));
+    }
     if (which == 4 || which == 5) {
       // assembly/factorisation overwrote the factored blocks: refactor so the
       // cases stay consistent for a following Newton step

@@ -31,10 +31,105 @@ constexpr int NT = EXPLICIT_BLOCK;
 constexpr int ROWS = NT / WAVE;   // rows (one wavefront each) per workgroup
 static_assert(ROWS == 4, "the row plan has one block per four rows");
 static_assert(CELL_BLOCK == NT, "cell plans have one thread per cell");
+constexpr int JSEG = IRS_J_SEGMENTS;        // row segments per column (segmented j smoothing)
+constexpr int JCOLS = WAVE / JSEG;          // columns per wavefront
+static_assert(ROWS * JCOLS == IRS_J_COLS_PER_BLOCK, "the segmented column plan has 32 columns per block");
 
-// Smoothing along j: one thread per (component, column i), forward
-// elimination into w, back substitution in place. r = R (+ P: the FAS forcing
-// held in xa, coarse levels).
+// Smoothing along j, for columns of up to 8 L cells: each column is split into
+// 8 segments of L rows held by 8 lanes (lane = segment * 8 + column, so that
+// the 8 lanes of one segment read 8 neighbouring columns). Both passes of the
+// elimination, y_j = m_j (r_j + eps y_{j-1}) and x_j = y_j - c_j x_{j+1}, are
+// first-order linear recursions: each lane runs its segment with a zero carry
+// (the affine map carry-in -> carry-out of the segment), a scan of those maps
+// over the 8 segments gives every segment its carry, and the lanes rerun their
+// segments with it. The column stays in registers between the passes: R (+ P)
+// is read once and the smoothed residual written once. r = R (+ P: the FAS
+// forcing held in xa, coarse levels).
+template <int L>
+__global__ void __launch_bounds__(NT) k_irs_j_seg(Plan plan, double eps, int forcing) {
+  const int e = plan_entry(plan, blockIdx.x);
+  const CaseDev& cd = plan.cases[plan.slots[e]];
+  const int ni = cd.ni, nc = cd.nc, n = cd.n;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int seg = lane / JCOLS, col = lane - seg * JCOLS;
+  const int t = (blockIdx.x - plan.blk_prefix[e]) * (ROWS * JCOLS) + wave * JCOLS + col;
+  const bool live = t < 4 * ni;
+  const int k = live ? t / ni : 0, i = live ? t - k * ni : 0;
+  const size_t base = size_t(k) * n + i;
+  const int j0 = seg * L;
+  const int len = live ? max(0, min(L, nc - j0)) : 0;
+  const gptr<const double> R = gp(cd.R), P = gp(ctp<double>(cd.xa));
+  const gptr<double> w = gp(tp<double>(cd.w));
+  const gptr<const double> m = gp(cd.irs), c = gp(cd.irs + nc);
+  double r[L];
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    r[q] = 0.0;
+    if (q < len) {
+      r[q] = R[base + size_t(j0 + q) * ni];
+      if (forcing) r[q] += P[base + size_t(j0 + q) * ni];
+    }
+  }
+  // forward: y_j = m_j r_j + (eps m_j) y_{j-1}
+  double A = 1.0, B = 0.0;
+#pragma unroll
+  for (int q = 0; q < L; ++q)
+    if (q < len) {
+      const double mj = m[j0 + q];
+      B = mj * (r[q] + eps * B);
+      A *= eps * mj;
+    }
+#pragma unroll
+  for (int d = 1; d < JSEG; d <<= 1) {
+    const int src = (lane - d * JCOLS) & (WAVE - 1);
+    const double Ap = __shfl(A, src, WAVE), Bp = __shfl(B, src, WAVE);
+    if (seg >= d) {
+      B = A * Bp + B;
+      A = A * Ap;
+    }
+  }
+  double y = __shfl(B, (lane - JCOLS) & (WAVE - 1), WAVE);
+  if (seg == 0) y = 0.0;
+#pragma unroll
+  for (int q = 0; q < L; ++q)
+    if (q < len) {
+      y = m[j0 + q] * (r[q] + eps * y);
+      r[q] = y;
+    }
+  // backward: x_j = y_j - c_j x_{j+1}
+  A = 1.0;
+  B = 0.0;
+#pragma unroll
+  for (int q = L - 1; q >= 0; --q)
+    if (q < len) {
+      const double cj = c[j0 + q];
+      B = r[q] - cj * B;
+      A *= -cj;
+    }
+#pragma unroll
+  for (int d = 1; d < JSEG; d <<= 1) {
+    const int src = (lane + d * JCOLS) & (WAVE - 1);
+    const double Ap = __shfl(A, src, WAVE), Bp = __shfl(B, src, WAVE);
+    if (seg + d < JSEG) {
+      B = A * Bp + B;
+      A = A * Ap;
+    }
+  }
+  double x = __shfl(B, (lane + JCOLS) & (WAVE - 1), WAVE);
+  if (seg == JSEG - 1) x = 0.0;
+#pragma unroll
+  for (int q = L - 1; q >= 0; --q)
+    if (q < len) {
+      x = r[q] - c[j0 + q] * x;
+      r[q] = x;
+    }
+#pragma unroll
+  for (int q = 0; q < L; ++q)
+    if (q < len) w[base + size_t(j0 + q) * ni] = r[q];
+}
+
+// Smoothing along j for longer columns: one thread per (component, column i),
+// forward elimination into w, back substitution in place.
 __global__ void __launch_bounds__(NT) k_irs_j(Plan plan, double eps, int forcing) {
   const int e = plan_entry(plan, blockIdx.x);
   const CaseDev& cd = plan.cases[plan.slots[e]];
@@ -328,8 +423,24 @@ int rk_rows_cm(int ni_max) {
 }
 size_t rk_rows_lds(int cm) { return size_t(ROWS) * (cm + 1) * WAVE * sizeof(double); }
 
+int irs_j_seg_len(int nc_max) {
+  int l = 1;
+  while (l * JSEG < nc_max) l <<= 1;
+  return l <= IRS_J_SEG_MAX ? l : 0;
+}
+
 void launch_irs_j(const Plan& pcol, int nblocks, double eps, int forcing, hipStream_t s) {
   hipLaunchKernelGGL(k_irs_j, dim3(nblocks), dim3(NT), 0, s, pcol, eps, forcing);
+}
+
+void launch_irs_j_seg(const Plan& pseg, int nblocks, int seg_len, double eps, int forcing, hipStream_t s) {
+  switch (seg_len) {
+#define A2D_SEG(LL)                                                                                   \
+  case LL: hipLaunchKernelGGL(k_irs_j_seg<LL>, dim3(nblocks), dim3(NT), 0, s, pseg, eps, forcing); break;
+    A2D_SEG(1) A2D_SEG(2) A2D_SEG(4) A2D_SEG(8) A2D_SEG(16) A2D_SEG(32) A2D_SEG(64)
+#undef A2D_SEG
+    default: break;
+  }
 }
 
 void launch_rk_rows(const Plan& prow, int nblocks, int cm, double eps, double a, int first, int want_res,

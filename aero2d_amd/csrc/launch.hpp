@@ -137,8 +137,15 @@ void launch_copy_vec(const Plan& pv, int nblocks, int src, int dst, int prec, hi
 constexpr int EXPLICIT_BLOCK = 256;   // threads per workgroup of the explicit kernels
 int rk_rows_cm(int ni_max);
 size_t rk_rows_lds(int cm);
-// Smoothing along j of R (+ the forcing in xa) into w.
+// Smoothing along j of R (+ the forcing in xa) into w. The segmented kernel
+// (columns of up to 8 * IRS_J_SEG_MAX cells; plan pseg, 32 columns per block)
+// keeps each column in registers; seg_len = irs_j_seg_len(largest nc), 0 when
+// the columns are too long, then launch_irs_j (plan pcol) instead, which keeps
+// the first pass's values in w.
+constexpr int IRS_J_SEGMENTS = 8, IRS_J_COLS_PER_BLOCK = 32, IRS_J_SEG_MAX = 64;
+int irs_j_seg_len(int nc_max);
 void launch_irs_j(const Plan& pcol, int nblocks, double eps, int forcing, hipStream_t s);
+void launch_irs_j_seg(const Plan& pseg, int nblocks, int seg_len, double eps, int forcing, hipStream_t s);
 // One Runge-Kutta stage, U = U0 - a x / diag: x the smoothed residual (from w,
 // smoothing along i; eps > 0) or R (+ P when forcing; eps = 0). first: U0 =
 // U, kept in dU; want_res: the residual partials of the stage.

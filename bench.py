@@ -18,8 +18,9 @@ Jacobian-vector product; the FP64 residual; the FP64 coupled line sweep); a
 CPU baseline (the FP64 oracle on the host cores, bounded sample).
 
 Other workloads: config1 (one case), config2 (64 cases of 256 x 128, FP64
-vs mixed), config4 (256 cases of uneven sizes), config5 (serving), kernels /
-kernels3 (kernel timings alone, for counter runs).
+vs mixed), config4 (256 cases of uneven sizes), config5 (serving), explicit
+(the reference's explicit scheme), kernels / kernels3 (kernel timings alone,
+for counter runs).
 """
 from __future__ import annotations
 
@@ -464,6 +465,119 @@ def run_config4(a):
             "tile_shapes": {f"{k[0]}x{k[1]}": v for k, v in shapes.items()}, "by_size": sizes}
 
 
+# explicit scheme, algorithmic bytes per cell and launch (FP64): smoothing
+# along j reads R (4 words) and writes the smoothed residual (4); a later
+# Runge-Kutta stage reads it (4), U0 (4) and vol/dt (1) and writes U (4)
+IRS_J_BYTES_PER_CELL = 64
+RK_ROWS_BYTES_PER_CELL = 104
+
+
+def explicit_pool(cases, mg, knobs):
+    """FP64 cases of the explicit scheme, each with mg grid levels (every
+    other node of the level above), linked finest to coarsest."""
+    from aero2d_amd import _abi
+    pool = _abi.Pool(0, knobs)
+    ids = []
+    for c in cases:
+        x, y, prev = c.grid.x, c.grid.y, None
+        for _ in range(mg):
+            cid = pool.add_case(x, y, c.mach, c.alpha, _abi.PREC_FP64)
+            if prev is None:
+                ids.append(cid)
+            else:
+                pool.link_coarse(prev, cid)
+            prev, x, y = cid, x[::2, ::2], y[::2, ::2]
+    return pool, np.array(ids, np.int32)
+
+
+def run_explicit(a):
+    """The explicit scheme (the reference's EulerSolver.solve: RK4, residual
+    smoothing, FAS V-cycle), FP64: cycles/s of the transonic batch (32 x 512
+    x 256) with one and two grid levels, HIP-event rooflines of its kernels, a
+    converged solve with the reference's defaults on eight subsonic cases of
+    the reference's golden grid (NACA 0012, 128 x 48), and the oracle's cycle
+    time on the host as the CPU baseline."""
+    import torch
+    from aero2d_amd import EulerSolver, ImplicitEuler, _abi
+    from aero2d_amd.workloads import config1, config3
+    from oracle import explicit as oexplicit
+    cases = config3(a.cases or 32)
+    cells = sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases)
+    k = _abi.default_knobs()
+    out = {"metric": "explicit case-cycles/s (RK4 + residual smoothing [+ FAS])", "unit": "case-cycles/s",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": {"workload": f"{len(cases)} transonic cases on 512x256, explicit "
+                                                       "cycles, CFL 2.5, irs_eps 0.6"}}
+    cyc = max(2, a.steps)
+    for mg in (1, 2):
+        pool, ids = explicit_pool(cases, mg, k)
+        try:
+            pool.explicit_cycles(ids, max(1, a.warmup), True)
+            t = time.perf_counter()
+            r = pool.explicit_cycles(ids, cyc, False)
+            dt = time.perf_counter() - t
+            entry = {"ms_per_cycle": 1e3 * dt / cyc, "case_cycles_per_s": len(ids) * cyc / dt,
+                     "finite": bool(np.all(np.isfinite(r)))}
+            if mg == 1:
+                kt = {}
+                for name, w in (("residual", 0), ("irs_j", 11), ("rk_rows", 12)):
+                    ms = np.zeros(1)
+                    _abi.call("a2d_time_kernel", pool.handle, _abi.iptr(ids), len(ids), w, 5, _abi.dptr(ms))
+                    kt[name] = float(ms[0])
+                entry["kernel_ms"] = kt
+                for name, bpc in (("irs_j", IRS_J_BYTES_PER_CELL), ("rk_rows", RK_ROWS_BYTES_PER_CELL),
+                                  ("residual", RES_BYTES_PER_CELL)):
+                    ach = bpc * cells / (kt[name] * 1e-3) / 1e9
+                    entry["roofline_" + name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                                                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                                                 "bytes_per_launch": bpc * cells, "ms_per_launch": kt[name]}
+                entry["stage_kernels_ms"] = kt["residual"] + kt["irs_j"] + kt["rk_rows"]
+            out[f"mg{mg}"] = entry
+            print(f"# explicit mg{mg}: {json.dumps(entry)}", file=sys.stderr, flush=True)
+        finally:
+            pool.close()
+    out["value"] = out["mg1"]["case_cycles_per_s"]
+    out["ms_per_step"] = out["mg1"]["ms_per_cycle"]
+    out["steps"], out["warmup"] = cyc, max(1, a.warmup)
+    # converged solves: two grid levels to res/res0 < 1e-9 (chunks of 200
+    # cycles), and the reference's defaults (one level, 4000 cycles, res/res0
+    # < 1e-7 or Cl, Cd settled), against the implicit FP64 steady state
+    g = config1()[0].grid
+    machs = np.array([0.3, 0.4, 0.5, 0.5, 0.55, 0.6, 0.6, 0.65])
+    alphas = np.array([0.0, 1.0, 2.0, 3.0, 1.5, 0.0, 2.0, 1.0])
+    ri = ImplicitEuler(EulerSolver(g, dtype=np.float64)).solve(machs, alphas, n_iter=200, tol=1e-12, verbose=False)
+    for key, mg, kw in (("solve_mg2", 2, dict(n_iter=20000, tol=1e-9, tol_cl=0.0, tol_cd=0.0)),
+                        ("solve_reference_defaults", 1, {})):
+        s = EulerSolver(g, dtype=np.float64, mg_levels=mg)
+        t = time.perf_counter()
+        r = s.solve(machs, alphas, verbose=False, **kw)
+        dt = time.perf_counter() - t
+        drop = r.res_hist[-1] / r.res_hist[0]
+        out[key] = {"workload": f"8 subsonic NACA 0012 cases (M 0.3-0.65) on {g.x.shape[0]}x{g.x.shape[1] - 1}, "
+                                f"{mg} grid level(s), {kw or 'reference defaults'}",
+                    "wall_s": dt, "cycles": s.iterations, "worst_log10_res_drop": float(np.max(np.log10(drop))),
+                    "max_dcl_vs_implicit_fp64": float(np.max(np.abs(r.cl - ri.cl))),
+                    "max_dcd_vs_implicit_fp64": float(np.max(np.abs(r.cd - ri.cd)))}
+        print(f"# explicit {key}: {json.dumps(out[key])}", file=sys.stderr, flush=True)
+    if not a.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0))
+        torch.set_num_threads(cores)
+        c = cases[0]
+        lev = oexplicit.make_levels(c.grid.x, c.grid.y)
+        from oracle import flow
+        winf = flow.freestream(torch.tensor(c.mach, dtype=torch.float64), torch.tensor(c.alpha, dtype=torch.float64))
+        U = flow.init_state(winf, lev.ni, lev.nc)
+        t = time.perf_counter()
+        n = 2
+        for _ in range(n):
+            U, _ = oexplicit.cycle(U, winf, torch.zeros((), dtype=torch.float64), lev, 2.5, 0.6, 0.02, True)
+        dt = (time.perf_counter() - t) / n
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "case-cycles/s", "cores": cores, "kind": "port",
+                               "sample": f"{n} explicit cycles (one grid level) of one 512x256 case by the "
+                                         f"oracle, {dt:.2f} s/cycle"}
+    return out
+
+
 def run_config5(a):
     """Serving: open-loop Poisson streams at several rates, mixed and FP64.
     Every released request counts in the latency percentiles (an unfinished
@@ -505,7 +619,8 @@ def run_config5(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="config3",
-                    choices=["config1", "config2", "config3", "config4", "config5", "kernels", "kernels3"])
+                    choices=["config1", "config2", "config3", "config4", "config5", "kernels", "kernels3",
+                             "explicit"])
     ap.add_argument("--rates", default="1,2,4,10,25,100,400", help="config5: mixed-form request rates")
     ap.add_argument("--rates64", default="0.25,0.5,1,10,100", help="config5: FP64 request rates")
     ap.add_argument("--horizon", type=float, default=60.0)
@@ -533,7 +648,7 @@ def main():
 
     if a.workload != "config2":
         fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,
-              "kernels": run_kernels, "kernels3": run_kernels3}[a.workload]
+              "kernels": run_kernels, "kernels3": run_kernels3, "explicit": run_explicit}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 

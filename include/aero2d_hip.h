@@ -197,6 +197,9 @@ int a2d_explicit_cycles(int64_t pool, const int* case_ids, int n, int ncycles, i
  *   6 Gram-Schmidt update (k = 25)   7 plain vector copy (4 words/cell each way)
  *   8 the per-cell pass of a coupled line sweep   9 the line recurrences alone
  *   10 one whole preconditioner application (all sweeps, in the pool's order)
+ *   11 explicit scheme: residual smoothing along j (FP64 cases)
+ *   12 explicit scheme: a later Runge-Kutta stage, smoothing along i and the
+ *      update (FP64 cases; the state is restored afterwards)
  * The cases must have taken a Newton step (their blocks and Krylov vectors
  * hold data); the state is not changed. The ids must be distinct. */
 int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int reps, double* ms);

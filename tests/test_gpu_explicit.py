@@ -33,13 +33,26 @@ def _oracle_cycle(g, U, mach, alpha, eps, order, mg, ncycles=1, fresh=True):
     return Ut.numpy(), float(res), float(cl), float(cd), float(circ)
 
 
-GRIDS = [(17, 9), (101, 9), (257, 17), (513, 9)]   # ni = 32, 200, 512, 1024 (every lane-chunk width)
+def annulus(ni, nc, R=40.0):
+    """O-grid between a circle of radius 0.5 and one of radius R (tall cases
+    the airfoil grid generator does not make)."""
+    from aero2d_amd.grid import OGrid
+    th = 2 * np.pi * np.arange(ni) / ni
+    r = np.geomspace(0.5, R, nc + 1)
+    return OGrid(r[None, :] * np.cos(th)[:, None] + 0.5, r[None, :] * np.sin(th)[:, None])
+
+
+# ni = 32, 200, 512, 1024 (every lane-chunk width of the i smoothing); nc = 8
+# .. 300 (segment lengths 1 .. 64 of the j smoothing, partial segments), and
+# nc = 520 (longer columns: the unsegmented j kernel)
+GRIDS = [(17, 9), (101, 9), (257, 17), (513, 9), ("annulus", 64, 256), ("annulus", 32, 300),
+         ("annulus", 16, 520)]
 
 
 @pytest.mark.parametrize("eps", [0.6, 0.0])
 @pytest.mark.parametrize("grid", GRIDS)
 def test_one_iteration_matches_oracle(grid, eps):
-    g = small_grid(*grid)
+    g = annulus(*grid[1:]) if grid[0] == "annulus" else small_grid(*grid)
     s = EulerSolver(g, dtype=np.float64, irs_eps=eps)
     mach, alpha = 0.6, 1.5
     U = perturbed_state(s.ni, s.nc, mach, alpha, 11)
