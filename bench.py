@@ -545,9 +545,12 @@ def run_explicit(a):
     g = config1()[0].grid
     machs = np.array([0.3, 0.4, 0.5, 0.5, 0.55, 0.6, 0.6, 0.65])
     alphas = np.array([0.0, 1.0, 2.0, 3.0, 1.5, 0.0, 2.0, 1.0])
-    ri = ImplicitEuler(EulerSolver(g, dtype=np.float64)).solve(machs, alphas, n_iter=200, tol=1e-12, verbose=False)
-    for key, mg, kw in (("solve_mg2", 2, dict(n_iter=20000, tol=1e-9, tol_cl=0.0, tol_cd=0.0)),
-                        ("solve_reference_defaults", 1, {})):
+    solves = [] if a.no_solves else [("solve_mg2", 2, dict(n_iter=20000, tol=1e-9, tol_cl=0.0, tol_cd=0.0)),
+                                      ("solve_reference_defaults", 1, {})]
+    if solves:
+        ri = ImplicitEuler(EulerSolver(g, dtype=np.float64)).solve(machs, alphas, n_iter=200, tol=1e-12,
+                                                                   verbose=False)
+    for key, mg, kw in solves:
         s = EulerSolver(g, dtype=np.float64, mg_levels=mg)
         t = time.perf_counter()
         r = s.solve(machs, alphas, verbose=False, **kw)
@@ -576,6 +579,26 @@ def run_explicit(a):
                                "sample": f"{n} explicit cycles (one grid level) of one 512x256 case by the "
                                          f"oracle, {dt:.2f} s/cycle"}
     return out
+
+
+def run_explicit_kernels(a):
+    """The explicit scheme's kernels alone on the transonic batch (32 x 512 x
+    256 cells, FP64, one grid level), after one cycle, with the plain copy for
+    counter calibration (used under rocprofv3 for the per-dispatch counters)."""
+    from aero2d_amd import _abi
+    from aero2d_amd.workloads import config3
+    cases = config3(a.cases or 32)
+    pool, ids = explicit_pool(cases, 1, _abi.default_knobs())
+    try:
+        pool.explicit_cycles(ids, 1, True)
+        kt = {}
+        for name, w in (("residual", 0), ("irs_j", 11), ("rk_rows", 12), ("copy", 7)):
+            ms = np.zeros(1)
+            _abi.call("a2d_time_kernel", pool.handle, _abi.iptr(ids), len(ids), w, 5, _abi.dptr(ms))
+            kt[name] = float(ms[0])
+    finally:
+        pool.close()
+    return {"kernel_ms": kt, "cells": sum(c.grid.x.shape[0] * (c.grid.x.shape[1] - 1) for c in cases)}
 
 
 def run_config5(a):
@@ -620,7 +643,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="config3",
                     choices=["config1", "config2", "config3", "config4", "config5", "kernels", "kernels3",
-                             "explicit"])
+                             "explicit", "explicit_kernels"])
     ap.add_argument("--rates", default="1,2,4,10,25,100,400", help="config5: mixed-form request rates")
     ap.add_argument("--rates64", default="0.25,0.5,1,10,100", help="config5: FP64 request rates")
     ap.add_argument("--horizon", type=float, default=60.0)
@@ -638,6 +661,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="config3: the mixed solve alone")
+    ap.add_argument("--no-solves", action="store_true", help="explicit: cycle timing only")
     ap.add_argument("--no-restart-checks", action="store_true",
                     help="config3: skip the cross-restart checks of cases outside the accuracy bar")
     a = ap.parse_args()
@@ -648,7 +672,8 @@ def main():
 
     if a.workload != "config2":
         fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,
-              "kernels": run_kernels, "kernels3": run_kernels3, "explicit": run_explicit}[a.workload]
+              "kernels": run_kernels, "kernels3": run_kernels3, "explicit": run_explicit,
+              "explicit_kernels": run_explicit_kernels}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 
