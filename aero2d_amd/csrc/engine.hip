@@ -1371,7 +1371,7 @@ int a2d_explicit_cycles(int64_t pool, const int* case_ids, int n, int ncycles, i
       }
     check_batch(p, all);   // no case on two levels or twice on one
     const double eps = p.knobs.irs_eps;
-    if (eps > 0 && rk_rows_lds(rk_rows_cm(ni_max)) > size_t(g_lds_block))
+    if (eps > 0 && (rk_rows_cm(ni_max) == 0 || rk_rows_lds(rk_rows_cm(ni_max)) > size_t(g_lds_block)))
       fail(-6, "rows too long for the smoothing kernel (" + std::to_string(ni_max) + " cells around)");
     if (eps > 0)
       for (int s : all) irs_table(p, p.cases[s], eps);
@@ -1412,7 +1412,7 @@ int a2d_time_kernel(int64_t pool, const int* case_ids, int n, int which, int rep
     const double eps = p.knobs.irs_eps > 0 ? p.knobs.irs_eps : 0.6;
     if (which == 11 || which == 12)
       for (int s : slots) irs_table(p, p.cases[s], eps);
-    if (which == 12 && rk_rows_lds(rk_rows_cm(b.ni_max)) > size_t(g_lds_block))
+    if (which == 12 && (rk_rows_cm(b.ni_max) == 0 || rk_rows_lds(rk_rows_cm(b.ni_max)) > size_t(g_lds_block)))
       fail(-6, "rows too long for the smoothing kernel");
     // a Runge-Kutta stage overwrites U: keep it in xb (FP64 scratch) meanwhile
     if (which == 12)

@@ -419,7 +419,7 @@ void run_rows(const Plan& p, int nblocks, double lam, double scale, double a, in
 int rk_rows_cm(int ni_max) {
   int cm = 1;
   while (cm * WAVE < ni_max) cm <<= 1;
-  return cm;
+  return cm <= RK_ROWS_CM_MAX ? cm : 0;
 }
 size_t rk_rows_lds(int cm) { return size_t(ROWS) * (cm + 1) * WAVE * sizeof(double); }
 
@@ -454,7 +454,8 @@ void launch_rk_rows(const Plan& prow, int nblocks, int cm, double eps, double a,
       case 4: run_rows<4>(prow, nblocks, lam, scale, a, first, want_res, s); break;
       case 8: run_rows<8>(prow, nblocks, lam, scale, a, first, want_res, s); break;
       case 16: run_rows<16>(prow, nblocks, lam, scale, a, first, want_res, s); break;
-      default: run_rows<32>(prow, nblocks, lam, scale, a, first, want_res, s); break;
+      case 32: run_rows<32>(prow, nblocks, lam, scale, a, first, want_res, s); break;
+      default: break;   // the engine refuses longer rows before launching
     }
   } else {
     hipLaunchKernelGGL((k_rk_rows<1, false>), dim3(nblocks), dim3(NT), 0, s, prow, 0.0, 0.0, a, first, want_res,

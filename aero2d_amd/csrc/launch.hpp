@@ -135,6 +135,8 @@ void launch_copy_vec(const Plan& pv, int nblocks, int src, int dst, int prec, hi
 // Elements per lane of the row kernel for rows of up to ni_max cells, and the
 // LDS bytes its workgroup then takes.
 constexpr int EXPLICIT_BLOCK = 256;   // threads per workgroup of the explicit kernels
+// 0 when rows are longer than 64 RK_ROWS_CM_MAX cells (not supported).
+constexpr int RK_ROWS_CM_MAX = 32;
 int rk_rows_cm(int ni_max);
 size_t rk_rows_lds(int cm);
 // Smoothing along j of R (+ the forcing in xa) into w. The segmented kernel

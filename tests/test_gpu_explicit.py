@@ -178,3 +178,11 @@ def test_explicit_refusals():
         pool.close()
     with pytest.raises(ValueError):
         EulerSolver(small_grid(17, 8), dtype=np.float64, mg_levels=2)   # 32 x 7 cells
+    long_rows = EulerSolver(annulus(2050, 2), dtype=np.float64)          # rows beyond 2048 cells
+    pool, ids = long_rows.explicit_batch([0.5], [1.0])
+    try:
+        with pytest.raises(_abi.HipError) as e:
+            pool.explicit_cycles(ids, 1, True)
+        assert e.value.code == -6
+    finally:
+        pool.close()
