@@ -79,12 +79,12 @@ def time_kernels(cases, precision, reps=20, knobs=None):
     return res
 
 
-# HBM bytes per launch of the default batch (64 cases, 256 x 128, FP64) from
+# HBM bytes per launch of the 64-case batch (64 cases, 256 x 128, FP64) from
 # two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE, then
-# WRITE_SIZE), fetch scaled by 1.9944 (the copy kernel's known bytes over what
+# WRITE_SIZE), fetch scaled by 1.9936 (the copy kernel's known bytes over what
 # the counter reads for it) and write by 1.0 (profiles/traffic_config2_fp64_pmc.txt).
 TRAFFIC_CELLS = 64 * 256 * 128
-MEASURED_TRAFFIC = {"line_sweep": 1492.1e6, "residual": 404.9e6}
+MEASURED_TRAFFIC = {"line_sweep": 1483.1e6, "residual": 253.1e6}
 # HBM bytes per launch at the transonic batch (32 x 512 x 256 cells): FETCH_SIZE
 # x 1.9967 + WRITE_SIZE of `bench.py --workload kernels3`
 # (profiles/kernels_config3_counters.txt)
