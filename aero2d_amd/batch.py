@@ -328,7 +328,8 @@ def cross_restart(case: Case, U: np.ndarray, precision: str = "fp64", n_iter: in
     takes n_iter Newton steps; a steady U stays put. The restart begins at the
     path's largest CFL (where the solve that produced U ended), not at cfl0.
     Returns Cl/Cd/Cm of U and of the restarted state, their differences, and
-    the restarted state's residual components (residual_components)."""
+    the residual components (residual_components) of U and of the restarted
+    state."""
     if "cfl0" not in path:
         cmax, clate, _ = cfl_policy(precision, path.get("cfl_max"), path.get("cfl_late"), path.get("late_below"))
         path["cfl0"] = max(cmax, clate or 0.0)
@@ -339,4 +340,5 @@ def cross_restart(case: Case, U: np.ndarray, precision: str = "fp64", n_iter: in
             "cm": [float(r0.cm[0]), float(r1.cm[0])],
             "dcl": abs(float(r1.cl[0] - r0.cl[0])), "dcd": abs(float(r1.cd[0] - r0.cd[0])),
             "dcm": abs(float(r1.cm[0] - r0.cm[0])), "residual": comp.tolist(),
+            "residual_start": residual_components(case, U).tolist(),
             "newton_steps": int(r1.iterations[0])}

@@ -248,9 +248,13 @@ __host__ __device__ __forceinline__ size_t blk_off(int slot, int m, size_t cell,
   return (size_t(slot) * pstride + cell) * 16 + m;
 }
 
-// Column of line i in the preconditioner's block storage. With the red-black
-// line order the blocks are kept colour-split (even lines, then odd lines) so
-// that a half sweep reads them contiguously; otherwise column = i.
+// Column of line i in the preconditioner's block storage and in the Krylov
+// vectors (x0, w, basis, P, vc, xa, xb). With the red-black line order both
+// are kept colour-split (even lines, then odd lines, in every row) so that a
+// half sweep reads its blocks, its own lines' vector values and the other
+// colour's neighbours as contiguous runs and writes whole cache lines;
+// otherwise column = i. The state, residual, update and vol/dtau keep the
+// natural order.
 __host__ __device__ __forceinline__ int bcol(int i, int ni, int split) {
   return split ? (i & 1) * (ni >> 1) + (i >> 1) : i;
 }

@@ -594,7 +594,7 @@ void enq_newton(Pool& p, BatchPlan& b) {
   }
   if (K > 0) launch_normalize(b.pc, K - 1, kn, prec, s);
   launch_lstsq(b.pc, s);
-  launch_dupdate(b.pvec, b.nvec, prec, bh, s);
+  launch_dupdate(b.pvec, b.nvec, prec, bh, kn.precond_rb, s);
   launch_limiter(b.pc, kn, s);
   if (kn.positivity) launch_positivity(b.pvec, b.nvec, b.pc, s);
   launch_apply(b.pvec, b.nvec, s);
@@ -668,28 +668,30 @@ void enq_cycle(Pool& p, std::vector<BatchPlan*>& L, int l, const KnobsDev& kn) {
 }
 
 // ---- host <-> device field transfer -------------------------------------------
-// Reference layout (4, ni, nc) j-fastest <-> device [4][nc][ni] i-fastest.
-void upload_field(Pool& p, const CaseHost& c, const double* src, void* dst, int prec, int nv) {
+// Reference layout (4, ni, nc) j-fastest <-> device [4][nc][ni] i-fastest;
+// split: a Krylov-space vector under the red-black line order (column bcol).
+void upload_field(Pool& p, const CaseHost& c, const double* src, void* dst, int prec, int nv, int split = 0) {
   const int ni = c.ni, nc = c.nc;
   const size_t n = size_t(ni) * nc;
   if (prec == 0) {
     p.stage_d.resize(nv * n);
     for (int v = 0; v < nv; ++v)
       for (int i = 0; i < ni; ++i)
-        for (int j = 0; j < nc; ++j) p.stage_d[v * n + size_t(j) * ni + i] = src[v * n + size_t(i) * nc + j];
+        for (int j = 0; j < nc; ++j)
+          p.stage_d[v * n + size_t(j) * ni + bcol(i, ni, split)] = src[v * n + size_t(i) * nc + j];
     HC(hipMemcpyAsync(dst, p.stage_d.data(), nv * n * 8, hipMemcpyHostToDevice, p.stream));
   } else {
     p.stage_f.resize(nv * n);
     for (int v = 0; v < nv; ++v)
       for (int i = 0; i < ni; ++i)
         for (int j = 0; j < nc; ++j)
-          p.stage_f[v * n + size_t(j) * ni + i] = float(src[v * n + size_t(i) * nc + j]);
+          p.stage_f[v * n + size_t(j) * ni + bcol(i, ni, split)] = float(src[v * n + size_t(i) * nc + j]);
     HC(hipMemcpyAsync(dst, p.stage_f.data(), nv * n * 4, hipMemcpyHostToDevice, p.stream));
   }
   HC(hipStreamSynchronize(p.stream));
 }
 
-void download_field(Pool& p, const CaseHost& c, const void* src, double* dst, int prec, int nv) {
+void download_field(Pool& p, const CaseHost& c, const void* src, double* dst, int prec, int nv, int split = 0) {
   const int ni = c.ni, nc = c.nc;
   const size_t n = size_t(ni) * nc;
   if (prec == 0) {
@@ -698,14 +700,16 @@ void download_field(Pool& p, const CaseHost& c, const void* src, double* dst, in
     HC(hipStreamSynchronize(p.stream));
     for (int v = 0; v < nv; ++v)
       for (int i = 0; i < ni; ++i)
-        for (int j = 0; j < nc; ++j) dst[v * n + size_t(i) * nc + j] = p.stage_d[v * n + size_t(j) * ni + i];
+        for (int j = 0; j < nc; ++j)
+          dst[v * n + size_t(i) * nc + j] = p.stage_d[v * n + size_t(j) * ni + bcol(i, ni, split)];
   } else {
     p.stage_f.resize(nv * n);
     HC(hipMemcpyAsync(p.stage_f.data(), src, nv * n * 4, hipMemcpyDeviceToHost, p.stream));
     HC(hipStreamSynchronize(p.stream));
     for (int v = 0; v < nv; ++v)
       for (int i = 0; i < ni; ++i)
-        for (int j = 0; j < nc; ++j) dst[v * n + size_t(i) * nc + j] = p.stage_f[v * n + size_t(j) * ni + i];
+        for (int j = 0; j < nc; ++j)
+          dst[v * n + size_t(i) * nc + j] = p.stage_f[v * n + size_t(j) * ni + bcol(i, ni, split)];
   }
 }
 
@@ -1217,10 +1221,10 @@ int a2d_jvp(int64_t pool, int case_id, const double* v, double cfl, int with_dia
     const KnobsDev kn = to_dev(p.knobs);
     set_cfl_one(p, b, cfl);
     enq_prologue(p, b, kn);
-    upload_field(p, c, v, c.dev.xa, c.prec, 4);
+    upload_field(p, c, v, c.dev.xa, c.prec, 4, kn.precond_rb);
     launch_dcirc(b.pc, VEC_XA, 0, -1, kn, c.prec, p.stream);
     enq_jvp(p, b, kn, VEC_XA, VEC_W, 0, -1, with_diag, 0);
-    download_field(p, c, c.dev.w, out, c.prec, 4);
+    download_field(p, c, c.dev.w, out, c.prec, 4, kn.precond_rb);
   });
 }
 
@@ -1274,9 +1278,10 @@ int a2d_precond_apply(int64_t pool, int case_id, const double* r, double* x) {
     Pool& p = get_pool(pool);
     CaseHost& c = get_case(p, case_id);
     BatchPlan& b = get_plan(p, {case_id});
-    upload_field(p, c, r, c.dev.w, c.prec, 4);
+    const int split = p.knobs.precond_order == 1;
+    upload_field(p, c, r, c.dev.w, c.prec, 4, split);
     enq_precond(p, b, VEC_W, VEC_X0, 0, 0);
-    download_field(p, c, c.dev.x0, x, c.prec, 4);
+    download_field(p, c, c.dev.x0, x, c.prec, 4, split);
   });
 }
 

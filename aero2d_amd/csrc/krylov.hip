@@ -384,9 +384,12 @@ __global__ void __launch_bounds__(WAVE) k_lstsq(Plan pc) {
 
 // dU = x0 + sum_j y_j s_j V_j (stored FP64), with the step-limiter partials:
 // max over cells of |rho_new - rho|/rho and |p_new - p|/p for U + dU, and the
-// count of non-finite entries of dU (aero2d/implicit.py:186-189).
+// count of non-finite entries of dU (aero2d/implicit.py:186-189). Cells run
+// in the natural order of U and dU; split: x0 and the basis are colour-split
+// (red-black line order), read at column bcol (two contiguous runs per
+// wavefront).
 template <class T, bool BH>
-__global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
+__global__ void __launch_bounds__(NT) k_dupdate(Plan plan, int split) {
   using VT = BasisT<T, BH>;
   __shared__ double coef[KMAX];
   __shared__ double smax[NT / WAVE], scnt[NT / WAVE];
@@ -394,7 +397,7 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   const CaseDev& cd = plan.cases[plan.slots[e]];
   const CaseScal* sc = cd.sc;
   if (sc->parked) return;
-  const int n = cd.n, kd = sc->kdone;
+  const int n = cd.n, kd = sc->kdone, ni = cd.ni;
   const int lb = blockIdx.x - plan.blk_prefix[e];
   for (int j = threadIdx.x; j < kd; j += NT) coef[j] = sc->y[j] * sc->s[j];
   __syncthreads();
@@ -407,16 +410,21 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
   for (int m = 0; m < PER; ++m) {
     const int c = lb * VEC_CHUNK + threadIdx.x + m * NT;
     if (c >= n) continue;
+    int cv = c;
+    if (split) {
+      const int row = c / ni, i = c - row * ni;
+      cv = row * ni + bcol(i, ni, 1);
+    }
     double du[4], u[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) du[v] = double(x0[(size_t)v * n + c]);
+    for (int v = 0; v < 4; ++v) du[v] = double(x0[(size_t)v * n + cv]);
     int j = 0;
     for (; j + 4 <= kd; j += 4) {
       VT vv[4][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) vv[q][v] = Vb[(size_t)(j + q) * vs + (size_t)v * n + c];
+        for (int v = 0; v < 4; ++v) vv[q][v] = Vb[(size_t)(j + q) * vs + (size_t)v * n + cv];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -425,7 +433,7 @@ __global__ void __launch_bounds__(NT) k_dupdate(Plan plan) {
     for (; j < kd; ++j) {
       const double cj = coef[j];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) du[v] += cj * double(T(Vb[(size_t)j * vs + (size_t)v * n + c]));
+      for (int v = 0; v < 4; ++v) du[v] += cj * double(T(Vb[(size_t)j * vs + (size_t)v * n + cv]));
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -621,10 +629,10 @@ void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec
 void launch_lstsq(const Plan& pc, hipStream_t s) {
   hipLaunchKernelGGL(k_lstsq, dim3(pc.nb), dim3(WAVE), 0, s, pc);
 }
-void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, hipStream_t s) {
-  if (prec == 0) hipLaunchKernelGGL((k_dupdate<double, false>), dim3(nblocks), dim3(NT), 0, s, pv);
-  else if (bh) hipLaunchKernelGGL((k_dupdate<float, true>), dim3(nblocks), dim3(NT), 0, s, pv);
-  else hipLaunchKernelGGL((k_dupdate<float, false>), dim3(nblocks), dim3(NT), 0, s, pv);
+void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, int split, hipStream_t s) {
+  if (prec == 0) hipLaunchKernelGGL((k_dupdate<double, false>), dim3(nblocks), dim3(NT), 0, s, pv, split);
+  else if (bh) hipLaunchKernelGGL((k_dupdate<float, true>), dim3(nblocks), dim3(NT), 0, s, pv, split);
+  else hipLaunchKernelGGL((k_dupdate<float, false>), dim3(nblocks), dim3(NT), 0, s, pv, split);
 }
 void launch_store_v0(const Plan& pv, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_store_v0, dim3(nblocks), dim3(NT), 0, s, pv);

@@ -115,7 +115,8 @@ void launch_update(const Plan& pv, int nblocks, int k, int wvec, int gate, int r
 void launch_normalize(const Plan& pc, int k, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_krylov_begin(const Plan& pc, int stage, const KnobsDev& kn, int prec, hipStream_t s);
 void launch_lstsq(const Plan& pc, hipStream_t s);
-void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, hipStream_t s);
+// split: x0 and the basis are colour-split (red-black line order).
+void launch_dupdate(const Plan& pv, int nblocks, int prec, int bh, int split, hipStream_t s);
 // FP16 basis: V_raw_0 = vsig * P (and vc) after the V_0 sweep.
 void launch_store_v0(const Plan& pv, int nblocks, hipStream_t s);
 void launch_limiter(const Plan& pc, const KnobsDev& kn, hipStream_t s);

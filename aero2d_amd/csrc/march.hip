@@ -82,12 +82,14 @@ struct Raw {
   T d[TAN ? 4 : 1];
 };
 
+// idx: the cell in the state's natural order; vidx: the same cell in the
+// tangent vector's order (colour-split under the red-black line order).
 template <class T, bool TAN>
-__device__ __forceinline__ void fetch(Raw<T, TAN>& a, gptr<const T> U, gptr<const T> v, int n, int idx) {
+__device__ __forceinline__ void fetch(Raw<T, TAN>& a, gptr<const T> U, gptr<const T> v, int n, int idx, int vidx) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     a.q[k] = U[k * n + idx];
-    if constexpr (TAN) a.d[k] = v[k * n + idx];
+    if constexpr (TAN) a.d[k] = v[k * n + vidx];
   }
 }
 
@@ -148,6 +150,7 @@ __global__ void __launch_bounds__(MNT, MINW) k_march(Plan plan, KnobsDev kn, int
   const int jend = min(j0 + MH, nc);               // rows j0 .. jend-1 are output
   int col = (strip * MOWN - 2 + lane) % ni;
   if (col < 0) col += ni;
+  const int vcol = bcol(col, ni, kn.precond_rb);   // the column in the Krylov vectors
   const bool own = lane >= 2 && lane < MW - 2 && strip * MOWN + lane - 2 < ni;
 
   gptr<const T> U, vol, Si, Sj, xf, yf;
@@ -209,11 +212,12 @@ __global__ void __launch_bounds__(MNT, MINW) k_march(Plan plan, KnobsDev kn, int
 
   const int rfirst = j0 - 2, rlast = jend + 1;
   Raw<T, TAN> nxt;
-  if (rfirst < nc) fetch<T, TAN>(nxt, U, vt, n, src_row(rfirst) * ni + col);
+  if (rfirst < nc) fetch<T, TAN>(nxt, U, vt, n, src_row(rfirst) * ni + col, src_row(rfirst) * ni + vcol);
   for (int r = rfirst; r <= rlast; ++r) {
     const Raw<T, TAN> cur = nxt;
     // prefetch the next row's cells (a row below the far field reuses registers)
-    if (r + 1 <= rlast && r + 1 < nc) fetch<T, TAN>(nxt, U, vt, n, src_row(r + 1) * ni + col);
+    if (r + 1 <= rlast && r + 1 < nc)
+      fetch<T, TAN>(nxt, U, vt, n, src_row(r + 1) * ni + col, src_row(r + 1) * ni + vcol);
     A = B;
     B = C;
     C = make(r, cur, B);
@@ -276,15 +280,17 @@ __global__ void __launch_bounds__(MNT, MINW) k_march(Plan plan, KnobsDev kn, int
       } else {
         if (own) {
           const gptr<T> out = vec_gptr<T>(cd, vout);
+          const int vidx = c * ni + vcol;
           const T dg = use_diag ? gp(ctp<T>(cd.diag))[idx] : T(0);
           if (mode == 0) {
             const T s = (kidx >= 0) ? T(sc->s[kidx]) : T(1);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) out[k * n + idx] = s * (dg * vt[k * n + idx] + res[k]);
+            for (int k = 0; k < 4; ++k) out[k * n + vidx] = s * (dg * vt[k * n + vidx] + res[k]);
           } else {
             const gptr<const double> R = gp(cd.R);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) out[k * n + idx] = T(-R[k * n + idx]) - (dg * vt[k * n + idx] + res[k]);
+            for (int k = 0; k < 4; ++k)
+              out[k * n + vidx] = T(-R[k * n + idx]) - (dg * vt[k * n + vidx] + res[k]);
           }
         }
       }

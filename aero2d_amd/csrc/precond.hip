@@ -599,7 +599,12 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   // step's cell rowc 16 + cblk[slot] (T storage) or + chb[slot] (FP16 copy).
   const unsigned psu = (unsigned)B.ps, ru = (unsigned)r;
   const unsigned cgd = colu * 4u + ru;
-  const unsigned cxm = rn + (unsigned)im, cxp = rn + (unsigned)ip, csrc = rn + (unsigned)ic;
+  // vectors are colour-split in the red-black order (column bcol), so a half
+  // sweep's own lines sit at its block column and the neighbours in the other
+  // colour's contiguous run; -R keeps the natural order
+  const int split = color >= 0;
+  const unsigned cxm = rn + (unsigned)bcol(im, ni, split), cxp = rn + (unsigned)bcol(ip, ni, split);
+  const unsigned csrc = rn + (unsigned)(split ? col : ic), csrcR = rn + (unsigned)ic;
   auto cblk = [&](int slot) -> unsigned { return ((unsigned)slot * psu + colu) * 16u + ru * 4u; };
   auto chb = [&](int slot) -> unsigned { return ((unsigned)(slot - 1) * psu + colu) * 16u + ru * 4u; };
   // row r of block `slot` of the cell in row rowc, from the T storage or
@@ -619,9 +624,8 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   // the source as loaded (-R stays FP64 until consumed) and as T
   using SR = typename std::conditional<GS == GS_NEGR, double, T>::type;
   auto load_src = [&](int jj) -> SR {
-    const unsigned idx = rn + (unsigned)jj * niu + (unsigned)ic;
-    if constexpr (GS == GS_NEGR) return *at(sR, idx);
-    else return *at(sV, idx);
+    if constexpr (GS == GS_NEGR) return *at(sR, (unsigned)jj * niu + csrcR);
+    else return *at(sV, (unsigned)jj * niu + csrc);
   };
   auto src_val = [&](SR v) -> T {
     if constexpr (GS == GS_NEGR) return T(-v);
@@ -662,7 +666,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
     if constexpr (GS == GS_G0) {
       bs[k] = *at(g0p, rowc * 4u + cgd);
     } else {
-      if constexpr (GS == GS_NEGR) bs[k] = *at(sR, rowc + csrc);
+      if constexpr (GS == GS_NEGR) bs[k] = *at(sR, rowc + csrcR);
       else bs[k] = *at(sV, rowc + csrc);
       load_row(brow(0, rowc), b0[k]);
     }
@@ -734,8 +738,8 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
       if (live && half == 0) *at(g0p, cell * 4u + (unsigned)r) = gs;
     }
     if constexpr (COUP) {
-      xmv = *at(xnbp, rn + (unsigned)m * niu + (unsigned)im);
-      xpv = *at(xnbp, rn + (unsigned)m * niu + (unsigned)ip);
+      xmv = *at(xnbp, (unsigned)m * niu + cxm);
+      xpv = *at(xnbp, (unsigned)m * niu + cxp);
       crow_t(3, cell, s3r);
       crow_t(4, cell, s4r);
     }
@@ -748,7 +752,7 @@ __global__ void __launch_bounds__(8 * SL) k_sweep(Plan plan, int src, int xnb, i
   }
   double nrm = 0.0;
   if (live && half == 0) {
-    *at(xv, rn + (unsigned)m * niu + (unsigned)ic) = xr;
+    *at(xv, (unsigned)m * niu + csrc) = xr;
     nrm += double(xr) * double(xr);
   }
   // ---- away from the middle: x_j = d_j - S1_j x_prev (ring of PFB steps) -------

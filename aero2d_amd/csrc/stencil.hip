@@ -60,13 +60,15 @@ __device__ __forceinline__ CaseView<T> view(const CaseDev& cd) {
 }
 
 // Primitive state of cell (i, j) as scalar type S (tangent from v when dual).
+// idx: the cell in the state's natural order; vidx: the same cell in the
+// tangent vector's order (colour-split under the red-black line order).
 template <class S, class T>
-__device__ __forceinline__ W4<S> load_prim(gptr<const T> U, gptr<const T> v, int n, int idx) {
+__device__ __forceinline__ W4<S> load_prim(gptr<const T> U, gptr<const T> v, int n, int idx, int vidx) {
   if constexpr (std::is_same<S, T>::value) {
     return cons_to_prim<S>(U[idx], U[n + idx], U[2 * n + idx], U[3 * n + idx]);
   } else {
-    return cons_to_prim<S>(S(U[idx], v[idx]), S(U[n + idx], v[n + idx]),
-                           S(U[2 * n + idx], v[2 * n + idx]), S(U[3 * n + idx], v[3 * n + idx]));
+    return cons_to_prim<S>(S(U[idx], v[vidx]), S(U[n + idx], v[n + vidx]),
+                           S(U[2 * n + idx], v[2 * n + vidx]), S(U[3 * n + idx], v[3 * n + vidx]));
   }
 }
 
@@ -113,20 +115,21 @@ __global__ void __launch_bounds__(NTS) k_stencil(Plan plan, KnobsDev kn, int vin
     const int hi = c % HI, hj = c / HI;
     const int i = wrap(i0 - 2 + hi, ni);
     const int j = j0 - 2 + hj;
+    const int vi = bcol(i, ni, kn.precond_rb);     // the column in the Krylov vectors
     W4<S> w;
     if (j >= 0 && j < nc) {
-      w = load_prim<S, T>(cv.U, vt, n, j * ni + i);
+      w = load_prim<S, T>(cv.U, vt, n, j * ni + i, j * ni + vi);
     } else if (j < 0) {
       // mirror ghosts: j = -1 mirrors cell 0, j = -2 mirrors cell 1
       int jm = (j == -1) ? 0 : 1;
       if (jm >= nc) jm = nc - 1;
       const T sx = cv.Sj[i], sy = cv.Sj[(nc + 1) * ni + i];
       const T area = sqrt(sx * sx + sy * sy);
-      w = mirror(load_prim<S, T>(cv.U, vt, n, jm * ni + i), sx / area, sy / area);
+      w = mirror(load_prim<S, T>(cv.U, vt, n, jm * ni + i, jm * ni + vi), sx / area, sy / area);
     } else if (j <= nc + 1) {
       // both far-field ghost layers equal the characteristic boundary state
       const T sx = cv.Sj[nc * ni + i], sy = cv.Sj[(nc + 1) * ni + nc * ni + i];
-      w = farfield_ghost<S, T>(load_prim<S, T>(cv.U, vt, n, (nc - 1) * ni + i), sx, sy, inf,
+      w = farfield_ghost<S, T>(load_prim<S, T>(cv.U, vt, n, (nc - 1) * ni + i, (nc - 1) * ni + vi), sx, sy, inf,
                                cv.xf[i], cv.yf[i], circ);
     } else {
       w = {K<S>(1.0), K<S>(0.0), K<S>(0.0), K<S>(1.0)};  // outside the grid, never read
@@ -327,16 +330,17 @@ __global__ void __launch_bounds__(NTS) k_stencil(Plan plan, KnobsDev kn, int vin
       }
     } else {
       const gptr<T> out = vec_gptr<T>(cd, vout);
+      const int vidx = j * ni + bcol(i, ni, kn.precond_rb);
       const T dg = use_diag ? gp(ctp<T>(cd.diag))[idx] : T(0);
       if (mode == 0) {
         const T s = (kidx >= 0) ? T(sc->s[kidx]) : T(1);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) out[k * n + idx] = s * (dg * vt[k * n + idx] + r[k]);
+        for (int k = 0; k < 4; ++k) out[k * n + vidx] = s * (dg * vt[k * n + vidx] + r[k]);
       } else {
         const gptr<const double> R = gp(cd.R);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          out[k * n + idx] = T(-R[k * n + idx]) - (dg * vt[k * n + idx] + r[k]);
+          out[k * n + vidx] = T(-R[k * n + idx]) - (dg * vt[k * n + vidx] + r[k]);
       }
     }
   }
@@ -431,8 +435,9 @@ __global__ void __launch_bounds__(NT) k_dcirc(Plan plan, int vin, int gate, int 
   double dfx = 0.0, dfy = 0.0;
   for (int i = threadIdx.x; i < ni; i += NT) {
     using D = Dual<T>;
-    const W4<D> w0 = load_prim<D, T>(cv.U, v, n, i);
-    const W4<D> w1 = load_prim<D, T>(cv.U, v, n, ni + i);
+    const int vi = bcol(i, ni, kn.precond_rb);
+    const W4<D> w0 = load_prim<D, T>(cv.U, v, n, i, vi);
+    const W4<D> w1 = load_prim<D, T>(cv.U, v, n, ni + i, ni + vi);
     const double dpw = 1.5 * double(w0.p.d) - 0.5 * double(w1.p.d);
     const double dcp = dpw / q;
     dfx += dcp * cd.Sj[i];
