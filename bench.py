@@ -88,7 +88,7 @@ MEASURED_TRAFFIC = {"line_sweep": 1483.1e6, "residual": 253.1e6}
 # HBM bytes per launch at the transonic batch (32 x 512 x 256 cells): FETCH_SIZE
 # x 1.9967 + WRITE_SIZE of `bench.py --workload kernels3`
 # (profiles/kernels_config3_counters.txt)
-MEASURED_TRAFFIC_C3 = {"sweep_rb_half": 469.8e6, "jvp": 438.0e6, "residual": 674.7e6}
+MEASURED_TRAFFIC_C3 = {"sweep_rb_half": 402.6e6, "jvp": 442.9e6, "residual": 674.6e6}
 
 
 def rooflines(kt, cells):
