@@ -400,6 +400,14 @@ def run_config3(a):
     v32 = float(r32.converged.sum() / r32.wall)
     acc = accuracy(rmx, r64, cases, restart_checks=not a.no_restart_checks)
     acc32 = accuracy(r32, r64, cases, restart_checks=False)
+    # precision alone: the mixed form against FP64 on the mixed form's solver path
+    both_m = rmx.converged & r64m.converged
+    dm = {k: np.abs(getattr(rmx, k) - getattr(r64m, k)) for k in ("cl", "cd", "cm")}
+    within_m = both_m & (dm["cl"] <= 1e-5) & (dm["cd"] <= 1e-6)
+    acc_same = {"cases_both_converged": int(both_m.sum()), "cases_within_bar": int(within_m.sum()),
+                "outside_bar": [int(k) for k in np.nonzero(both_m & ~within_m)[0]],
+                "max_within_bar": {k: float(v[within_m].max()) if within_m.any() else None for k, v in dm.items()},
+                "per_case_dcl": dm["cl"].tolist()}
     out = {"metric": METRIC, "value": vmx, "unit": "cases/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": 1e3 * float(np.mean(walls)), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 outer", "data": "synthetic",
@@ -416,7 +424,8 @@ def run_config3(a):
                     "basis and the factored couplings) against the FP64 reference algorithm; "
                     "mixed_fp32_storage is the form with FP32 storage",
            "mixed": form_entry(rmx, walls, convs) | {"knobs": SOLVER_DEFAULTS["mixed"],
-                                                     "accuracy_vs_fp64": acc},
+                                                     "accuracy_vs_fp64": acc,
+                                                     "accuracy_vs_fp64_same_path": acc_same},
            "fp64": form_entry(r64) | {"knobs": SOLVER_DEFAULTS["fp64"]},
            "fp64_mixed_path": form_entry(r64m),
            "mixed_fp32_storage": form_entry(r32) | {"speedup_vs_fp64": v32 / v64 if v64 > 0 else None,
