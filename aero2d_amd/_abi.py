@@ -22,6 +22,14 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _i64 = ctypes.c_int64
 
 
+class RansParams(ctypes.Structure):
+    """a2d_rans_params (include/aero2d_hip.h)."""
+    _fields_ = [("mach", ctypes.c_double), ("re", ctypes.c_double), ("chord", ctypes.c_double),
+                ("xref_x", ctypes.c_double), ("xref_y", ctypes.c_double), ("chi_inf", ctypes.c_double),
+                ("limiter_k", ctypes.c_double), ("zmin", ctypes.c_double), ("order", ctypes.c_int),
+                ("turbulent", ctypes.c_int), ("vortex_bc", ctypes.c_int), ("limiter_zone", ctypes.c_int)]
+
+
 class Knobs(ctypes.Structure):
     """a2d_knobs (include/aero2d_hip.h)."""
     _fields_ = [("cfl_explicit", ctypes.c_double), ("lim_k", ctypes.c_double),
@@ -71,6 +79,28 @@ SIGNATURES = {
     "a2d_hllc_flux": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
     "a2d_hllc_flux_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp]),
     "a2d_time_kernel": (ctypes.c_int, [_i64, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "a2d_rans_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp,
+                                       _ip, _dp, _ip, _dp, _dp, ctypes.POINTER(RansParams),
+                                       ctypes.POINTER(_i64)]),
+    "a2d_rans_destroy": (ctypes.c_int, [_i64]),
+    "a2d_rans_set_state": (ctypes.c_int, [_i64, _dp]),
+    "a2d_rans_get_state": (ctypes.c_int, [_i64, _dp]),
+    "a2d_rans_freeze_limiter": (ctypes.c_int, [_i64, _dp]),
+    "a2d_rans_limiter": (ctypes.c_int, [_i64, _dp]),
+    "a2d_rans_residual": (ctypes.c_int, [_i64, ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]),
+    "a2d_rans_jvp": (ctypes.c_int, [_i64, ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]),
+    "a2d_rans_local_dt": (ctypes.c_int, [_i64, ctypes.c_double, _dp]),
+    "a2d_rans_forces": (ctypes.c_int, [_i64, ctypes.c_double, _dp, _dp, _dp]),
+    "a2d_rans_jac1": (ctypes.c_int, [_i64, _dp, ctypes.c_double, _dp, _dp]),
+    "a2d_rans_ilu_setup": (ctypes.c_int, [_i64, ctypes.c_int, _ip, _ip, _ip, _ip]),
+    "a2d_rans_ilu_info": (ctypes.c_int, [_i64, _ip]),
+    "a2d_rans_ilu_factor": (ctypes.c_int, [_i64, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _ip]),
+    "a2d_rans_ilu_apply": (ctypes.c_int, [_i64, _dp, _dp]),
+    "a2d_rans_newton_step": (ctypes.c_int, [_i64, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, _dp]),
+    "a2d_rans_newton_dq": (ctypes.c_int, [_i64, _dp]),
+    "a2d_rans_keep": (ctypes.c_int, [_i64, ctypes.c_int]),
 }
 
 _lib = None

@@ -734,6 +734,16 @@ int guarded(F&& f) {
 }
 
 }  // namespace
+
+// Shared with the other translation units of the library (api.hpp): the
+// thread-local error text behind a2d_last_error, the library lock and the
+// device chosen by a2d_init.
+void set_error_text(const std::string& s) { g_err = s; }
+std::recursive_mutex& api_mutex() { return g_mu; }
+int require_device() {
+  ensure_init();
+  return g_device;
+}
 }  // namespace a2d
 
 using namespace a2d;

@@ -1,0 +1,1819 @@
+// RANS (Spalart-Allmaras, SA-neg) Newton-Krylov core on an unstructured mesh:
+// residual, exact Jacobian-vector product, first-order Jacobian blocks, block
+// ILU(k) factorisation and triangular solves, GMRES, local pseudo-time step,
+// step limit and forces; the C ABI declared in include/aero2d_hip.h
+// (a2d_rans_*). Reference: aero2d/rans.py (RansSolver, SA path),
+// aero2d/bilu.py (BlockILU0), aero2d/krylov.py (gmres).
+//
+// Device layout of one solver (all FP64):
+//   Q, R, dq, x, ... [5][nc]        state and vectors, structure of arrays
+//   g, gt            [10][nc]       least-squares gradients (k, d) -> 2k+d, and tangents
+//   phi, phit        [5][nc]        limiter and its tangent
+//   F, Ft            [5][nfi]       interior face fluxes times face length
+//   B                [nnz][5][5]    ILU blocks in the permuted order, row-major blocks
+//   y                [n][5]         ILU work vector in the permuted order
+// Mesh arrays are structure-of-arrays per face set (interior, wall, far);
+// each cell lists its faces (CSR), so every gather is per cell in a fixed
+// order and results do not depend on scheduling (no atomics).
+#include "../../include/aero2d_hip.h"
+#include "api.hpp"
+#include "rans_physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace a2d {
+namespace rans {
+namespace {
+
+struct Fail {
+  int code;
+};
+[[noreturn]] void fail(int code, const std::string& msg) {
+  set_error_text(msg);
+  throw Fail{code};
+}
+void hc(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(-10, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HC(x) hc((x), #x)
+
+template <class F>
+int guarded(F&& f) {
+  std::lock_guard<std::recursive_mutex> lk(api_mutex());
+  try {
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error_text(e.what());
+    return -99;
+  }
+}
+
+constexpr int NFG = 11;   // interior face: nx ny A rLx rLy rRx rRy dx dy dist w
+constexpr int NWG = 9;    // wall face: nx ny A rx ry dn w fcx fcy
+constexpr int NFF = 7;    // far face: nx ny A rx ry fcx fcy
+constexpr int NCG = 6;    // cell: vol wd M00 M01 M11 in_limiter_zone
+constexpr int TPB = 256;
+constexpr int GMRES_MAX = 120;   // largest Krylov basis of one linear solve
+
+struct MeshDev {
+  int nc, nfi, nfw, nff;
+  const int *iL, *iR;
+  const double* fg;    // [NFG][nfi]
+  const int* cw;
+  const double* wg;    // [NWG][nfw]
+  const int* cf;
+  const double* ffg;   // [NFF][nff]
+  const double* cg;    // [NCG][nc]
+  const unsigned char* far_cell;
+  const int *cptr, *cfac;   // interior faces of a cell: f (cell is left) or ~f (right)
+  const int *wptr, *wfac;
+  const int *fptr, *ffac;
+};
+
+// ---- loads and stores of plain or dual values ---------------------------------
+template <class S> struct is_dual { static constexpr bool v = false; };
+template <class T> struct is_dual<Dual<T>> { static constexpr bool v = true; };
+
+template <class S>
+__device__ __forceinline__ S ld(const double* a, const double* at, int i) {
+  if constexpr (is_dual<S>::v) return S(a[i], at ? at[i] : 0.0);
+  else return a[i];
+}
+template <class S>
+__device__ __forceinline__ void st(double* a, double* at, int i, S x) {
+  a[i] = val(x);
+  if constexpr (is_dual<S>::v) { if (at) at[i] = x.d; }
+}
+
+template <class S>
+__device__ __forceinline__ void load_w(const double* Q, const double* V, int nc, int c, S w[5]) {
+  S q[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[k] = ld<S>(Q, V, k * nc + c);
+  prim(q, w);
+}
+
+// Gradients of cell c as used by the fluxes: at second order multiplied by the
+// limiter (when one is applied) and zero in cells on the far-field boundary
+// (aero2d/rans.py:870-877).
+template <class S>
+__device__ __forceinline__ void load_g(const MeshDev& m, const double* g, const double* gt, const double* phi,
+                                       const double* phit, int order, int c, S G[5][2]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int d = 0; d < 2; ++d) G[k][d] = ld<S>(g, gt, (2 * k + d) * m.nc + c);
+  if (order >= 2) {
+    if (phi) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        S p = ld<S>(phi, phit, k * m.nc + c);
+        G[k][0] = G[k][0] * p;
+        G[k][1] = G[k][1] * p;
+      }
+    }
+    if (m.far_cell[c]) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) G[k][0] = G[k][1] = K<S>(0.0);
+    }
+  }
+}
+
+// ---- residual pieces ----------------------------------------------------------------
+// Least-squares gradients of the primitives (aero2d/rans.py:809-819).
+template <class S>
+__global__ void k_grad(MeshDev m, const double* Q, const double* V, double* g, double* gt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  S w[5];
+  load_w<S>(Q, V, m.nc, c, w);
+  S rhs[5][2];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rhs[k][0] = rhs[k][1] = K<S>(0.0);
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    const bool left = code >= 0;
+    const int f = left ? code : ~code;
+    const int nb = left ? m.iR[f] : m.iL[f];
+    S wn[5];
+    load_w<S>(Q, V, m.nc, nb, wn);
+    const double wf = m.fg[10 * m.nfi + f], dx = m.fg[7 * m.nfi + f], dy = m.fg[8 * m.nfi + f];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      S dphi = left ? wn[k] - w[k] : w[k] - wn[k];       // w[iR] - w[iL]
+      S a = K<S>(wf) * dphi;
+      rhs[k][0] = rhs[k][0] + a * dx;
+      rhs[k][1] = rhs[k][1] + a * dy;
+    }
+  }
+  for (int e = m.wptr[c]; e < m.wptr[c + 1]; ++e) {
+    const int f = m.wfac[e];
+    const double ww = m.wg[6 * m.nfw + f], rx = m.wg[3 * m.nfw + f], ry = m.wg[4 * m.nfw + f];
+    S wv[5] = {w[0], K<S>(0.0), K<S>(0.0), w[3], K<S>(0.0)};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      S a = K<S>(ww) * (wv[k] - w[k]);
+      rhs[k][0] = rhs[k][0] + a * rx;
+      rhs[k][1] = rhs[k][1] + a * ry;
+    }
+  }
+  const double M00 = m.cg[2 * m.nc + c], M01 = m.cg[3 * m.nc + c], M11 = m.cg[4 * m.nc + c];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    st<S>(g, gt, (2 * k) * m.nc + c, K<S>(M00) * rhs[k][0] + K<S>(M01) * rhs[k][1]);
+    st<S>(g, gt, (2 * k + 1) * m.nc + c, K<S>(M01) * rhs[k][0] + K<S>(M11) * rhs[k][1]);
+  }
+}
+
+// Venkatakrishnan limiter (aero2d/rans.py:983-1022) in the limiter zone, 1
+// elsewhere (aero2d/rans.py:869-871).
+template <class S>
+__global__ void k_limit(MeshDev m, Params P, const double* Q, const double* V, const double* g,
+                        const double* gt, double* phi, double* phit) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  const bool zone = P.limiter_zone == 2 || (P.limiter_zone == 1 && m.cg[5 * m.nc + c] != 0.0);
+  if (!zone) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) st<S>(phi, phit, k * m.nc + c, K<S>(1.0));
+    return;
+  }
+  S w[5];
+  load_w<S>(Q, V, m.nc, c, w);
+  Ext<S> mx[5], mn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) mx[k] = mn[k] = Ext<S>{w[k], 1};
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    const int f = code >= 0 ? code : ~code;
+    const int nb = code >= 0 ? m.iR[f] : m.iL[f];
+    S wn[5];
+    load_w<S>(Q, V, m.nc, nb, wn);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      ext_push_max(mx[k], wn[k]);
+      ext_push_min(mn[k], wn[k]);
+    }
+  }
+  for (int e = m.wptr[c]; e < m.wptr[c + 1]; ++e) {
+    S wv[5] = {w[0], K<S>(0.0), K<S>(0.0), w[3], K<S>(0.0)};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      ext_push_max(mx[k], wv[k]);
+      ext_push_min(mn[k], wv[k]);
+    }
+  }
+  S G[5][2];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    G[k][0] = ld<S>(g, gt, (2 * k) * m.nc + c);
+    G[k][1] = ld<S>(g, gt, (2 * k + 1) * m.nc + c);
+  }
+  const double h = sqrt(m.cg[c]);
+  const double kh = P.limiter_k * h;
+  const double kh3 = kh * kh * kh;
+  const double scale[5] = {1.0, P.mach, P.mach, 1.0 / GAM, P.chi_inf > 1.0 ? P.chi_inf : 1.0};
+  S dmx[5], dmn[5];
+  Ext<S> lim[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    dmx[k] = ext_value(mx[k]) - w[k];
+    dmn[k] = ext_value(mn[k]) - w[k];
+    lim[k] = Ext<S>{K<S>(1.0), 1};
+  }
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    const bool left = code >= 0;
+    const int f = left ? code : ~code;
+    const double rx = m.fg[(left ? 3 : 5) * m.nfi + f], ry = m.fg[(left ? 4 : 6) * m.nfi + f];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      S d2 = G[k][0] * rx + G[k][1] * ry;
+      ext_push_min(lim[k], venkat_psi(d2, dmx[k], dmn[k], kh3 * scale[k] * scale[k]));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) st<S>(phi, phit, k * m.nc + c, ext_value(lim[k]));
+}
+
+// Far-field state: freestream plus the Prandtl-Glauert point vortex of
+// circulation gam at xref, isentropic rho and p (aero2d/rans.py:830-858).
+__global__ void k_far_state(MeshDev m, Params P, double w0, double w1, double w2, double w3, double w4,
+                            double gam, double* wgo) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nff) return;
+  const double x = m.ffg[5 * m.nff + f] - P.xref_x, y = m.ffg[6 * m.nff + f] - P.xref_y;
+  const double M = P.mach;
+  const double b = sqrt(1.0 - M * M);
+  const double vm = sqrt(w1 * w1 + w2 * w2) + 1e-30;
+  const double ca = w1 / vm, sa = w2 / vm;
+  const double xr = x * ca + y * sa, yr = -x * sa + y * ca;
+  const double th = atan2(yr, xr);
+  const double r = sqrt(xr * xr + yr * yr);
+  const double st_ = sin(th), ct_ = cos(th);
+  const double fac = gam * b / (2.0 * M_PI * r * (1.0 - M * M * st_ * st_));
+  const double dur = fac * st_, dvr = -fac * ct_;
+  const double U = w1 + (dur * ca - dvr * sa), Vv = w2 + (dur * sa + dvr * ca);
+  const double q2 = U * U + Vv * Vv;
+  const double T = 1.0 + 0.5 * (GAM - 1.0) * (M * M - q2);
+  const double rho = pow(T, 1.0 / (GAM - 1.0));
+  wgo[f] = rho;
+  wgo[m.nff + f] = U;
+  wgo[2 * m.nff + f] = Vv;
+  wgo[3 * m.nff + f] = rho * T / GAM;
+  wgo[4 * m.nff + f] = w4;
+  (void)w0;
+  (void)w3;
+}
+
+// Interior face flux: reconstruction, LM-Roe, viscous flux with the face
+// gradient (cell mean plus the correction along the line of centres), times
+// the face length (aero2d/rans.py:878-919).
+template <class S>
+__global__ void k_face(MeshDev m, Params P, int order, const double* Q, const double* V, const double* g,
+                       const double* gt, const double* phi, const double* phit, double* F, double* Ft) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nfi) return;
+  const int L = m.iL[f], R = m.iR[f];
+  const int nf = m.nfi;
+  const double nx = m.fg[f], ny = m.fg[nf + f], A = m.fg[2 * nf + f];
+  S wL0[5], wR0[5], GL[5][2], GR[5][2];
+  load_w<S>(Q, V, m.nc, L, wL0);
+  load_w<S>(Q, V, m.nc, R, wR0);
+  load_g<S>(m, g, gt, phi, phit, order, L, GL);
+  load_g<S>(m, g, gt, phi, phit, order, R, GR);
+  S wL[5], wR[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    wL[k] = wL0[k];
+    wR[k] = wR0[k];
+  }
+  if (order >= 2) {
+    const double rLx = m.fg[3 * nf + f], rLy = m.fg[4 * nf + f];
+    const double rRx = m.fg[5 * nf + f], rRy = m.fg[6 * nf + f];
+    S a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = wL0[k] + (GL[k][0] * rLx + GL[k][1] * rLy);
+      b[k] = wR0[k] + (GR[k][0] * rRx + GR[k][1] * rRy);
+    }
+    if (val(a[0]) > 0.2 * val(wL0[0]) && val(a[3]) > 0.2 * val(wL0[3]))
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wL[k] = a[k];
+    if (val(b[0]) > 0.2 * val(wR0[0]) && val(b[3]) > 0.2 * val(wR0[3]))
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wR[k] = b[k];
+  }
+  S Fc[5];
+  roe(P, wL, wR, nx, ny, Fc);
+  S wf[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) wf[k] = K<S>(0.5) * (wL0[k] + wR0[k]);
+  const S TL = K<S>(GAM) * wL0[3] / wL0[0], TR = K<S>(GAM) * wR0[3] / wR0[0];
+  S gTL[2], gTR[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    gTL[d] = K<S>(GAM) * (GL[3][d] / wL0[0] - (wL0[3] / (wL0[0] * wL0[0])) * GL[0][d]);
+    gTR[d] = K<S>(GAM) * (GR[3][d] / wR0[0] - (wR0[3] / (wR0[0] * wR0[0])) * GR[0][d]);
+  }
+  const double dist = m.fg[9 * nf + f];
+  const double ex = m.fg[7 * nf + f] / dist, ey = m.fg[8 * nf + f] / dist;
+  auto fgrad = [&](S pl, S pr, const S gl[2], const S gr[2], S out[2]) {
+    S ga0 = K<S>(0.5) * (gl[0] + gr[0]), ga1 = K<S>(0.5) * (gl[1] + gr[1]);
+    S corr = (pr - pl) / K<S>(dist) - (ga0 * ex + ga1 * ey);
+    out[0] = ga0 + corr * ex;
+    out[1] = ga1 + corr * ey;
+  };
+  S gu[2], gv[2], gTf[2], gc[2];
+  fgrad(wL0[1], wR0[1], GL[1], GR[1], gu);
+  fgrad(wL0[2], wR0[2], GL[2], GR[2], gv);
+  fgrad(TL, TR, gTL, gTR, gTf);
+  fgrad(wL0[4], wR0[4], GL[4], GR[4], gc);
+  S Fv[5];
+  visc_flux(P, wf, gu, gv, gTf, gc, nx, ny, Fv);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) st<S>(F, Ft, k * nf + f, (Fc[k] - Fv[k]) * K<S>(A));
+}
+
+// Cell residual: signed sum of interior face fluxes, wall and far-field faces,
+// minus the SA source times the volume (aero2d/rans.py:920-964). With vdt
+// non-null the output is the Newton matrix times V: the tangent plus
+// vol/dt V (vol/(dt ct) on the turbulence equation).
+template <class S>
+__global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const double* V, const double* g,
+                       const double* gt, const double* phi, const double* phit, const double* F,
+                       const double* Ft, const double* wgf, double* Rv, double* Rt, const double* vdt,
+                       double ct_inv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  S w[5], G[5][2];
+  load_w<S>(Q, V, m.nc, c, w);
+  load_g<S>(m, g, gt, phi, phit, order, c, G);
+  S R[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) R[k] = K<S>(0.0);
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    if (code >= 0) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) R[k] = R[k] + ld<S>(F, Ft, k * m.nfi + code);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) R[k] = R[k] - ld<S>(F, Ft, k * m.nfi + ~code);
+    }
+  }
+  const int nw = m.nfw;
+  if (m.wptr[c] < m.wptr[c + 1]) {
+    S gT[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) gT[d] = K<S>(GAM) * (G[3][d] / w[0] - (w[3] / (w[0] * w[0])) * G[0][d]);
+    for (int e = m.wptr[c]; e < m.wptr[c + 1]; ++e) {
+      const int f = m.wfac[e];
+      const double nx = m.wg[f], ny = m.wg[nw + f], A = m.wg[2 * nw + f];
+      const double rx = m.wg[3 * nw + f], ry = m.wg[4 * nw + f], dn = m.wg[5 * nw + f];
+      S pw = w[3];
+      if (order >= 2) pw = w[3] + (G[3][0] * rx + G[3][1] * ry);
+      if (!(val(pw) > 0.2 * val(w[3]))) pw = w[3];
+      auto wgrad = [&](S phii, const S gi[2], S out[2]) {
+        S corr = (K<S>(0.0) - phii) / K<S>(dn) - (gi[0] * nx + gi[1] * ny);
+        out[0] = gi[0] + corr * nx;
+        out[1] = gi[1] + corr * ny;
+      };
+      S gu[2], gv[2], gc[2], gTw[2];
+      wgrad(w[1], G[1], gu);
+      wgrad(w[2], G[2], gv);
+      wgrad(w[4], G[4], gc);
+      S tn = gT[0] * nx + gT[1] * ny;
+      gTw[0] = gT[0] - tn * nx;
+      gTw[1] = gT[1] - tn * ny;
+      S wv[5] = {w[0], K<S>(0.0), K<S>(0.0), w[3], K<S>(0.0)};
+      S Fv[5];
+      visc_flux(P, wv, gu, gv, gTw, gc, nx, ny, Fv);
+      R[0] = R[0] + (K<S>(0.0) - Fv[0]) * K<S>(A);
+      R[1] = R[1] + (pw * nx - Fv[1]) * K<S>(A);
+      R[2] = R[2] + (pw * ny - Fv[2]) * K<S>(A);
+      R[3] = R[3] + (K<S>(0.0) - Fv[3]) * K<S>(A);
+      R[4] = R[4] + (K<S>(0.0) - Fv[4]) * K<S>(A);
+    }
+  }
+  const int nff = m.nff;
+  for (int e = m.fptr[c]; e < m.fptr[c + 1]; ++e) {
+    const int f = m.ffac[e];
+    const double nx = m.ffg[f], ny = m.ffg[nff + f], A = m.ffg[2 * nff + f];
+    S wi[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) wi[k] = w[k];
+    if (order >= 2) {
+      const double rx = m.ffg[3 * nff + f], ry = m.ffg[4 * nff + f];
+      S a[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a[k] = w[k] + (G[k][0] * rx + G[k][1] * ry);
+      if (val(a[0]) > 0.2 * val(w[0]) && val(a[3]) > 0.2 * val(w[3]))
+#pragma unroll
+        for (int k = 0; k < 5; ++k) wi[k] = a[k];
+    }
+    S wb[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) wb[k] = K<S>(wgf[k * nff + f]);
+    S Ff[5];
+    roe(P, wi, wb, nx, ny, Ff);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) R[k] = R[k] + Ff[k] * K<S>(A);
+  }
+  const double vol = m.cg[c];
+  if (P.turbulent) {
+    R[4] = R[4] - sa_source(P, w, G, m.cg[m.nc + c]) * K<S>(vol);
+  } else {
+    R[4] = ld<S>(Q, V, 4 * m.nc + c) * K<S>(vol);
+  }
+  if constexpr (is_dual<S>::v) {
+    if (vdt) {
+      const double d = vdt[c];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const double dk = k == 4 ? d * ct_inv : d;
+        Rt[k * m.nc + c] = R[k].d + dk * V[k * m.nc + c];
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) st<S>(Rv, Rt, k * m.nc + c, R[k]);
+}
+
+// Per-equation sums of (R_k / vol)^2 over blocks of cells, then in order.
+__global__ void k_rnorm_part(int nc, const double* R, const double* vol, double* part) {
+  __shared__ double sh[5][TPB];
+  const int c = blockIdx.x * TPB + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double v = 0.0;
+    if (c < nc) {
+      const double x = R[k * nc + c] / vol[c];
+      v = x * x;
+    }
+    sh[k][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// Sums nparts partials of each of nsum rows in order (one lane per row).
+__global__ void k_sum_rows(int nsum, int nparts, const double* part, double* out) {
+  const int r = threadIdx.x;
+  if (r >= nsum) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += part[r * nparts + p];
+  out[r] = s;
+}
+
+// Local pseudo-time step (aero2d/rans.py:1206-1249, SA): vdt = vol / dt.
+__global__ void k_local_dt(MeshDev m, Params P, const double* Q, double cfl, double* dt, double* vdt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  double w[5];
+  load_w<double>(Q, nullptr, m.nc, c, w);
+  const double a = sqrt(GAM * w[3] / w[0]);
+  double lam = 0.0, lv = 0.0;
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    const bool left = code >= 0;
+    const int f = left ? code : ~code;
+    double wn[5];
+    load_w<double>(Q, nullptr, m.nc, left ? m.iR[f] : m.iL[f], wn);
+    const double an = sqrt(GAM * wn[3] / wn[0]);
+    const double uL = left ? w[1] : wn[1], uR = left ? wn[1] : w[1];
+    const double vL = left ? w[2] : wn[2], vR = left ? wn[2] : w[2];
+    const double aL = left ? a : an, aR = left ? an : a;
+    const double nx = m.fg[f], ny = m.fg[m.nfi + f], A = m.fg[2 * m.nfi + f];
+    const double un = fabs(0.5 * ((uL + uR) * nx + (vL + vR) * ny));
+    lam += (un + 0.5 * (aL + aR)) * A;
+    lv += A * A;
+  }
+  for (int e = m.fptr[c]; e < m.fptr[c + 1]; ++e) {
+    const int f = m.ffac[e];
+    const double nx = m.ffg[f], ny = m.ffg[m.nff + f], A = m.ffg[2 * m.nff + f];
+    lam += (fabs(w[1] * nx + w[2] * ny) + a) * A;
+  }
+  for (int e = m.wptr[c]; e < m.wptr[c + 1]; ++e) {
+    const double A = m.wg[2 * m.nfw + m.wfac[e]];
+    lam += a * A;
+    lv += A * A;
+  }
+  const double T = GAM * w[3] / w[0];
+  const double mu = P.mu_inf * T * sqrt(T) * (1.0 + SUTH) / (T + SUTH);
+  double nut = 0.0;
+  if (P.turbulent) {
+    const double nt = (w[4] > 0.0 ? w[4] : 0.0) * P.nu_inf;
+    const double x = nt * w[0] / mu;
+    const double X3 = x * x * x;
+    nut = nt * X3 / (X3 + CV1 * CV1 * CV1);
+  }
+  const double vol = m.cg[c];
+  lam += 1.4 * (mu / w[0] / PR + nut / PRT) * lv / vol;
+  const double d = cfl * vol / lam;
+  dt[c] = d;
+  vdt[c] = vol / d;
+}
+
+// ---- first-order Jacobian blocks (aero2d/rans.py:1032-1126) --------------------------
+// Interior face: F(qL, qR) of the first-order face flux with the two-point
+// face gradient; columns by forward duals. Writes the off-diagonal blocks
+// (iL,iR) = dF/dqR A and (iR,iL) = -dF/dqL A into the ILU storage and keeps
+// both for the diagonal gather.
+__global__ void k_jac_int(MeshDev m, Params P, const double* Q, double* J, double* B, const int* posLR,
+                          const int* posRL) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nfi) return;
+  using D = Dual<double>;
+  const int L = m.iL[f], R = m.iR[f];
+  const int nf = m.nfi;
+  const double nx = m.fg[f], ny = m.fg[nf + f], A = m.fg[2 * nf + f];
+  const double dist = m.fg[9 * nf + f];
+  const double ex = m.fg[7 * nf + f] / dist, ey = m.fg[8 * nf + f] / dist;
+  double qL[5], qR[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    qL[k] = Q[k * m.nc + L];
+    qR[k] = Q[k * m.nc + R];
+  }
+  double* jl = J + size_t(f) * 50;
+  double* jr = jl + 25;
+  for (int s = 0; s < 10; ++s) {
+    D a[5], b[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      a[k] = D(qL[k], s == k ? 1.0 : 0.0);
+      b[k] = D(qR[k], s == k + 5 ? 1.0 : 0.0);
+    }
+    D wL[5], wR[5];
+    prim(a, wL);
+    prim(b, wR);
+    D Fc[5];
+    roe(P, wL, wR, nx, ny, Fc);
+    const D TL = K<D>(GAM) * wL[3] / wL[0], TR = K<D>(GAM) * wR[3] / wR[0];
+    auto g2 = [&](D x, D y, D out[2]) {
+      D t = (y - x) / K<D>(dist);
+      out[0] = t * ex;
+      out[1] = t * ey;
+    };
+    D gu[2], gv[2], gT[2], gc[2], wf[5];
+    g2(wL[1], wR[1], gu);
+    g2(wL[2], wR[2], gv);
+    g2(TL, TR, gT);
+    g2(wL[4], wR[4], gc);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) wf[k] = K<D>(0.5) * (wL[k] + wR[k]);
+    D Fv[5];
+    visc_flux(P, wf, gu, gv, gT, gc, nx, ny, Fv);
+    double* col = s < 5 ? jl : jr;
+    const int j = s < 5 ? s : s - 5;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double v = (Fc[i].d - Fv[i].d) * A;
+      if (!P.turbulent && i == 4) v = 0.0;
+      col[i * 5 + j] = v;
+    }
+  }
+  if (B)
+    for (int t = 0; t < 25; ++t) {
+      B[size_t(posLR[f]) * 25 + t] = jr[t];
+      B[size_t(posRL[f]) * 25 + t] = -jl[t];
+    }
+}
+
+// Wall face flux Jacobian (no slip, adiabatic, one-sided gradients).
+__global__ void k_jac_wall(MeshDev m, Params P, const double* Q, double* JW) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nfw) return;
+  using D = Dual<double>;
+  const int c = m.cw[f];
+  const int nw = m.nfw;
+  const double nx = m.wg[f], ny = m.wg[nw + f], A = m.wg[2 * nw + f], dn = m.wg[5 * nw + f];
+  for (int s = 0; s < 5; ++s) {
+    D q[5], wi[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = D(Q[k * m.nc + c], s == k ? 1.0 : 0.0);
+    prim(q, wi);
+    D p = wi[3];
+    auto gw = [&](D x, D out[2]) {
+      D t = (K<D>(0.0) - x) / K<D>(dn);
+      out[0] = t * nx;
+      out[1] = t * ny;
+    };
+    D gu[2], gv[2], gc[2];
+    D gT[2] = {K<D>(0.0), K<D>(0.0)};
+    gw(wi[1], gu);
+    gw(wi[2], gv);
+    gw(wi[4], gc);
+    D wf[5] = {wi[0], K<D>(0.0), K<D>(0.0), p, K<D>(0.0)};
+    D Fv[5];
+    visc_flux(P, wf, gu, gv, gT, gc, nx, ny, Fv);
+    D Fc[5] = {K<D>(0.0), p * nx, p * ny, K<D>(0.0), K<D>(0.0)};
+#pragma unroll
+    for (int i = 0; i < 5; ++i) JW[size_t(f) * 25 + i * 5 + s] = (Fc[i].d - Fv[i].d) * A;
+  }
+}
+
+// Far-field face flux Jacobian (Roe flux against the fixed boundary state).
+__global__ void k_jac_far(MeshDev m, Params P, const double* Q, const double* wgf, double* JF) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nff) return;
+  using D = Dual<double>;
+  const int c = m.cf[f];
+  const int n = m.nff;
+  const double nx = m.ffg[f], ny = m.ffg[n + f], A = m.ffg[2 * n + f];
+  D wb[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) wb[k] = K<D>(wgf[k * n + f]);
+  for (int s = 0; s < 5; ++s) {
+    D q[5], wi[5], F[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = D(Q[k * m.nc + c], s == k ? 1.0 : 0.0);
+    prim(q, wi);
+    roe(P, wi, wb, nx, ny, F);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) JF[size_t(f) * 25 + i * 5 + s] = F[i].d * A;
+  }
+}
+
+// Diagonal blocks: sum of the cell's face blocks, the decreasing part of the
+// SA source on (4,4) at frozen gradients, vol/dt on the diagonal (vol/(dt ct)
+// on the turbulence equation) (aero2d/rans.py:1107-1144).
+__global__ void k_jac_diag(MeshDev m, Params P, const double* Q, const double* g, const double* J,
+                           const double* JW, const double* JF, const double* vdt, double ct_inv, double* B,
+                           const int* posD, double* Dout) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  double Dm[25];
+#pragma unroll
+  for (int t = 0; t < 25; ++t) Dm[t] = 0.0;
+  for (int e = m.cptr[c]; e < m.cptr[c + 1]; ++e) {
+    const int code = m.cfac[e];
+    if (code >= 0) {
+      const double* jl = J + size_t(code) * 50;
+#pragma unroll
+      for (int t = 0; t < 25; ++t) Dm[t] += jl[t];
+    } else {
+      const double* jr = J + size_t(~code) * 50 + 25;
+#pragma unroll
+      for (int t = 0; t < 25; ++t) Dm[t] -= jr[t];
+    }
+  }
+  for (int e = m.wptr[c]; e < m.wptr[c + 1]; ++e) {
+    const double* jw = JW + size_t(m.wfac[e]) * 25;
+#pragma unroll
+    for (int t = 0; t < 25; ++t) Dm[t] += jw[t];
+  }
+  for (int e = m.fptr[c]; e < m.fptr[c + 1]; ++e) {
+    const double* jf = JF + size_t(m.ffac[e]) * 25;
+#pragma unroll
+    for (int t = 0; t < 25; ++t) Dm[t] += jf[t];
+  }
+  const double vol = m.cg[c];
+  if (P.turbulent) {
+    using D = Dual<double>;
+    D q[5], w[5], G[5][2];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = D(Q[k * m.nc + c], k == 4 ? 1.0 : 0.0);
+    prim(q, w);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      G[k][0] = K<D>(g[(2 * k) * m.nc + c]);
+      G[k][1] = K<D>(g[(2 * k + 1) * m.nc + c]);
+    }
+    const double ds = sa_source(P, w, G, m.cg[m.nc + c]).d;
+    Dm[24] += vol * (-ds > 0.0 ? -ds : 0.0);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) Dm[20 + j] = 0.0;
+    Dm[24] = vol;
+  }
+  if (Dout) {
+#pragma unroll
+    for (int t = 0; t < 25; ++t) Dout[size_t(c) * 25 + t] = Dm[t];
+  }
+  if (vdt) {
+    const double d = vdt[c];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Dm[k * 6] += k == 4 ? d * ct_inv : d;
+  }
+  if (B) {
+#pragma unroll
+    for (int t = 0; t < 25; ++t) B[size_t(posD[c]) * 25 + t] = Dm[t];
+  }
+}
+
+// ---- block ILU in level order ------------------------------------------------------
+// One workgroup walks the levels; rows of a level are independent. In the
+// factorisation a group of 32 lanes owns a row, lane t < 25 owning element
+// (t/5, t%5) of every 5x5 block product; row elements are exchanged by lane
+// shuffles. Lane 0 of the group inverts the diagonal block (Gauss-Jordan,
+// partial pivoting).
+struct IluDev {
+  int n, nnz;
+  const int *rstart, *rdiag, *rend, *col;
+  const int *pptr, *pt, *pe2;        // per lower entry: its update pairs (target, row-k entry)
+  int nlev, nlevu;
+  const int *lptr, *lrows, *uptr, *urows;
+  const int* perm;                   // perm[new] = old
+};
+
+constexpr int ILU_TPB = 512;
+
+__global__ __launch_bounds__(ILU_TPB) void k_ilu_factor(IluDev L, double* B, double* invU) {
+  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31, ngrp = blockDim.x >> 5;
+  const bool act = lane < 25;
+  const int r = act ? lane / 5 : 4, cc = act ? lane % 5 : 4;
+  for (int l = 0; l < L.nlev; ++l) {
+    for (int p = L.lptr[l] + grp; p < L.lptr[l + 1]; p += ngrp) {
+      const int i = L.lrows[p];
+      const int d = L.rdiag[i];
+      for (int e = L.rstart[i]; e < d; ++e) {
+        const int k = L.col[e];
+        const double be = B[size_t(e) * 25 + r * 5 + cc];
+        double s = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) s += __shfl(be, r * 5 + mm, 32) * invU[size_t(k) * 25 + mm * 5 + cc];
+        if (act) B[size_t(e) * 25 + lane] = s;
+        for (int q = L.pptr[e]; q < L.pptr[e + 1]; ++q) {
+          const size_t t = size_t(L.pt[q]) * 25, e2 = size_t(L.pe2[q]) * 25;
+          double acc = 0.0;
+#pragma unroll
+          for (int mm = 0; mm < 5; ++mm) acc += __shfl(s, r * 5 + mm, 32) * B[e2 + mm * 5 + cc];
+          if (act) B[t + lane] -= acc;
+        }
+      }
+      __threadfence_block();   // the row's block updates by the other lanes are visible to lane 0
+      if (lane == 0) {
+        double a[5][10];
+        for (int x = 0; x < 5; ++x)
+          for (int y = 0; y < 5; ++y) {
+            a[x][y] = B[size_t(d) * 25 + x * 5 + y];
+            a[x][5 + y] = x == y ? 1.0 : 0.0;
+          }
+        for (int col = 0; col < 5; ++col) {
+          int piv = col;
+          double best = fabs(a[col][col]);
+          for (int x = col + 1; x < 5; ++x)
+            if (fabs(a[x][col]) > best) {
+              best = fabs(a[x][col]);
+              piv = x;
+            }
+          if (piv != col)
+            for (int y = 0; y < 10; ++y) {
+              double t = a[col][y];
+              a[col][y] = a[piv][y];
+              a[piv][y] = t;
+            }
+          const double inv = 1.0 / a[col][col];
+          for (int y = 0; y < 10; ++y) a[col][y] *= inv;
+          for (int x = 0; x < 5; ++x)
+            if (x != col) {
+              const double fct = a[x][col];
+              for (int y = 0; y < 10; ++y) a[x][y] -= fct * a[col][y];
+            }
+        }
+        for (int x = 0; x < 5; ++x)
+          for (int y = 0; y < 5; ++y) invU[size_t(i) * 25 + x * 5 + y] = a[x][5 + y];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// x (SoA [5][n], original order) -> y ([n][5], permuted order).
+__global__ void k_perm_in(int n, const int* perm, const double* b, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[i];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) y[size_t(i) * 5 + k] = b[size_t(k) * n + o];
+}
+__global__ void k_perm_out(int n, const int* perm, const double* y, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[i];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[size_t(k) * n + o] = y[size_t(i) * 5 + k];
+}
+
+// Forward (unit lower) and backward (upper, then invU) sweeps over the levels,
+// in place on y; eight lanes per row, lane k < 5 owning component k
+// (aero2d/bilu.py:148-166).
+__global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B, const double* invU,
+                                                       double* y) {
+  const int grp = threadIdx.x >> 3, lane = threadIdx.x & 7, ngrp = blockDim.x >> 3;
+  const int k = lane < 5 ? lane : 4;
+  for (int l = 0; l < L.nlev; ++l) {
+    for (int p = L.lptr[l] + grp; p < L.lptr[l + 1]; p += ngrp) {
+      const int i = L.lrows[p];
+      double acc = 0.0;
+      for (int e = L.rstart[i]; e < L.rdiag[i]; ++e) {
+        const double* b = B + size_t(e) * 25 + k * 5;
+        const double* yc = y + size_t(L.col[e]) * 5;
+        double s = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) s += b[mm] * yc[mm];
+        acc += s;
+      }
+      if (lane < 5) y[size_t(i) * 5 + k] -= acc;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < L.nlevu; ++l) {
+    for (int p = L.uptr[l] + grp; p < L.uptr[l + 1]; p += ngrp) {
+      const int i = L.urows[p];
+      double acc = 0.0;
+      for (int e = L.rdiag[i] + 1; e < L.rend[i]; ++e) {
+        const double* b = B + size_t(e) * 25 + k * 5;
+        const double* xc = y + size_t(L.col[e]) * 5;
+        double s = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) s += b[mm] * xc[mm];
+        acc += s;
+      }
+      const double t = y[size_t(i) * 5 + k] - acc;
+      double xi = 0.0;
+#pragma unroll
+      for (int mm = 0; mm < 5; ++mm) xi += invU[size_t(i) * 25 + k * 5 + mm] * __shfl(t, mm, 8);
+      if (lane < 5) y[size_t(i) * 5 + k] = xi;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- vector kernels -----------------------------------------------------------------
+constexpr int VCHUNK = 2048;
+
+// part[j][b] = sum over block b's chunk of V_j . w, for j < nvec.
+__global__ void k_mdot(int N, const double* Vb, size_t ldv, int nvec, const double* w, double* part) {
+  __shared__ double sh[TPB];
+  const int j = blockIdx.y;
+  const double* v = Vb + size_t(j) * ldv;
+  const int b0 = blockIdx.x * VCHUNK;
+  const int b1 = min(N, b0 + VCHUNK);
+  double s = 0.0;
+  for (int i = b0 + threadIdx.x; i < b1; i += TPB) s += v[i] * w[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int t = TPB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) sh[threadIdx.x] += sh[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[size_t(j) * gridDim.x + blockIdx.x] = sh[0];
+  (void)nvec;
+}
+
+// w -= sum_j h_j V_j (the sum first, in order of j).
+__global__ void k_gs_update(int N, const double* Vb, size_t ldv, int nvec, const double* h, double* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  for (int j = 0; j < nvec; ++j) s += h[j] * Vb[size_t(j) * ldv + i];
+  w[i] -= s;
+}
+
+// out = a * x (scale) or out = x + a*y style helpers.
+__global__ void k_scale(int N, double a, const double* x, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[i] = a * x[i];
+}
+__global__ void k_negate(int N, const double* x, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[i] = -x[i];
+}
+__global__ void k_add(int N, const double* x, const double* y, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[i] = x[i] + y[i];
+}
+
+// Relative decrease of rho and p under the full update, per cell
+// (aero2d/rans.py:1822-1826); block partial max and count of cells >= 0.2.
+__global__ void k_step_rel(int nc, const double* Q, const double* dq, double* reli, double* pmax,
+                           double* pcnt) {
+  __shared__ double smax[TPB], scnt[TPB];
+  const int c = blockIdx.x * TPB + threadIdx.x;
+  double rl = -INFINITY, cnt = 0.0;
+  if (c < nc) {
+    double q[5], qn[5], w[5], wn[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      q[k] = Q[k * nc + c];
+      qn[k] = q[k] + dq[k * nc + c];
+    }
+    prim(q, w);
+    prim(qn, wn);
+    const double a = -(wn[0] - w[0]) / w[0], b = -(wn[3] - w[3]) / w[3];
+    rl = a > b ? a : b;
+    if (a != a || b != b) rl = NAN;
+    reli[c] = rl;
+    cnt = rl >= 0.2 ? 1.0 : 0.0;
+  }
+  smax[threadIdx.x] = rl;
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int t = TPB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) {
+      const double o = smax[threadIdx.x + t], s = smax[threadIdx.x];
+      smax[threadIdx.x] = (o != o || s != s) ? NAN : (o > s ? o : s);
+      scnt[threadIdx.x] += scnt[threadIdx.x + t];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmax[blockIdx.x] = smax[0];
+    pcnt[blockIdx.x] = scnt[0];
+  }
+}
+
+__global__ void k_max_rows(int nparts, const double* pmax, const double* pcnt, double* out) {
+  if (threadIdx.x != 0) return;
+  double m = -INFINITY, c = 0.0;
+  for (int p = 0; p < nparts; ++p) {
+    const double x = pmax[p];
+    m = (x != x || m != m) ? NAN : (x > m ? x : m);
+    c += pcnt[p];
+  }
+  out[0] = m;
+  out[1] = c;
+}
+
+// Q += lam dq (whole step) or Q += lam_c dq with lam_c = 1 below 0.2 and
+// 0.2 / rel above (only the few cells that ask for more).
+__global__ void k_apply(int nc, double* Q, const double* dq, const double* reli, double lam, int per_cell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double s = lam;
+  if (per_cell) {
+    const double r = reli[c];
+    s = r < 0.2 ? 1.0 : 0.2 / (r > 1e-300 ? r : 1e-300);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Q[k * nc + c] += s * dq[k * nc + c];
+}
+
+// Wall forces by faces (aero2d/rans.py:1251-1296): per face the pressure and
+// friction forces and the moment arm, summed in order by one block.
+__global__ void k_forces(MeshDev m, Params P, const double* Q, const double* g, double ca, double sa,
+                         double* out, double* cp, double* cfo) {
+  __shared__ double sh[6][TPB];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const double q = 0.5 * P.mach * P.mach;
+  const int nw = m.nfw;
+  for (int f = threadIdx.x; f < nw; f += TPB) {
+    const int c = m.cw[f];
+    double w[5];
+    load_w<double>(Q, nullptr, m.nc, c, w);
+    const double nx = m.wg[f], ny = m.wg[nw + f], A = m.wg[2 * nw + f];
+    const double rx = m.wg[3 * nw + f], ry = m.wg[4 * nw + f], dn = m.wg[5 * nw + f];
+    const double pw = w[3] + (g[6 * m.nc + c] * rx + g[7 * m.nc + c] * ry);
+    const double T = GAM * w[3] / w[0];
+    const double mu = P.mu_inf * T * sqrt(T) * (1.0 + SUTH) / (T + SUTH);
+    const double ut = w[1] * (-ny) + w[2] * nx;
+    const double tau = mu * ut / dn;
+    const double fxp = (pw - 1.0 / GAM) * nx * A, fyp = (pw - 1.0 / GAM) * ny * A;
+    const double fxv = tau * (-ny) * A, fyv = tau * nx * A;
+    const double xm = m.wg[7 * nw + f] - P.xref_x, ym = m.wg[8 * nw + f] - P.xref_y;
+    acc[0] += -fxp * sa + fyp * ca;
+    acc[1] += fxp * ca + fyp * sa;
+    acc[2] += -fxv * sa + fyv * ca;
+    acc[3] += fxv * ca + fyv * sa;
+    acc[4] += xm * (fyp + fyv) - ym * (fxp + fxv);
+    if (cp) cp[f] = (pw - 1.0 / GAM) / q;
+    if (cfo) cfo[f] = tau / q;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int t = TPB / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double qc = q * P.chord;
+    const double lp = sh[0][0] / qc, dp = sh[1][0] / qc, lv = sh[2][0] / qc, dv = sh[3][0] / qc;
+    out[0] = lp + lv;
+    out[1] = dp + dv;
+    out[2] = dp;
+    out[3] = dv;
+    out[4] = -sh[4][0] / (qc * P.chord);
+  }
+}
+
+__global__ void k_count_nonfinite(int n, const double* x, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !isfinite(x[i])) atomicAdd(out, 1);
+}
+
+// ---- host side ----------------------------------------------------------------------
+template <class T>
+T* dalloc(size_t n) {
+  T* p = nullptr;
+  if (n == 0) n = 1;
+  HC(hipMalloc(&p, n * sizeof(T)));
+  return p;
+}
+template <class T>
+T* dupload(const std::vector<T>& v) {
+  T* p = dalloc<T>(v.size());
+  if (!v.empty()) HC(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+struct Solver {
+  MeshDev m;
+  Params P;
+  int order = 2;
+  hipStream_t st = nullptr;
+  std::vector<void*> owned;
+  // state and work vectors [5][nc]
+  double *Q, *Qprev, *Qbest, *R, *dq, *dq2, *tmp, *tmp2, *bvec;
+  double *g, *gt, *phi, *phit, *phi_frozen, *zero5, *F, *Ft, *wgf;
+  double *dt, *vdt, *reli, *part, *small, *cpbuf;
+  int nparts_cell;
+  bool frozen = false;
+  std::vector<unsigned char> in_zone;   // host copy of the trailing-edge limiter zone
+  // Jacobian and ILU
+  double *J = nullptr, *JW = nullptr, *JF = nullptr;
+  double *B = nullptr, *invU = nullptr, *y = nullptr;
+  int *posD = nullptr, *posLR = nullptr, *posRL = nullptr;
+  IluDev ilu{};
+  bool has_ilu = false;
+  int* nonfinite = nullptr;
+  // Krylov basis
+  double* Vb = nullptr;
+  int vcap = 0;
+  double* hdev = nullptr;
+  int N() const { return 5 * m.nc; }
+
+  template <class T>
+  T* own(T* p) {
+    owned.push_back((void*)p);
+    return p;
+  }
+  ~Solver() {
+    for (void* p : owned) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+std::map<int64_t, std::unique_ptr<Solver>> g_solvers;
+int64_t g_next = 1;
+
+Solver& get(int64_t h) {
+  auto it = g_solvers.find(h);
+  if (it == g_solvers.end()) fail(-3, "unknown RANS solver handle");
+  require_device();
+  return *it->second;
+}
+
+inline int nblk(int n, int t = TPB) { return (n + t - 1) / t; }
+
+// Residual (or, with V, its tangent along V) of the state Q into Rv/Rt.
+// Limiter: none at first order or with the zone off; frozen when set; else
+// evaluated (and differentiated) from Q.
+void eval(Solver& s, int order, const double* V, double* Rv, double* Rt, const double* vdt, double ct_inv) {
+  const MeshDev& m = s.m;
+  const bool lim = order >= 2 && s.P.limiter_k > 0 && s.P.limiter_zone != 0;
+  const double* phi = nullptr;
+  const double* phit = nullptr;
+  if (V) {
+    hipLaunchKernelGGL(k_grad<Dual<double>>, nblk(m.nc), TPB, 0, s.st, m, s.Q, V, s.g, s.gt);
+    if (lim && s.frozen) {
+      phi = s.phi_frozen;
+      phit = s.zero5;
+    } else if (lim) {
+      hipLaunchKernelGGL(k_limit<Dual<double>>, nblk(m.nc), TPB, 0, s.st, m, s.P, s.Q, V, s.g, s.gt, s.phi,
+                         s.phit);
+      phi = s.phi;
+      phit = s.phit;
+    }
+    hipLaunchKernelGGL(k_face<Dual<double>>, nblk(m.nfi), TPB, 0, s.st, m, s.P, order, s.Q, V, s.g, s.gt, phi,
+                       phit, s.F, s.Ft);
+    hipLaunchKernelGGL(k_cell<Dual<double>>, nblk(m.nc), TPB, 0, s.st, m, s.P, order, s.Q, V, s.g, s.gt, phi,
+                       phit, s.F, s.Ft, s.wgf, Rv, Rt, vdt, ct_inv);
+  } else {
+    hipLaunchKernelGGL(k_grad<double>, nblk(m.nc), TPB, 0, s.st, m, s.Q, nullptr, s.g, nullptr);
+    if (lim && s.frozen) {
+      phi = s.phi_frozen;
+    } else if (lim) {
+      hipLaunchKernelGGL(k_limit<double>, nblk(m.nc), TPB, 0, s.st, m, s.P, s.Q, nullptr, s.g, nullptr, s.phi,
+                         nullptr);
+      phi = s.phi;
+    }
+    hipLaunchKernelGGL(k_face<double>, nblk(m.nfi), TPB, 0, s.st, m, s.P, order, s.Q, nullptr, s.g, nullptr,
+                       phi, nullptr, s.F, nullptr);
+    hipLaunchKernelGGL(k_cell<double>, nblk(m.nc), TPB, 0, s.st, m, s.P, order, s.Q, nullptr, s.g, nullptr,
+                       phi, nullptr, s.F, nullptr, s.wgf, Rv, nullptr, nullptr, 1.0);
+  }
+  HC(hipGetLastError());
+}
+
+void far_state(Solver& s, const double* winf, double gam) {
+  hipLaunchKernelGGL(k_far_state, nblk(s.m.nff), TPB, 0, s.st, s.m, s.P, winf[0], winf[1], winf[2], winf[3],
+                     winf[4], gam, s.wgf);
+}
+
+void rnorm(Solver& s, const double* R, double* out5) {
+  hipLaunchKernelGGL(k_rnorm_part, s.nparts_cell, TPB, 0, s.st, s.m.nc, R, s.m.cg, s.part);
+  hipLaunchKernelGGL(k_sum_rows, 1, 64, 0, s.st, 5, s.nparts_cell, s.part, s.small);
+  double h[5];
+  HC(hipMemcpyAsync(h, s.small, 5 * sizeof(double), hipMemcpyDeviceToHost, s.st));
+  HC(hipStreamSynchronize(s.st));
+  for (int k = 0; k < 5; ++k) out5[k] = std::sqrt(h[k] / s.m.nc);
+}
+
+// dot products V_j . w for j < nvec into s.hdev[off .. off+nvec)
+void mdot(Solver& s, const double* Vb, size_t ldv, int nvec, const double* w, int off) {
+  const int N = s.N();
+  const int nb = (N + VCHUNK - 1) / VCHUNK;
+  hipLaunchKernelGGL(k_mdot, dim3(nb, nvec), TPB, 0, s.st, N, Vb, ldv, nvec, w, s.part);
+  hipLaunchKernelGGL(k_sum_rows, 1, 64 * ((nvec + 63) / 64), 0, s.st, nvec, nb, s.part, s.hdev + off);
+}
+
+void ilu_solve(Solver& s, const double* b, double* x) {
+  const int n = s.m.nc;
+  hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s.st, n, s.ilu.perm, b, s.y);
+  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, 0, s.st, s.ilu, s.B, s.invU, s.y);
+  hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s.st, n, s.ilu.perm, s.y, x);
+}
+
+void matvec(Solver& s, int order, const double* v, double* out, double ct_inv) {
+  eval(s, order, v, nullptr, out, s.vdt, ct_inv);
+}
+
+// Preconditioned GMRES without restarts (aero2d/krylov.py:14-87): left
+// preconditioning minimises |M^-1 (b - A x)|, right solves A M^-1 u = b and
+// returns x = M^-1 u. Arnoldi with CGS2; Givens rotations and the triangular
+// solve on the host (one small copy per step). Returns the iterations done;
+// info = 0 when the relative estimate reached rtol.
+int gmres(Solver& s, int order, double ct_inv, const double* b, double* x, double rtol, int maxiter, bool right,
+          int* info_out) {
+  const int N = s.N();
+  const int m = maxiter;
+  if (m + 1 > s.vcap) {
+    if (s.Vb) (void)hipFree(s.Vb);
+    s.Vb = dalloc<double>(size_t(m + 1) * N);
+    s.vcap = m + 1;
+  }
+  const size_t ld = size_t(N);
+  auto V = [&](int j) { return s.Vb + size_t(j) * ld; };
+  double* w = s.tmp;
+  // r = P b (left) or b (right); x0 = 0
+  if (right) HC(hipMemcpyAsync(V(0), b, N * sizeof(double), hipMemcpyDeviceToDevice, s.st));
+  else ilu_solve(s, b, V(0));
+  mdot(s, V(0), ld, 1, V(0), 0);
+  double bb;
+  HC(hipMemcpyAsync(&bb, s.hdev, sizeof(double), hipMemcpyDeviceToHost, s.st));
+  HC(hipStreamSynchronize(s.st));
+  const double beta = std::sqrt(bb);
+  HC(hipMemsetAsync(x, 0, N * sizeof(double), s.st));
+  if (beta == 0.0 || !std::isfinite(beta)) {
+    *info_out = 0;
+    return 0;
+  }
+  hipLaunchKernelGGL(k_scale, nblk(N), TPB, 0, s.st, N, 1.0 / beta, V(0), V(0));
+  std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1, 0.0), hbuf(2 * (m + 1) + 1);
+  auto Hij = [&](int i, int j) -> double& { return H[size_t(i) * m + j]; };
+  gv[0] = beta;
+  int k_done = 0, info = m;
+  for (int j = 0; j < m; ++j) {
+    if (right) {
+      ilu_solve(s, V(j), s.tmp2);
+      matvec(s, order, s.tmp2, w, ct_inv);
+    } else {
+      matvec(s, order, V(j), s.tmp2, ct_inv);
+      ilu_solve(s, s.tmp2, w);
+    }
+    mdot(s, s.Vb, ld, j + 1, w, 0);
+    hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s.st, N, s.Vb, ld, j + 1, s.hdev, w);
+    mdot(s, s.Vb, ld, j + 1, w, j + 1);
+    hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s.st, N, s.Vb, ld, j + 1, s.hdev + j + 1, w);
+    mdot(s, w, ld, 1, w, 2 * (j + 1));
+    HC(hipMemcpyAsync(hbuf.data(), s.hdev, (2 * (j + 1) + 1) * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipStreamSynchronize(s.st));
+    const double hn = std::sqrt(hbuf[2 * (j + 1)]);
+    for (int i = 0; i <= j; ++i) Hij(i, j) = hbuf[i] + hbuf[j + 1 + i];
+    Hij(j + 1, j) = hn;
+    for (int i = 0; i < j; ++i) {
+      const double t = cs[i] * Hij(i, j) + sn[i] * Hij(i + 1, j);
+      Hij(i + 1, j) = -sn[i] * Hij(i, j) + cs[i] * Hij(i + 1, j);
+      Hij(i, j) = t;
+    }
+    const double d = std::hypot(Hij(j, j), Hij(j + 1, j));
+    if (d == 0.0) {
+      k_done = j;
+      break;
+    }
+    cs[j] = Hij(j, j) / d;
+    sn[j] = Hij(j + 1, j) / d;
+    Hij(j, j) = d;
+    Hij(j + 1, j) = 0.0;
+    gv[j + 1] = -sn[j] * gv[j];
+    gv[j] = cs[j] * gv[j];
+    k_done = j + 1;
+    const double rel = std::fabs(gv[j + 1]) / beta;
+    if (!std::isfinite(rel)) break;
+    if (rel <= rtol || hn == 0.0) {
+      info = 0;
+      break;
+    }
+    hipLaunchKernelGGL(k_scale, nblk(N), TPB, 0, s.st, N, 1.0 / hn, w, V(j + 1));
+  }
+  *info_out = info;
+  const int k = k_done;
+  if (k == 0) return 0;
+  std::vector<double> yv(k);
+  for (int i = k - 1; i >= 0; --i) {
+    double t = gv[i];
+    for (int c = i + 1; c < k; ++c) t -= Hij(i, c) * yv[c];
+    yv[i] = t / Hij(i, i);
+  }
+  // x = sum_j y_j V_j  (through k_gs_update with -y: x = 0 - sum(-y_j V_j))
+  for (auto& v : yv) v = -v;
+  HC(hipMemcpyAsync(s.hdev, yv.data(), k * sizeof(double), hipMemcpyHostToDevice, s.st));
+  if (right) {
+    HC(hipMemsetAsync(s.tmp2, 0, N * sizeof(double), s.st));
+    hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s.st, N, s.Vb, ld, k, s.hdev, s.tmp2);
+    ilu_solve(s, s.tmp2, x);
+  } else {
+    hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s.st, N, s.Vb, ld, k, s.hdev, x);
+  }
+  HC(hipStreamSynchronize(s.st));
+  return k;
+}
+
+double norm2(Solver& s, const double* v) {
+  mdot(s, v, size_t(s.N()), 1, v, 0);
+  double h;
+  HC(hipMemcpyAsync(&h, s.hdev, sizeof(double), hipMemcpyDeviceToHost, s.st));
+  HC(hipStreamSynchronize(s.st));
+  return std::sqrt(h);
+}
+
+// |A dq + R| / |R| with A the Newton matrix at the current step.
+double lin_residual(Solver& s, int order, double ct_inv, const double* dq, double rnorm) {
+  matvec(s, order, dq, s.tmp2, ct_inv);
+  hipLaunchKernelGGL(k_add, nblk(s.N()), TPB, 0, s.st, s.N(), s.tmp2, s.R, s.tmp2);
+  return norm2(s, s.tmp2) / std::max(rnorm, 1e-300);
+}
+
+void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Dout, double* OffOut) {
+  if (!Dout && !s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
+  const MeshDev& m = s.m;
+  double* B = Dout ? nullptr : s.B;
+  far_state(s, winf, gam);
+  hipLaunchKernelGGL(k_grad<double>, nblk(m.nc), TPB, 0, s.st, m, s.Q, nullptr, s.g, nullptr);
+  if (B) HC(hipMemsetAsync(B, 0, size_t(s.ilu.nnz) * 25 * sizeof(double), s.st));
+  hipLaunchKernelGGL(k_jac_int, nblk(m.nfi), TPB, 0, s.st, m, s.P, s.Q, s.J, B, s.posLR, s.posRL);
+  hipLaunchKernelGGL(k_jac_wall, nblk(m.nfw), TPB, 0, s.st, m, s.P, s.Q, s.JW);
+  hipLaunchKernelGGL(k_jac_far, nblk(m.nff), TPB, 0, s.st, m, s.P, s.Q, s.wgf, s.JF);
+  hipLaunchKernelGGL(k_jac_diag, nblk(m.nc), TPB, 0, s.st, m, s.P, s.Q, s.g, s.J, s.JW, s.JF,
+                     Dout ? nullptr : s.vdt, ct_inv, B, s.posD, s.tmp2);
+  HC(hipGetLastError());
+  if (Dout) {
+    std::vector<double> Jh(size_t(m.nfi) * 50);
+    HC(hipMemcpyAsync(Dout, s.tmp2, size_t(m.nc) * 25 * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipMemcpyAsync(Jh.data(), s.J, Jh.size() * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipStreamSynchronize(s.st));
+    for (int f = 0; f < m.nfi; ++f)
+      for (int t = 0; t < 25; ++t) {
+        OffOut[size_t(f) * 25 + t] = Jh[size_t(f) * 50 + 25 + t];
+        OffOut[size_t(m.nfi + f) * 25 + t] = -Jh[size_t(f) * 50 + t];
+      }
+    return;
+  }
+  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, 0, s.st, s.ilu, s.B, s.invU);
+  HC(hipGetLastError());
+}
+
+int count_nonfinite(Solver& s, const double* x, int n) {
+  HC(hipMemsetAsync(s.nonfinite, 0, sizeof(int), s.st));
+  hipLaunchKernelGGL(k_count_nonfinite, nblk(n), TPB, 0, s.st, n, x, s.nonfinite);
+  int h = 0;
+  HC(hipMemcpyAsync(&h, s.nonfinite, sizeof(int), hipMemcpyDeviceToHost, s.st));
+  HC(hipStreamSynchronize(s.st));
+  return h;
+}
+
+void to_soa(int nc, const double* aos, std::vector<double>& soa) {
+  soa.resize(size_t(nc) * 5);
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 5; ++k) soa[size_t(k) * nc + c] = aos[size_t(c) * 5 + k];
+}
+void to_aos(int nc, const std::vector<double>& soa, double* aos) {
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 5; ++k) aos[size_t(c) * 5 + k] = soa[size_t(k) * nc + c];
+}
+
+void upload_cells(Solver& s, const double* aos, double* dst) {
+  std::vector<double> soa;
+  to_soa(s.m.nc, aos, soa);
+  HC(hipMemcpyAsync(dst, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, s.st));
+  HC(hipStreamSynchronize(s.st));
+}
+void download_cells(Solver& s, const double* src, double* aos) {
+  std::vector<double> soa(size_t(s.m.nc) * 5);
+  HC(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s.st));
+  HC(hipStreamSynchronize(s.st));
+  to_aos(s.m.nc, soa, aos);
+}
+
+}  // namespace
+}  // namespace rans
+}  // namespace a2d
+
+using namespace a2d::rans;
+
+extern "C" {
+
+int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const int* face_r,
+                    const double* face_geom, const int* wall_cell, const double* wall_geom, const int* far_cell,
+                    const double* far_geom, const double* cell_geom, const a2d_rans_params* prm,
+                    int64_t* handle_out) {
+  return guarded([&] {
+    a2d::require_device();
+    if (nc < 1 || nfi < 1 || nfw < 1 || nff < 1) fail(-6, "a RANS mesh needs cells, interior, wall and far faces");
+    if (!prm || !handle_out) fail(-6, "null argument");
+    if (prm->order != 1 && prm->order != 2) fail(-6, "order must be 1 or 2");
+    if (prm->limiter_zone < 0 || prm->limiter_zone > 2) fail(-6, "limiter_zone must be 0, 1 or 2");
+    if (!(prm->mach > 0.0 && prm->mach < 1.0)) fail(-6, "mach must be in (0, 1)");
+    if (int64_t(nc) * 5 * 61 > (int64_t(1) << 31)) fail(-6, "mesh too large for 32-bit vector indexing");
+    for (int f = 0; f < nfi; ++f)
+      if (face_l[f] < 0 || face_l[f] >= nc || face_r[f] < 0 || face_r[f] >= nc || face_l[f] == face_r[f])
+        fail(-6, "interior face with a bad cell index");
+    for (int f = 0; f < nfw; ++f)
+      if (wall_cell[f] < 0 || wall_cell[f] >= nc) fail(-6, "wall face with a bad cell index");
+    for (int f = 0; f < nff; ++f)
+      if (far_cell[f] < 0 || far_cell[f] >= nc) fail(-6, "far-field face with a bad cell index");
+    auto s = std::make_unique<Solver>();
+    HC(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    MeshDev& m = s->m;
+    m.nc = nc;
+    m.nfi = nfi;
+    m.nfw = nfw;
+    m.nff = nff;
+    auto soa = [](const double* aos, int n, int w) {
+      std::vector<double> v(size_t(n) * w);
+      for (int i = 0; i < n; ++i)
+        for (int k = 0; k < w; ++k) v[size_t(k) * n + i] = aos[size_t(i) * w + k];
+      return v;
+    };
+    m.iL = s->own(dupload(std::vector<int>(face_l, face_l + nfi)));
+    m.iR = s->own(dupload(std::vector<int>(face_r, face_r + nfi)));
+    m.fg = s->own(dupload(soa(face_geom, nfi, NFG)));
+    m.cw = s->own(dupload(std::vector<int>(wall_cell, wall_cell + nfw)));
+    m.wg = s->own(dupload(soa(wall_geom, nfw, NWG)));
+    m.cf = s->own(dupload(std::vector<int>(far_cell, far_cell + nff)));
+    m.ffg = s->own(dupload(soa(far_geom, nff, NFF)));
+    m.cg = s->own(dupload(soa(cell_geom, nc, NCG)));
+    // cell -> faces lists, in face order (interior, wall, far)
+    std::vector<int> cnt(nc + 1, 0), wc(nc + 1, 0), fc(nc + 1, 0);
+    for (int f = 0; f < nfi; ++f) {
+      cnt[face_l[f] + 1]++;
+      cnt[face_r[f] + 1]++;
+    }
+    for (int f = 0; f < nfw; ++f) wc[wall_cell[f] + 1]++;
+    for (int f = 0; f < nff; ++f) fc[far_cell[f] + 1]++;
+    for (int c = 0; c < nc; ++c) {
+      cnt[c + 1] += cnt[c];
+      wc[c + 1] += wc[c];
+      fc[c + 1] += fc[c];
+    }
+    std::vector<int> cf(cnt[nc]), wf(wc[nc]), ff(fc[nc]);
+    {
+      std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+      for (int f = 0; f < nfi; ++f) {
+        cf[pos[face_l[f]]++] = f;
+        cf[pos[face_r[f]]++] = ~f;
+      }
+      std::vector<int> pw(wc.begin(), wc.end() - 1), pf(fc.begin(), fc.end() - 1);
+      for (int f = 0; f < nfw; ++f) wf[pw[wall_cell[f]]++] = f;
+      for (int f = 0; f < nff; ++f) ff[pf[far_cell[f]]++] = f;
+    }
+    s->in_zone.assign(nc, 0);
+    for (int c = 0; c < nc; ++c) s->in_zone[c] = cell_geom[size_t(c) * NCG + 5] != 0.0;
+    std::vector<unsigned char> farc(nc, 0);
+    for (int f = 0; f < nff; ++f) farc[far_cell[f]] = 1;
+    m.cptr = s->own(dupload(cnt));
+    m.cfac = s->own(dupload(cf));
+    m.wptr = s->own(dupload(wc));
+    m.wfac = s->own(dupload(wf));
+    m.fptr = s->own(dupload(fc));
+    m.ffac = s->own(dupload(ff));
+    m.far_cell = s->own(dupload(farc));
+    Params& P = s->P;
+    P.mach = prm->mach;
+    P.mu_inf = prm->mach * prm->chord / prm->re;
+    P.nu_inf = P.mu_inf;
+    P.chi_inf = prm->turbulent ? prm->chi_inf : 0.0;
+    P.limiter_k = prm->limiter_k;
+    P.xref_x = prm->xref_x;
+    P.xref_y = prm->xref_y;
+    P.zmin = prm->zmin;
+    P.chord = prm->chord;
+    P.turbulent = prm->turbulent;
+    P.limiter_zone = prm->limiter_zone;
+    s->order = prm->order;
+    const size_t n5 = size_t(nc) * 5;
+    auto& S = *s;
+    S.Q = S.own(dalloc<double>(n5));
+    S.Qprev = S.own(dalloc<double>(n5));
+    S.Qbest = S.own(dalloc<double>(n5));
+    S.R = S.own(dalloc<double>(n5));
+    S.dq = S.own(dalloc<double>(n5));
+    S.tmp = S.own(dalloc<double>(n5));
+    S.tmp2 = S.own(dalloc<double>(std::max(n5, size_t(nc) * 25)));
+    S.bvec = S.own(dalloc<double>(n5));
+    S.g = S.own(dalloc<double>(2 * n5));
+    S.gt = S.own(dalloc<double>(2 * n5));
+    S.phi = S.own(dalloc<double>(n5));
+    S.phit = S.own(dalloc<double>(n5));
+    S.phi_frozen = S.own(dalloc<double>(n5));
+    S.zero5 = S.own(dalloc<double>(n5));
+    HC(hipMemset(S.zero5, 0, n5 * sizeof(double)));
+    S.F = S.own(dalloc<double>(size_t(nfi) * 5));
+    S.Ft = S.own(dalloc<double>(size_t(nfi) * 5));
+    S.wgf = S.own(dalloc<double>(size_t(nff) * 5));
+    S.dt = S.own(dalloc<double>(nc));
+    S.vdt = S.own(dalloc<double>(nc));
+    S.reli = S.own(dalloc<double>(nc));
+    S.cpbuf = S.own(dalloc<double>(2 * size_t(nfw)));
+    S.nparts_cell = nblk(nc);
+    const int nbv = (5 * nc + VCHUNK - 1) / VCHUNK;
+    S.part = S.own(dalloc<double>(std::max(size_t(5) * S.nparts_cell, size_t(GMRES_MAX + 1) * nbv) +
+                                  2 * S.nparts_cell));
+    S.small = S.own(dalloc<double>(64));
+    S.hdev = S.own(dalloc<double>(2 * (GMRES_MAX + 1) + 8));
+    S.dq2 = S.own(dalloc<double>(n5));
+    S.J = S.own(dalloc<double>(size_t(nfi) * 50));
+    S.JW = S.own(dalloc<double>(size_t(nfw) * 25));
+    S.JF = S.own(dalloc<double>(size_t(nff) * 25));
+    S.nonfinite = S.own(dalloc<int>(1));
+    const int64_t h = g_next++;
+    g_solvers[h] = std::move(s);
+    *handle_out = h;
+  });
+}
+
+int a2d_rans_destroy(int64_t h) {
+  return guarded([&] {
+    get(h);
+    auto& s = *g_solvers[h];
+    if (s.Vb) (void)hipFree(s.Vb);
+    s.Vb = nullptr;
+    g_solvers.erase(h);
+  });
+}
+
+int a2d_rans_set_state(int64_t h, const double* Q) {
+  return guarded([&] {
+    Solver& s = get(h);
+    upload_cells(s, Q, s.Q);
+  });
+}
+
+int a2d_rans_get_state(int64_t h, double* Q) {
+  return guarded([&] {
+    Solver& s = get(h);
+    download_cells(s, s.Q, Q);
+  });
+}
+
+int a2d_rans_freeze_limiter(int64_t h, const double* phi) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (!phi) {
+      s.frozen = false;
+      return;
+    }
+    // outside the limiter zone the frozen limiter is 1, as for a live one
+    // (aero2d/rans.py:868-871)
+    std::vector<double> p(phi, phi + size_t(s.m.nc) * 5);
+    if (s.P.limiter_zone == 1)
+      for (int c = 0; c < s.m.nc; ++c)
+        if (!s.in_zone[c])
+          for (int k = 0; k < 5; ++k) p[size_t(c) * 5 + k] = 1.0;
+    upload_cells(s, p.data(), s.phi_frozen);
+    s.frozen = true;
+  });
+}
+
+int a2d_rans_limiter(int64_t h, double* phi) {
+  return guarded([&] {
+    Solver& s = get(h);
+    hipLaunchKernelGGL(k_grad<double>, nblk(s.m.nc), TPB, 0, s.st, s.m, s.Q, nullptr, s.g, nullptr);
+    Params P = s.P;
+    P.limiter_zone = 2;
+    hipLaunchKernelGGL(k_limit<double>, nblk(s.m.nc), TPB, 0, s.st, s.m, P, s.Q, nullptr, s.g, nullptr, s.phi,
+                       nullptr);
+    HC(hipGetLastError());
+    download_cells(s, s.phi, phi);
+  });
+}
+
+int a2d_rans_residual(int64_t h, int order, const double* winf, double gam, double* R, double* rn5) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
+    far_state(s, winf, gam);
+    eval(s, order, nullptr, s.R, nullptr, nullptr, 1.0);
+    if (rn5) rnorm(s, s.R, rn5);
+    if (R) download_cells(s, s.R, R);
+    HC(hipStreamSynchronize(s.st));
+  });
+}
+
+int a2d_rans_jvp(int64_t h, int order, const double* winf, double gam, const double* v, double* out) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
+    upload_cells(s, v, s.bvec);
+    far_state(s, winf, gam);
+    eval(s, order, s.bvec, s.tmp, s.tmp, nullptr, 1.0);
+    download_cells(s, s.tmp, out);
+  });
+}
+
+int a2d_rans_local_dt(int64_t h, double cfl, double* dt) {
+  return guarded([&] {
+    Solver& s = get(h);
+    hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
+    HC(hipGetLastError());
+    HC(hipMemcpyAsync(dt, s.dt, s.m.nc * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipStreamSynchronize(s.st));
+  });
+}
+
+int a2d_rans_forces(int64_t h, double alpha_deg, double* out5, double* cp, double* cf) {
+  return guarded([&] {
+    Solver& s = get(h);
+    hipLaunchKernelGGL(k_grad<double>, nblk(s.m.nc), TPB, 0, s.st, s.m, s.Q, nullptr, s.g, nullptr);
+    const double a = alpha_deg * M_PI / 180.0;
+    double* dcp = cp ? s.cpbuf : nullptr;
+    double* dcf = cf ? s.cpbuf + s.m.nfw : nullptr;
+    hipLaunchKernelGGL(k_forces, 1, TPB, 0, s.st, s.m, s.P, s.Q, s.g, std::cos(a), std::sin(a), s.small, dcp,
+                       dcf);
+    HC(hipGetLastError());
+    HC(hipMemcpyAsync(out5, s.small, 5 * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    if (cp) HC(hipMemcpyAsync(cp, dcp, s.m.nfw * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    if (cf) HC(hipMemcpyAsync(cf, dcf, s.m.nfw * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipStreamSynchronize(s.st));
+  });
+}
+
+int a2d_rans_jac1(int64_t h, const double* winf, double gam, double* D, double* off) {
+  return guarded([&] {
+    Solver& s = get(h);
+    factor(s, winf, gam, 1.0, D, off);
+  });
+}
+
+int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, const int* cols, const int* pos_src) {
+  return guarded([&] {
+    Solver& s = get(h);
+    const int n = s.m.nc;
+    const int nfi = s.m.nfi;
+    if (nnz < n) fail(-6, "pattern smaller than its diagonal");
+    std::vector<int> rstart(n, 0), rend(n, 0), rdiag(n, -1);
+    for (int e = 0; e < nnz; ++e) {
+      if (rows[e] < 0 || rows[e] >= n || cols[e] < 0 || cols[e] >= n) fail(-6, "pattern index out of range");
+      if (e > 0 && (rows[e] < rows[e - 1] || (rows[e] == rows[e - 1] && cols[e] <= cols[e - 1])))
+        fail(-6, "pattern must be sorted by row, then column, without repeats");
+    }
+    {
+      int e = 0;
+      for (int i = 0; i < n; ++i) {
+        rstart[i] = e;
+        while (e < nnz && rows[e] == i) {
+          if (cols[e] == i) rdiag[i] = e;
+          ++e;
+        }
+        rend[i] = e;
+        if (rdiag[i] < 0) fail(-6, "pattern misses a diagonal block");
+      }
+    }
+    std::vector<int> permv(perm, perm + n), seen(n, 0);
+    for (int i = 0; i < n; ++i) {
+      if (permv[i] < 0 || permv[i] >= n || seen[permv[i]]) fail(-6, "perm is not a permutation");
+      seen[permv[i]] = 1;
+    }
+    for (int q = 0; q < n + 2 * nfi; ++q)
+      if (pos_src[q] < 0 || pos_src[q] >= nnz) fail(-6, "block position out of range");
+    // update pairs of every lower entry e = (i,k): (i,j) -= (i,k)(k,j) for the
+    // upper entries (k,j) whose (i,j) is in the pattern
+    std::vector<int> pptr(nnz + 1, 0), pt, pe2;
+    for (int i = 0; i < n; ++i) {
+      for (int e = rstart[i]; e < rdiag[i]; ++e) {
+        const int k = cols[e];
+        for (int e2 = rdiag[k] + 1; e2 < rend[k]; ++e2) {
+          const int j = cols[e2];
+          const int* b = cols + rstart[i];
+          const int* en = cols + rend[i];
+          const int* it = std::lower_bound(b, en, j);
+          if (it != en && *it == j) {
+            pt.push_back(int(it - cols));
+            pe2.push_back(e2);
+          }
+        }
+        pptr[e + 1] = int(pt.size());
+      }
+      for (int e = rdiag[i]; e < rend[i]; ++e) pptr[e + 1] = int(pt.size());
+    }
+    // levels: forward by the lower pattern, backward by the upper pattern
+    std::vector<int> lev(n, 0), levu(n, 0);
+    int nlev = 0, nlevu = 0;
+    for (int i = 0; i < n; ++i) {
+      int l = 0;
+      for (int e = rstart[i]; e < rdiag[i]; ++e) l = std::max(l, lev[cols[e]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      int l = 0;
+      for (int e = rdiag[i] + 1; e < rend[i]; ++e) l = std::max(l, levu[cols[e]] + 1);
+      levu[i] = l;
+      nlevu = std::max(nlevu, l + 1);
+    }
+    auto group = [&](const std::vector<int>& lv, int nl, std::vector<int>& ptr, std::vector<int>& rws) {
+      ptr.assign(nl + 1, 0);
+      for (int i = 0; i < n; ++i) ptr[lv[i] + 1]++;
+      for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+      rws.assign(n, 0);
+      std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+      for (int i = 0; i < n; ++i) rws[pos[lv[i]]++] = i;
+    };
+    std::vector<int> lptr, lrows, uptr, urows;
+    group(lev, nlev, lptr, lrows);
+    group(levu, nlevu, uptr, urows);
+    // release an earlier setup
+    if (s.has_ilu) fail(-6, "the ILU pattern is already set");
+    IluDev& L = s.ilu;
+    L.n = n;
+    L.nnz = nnz;
+    L.rstart = s.own(dupload(rstart));
+    L.rdiag = s.own(dupload(rdiag));
+    L.rend = s.own(dupload(rend));
+    L.col = s.own(dupload(std::vector<int>(cols, cols + nnz)));
+    L.pptr = s.own(dupload(pptr));
+    L.pt = s.own(dupload(pt));
+    L.pe2 = s.own(dupload(pe2));
+    L.nlev = nlev;
+    L.nlevu = nlevu;
+    L.lptr = s.own(dupload(lptr));
+    L.lrows = s.own(dupload(lrows));
+    L.uptr = s.own(dupload(uptr));
+    L.urows = s.own(dupload(urows));
+    L.perm = s.own(dupload(permv));
+    s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
+    s.posLR = s.own(dupload(std::vector<int>(pos_src + n, pos_src + n + nfi)));
+    s.posRL = s.own(dupload(std::vector<int>(pos_src + n + nfi, pos_src + n + 2 * nfi)));
+    s.B = s.own(dalloc<double>(size_t(nnz) * 25));
+    s.invU = s.own(dalloc<double>(size_t(n) * 25));
+    s.y = s.own(dalloc<double>(size_t(n) * 5));
+    s.has_ilu = true;
+  });
+}
+
+int a2d_rans_ilu_info(int64_t h, int* out4) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (!s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
+    out4[0] = s.ilu.nnz;
+    out4[1] = s.ilu.nlev;
+    out4[2] = s.ilu.nlevu;
+    out4[3] = 0;
+  });
+}
+
+int a2d_rans_ilu_factor(int64_t h, const double* winf, double gam, double cfl, double turb_factor, int* nonfinite) {
+  return guarded([&] {
+    Solver& s = get(h);
+    hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
+    factor(s, winf, gam, 1.0 / turb_factor, nullptr, nullptr);
+    if (nonfinite) *nonfinite = count_nonfinite(s, s.invU, s.m.nc * 25);
+    HC(hipStreamSynchronize(s.st));
+  });
+}
+
+int a2d_rans_ilu_apply(int64_t h, const double* b, double* x) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (!s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
+    upload_cells(s, b, s.bvec);
+    ilu_solve(s, s.bvec, s.tmp);
+    download_cells(s, s.tmp, x);
+  });
+}
+
+int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
+                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
+    if (gmres_iter < 1 || gmres_iter > GMRES_MAX) fail(-6, "gmres_iter must be in [1, 120]");
+    if (!(turb_factor > 0.0)) fail(-6, "turb_factor must be positive");
+    const int N = s.N();
+    const double ct_inv = 1.0 / turb_factor;
+    for (int k = 0; k < 12; ++k) out[k] = 0.0;
+    // residual of the state (the Newton right-hand side)
+    far_state(s, winf, gam);
+    eval(s, order, nullptr, s.R, nullptr, nullptr, 1.0);
+    // pseudo-time step, first-order Jacobian + vol/dt, ILU
+    hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
+    factor(s, winf, gam, ct_inv, nullptr, nullptr);
+    if (count_nonfinite(s, s.invU, s.m.nc * 25) > 0) {
+      out[5] = 1.0;                       // ILU not finite: no step
+      return;
+    }
+    const double rnorm = norm2(s, s.R);
+    hipLaunchKernelGGL(k_negate, nblk(N), TPB, 0, s.st, N, s.R, s.bvec);
+    far_state(s, winf, gam);
+    int info = 0;
+    int its = gmres(s, order, ct_inv, s.bvec, s.dq, gmres_tol, gmres_iter, right != 0, &info);
+    double lin = lin_residual(s, order, ct_inv, s.dq, rnorm);
+    int total = its;
+    if (!right && !(std::isfinite(lin) && lin < 0.5)) {
+      // left preconditioning can stop on |M^-1 r| while the true residual is large:
+      // retry with right preconditioning and keep the better (aero2d/rans.py:1791-1801)
+      int info2 = 0;
+      int its2 = gmres(s, order, ct_inv, s.bvec, s.dq2, gmres_tol, gmres_iter, true, &info2);
+      const double lin2 = lin_residual(s, order, ct_inv, s.dq2, rnorm);
+      total += its2;
+      if (std::isfinite(lin2) && (!std::isfinite(lin) || lin2 < lin)) {
+        HC(hipMemcpyAsync(s.dq, s.dq2, N * sizeof(double), hipMemcpyDeviceToDevice, s.st));
+        lin = lin2;
+        info = info2;
+      }
+    }
+    out[0] = lin;
+    out[1] = double(total);
+    out[2] = double(info);
+    if (!std::isfinite(lin) || lin > 0.9) return;   // linear solve failed: no step
+    // step limit (aero2d/rans.py:1822-1847)
+    const int nb = s.nparts_cell;
+    double* pmax = s.part;
+    double* pcnt = s.part + nb;
+    hipLaunchKernelGGL(k_step_rel, nb, TPB, 0, s.st, s.m.nc, s.Q, s.dq, s.reli, pmax, pcnt);
+    hipLaunchKernelGGL(k_max_rows, 1, 64, 0, s.st, nb, pmax, pcnt, s.small);
+    double mc[2];
+    HC(hipMemcpyAsync(mc, s.small, 2 * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    HC(hipStreamSynchronize(s.st));
+    const double rel = mc[0];
+    const double lam = rel < 0.2 ? 1.0 : 0.2 / rel;
+    const int nbig = int(mc[1]);
+    const bool per_cell = lam < 1.0 && step_limit_cells > 0 && nbig <= step_limit_cells;
+    HC(hipMemcpyAsync(s.Qprev, s.Q, N * sizeof(double), hipMemcpyDeviceToDevice, s.st));
+    hipLaunchKernelGGL(k_apply, nblk(s.m.nc), TPB, 0, s.st, s.m.nc, s.Q, s.dq, s.reli, lam, per_cell ? 1 : 0);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(s.st));
+    out[3] = lam;
+    out[4] = rel;
+    out[6] = 1.0;                        // update applied
+    out[7] = double(nbig);
+    out[8] = per_cell ? 1.0 : 0.0;
+  });
+}
+
+int a2d_rans_newton_dq(int64_t h, double* dq) {
+  return guarded([&] {
+    Solver& s = get(h);
+    download_cells(s, s.dq, dq);
+  });
+}
+
+int a2d_rans_keep(int64_t h, int what) {
+  return guarded([&] {
+    Solver& s = get(h);
+    const size_t bytes = size_t(s.N()) * sizeof(double);
+    switch (what) {
+      case 0: HC(hipMemcpyAsync(s.Qbest, s.Q, bytes, hipMemcpyDeviceToDevice, s.st)); break;   // save best
+      case 1: HC(hipMemcpyAsync(s.Q, s.Qbest, bytes, hipMemcpyDeviceToDevice, s.st)); break;   // restore best
+      case 2: HC(hipMemcpyAsync(s.Q, s.Qprev, bytes, hipMemcpyDeviceToDevice, s.st)); break;   // undo the step
+      default: fail(-6, "what must be 0 (save best), 1 (restore best) or 2 (undo the last step)");
+    }
+    HC(hipStreamSynchronize(s.st));
+  });
+}
+
+}  // extern "C"

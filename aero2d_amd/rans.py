@@ -1,0 +1,729 @@
+"""RANS (Spalart-Allmaras, SA-neg) Newton-Krylov solver on an unstructured mesh.
+
+Host side of the HIP library's RANS core (the a2d_rans_* entry points of
+include/aero2d_hip.h). The class keeps the reference's API and defaults for
+the fully turbulent SA model (aero2d/rans.py: RansSolver, RansResult,
+Convergence, RansControl): geometry set-up on the host (NumPy, as
+RansSolver._setup), the block-ILU structure on the host (reverse Cuthill-McKee
+order and fill pattern, as BlockILU0.__init__), and every field operation on
+the device: residual, exact Jacobian-vector product, first-order Jacobian,
+ILU factorisation and solves, GMRES, step limit and update, forces. The
+pseudo-transient continuation loop (CFL growth, first-order start, rollback,
+step rejection, convergence rule, limiter freezing, turbulence step boost)
+follows RansSolver.solve (aero2d/rans.py:1298-1900).
+
+Out of this build: k-omega SST, the AFT transition model, SA-RC, laminar
+regions, the unsteady solve, the exact LU preconditioner and the gmsh mesh
+generator (aero2d_amd.rans_mesh builds hybrid meshes from O-grids instead).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .rans_mesh import Mesh
+
+__all__ = ["RansSolver", "RansResult", "Convergence", "RansControl", "ilu_structure"]
+
+GAM = 1.4
+
+
+@dataclass
+class RansResult:
+    """aero2d/rans.py:48-62."""
+    alpha: float
+    cl: float
+    cd: float
+    cdp: float
+    cdf: float
+    cm: float
+    converged: bool
+    iterations: int
+    residual: list = field(default_factory=list)
+    elements: list = field(default_factory=list)
+    state: object = None
+    status: str = ""
+    history: dict = field(default_factory=dict)
+
+
+@dataclass
+class Convergence:
+    """Stop rule between angles (aero2d/rans.py:65-80): the last `hold`
+    second-order iterations at most 10^-orders of the freestream residual with
+    Cl and Cd steady; or a deep drop with forces steady for 3 iterations; or,
+    with stall_ok, a stalled residual with steady forces."""
+    orders: float = 4.5
+    hold: int = 20
+    cl_tol: float = 2e-4
+    cd_tol: float = 2e-5
+    stall_ok: bool = True
+
+
+class RansControl:
+    """Live control of a running solve (aero2d/rans.py:83-99): monitor(info)
+    per iteration; skip / stop end the solve; cfl_factor scales the CFL once;
+    rollback returns to the best state; cfl_set sets the CFL (held with
+    cfl_hold)."""
+
+    def __init__(self, monitor=None):
+        self.monitor = monitor
+        self.skip = False
+        self.stop = False
+        self.cfl_factor = 1.0
+        self.rollback = False
+        self.pause = False
+        self.cfl_set = None
+        self.cfl_hold = True
+
+
+def ilu_structure(n, rows, cols, fill=0, perm=None):
+    """Block ILU(k) structure as BlockILU0.__init__ forms it (aero2d/bilu.py:14-59):
+    perm (reverse Cuthill-McKee of the symmetrised pattern unless given,
+    perm[new] = old), the pattern in the new numbering grown `fill` times by
+    P <- P + tril(P, -1) triu(P, 1), sorted by row then column, and the
+    position of every given block in it. Returns (perm, r, c, src_pos)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    rows, cols = np.asarray(rows), np.asarray(cols)
+    if perm is None:
+        G = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+        perm = reverse_cuthill_mckee((G + G.T).tocsr(), symmetric_mode=True)
+    perm = np.asarray(perm)
+    inv = np.empty(n, int)
+    inv[perm] = np.arange(n)
+    r0, c0 = inv[rows], inv[cols]
+    P = sp.csr_matrix((np.ones(len(r0)), (r0, c0)), shape=(n, n))
+    for _ in range(int(fill)):
+        P = (P + sp.tril(P, -1).tocsr() @ sp.triu(P, 1).tocsr()).tocsr()
+    P = P.tocoo()
+    r, c = P.row.astype(np.int64), P.col.astype(np.int64)
+    o = np.lexsort((c, r))
+    r, c = r[o], c[o]
+    key = r * n + c
+    src_pos = np.searchsorted(key, r0.astype(np.int64) * n + c0)
+    return perm, r, c, src_pos
+
+
+class RansSolver:
+    """SA RANS on a Mesh (aero2d/rans.py:187-326 for the constructor and the
+    defaults kept here)."""
+
+    def __init__(self, mesh: Mesh, mach: float, re: float, chord: float = 1.0, xref=(0.25, 0.0),
+                 chi_inf: float = 3.0, order: int = 2, vortex_bc: bool = True, turbulent: bool = True):
+        self.m = mesh
+        self.mach, self.re, self.chord = float(mach), float(re), float(chord)
+        self.mu_inf = self.mach * chord / self.re
+        self.nu_inf = self.mu_inf
+        self.xref = tuple(xref)
+        self.chi_inf = chi_inf if turbulent else 0.0
+        self.order = order
+        self.vortex_bc = vortex_bc
+        self.turbulent = turbulent
+        self.limiter_k = 5.0
+        self.cfl_reset_on_order = True
+        self.limiter_zone = "te"
+        self.limiter_te_radius = 0.05
+        self.cfl_mode = "grow"
+        self.cfl_growth = 1.25
+        self.cfl_formula = "min(5*1.2**n, 1000)"
+        self.cfl_turb = None
+        self.turb_boost = 300.0
+        self.turb_boost_cfl = 10.0
+        self.step_limit_cells = 100
+        self.dt_visc_c = 1.0
+        self.conv = Convergence()
+        self.cfl_start = 5.0
+        self.cfl_limit = 1e6
+        self.first_order = 30
+        self.rollback_factor = 100.0
+        self.rollback_auto = True
+        self.cfl_lin_back = 0.7
+        self.rollback_first_order = 15
+        self.step_reject = 3.0
+        self.max_rejects = 6
+        self.gmres_tol = 0.05
+        self.max_rollbacks = 4
+        self.freeze_limiter = True
+        self.prec_fill = 3
+        self.right_prec = False
+        self.zmin = 0.0
+        self.gamma_vortex = 0.0
+        self.nv = 5
+        self._h = None
+        self._ilu_fill = None
+        self._setup()
+
+    # ------------------------------------------------------------------ set-up
+    def _setup(self):
+        """Face sets, least-squares matrices and the limiter zone, as
+        RansSolver._setup (aero2d/rans.py:328-425)."""
+        m = self.m
+        nc = m.n_cells
+        self.nc = nc
+        fi = np.where(m.face_tag == 0)[0]
+        fw = np.where(m.face_tag == 1)[0]
+        ff = np.where(m.face_tag == 2)[0]
+        L, R = m.face_cells[fi, 0], m.face_cells[fi, 1]
+        xc = m.centroid
+        self.iL, self.iR = L, R
+        self.n_i, self.A_i = m.face_normal[fi], m.face_area[fi]
+        self.rL = m.face_center[fi] - xc[L]
+        self.rR = m.face_center[fi] - xc[R]
+        d = xc[R] - xc[L]
+        self.d_i = d
+        self.dist_i = np.hypot(d[:, 0], d[:, 1])
+        cw = m.face_cells[fw, 0]
+        self.cw = cw
+        self.n_w, self.A_w = m.face_normal[fw], m.face_area[fw]
+        self.r_w = m.face_center[fw] - xc[cw]
+        self.dn_w = np.maximum(np.einsum("ij,ij->i", self.r_w, self.n_w), 1e-14)
+        self.body_w = m.face_body[fw]
+        self.fc_w = m.face_center[fw]
+        cf = m.face_cells[ff, 0]
+        self.cf = cf
+        self.n_f, self.A_f = m.face_normal[ff], m.face_area[ff]
+        self.r_f = m.face_center[ff] - xc[cf]
+        self.fc_f = m.face_center[ff]
+        w_i = 1.0 / self.dist_i**2
+        Mxx, Mxy, Myy = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+        for idx in (L, R):
+            np.add.at(Mxx, idx, w_i * d[:, 0] ** 2)
+            np.add.at(Mxy, idx, w_i * d[:, 0] * d[:, 1])
+            np.add.at(Myy, idx, w_i * d[:, 1] ** 2)
+        dw = self.r_w
+        ww = 1.0 / np.maximum(np.sum(dw**2, 1), 1e-30)
+        np.add.at(Mxx, cw, ww * dw[:, 0] ** 2)
+        np.add.at(Mxy, cw, ww * dw[:, 0] * dw[:, 1])
+        np.add.at(Myy, cw, ww * dw[:, 1] ** 2)
+        det = Mxx * Myy - Mxy**2
+        det = np.where(np.abs(det) < 1e-300, 1e-300, det)
+        self.Minv = np.stack([np.stack([Myy / det, -Mxy / det], -1),
+                              np.stack([-Mxy / det, Mxx / det], -1)], -2)
+        self.w_i, self.w_w = w_i, ww
+        self.vol = m.volume
+        self.wall_d = np.maximum(m.wall_dist, 1e-12)
+        near = np.zeros(nc, bool)
+        for b in np.unique(self.body_w):
+            pts = self.fc_w[self.body_w == b]
+            te = pts[np.argmax(pts[:, 0])]
+            near |= np.hypot(xc[:, 0] - te[0], xc[:, 1] - te[1]) < self.limiter_te_radius * self.chord
+        self.lim_te = near
+
+    def _device(self):
+        """The library's solver for this mesh and these constants (created once)."""
+        if self._h is not None:
+            return self._h
+        _abi.lib()
+        nc = self.nc
+        fgeo = np.column_stack([self.n_i, self.A_i, self.rL, self.rR, self.d_i, self.dist_i, self.w_i])
+        wgeo = np.column_stack([self.n_w, self.A_w, self.r_w, self.dn_w, self.w_w, self.fc_w])
+        ffgeo = np.column_stack([self.n_f, self.A_f, self.r_f, self.fc_f])
+        cgeo = np.column_stack([self.vol, self.wall_d, self.Minv[:, 0, 0], self.Minv[:, 0, 1],
+                                self.Minv[:, 1, 1], self.lim_te.astype(float)])
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)   # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)     # noqa: E731
+        zone = {"off": 0, "te": 1, "all": 2}[self.limiter_zone]
+        prm = _abi.RansParams(self.mach, self.re, self.chord, self.xref[0], self.xref[1], self.chi_inf,
+                              self.limiter_k, self.zmin, int(self.order), int(bool(self.turbulent)),
+                              int(bool(self.vortex_bc)), zone)
+        arrs = [i32(self.iL), i32(self.iR), f64(fgeo), i32(self.cw), f64(wgeo), i32(self.cf), f64(ffgeo),
+                f64(cgeo)]
+        self._keep = arrs
+        h = _abi._i64(0)
+        _abi.call("a2d_rans_create", nc, len(self.iL), len(self.cw), len(self.cf), _abi.iptr(arrs[0]),
+                  _abi.iptr(arrs[1]), _abi.dptr(arrs[2]), _abi.iptr(arrs[3]), _abi.dptr(arrs[4]),
+                  _abi.iptr(arrs[5]), _abi.dptr(arrs[6]), _abi.dptr(arrs[7]), ctypes.byref(prm), ctypes.byref(h))
+        self._h = h.value
+        return self._h
+
+    def close(self):
+        if self._h is not None:
+            try:
+                _abi.call("a2d_rans_destroy", self._h)
+            finally:
+                self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self):
+        """Block rows/cols of the first-order Jacobian: the diagonal, then
+        (iL, iR), then (iR, iL) (aero2d/rans.py:1128-1144)."""
+        nc = self.nc
+        return (np.concatenate([np.arange(nc), self.iL, self.iR]),
+                np.concatenate([np.arange(nc), self.iR, self.iL]))
+
+    def _ensure_ilu(self):
+        if self._ilu_fill == int(self.prec_fill):
+            return
+        if self._ilu_fill is not None:
+            raise RuntimeError("the ILU fill level of a solver is fixed once set")
+        h = self._device()
+        rows, cols = self.pattern()
+        perm, r, c, pos = ilu_structure(self.nc, rows, cols, fill=int(self.prec_fill))
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)     # noqa: E731
+        a = [i32(perm), i32(r), i32(c), i32(pos)]
+        _abi.call("a2d_rans_ilu_setup", h, len(r), *[_abi.iptr(x) for x in a])
+        self._ilu_fill = int(self.prec_fill)
+        self.ilu_perm = perm
+
+    def ilu_info(self) -> dict:
+        self._ensure_ilu()
+        out = np.zeros(4, np.int32)
+        _abi.call("a2d_rans_ilu_info", self._device(), _abi.iptr(out))
+        return dict(nnz=int(out[0]), levels=int(out[1]), levels_back=int(out[2]))
+
+    # ---------------------------------------------------------------- physics
+    def prim(self, Q):
+        """(rho, rho u, rho v, rho E, rho chi) -> (rho, u, v, p, chi) (aero2d/rans.py:455-461)."""
+        Q = np.asarray(Q, float)
+        r = Q[:, 0]
+        u, v = Q[:, 1] / r, Q[:, 2] / r
+        p = (GAM - 1) * (Q[:, 3] - 0.5 * r * (u**2 + v**2))
+        return np.concatenate([np.stack([r, u, v, p], 1), Q[:, 4:] / r[:, None]], 1)
+
+    def cons(self, w):
+        """Inverse of prim (aero2d/rans.py:463-467)."""
+        w = np.asarray(w, float)
+        r, u, v, p = (w[:, k] for k in range(4))
+        return np.concatenate([np.stack([r, r * u, r * v, p / (GAM - 1) + 0.5 * r * (u**2 + v**2)], 1),
+                               w[:, 4:] * r[:, None]], 1)
+
+    def freestream(self, alpha):
+        """(1, M cos a, M sin a, 1/gamma, chi_inf) (aero2d/rans.py:469-475)."""
+        a = np.radians(alpha)
+        return np.array([1.0, self.mach * np.cos(a), self.mach * np.sin(a), 1.0 / GAM, self.chi_inf])
+
+    # ------------------------------------------------------------- device calls
+    def _set(self, Q):
+        Q = np.ascontiguousarray(Q, np.float64).reshape(self.nc, 5)
+        _abi.call("a2d_rans_set_state", self._device(), _abi.dptr(Q))
+
+    def _get(self):
+        Q = np.empty((self.nc, 5))
+        _abi.call("a2d_rans_get_state", self._device(), _abi.dptr(Q))
+        return Q
+
+    def _freeze(self, phi):
+        h = self._device()
+        if phi is None:
+            _abi.call("a2d_rans_freeze_limiter", h, None)
+        else:
+            p = np.ascontiguousarray(phi, np.float64).reshape(self.nc, 5)
+            _abi.call("a2d_rans_freeze_limiter", h, _abi.dptr(p))
+
+    def residual(self, Q, winf, gam=0.0, order=None, phi_frozen=None):
+        """R (nc, 5) (aero2d/rans.py:860-964)."""
+        order = self.order if order is None else int(order)
+        self._set(Q)
+        self._freeze(phi_frozen)
+        R = np.empty((self.nc, 5))
+        w = np.ascontiguousarray(winf, np.float64)
+        _abi.call("a2d_rans_residual", self._device(), order, _abi.dptr(w), float(gam), _abi.dptr(R), None)
+        return R
+
+    def jvp(self, Q, v, winf, gam=0.0, order=None, phi_frozen=None):
+        """jax.jvp of the residual along v (the jvp entries of _fns, aero2d/rans.py:1186-1191)."""
+        order = self.order if order is None else int(order)
+        self._set(Q)
+        self._freeze(phi_frozen)
+        out = np.empty((self.nc, 5))
+        v = np.ascontiguousarray(v, np.float64).reshape(self.nc, 5)
+        w = np.ascontiguousarray(winf, np.float64)
+        _abi.call("a2d_rans_jvp", self._device(), order, _abi.dptr(w), float(gam), _abi.dptr(v), _abi.dptr(out))
+        return out
+
+    def phi(self, Q):
+        """The Venkatakrishnan limiter of a state in every cell (aero2d/rans.py:1192-1194)."""
+        self._set(Q)
+        out = np.empty((self.nc, 5))
+        _abi.call("a2d_rans_limiter", self._device(), _abi.dptr(out))
+        return out
+
+    def local_dt(self, Q, cfl):
+        """Local pseudo-time step (aero2d/rans.py:1206-1249) for the flow
+        equations; the turbulence equation's is this times the step factor."""
+        self._set(Q)
+        dt = np.empty(self.nc)
+        _abi.call("a2d_rans_local_dt", self._device(), float(cfl), _abi.dptr(dt))
+        return dt
+
+    def _forces_dev(self, alpha):
+        out = np.zeros(5)
+        nw = len(self.cw)
+        cp, cf = np.empty(nw), np.empty(nw)
+        _abi.call("a2d_rans_forces", self._device(), float(alpha), _abi.dptr(out), _abi.dptr(cp), _abi.dptr(cf))
+        return out, cp, cf
+
+    def forces(self, Q, alpha):
+        """Pressure and friction forces (aero2d/rans.py:1251-1296)."""
+        self._set(Q)
+        return self._forces_dict(alpha)
+
+    def _forces_dict(self, alpha):
+        out, cp, cf = self._forces_dev(alpha)
+        q = 0.5 * self.mach**2
+        per = []
+        a = np.radians(alpha)
+        for b in range(int(self.body_w.max()) + 1):
+            k = self.body_w == b
+            # per-body forces from the per-face coefficients
+            n, A = self.n_w[k], self.A_w[k]
+            fxp = cp[k] * q * n[:, 0] * A
+            fyp = cp[k] * q * n[:, 1] * A
+            fxv = cf[k] * q * (-n[:, 1]) * A
+            fyv = cf[k] * q * n[:, 0] * A
+            per.append(dict(x=self.fc_w[k, 0], y=self.fc_w[k, 1], cp=cp[k], cf=cf[k], gam=None,
+                            cl=float(((-(fxp + fxv) * np.sin(a) + (fyp + fyv) * np.cos(a))).sum() / (q * self.chord)),
+                            cd=float((((fxp + fxv) * np.cos(a) + (fyp + fyv) * np.sin(a))).sum() / (q * self.chord))))
+        return dict(cl=float(out[0]), cd=float(out[1]), cdp=float(out[2]), cdf=float(out[3]), cm=float(out[4]),
+                    cp=cp, cf=cf, elements=per)
+
+    def jac1_blocks(self, Q, winf, gam):
+        """Diagonal blocks (nc,5,5) without V/dt and the off-diagonal blocks of
+        the pairs (iL,iR) then (iR,iL) (aero2d/rans.py:1107-1126)."""
+        self._set(Q)
+        D = np.empty((self.nc, 5, 5))
+        off = np.empty((2 * len(self.iL), 5, 5))
+        w = np.ascontiguousarray(winf, np.float64)
+        _abi.call("a2d_rans_jac1", self._device(), _abi.dptr(w), float(gam), _abi.dptr(D), _abi.dptr(off))
+        return D, off
+
+    def ilu_factor(self, Q, winf, gam, cfl, turb_factor=1.0):
+        """Factor vol/dt + J1 at the state (BlockILU0.factor); returns the count
+        of non-finite inverse-diagonal entries."""
+        self._ensure_ilu()
+        self._set(Q)
+        bad = ctypes.c_int(0)
+        w = np.ascontiguousarray(winf, np.float64)
+        _abi.call("a2d_rans_ilu_factor", self._device(), _abi.dptr(w), float(gam), float(cfl), float(turb_factor),
+                  ctypes.byref(bad))
+        return bad.value
+
+    def ilu_apply(self, b):
+        """BlockILU0.solve of b (nc, 5) with the last factorisation."""
+        b = np.ascontiguousarray(b, np.float64).reshape(self.nc, 5)
+        x = np.empty((self.nc, 5))
+        _abi.call("a2d_rans_ilu_apply", self._device(), _abi.dptr(b), _abi.dptr(x))
+        return x
+
+    STEP_FIELDS = ("lin", "gmres_iterations", "info", "lam", "rel", "ilu_nonfinite", "applied", "nbig",
+                   "per_cell")
+
+    def newton_step(self, order, winf, gam, cfl, gmres_tol, gmres_iter=60, turb_factor=1.0, right=None,
+                    Q=None):
+        """One Newton step at the device state (or at Q): the linear solve and
+        update of RansSolver.solve (aero2d/rans.py:1738-1858). Returns a dict of
+        the step's figures."""
+        self._ensure_ilu()
+        if Q is not None:
+            self._set(Q)
+        out = np.zeros(12)
+        w = np.ascontiguousarray(winf, np.float64)
+        r = self.right_prec if right is None else right
+        _abi.call("a2d_rans_newton_step", self._device(), int(order), _abi.dptr(w), float(gam), float(cfl),
+                  float(turb_factor), float(gmres_tol), int(gmres_iter), int(bool(r)), int(self.step_limit_cells),
+                  _abi.dptr(out))
+        d = {k: float(v) for k, v in zip(self.STEP_FIELDS, out)}
+        for k in ("gmres_iterations", "info", "ilu_nonfinite", "applied", "nbig", "per_cell"):
+            d[k] = int(d[k])
+        return d
+
+    def newton_dq(self):
+        dq = np.empty((self.nc, 5))
+        _abi.call("a2d_rans_newton_dq", self._device(), _abi.dptr(dq))
+        return dq
+
+    def _rn(self, order, winf, gam):
+        rn5 = np.zeros(5)
+        w = np.ascontiguousarray(winf, np.float64)
+        _abi.call("a2d_rans_residual", self._device(), int(order), _abi.dptr(w), float(gam), None, _abi.dptr(rn5))
+        return rn5
+
+    def _keep_state(self, what):
+        _abi.call("a2d_rans_keep", self._device(), int(what))
+
+    # ------------------------------------------------------------------ solve
+    def initial_state(self, winf):
+        """Freestream with the velocity faded to zero at the wall over 0.02 chord
+        (aero2d/rans.py:1380-1385)."""
+        f = np.clip(self.wall_d / 0.02, 0, 1)[:, None]
+        w0 = np.tile(winf, (self.nc, 1))
+        w0[:, 1:3] *= f
+        return self.cons(w0)
+
+    def solve(self, alpha, Q0=None, max_iter=300, tol=1e-7, cfl0=None, cfl_max=None, first_order_iters=None,
+              verbose=False, callback=None, gmres_tol=None, gmres_iter=60, conv=None, control=None,
+              alpha0=None) -> RansResult:
+        """Pseudo-transient Newton-Krylov solve to the Convergence rule or
+        max_iter iterations (aero2d/rans.py:1298-1900, the SA path)."""
+        crit = conv or self.conv
+        gmres_tol = self.gmres_tol if gmres_tol is None else gmres_tol
+        cfl0 = self.cfl_start if cfl0 is None else cfl0
+        cfl_max = self.cfl_limit if cfl_max is None else cfl_max
+        first_order_iters = self.first_order if first_order_iters is None else first_order_iters
+        rb_factor, st_reject = self.rollback_factor, self.step_reject
+        winf = self.freestream(alpha)
+        self._ensure_ilu()
+        if Q0 is None:
+            Q = self.initial_state(winf)
+        else:
+            Q = np.array(Q0, float)
+            if alpha0 is not None and abs(float(alpha) - float(alpha0)) > 1e-12:
+                Q = self._rotate_state(Q, float(alpha) - float(alpha0))
+        # reference residual of the rule: that of the faded freestream (aero2d/rans.py:1408-1414)
+        R_fs = self.residual(self.initial_state(winf), winf, 0.0, order=2)
+        rfs_eq = np.sqrt(np.mean((R_fs / self.vol[:, None]) ** 2, 0))
+        r_fs = float(rfs_eq.sum())
+        self._set(Q)
+        self._freeze(None)
+        warm = Q0 is not None
+        turning = warm and alpha0 is not None and abs(float(alpha) - float(alpha0)) > 1e-12
+        peak = [0.0, 0, None]
+        if warm and self.vortex_bc:
+            self.gamma_vortex = 0.5 * self._forces_dict(alpha)["cl"] * self.mach * self.chord
+        have_prev = False
+        prev_rn, prev_order = None, None
+        n_rej = ok_streak = n_tiny = 0
+        hist, fh = [], []
+        r0 = None
+        cfl = cfl0
+        self._last_ok, self._last_lin = True, 0.0
+        H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[])
+        events = []
+        best = dict(rn=np.inf, has=turning, it=0, cfl=cfl0, order=0)
+        if turning:
+            self._keep_state(0)
+        tb = 1.0
+        first_until = 0
+        n_back = 0
+        frozen = False
+        prev_order_used = None
+        conv_ok = False
+        status = ""
+        it = 0
+        for it in range(1, max_iter + 1):
+            if control is not None:
+                v_ = control.cfl_set
+                if v_ is not None and float(v_) > 0:
+                    control.cfl_set = None
+                    cfl = float(v_)
+                    self.cfl_hold_value = cfl if control.cfl_hold else None
+                sk, stp = bool(control.skip), bool(control.stop)
+                if sk or stp:
+                    status = "accepted by hand" if sk else "stopped"
+                    control.skip = False
+                    break
+                f_ = float(control.cfl_factor)
+                if f_ != 1.0:
+                    cfl = max(1.0, cfl * f_)
+                    control.cfl_factor = 1.0
+            order = 1 if (it <= first_order_iters and Q0 is None) or it <= first_until else self.order
+            if (order == 2 and prev_order_used == 1 and self.cfl_reset_on_order
+                    and getattr(self, "cfl_hold_value", None) is None):
+                cfl = cfl0
+            prev_order_used = order
+            gm = self.gamma_vortex if self.vortex_bc else 0.0
+            rn5 = self._rn(order, winf, gm)
+            rn = float(rn5.sum())
+            manual = control is not None and control.rollback
+            if manual:
+                control.rollback = False
+            if best["order"] != order:
+                best.update(rn=np.inf, order=order)
+            blowup = (not np.isfinite(rn)) or (self.rollback_auto and rn > rb_factor * best["rn"]
+                                               and it - best["it"] >= 3 and not (warm and it <= 8))
+            if (blowup or manual) and best["has"]:
+                n_back += int(not manual)
+                tb = 1.0
+                why = "by hand" if manual else ("NaN" if not np.isfinite(rn) else
+                                                 f"residual x{rn / best['rn']:.3g} of the lowest")
+                events.append((it, f"back to iteration {best['it']} ({why})", rn, []))
+                if verbose:
+                    print(f"  it {it:4d}: back to iteration {best['it']} ({why})", flush=True)
+                self._keep_state(1)
+                if n_back > self.max_rollbacks:
+                    status = f"unstable: {n_back - 1} rollbacks did not help (best state kept)"
+                    break
+                cfl = max(cfl0, min(cfl, best["cfl"]) * 0.1)
+                first_until = it + (self.rollback_first_order if not np.isfinite(rn) else 0)
+                if frozen:
+                    self._freeze(None)
+                    frozen = False
+                best.update(rn=np.inf, it=it)
+                continue
+            if (self.rollback_auto and have_prev and prev_order == order and n_rej < self.max_rejects
+                    and (not np.isfinite(rn) or rn > st_reject * prev_rn)
+                    and not (warm and it <= 8) and not manual):
+                n_rej += 1
+                self._keep_state(2)
+                cfl = max(cfl0, cfl * 0.3)
+                events.append((it, f"step undone (residual x{rn / prev_rn:.3g}), CFL -> {cfl:.3g}", rn, []))
+                if verbose:
+                    print(f"  it {it:4d}: step undone (residual x{rn / prev_rn:.3g}), CFL -> {cfl:.3g}",
+                          flush=True)
+                continue
+            n_rej = 0
+            ok_streak += 1
+            if not np.isfinite(rn):
+                status = "diverged"
+                break
+            if peak[2] is None:
+                peak[2] = rn
+            if rn > peak[0]:
+                peak[:2] = [rn, it]
+            past_peak = (peak[0] >= 5.0 * peak[2] and it - peak[1] >= 5) or it > 40
+            if rn < 0.95 * best["rn"] and (not turning or past_peak):
+                best.update(rn=rn, has=True, it=it, cfl=cfl, order=order)
+                self._keep_state(0)
+            if r0 is None or (order == 2 and len(hist) and hist[-1][1] == 1):
+                r0 = rn
+            hist.append((rn, order))
+            fr_ = self._forces_dict(alpha)
+            for k_, v_ in (("it", it), ("res", rn5), ("rn", rn), ("cl", fr_["cl"]), ("cd", fr_["cd"]),
+                           ("cfl", cfl)):
+                H[k_].append(v_)
+            if order == 2:
+                self._rref = max(getattr(self, "_rref", 0.0), rn)
+            if control is not None and control.monitor is not None:
+                control.monitor(dict(alpha=alpha, it=it, res=rn5, rn=rn, cl=fr_["cl"], cd=fr_["cd"], cfl=cfl,
+                                     rref=getattr(self, "_rref", rn), order=order, r_fs=r_fs, lin=self._last_lin))
+            if order == 2 and rn < tol * r0:
+                conv_ok = True
+                break
+            if order == 2:
+                fh.append((fr_["cl"], fr_["cd"]))
+                o2 = [h[0] for h in hist if h[1] == 2]
+                rref = r_fs
+                stalled = len(o2) > 60 and min(o2[-30:]) > 0.5 * min(o2[:-30])
+
+                def steady(win, tol_cl, tol_cd):
+                    if len(fh) <= win:
+                        return False
+                    cls = [f[0] for f in fh[-win - 1:]]
+                    cds = [f[1] for f in fh[-win - 1:]]
+                    return (max(cls) - min(cls) < tol_cl * max(0.2, abs(fh[-1][0]))
+                            and max(cds) - min(cds) < tol_cd)
+                hold = max(1, int(crit.hold))
+                lvl = 10.0 ** (-crit.orders) * rref
+                own = max(o2)
+                dropped = (not warm) or rn <= 1e-2 * own
+                if (len(o2) >= hold and max(o2[-hold:]) <= lvl and dropped
+                        and steady(hold, crit.cl_tol, crit.cd_tol)):
+                    conv_ok, status = True, "converged"
+                    break
+                if (len(o2) >= 8 and max(o2[-3:]) <= 10.0 ** (-1.5) * lvl and rn <= 1e-3 * own
+                        and steady(3, crit.cl_tol, crit.cd_tol)):
+                    conv_ok, status = True, "converged"
+                    break
+                if (crit.stall_ok and len(o2) > 40 and stalled and rn < 10.0 ** (1.0 - crit.orders) * rref
+                        and steady(max(hold, 30), crit.cl_tol, crit.cd_tol)):
+                    conv_ok, status = True, "converged (residual stalled, forces steady)"
+                    break
+                if (crit.stall_ok and len(o2) > 60 and stalled and rn < 10.0 ** -2.5 * own
+                        and steady(max(hold, 40), crit.cl_tol, crit.cd_tol)):
+                    conv_ok, status = True, "converged (forces steady)"
+                    break
+                if (self.freeze_limiter and not frozen and self.limiter_k > 0 and self.limiter_zone != "off"
+                        and stalled and steady(30, 0.005, 1e-3)):
+                    Qc = self._get()
+                    self._freeze(self.phi(Qc))
+                    self._set(Qc)
+                    frozen = True
+                    if verbose:
+                        print(f"  it {it:4d}: residual stalled at {rn:.2e}: limiter frozen", flush=True)
+            # CFL (aero2d/rans.py:1617-1680)
+            if self.cfl_mode == "formula":
+                raise NotImplementedError("cfl_mode 'formula' is not built; use 'grow', 'fixed' or 'ser'")
+            if len(hist) >= 2 and hist[-2][1] == order:
+                ok_ = self._last_ok
+                lin_ = self._last_lin
+                if self.cfl_mode == "fixed":
+                    cfl = min(cfl_max, cfl * self.cfl_growth) if ok_ and rn < 3.0 * hist[-2][0] \
+                        else max(cfl0, 0.5 * cfl)
+                elif self.cfl_mode == "ser":
+                    cfl = min(cfl_max, max(cfl0, cfl * min(2.0, hist[-2][0] / rn)))
+                else:
+                    grow = (self.cfl_growth if rn < 1.2 * hist[-2][0] and ok_ and lin_ < 0.25 else
+                            self.cfl_lin_back if lin_ >= 0.25 and ok_ and rn < 3.0 * hist[-2][0] else
+                            1.0 if rn < 3.0 * hist[-2][0] and ok_ else 0.5)
+                    cfl = min(cfl_max, max(cfl0, cfl * grow))
+            if getattr(self, "cfl_hold_value", None) is not None:
+                cfl = float(self.cfl_hold_value)
+            if self.turb_boost > 1.0:
+                ok_ = self._last_ok
+                fw_ = fh[-11:] if order >= 2 else []
+                calm = (len(fw_) == 11 and max(f[0] for f in fw_) - min(f[0] for f in fw_)
+                        < 2e-3 * max(0.2, abs(fw_[-1][0]))
+                        and max(f[1] for f in fw_) - min(f[1] for f in fw_) < 5e-4)
+                if calm and ok_ and cfl >= self.turb_boost_cfl and len(hist) >= 2 and rn < 1.2 * hist[-2][0]:
+                    tb = min(self.turb_boost, 1.5 * tb)
+                elif not ok_ or (len(hist) >= 2 and rn >= 1.2 * hist[-2][0]):
+                    tb = max(1.0, tb / 4.0)
+            ct = (self.cfl_turb if self.cfl_turb is not None else 1.0) * tb
+            gtol = gmres_tol if (warm and n_back == 0) else min(gmres_tol, 0.01)
+            st = self.newton_step(order, winf, gm, cfl, gtol, gmres_iter, turb_factor=ct)
+            if st["ilu_nonfinite"]:
+                if verbose:
+                    print(f"  it {it:4d}: ILU not finite, CFL {cfl:.3g} -> {max(cfl0, cfl / 10):.3g}", flush=True)
+                cfl = max(cfl0, cfl / 10)
+                continue
+            lin = st["lin"]
+            self._last_ok = bool(np.isfinite(lin) and lin < 0.5)
+            self._last_lin = lin
+            if not st["applied"]:
+                if verbose:
+                    print(f"  it {it:4d}: GMRES did not converge ({lin:.2g}), residual {rn:.3e}, CFL {cfl:.3g} -> "
+                          f"{max(cfl0, cfl / 4):.3g}", flush=True)
+                cfl = max(cfl0, cfl / 4)
+                continue
+            lam = st["lam"]
+            if lam < 1.0 and not st["per_cell"]:
+                cfl = max(cfl0 if n_tiny < 10 else 0.02 * cfl0, cfl * 0.5)
+            n_tiny = n_tiny + 1 if lam < 0.05 and not st["per_cell"] else 0
+            if n_tiny >= 60:
+                status = "stuck: the Newton step was cut to almost nothing for 60 iterations"
+                break
+            have_prev, prev_rn, prev_order = True, rn, order
+            if self.vortex_bc and it % 5 == 0:
+                self.gamma_vortex = 0.5 * self._forces_dict(alpha)["cl"] * self.mach * self.chord
+            if verbose and (it % 10 == 0 or it < 5):
+                fr = self._forces_dict(alpha)
+                print(f"  it {it:4d} ord {order} res {rn:.3e} cfl {cfl:.2e} lam {lam:.2f} "
+                      f"gmres {st['gmres_iterations']}:{lin:.1e} Cl {fr['cl']:.4f} Cd {fr['cd']:.5f}", flush=True)
+            if callback and callback(it, rn):
+                break
+        if status == "diverged" and best["has"]:
+            self._keep_state(1)
+            status = "diverged (best state kept)"
+        fr = self._forces_dict(alpha)
+        Hn = {k_: np.array(v_) for k_, v_ in H.items()}
+        if not status:
+            status = "converged" if conv_ok else "not converged within the iterations allowed"
+        Hn["rref"] = float(getattr(self, "_rref", 0.0) or (max(Hn["rn"]) if len(Hn["rn"]) else 1.0))
+        Hn["r_fs"] = r_fs
+        Hn["events"] = events
+        if n_back:
+            status += f"; rollbacks: {n_back}"
+        Q = self._get()
+        return RansResult(alpha, fr["cl"], fr["cd"], fr["cdp"], fr["cdf"], fr["cm"], conv_ok, it,
+                          [h[0] for h in hist], fr["elements"], Q, status, Hn)
+
+    def _rotate_state(self, Q, dalpha, inner=0.05, outer=0.5):
+        """Velocity turned by dalpha degrees, fully beyond `outer` chords from the
+        wall, fading to none at `inner` (aero2d/rans.py:1146-1158)."""
+        f = np.clip((self.wall_d / self.chord - inner) / max(outer - inner, 1e-9), 0.0, 1.0)
+        th = np.radians(dalpha) * f
+        c, s_ = np.cos(th), np.sin(th)
+        Q = Q.copy()
+        mu, mv = Q[:, 1].copy(), Q[:, 2].copy()
+        Q[:, 1] = c * mu - s_ * mv
+        Q[:, 2] = s_ * mu + c * mv
+        return Q

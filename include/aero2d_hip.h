@@ -214,6 +214,82 @@ int a2d_hllc_flux(int n, const double* WL, const double* WR, const double* S, do
  * on that array: do not call it from two threads at once. */
 int a2d_hllc_flux_noalloc(int n, const double* WL, const double* WR, const double* S, double* F);
 
+/* ---- RANS (Spalart-Allmaras, SA-neg) Newton-Krylov core on an unstructured mesh ----
+ * Replaces the jitted functions of RansSolver._fns (aero2d/rans.py:1169-1204),
+ * local_dt (1206-1249), forces (1251-1296), the linear solve and update of
+ * solve (1738-1858), BlockILU0 (aero2d/bilu.py:13-166) and gmres
+ * (aero2d/krylov.py:14-87). All arrays are FP64; per-cell arrays are (nc, 5)
+ * cell-major as the reference's Q, and are copied. A solver is one device
+ * context (mesh, state, Jacobian, ILU, Krylov basis) owned by the library. */
+typedef struct a2d_rans_params {
+  double mach, re, chord, xref_x, xref_y, chi_inf, limiter_k, zmin;
+  int order;          /* 1 or 2 (RansSolver.order) */
+  int turbulent;      /* 0: laminar, the rho chi equation is rho chi = 0 */
+  int vortex_bc;      /* informational: the host passes gam = 0 without it */
+  int limiter_zone;   /* 0 "off", 1 "te", 2 "all" */
+} a2d_rans_params;
+
+/* Mesh arrays as RansSolver._setup forms them (aero2d/rans.py:328-425):
+ * interior faces: face_l, face_r (nfi), face_geom (nfi, 11) =
+ *   nx ny A rLx rLy rRx rRy dx dy dist w_i   (d = xc[R] - xc[L], w_i = 1/dist^2);
+ * wall faces: wall_cell (nfw), wall_geom (nfw, 9) = nx ny A rx ry dn w_w fcx fcy;
+ * far faces: far_cell (nff), far_geom (nff, 7) = nx ny A rx ry fcx fcy;
+ * cells: cell_geom (nc, 6) = vol wall_d Minv00 Minv01 Minv11 in_te_zone.
+ * -6 for a bad index or size. */
+int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const int* face_r,
+                    const double* face_geom, const int* wall_cell, const double* wall_geom, const int* far_cell,
+                    const double* far_geom, const double* cell_geom, const a2d_rans_params* prm,
+                    int64_t* handle_out);
+int a2d_rans_destroy(int64_t h);
+int a2d_rans_set_state(int64_t h, const double* Q);   /* (nc, 5) */
+int a2d_rans_get_state(int64_t h, double* Q);
+/* Frozen limiter for the second-order residual and JVP (res2/jvp2 of _fns);
+ * NULL returns to the limiter of the state (res2f/jvp2f). */
+int a2d_rans_freeze_limiter(int64_t h, const double* phi);
+/* The Venkatakrishnan limiter of the state in every cell (the "phi" entry of _fns). */
+int a2d_rans_limiter(int64_t h, double* phi);
+/* residual(Q, winf, gam, order) (aero2d/rans.py:860-964): R (nc, 5) or NULL;
+ * rn5 (5) = sqrt(mean((R_k / vol)^2)) per equation, or NULL. winf (5). */
+int a2d_rans_residual(int64_t h, int order, const double* winf, double gam, double* R, double* rn5);
+/* jax.jvp of the residual along v (nc, 5) (jvp1/jvp2/jvp2f of _fns). */
+int a2d_rans_jvp(int64_t h, int order, const double* winf, double gam, const double* v, double* out);
+/* local_dt(Q, cfl) (aero2d/rans.py:1206-1249), dt (nc). */
+int a2d_rans_local_dt(int64_t h, double cfl, double* dt);
+/* forces(Q, alpha) (aero2d/rans.py:1251-1296): out5 = cl cd cdp cdf cm; cp, cf
+ * (nfw, per wall face) or NULL. */
+int a2d_rans_forces(int64_t h, double alpha_deg, double* out5, double* cp, double* cf);
+/* _jac1_blocks(Q, winf, gam) (aero2d/rans.py:1107-1126): D (nc, 5, 5) without
+ * vol/dt and the off-diagonal blocks (2 nfi, 5, 5) for (iL,iR) then (iR,iL). */
+int a2d_rans_jac1(int64_t h, const double* winf, double gam, double* D, double* off);
+/* BlockILU0 structure (aero2d/bilu.py:14-130): perm (nc, perm[new] = old),
+ * the pattern in the new numbering sorted by row then column (rows, cols:
+ * nnz), and pos_src (nc + 2 nfi): the pattern entry of each Jacobian block in
+ * the order of a2d_rans_jac1 (diagonal, (iL,iR), (iR,iL)). Once per solver. */
+int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, const int* cols, const int* pos_src);
+/* nnz, forward levels, backward levels, 0. */
+int a2d_rans_ilu_info(int64_t h, int* out4);
+/* Assemble vol/dt + J1 at the state and CFL (turbulence equation's step
+ * times turb_factor) and factor it (BlockILU0.factor); nonfinite = count of
+ * non-finite inverse-diagonal entries, or NULL. */
+int a2d_rans_ilu_factor(int64_t h, const double* winf, double gam, double cfl, double turb_factor, int* nonfinite);
+/* BlockILU0.solve of b (nc, 5) into x (nc, 5). */
+int a2d_rans_ilu_apply(int64_t h, const double* b, double* x);
+/* One Newton step of RansSolver.solve (aero2d/rans.py:1738-1858) at the
+ * state: R, dt, ILU of vol/dt + J1, GMRES on (vol/dt + dR/dQ) dq = -R (left
+ * preconditioning, retried with right preconditioning when the linear
+ * residual is not below 0.5, or right only with right = 1), then, when the
+ * linear residual is finite and <= 0.9, the step limit (rho, p fall by at
+ * most 20 %; only the cells that ask for more when there are at most
+ * step_limit_cells of them) and the update; the previous state is kept.
+ * out (12): lin, GMRES iterations, info, lam, rel, ilu_nonfinite, applied,
+ * cells beyond 0.2, per-cell limit used, 0, 0, 0. */
+int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
+                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out);
+/* The last Newton step's dq before the step limit (nc, 5). */
+int a2d_rans_newton_dq(int64_t h, double* dq);
+/* 0: keep the state as the best; 1: return to the best; 2: undo the last step. */
+int a2d_rans_keep(int64_t h, int what);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,735 @@
+"""Oracle: RANS (Spalart-Allmaras, SA-neg) residual and Newton-Krylov pieces on
+an unstructured mesh — TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+A PyTorch-CPU FP64 restatement of the fully turbulent SA path of
+aero2d/rans.py (the default turb_model "sa": no transition, no k-omega SST, no
+rotation correction), of aero2d/bilu.py BlockILU0 and of aero2d/krylov.py
+gmres. Arrays keep the reference's cell-first layout: Q (nc, 5) with
+(rho, rho u, rho v, rho E, rho chi), chi = nu~/nu_inf.
+
+Cited reference lines (aero2d/rans.py unless noted): constants 31-37; __init__
+defaults 188-326; _setup 328-425; prim/cons 455-467; freestream 469-475;
+mu_lam 477-478; roe 480-524; sa_terms 526-535; visc_flux 537-571 (SA branch);
+_lsq 600-607; sa_source 609-629; gradients 809-819; _wall_values 821-828;
+_far_state 830-858; residual 860-964; sources 966-981 (SA branch); venkat
+983-1022; _grad_T 1024-1028; _jac1_parts 1032-1105 (SA branch); _jac1_blocks
+1107-1126; local_dt 1206-1249; forces 1251-1296; the linear solve and update of
+solve 1738-1858.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+DT = torch.float64
+GAM = 1.4
+PR, PRT = 0.72, 0.9
+SUTH = 110.4 / 288.15
+CB1, SIG, CB2, KAP = 0.1355, 2.0 / 3.0, 0.622, 0.41
+CW1 = CB1 / KAP**2 + (1 + CB2) / SIG
+CW2, CW3, CV1, CV2, CV3, CN1 = 0.3, 2.0, 7.1, 0.7, 0.9, 16.0
+
+
+def _t(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=DT)
+
+
+class RansOracle:
+    """The SA RANS residual on a Mesh (aero2d_amd.rans_mesh.Mesh, the fields of
+    aero2d/rans_mesh.py:19-36). Keyword defaults follow RansSolver.__init__
+    (aero2d/rans.py:188-326): chi_inf 3, order 2, vortex_bc, turbulent,
+    limiter_k 5, limiter_zone "te" with radius 0.05 chord, turb_order 1,
+    dt_visc_c 1, zmin 0."""
+
+    def __init__(self, mesh, mach, re, chord=1.0, xref=(0.25, 0.0), chi_inf=3.0, order=2,
+                 vortex_bc=True, turbulent=True, limiter_zone="te", limiter_k=5.0):
+        self.m = mesh
+        self.mach, self.re, self.chord = float(mach), float(re), float(chord)
+        self.mu_inf = self.mach * chord / self.re
+        self.nu_inf = self.mu_inf
+        self.xref = xref
+        self.chi_inf = chi_inf if turbulent else 0.0
+        self.order = order
+        self.vortex_bc = vortex_bc
+        self.turbulent = turbulent
+        self.limiter_k = limiter_k
+        self.limiter_zone = limiter_zone
+        self.limiter_te_radius = 0.05
+        self.dt_visc_c = 1.0
+        self.zmin = 0.0
+        self.nv = 5
+        self._setup()
+
+    # ---------------------------------------------------------------- setup
+    def _setup(self):
+        """Face sets, least-squares matrices and the trailing-edge limiter zone (328-425)."""
+        m = self.m
+        nc = m.n_cells
+        self.nc = nc
+        fi = np.where(m.face_tag == 0)[0]
+        fw = np.where(m.face_tag == 1)[0]
+        ff = np.where(m.face_tag == 2)[0]
+        L, R = m.face_cells[fi, 0], m.face_cells[fi, 1]
+        xc = m.centroid
+        self.iL, self.iR = L, R
+        self.n_i, self.A_i = m.face_normal[fi], m.face_area[fi]
+        self.rL = m.face_center[fi] - xc[L]
+        self.rR = m.face_center[fi] - xc[R]
+        d = xc[R] - xc[L]
+        self.d_i = d
+        self.dist_i = np.hypot(d[:, 0], d[:, 1])
+        cw = m.face_cells[fw, 0]
+        self.cw = cw
+        self.n_w, self.A_w = m.face_normal[fw], m.face_area[fw]
+        self.r_w = m.face_center[fw] - xc[cw]
+        self.dn_w = np.maximum(np.einsum("ij,ij->i", self.r_w, self.n_w), 1e-14)
+        self.body_w = m.face_body[fw]
+        self.fc_w = m.face_center[fw]
+        cf = m.face_cells[ff, 0]
+        self.cf = cf
+        self.n_f, self.A_f = m.face_normal[ff], m.face_area[ff]
+        self.r_f = m.face_center[ff] - xc[cf]
+        self.fc_f = m.face_center[ff]
+        w_i = 1.0 / self.dist_i**2
+        Mxx, Mxy, Myy = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+        for idx in (L, R):
+            np.add.at(Mxx, idx, w_i * d[:, 0] ** 2)
+            np.add.at(Mxy, idx, w_i * d[:, 0] * d[:, 1])
+            np.add.at(Myy, idx, w_i * d[:, 1] ** 2)
+        dw = self.r_w
+        ww = 1.0 / np.maximum(np.sum(dw**2, 1), 1e-30)
+        np.add.at(Mxx, cw, ww * dw[:, 0] ** 2)
+        np.add.at(Mxy, cw, ww * dw[:, 0] * dw[:, 1])
+        np.add.at(Myy, cw, ww * dw[:, 1] ** 2)
+        det = Mxx * Myy - Mxy**2
+        det = np.where(np.abs(det) < 1e-300, 1e-300, det)
+        self.Minv = np.stack([np.stack([Myy / det, -Mxy / det], -1),
+                              np.stack([-Mxy / det, Mxx / det], -1)], -2)
+        self.w_i, self.w_w = w_i, ww
+        self.vol = m.volume
+        self.wall_d = np.maximum(m.wall_dist, 1e-12)
+        near = np.zeros(nc, bool)
+        for b in np.unique(self.body_w):
+            pts = self.fc_w[self.body_w == b]
+            te = pts[np.argmax(pts[:, 0])]
+            near |= np.hypot(xc[:, 0] - te[0], xc[:, 1] - te[1]) < self.limiter_te_radius * self.chord
+        self.lim_te = near
+        # torch copies
+        T = _t
+        self.J = dict(iL=torch.as_tensor(L), iR=torch.as_tensor(R), n_i=T(self.n_i), A_i=T(self.A_i),
+                      rL=T(self.rL), rR=T(self.rR), d_i=T(d), dist_i=T(self.dist_i), w_i=T(w_i),
+                      cw=torch.as_tensor(cw), n_w=T(self.n_w), A_w=T(self.A_w), r_w=T(self.r_w),
+                      dn_w=T(self.dn_w), w_w=T(ww), cf=torch.as_tensor(cf), n_f=T(self.n_f),
+                      A_f=T(self.A_f), r_f=T(self.r_f), fc_f=T(self.fc_f), Minv=T(self.Minv),
+                      vol=T(self.vol), wd=T(self.wall_d), lim_te=torch.as_tensor(near))
+        # limiter neighbour sets: each cell's interior neighbours and its wall faces,
+        # padded with -1 (the reference's scatter max/min of venkat, 983-1022)
+        self.nbr = _table(np.r_[L, R], np.r_[R, L], nc)
+        self.wtab = _table(cw, np.arange(len(cw)), nc)
+
+    # ----------------------------------------------------------- thermodynamics
+    def prim(self, Q):
+        """(rho, rho u, rho v, rho E, rho chi) -> (rho, u, v, p, chi) (455-461)."""
+        r = Q[:, 0]
+        u, v = Q[:, 1] / r, Q[:, 2] / r
+        p = (GAM - 1) * (Q[:, 3] - 0.5 * r * (u**2 + v**2))
+        return torch.cat([torch.stack([r, u, v, p], 1), Q[:, 4:] / r[:, None]], 1)
+
+    def cons(self, w):
+        """Inverse of prim (463-467)."""
+        r, u, v, p = (w[:, k] for k in range(4))
+        return torch.cat([torch.stack([r, r * u, r * v, p / (GAM - 1) + 0.5 * r * (u**2 + v**2)], 1),
+                          w[:, 4:] * r[:, None]], 1)
+
+    def freestream(self, alpha):
+        """(1, M cos a, M sin a, 1/gamma, chi_inf) (469-475)."""
+        a = math.radians(alpha)
+        return np.array([1.0, self.mach * math.cos(a), self.mach * math.sin(a), 1.0 / GAM, self.chi_inf])
+
+    def mu_lam(self, T):
+        """Sutherland's law in freestream units (477-478)."""
+        return self.mu_inf * T**1.5 * (1 + SUTH) / (T + SUTH)
+
+    # ----------------------------------------------------------------- fluxes
+    def roe(self, wL, wR, n):
+        """Roe flux with the low-Mach scaling z of the velocity jump (LM-Roe), entropy
+        fix 0.05 a, passive scalars upwinded by the mass flux (480-524)."""
+        nx, ny = n[:, 0], n[:, 1]
+        rL, uL, vL, pL = (wL[:, k] for k in range(4))
+        rR, uR, vR, pR = (wR[:, k] for k in range(4))
+        cL, cR = wL[:, 4:], wR[:, 4:]
+        unL, unR = uL * nx + vL * ny, uR * nx + vR * ny
+        HL = GAM / (GAM - 1) * pL / rL + 0.5 * (uL**2 + vL**2)
+        HR = GAM / (GAM - 1) * pR / rR + 0.5 * (uR**2 + vR**2)
+        FL = torch.stack([rL * unL, rL * uL * unL + pL * nx, rL * vL * unL + pL * ny, rL * HL * unL], 1)
+        FR = torch.stack([rR * unR, rR * uR * unR + pR * nx, rR * vR * unR + pR * ny, rR * HR * unR], 1)
+        sL, sR = torch.sqrt(rL), torch.sqrt(rR)
+        rh = sL * sR
+        uh = (sL * uL + sR * uR) / (sL + sR)
+        vh = (sL * vL + sR * vR) / (sL + sR)
+        Hh = (sL * HL + sR * HR) / (sL + sR)
+        q2 = uh**2 + vh**2
+        ah = torch.sqrt(torch.clamp_min((GAM - 1) * (Hh - 0.5 * q2), 1e-12))
+        unh = uh * nx + vh * ny
+        tx, ty = -ny, nx
+        uth = uh * tx + vh * ty
+        dr, dp = rR - rL, pR - pL
+        dun = unR - unL
+        dut = (uR - uL) * tx + (vR - vL) * ty
+        aL, aR = torch.sqrt(GAM * pL / rL), torch.sqrt(GAM * pR / rR)
+        z = torch.minimum(torch.ones_like(aL), torch.maximum(torch.sqrt(uL**2 + vL**2) / aL,
+                                                             torch.sqrt(uR**2 + vR**2) / aR))
+        z = torch.clamp_min(z, self.zmin)
+        a1 = (dp - rh * ah * z * dun) / (2 * ah**2)
+        a2 = dr - dp / ah**2
+        a3 = (dp + rh * ah * z * dun) / (2 * ah**2)
+        at = rh * dut
+
+        def fix(lam):
+            dl = 0.05 * ah
+            al = torch.abs(lam)
+            return torch.where(al < dl, (lam**2 + dl**2) / (2 * dl), al)
+        l1, l2, l3 = fix(unh - ah), torch.abs(unh), fix(unh + ah)
+        d0 = l1 * a1 + l2 * a2 + l3 * a3
+        d1 = l1 * a1 * (uh - ah * nx) + l2 * (a2 * uh + at * tx) + l3 * a3 * (uh + ah * nx)
+        d2 = l1 * a1 * (vh - ah * ny) + l2 * (a2 * vh + at * ty) + l3 * a3 * (vh + ah * ny)
+        d3 = l1 * a1 * (Hh - ah * unh) + l2 * (a2 * 0.5 * q2 + at * uth) + l3 * a3 * (Hh + ah * unh)
+        F = 0.5 * (FL + FR) - 0.5 * torch.stack([d0, d1, d2, d3], 1)
+        mf = F[:, 0]
+        F5 = 0.5 * mf[:, None] * (cL + cR) - 0.5 * torch.abs(mf)[:, None] * (cR - cL)
+        return torch.cat([F, F5], 1)
+
+    def sa_terms(self, rho, chi, T):
+        """mu, mu_t, X = nu~/nu, fv1, fn (526-535)."""
+        mu = self.mu_lam(T)
+        nu_t = chi * self.nu_inf
+        X = rho * nu_t / mu
+        fv1 = X**3 / (X**3 + CV1**3)
+        mut = torch.where(chi > 0, rho * nu_t * fv1, torch.zeros_like(chi))
+        fn = torch.where(chi >= 0, torch.ones_like(chi), (CN1 + X**3) / (CN1 - X**3))
+        return mu, mut, X, fv1, fn
+
+    def visc_flux(self, wf, gu, gv, gT, gc, n):
+        """Viscous flux through a face from face values and face gradients (537-571, SA)."""
+        rho, u, v, p, chi = (wf[:, k] for k in range(5))
+        T = GAM * p / rho
+        mu, mut, X, fv1, fn = self.sa_terms(rho, chi, T)
+        me = mu + mut
+        div = gu[:, 0] + gv[:, 1]
+        txx = me * (2 * gu[:, 0] - 2.0 / 3.0 * div)
+        tyy = me * (2 * gv[:, 1] - 2.0 / 3.0 * div)
+        txy = me * (gu[:, 1] + gv[:, 0])
+        nx, ny = n[:, 0], n[:, 1]
+        fx = txx * nx + txy * ny
+        fy = txy * nx + tyy * ny
+        k = (mu / PR + mut / PRT) / (GAM - 1)
+        fe = u * fx + v * fy + k * (gT[:, 0] * nx + gT[:, 1] * ny)
+        fs = (mu + rho * chi * self.nu_inf * fn) / SIG * (gc[:, 0] * nx + gc[:, 1] * ny)
+        return torch.stack([torch.zeros_like(fx), fx, fy, fe, fs], 1)
+
+    # --------------------------------------------------------------- gradients
+    def gradients(self, w, wall_vals):
+        """Weighted least-squares gradients of every primitive, walls by their values (809-819)."""
+        J = self.J
+        dphi = w[J["iR"]] - w[J["iL"]]
+        contrib = J["w_i"][:, None, None] * dphi[:, :, None] * J["d_i"][:, None, :]
+        rhs = torch.zeros((self.nc, w.shape[1], 2), dtype=DT)
+        rhs = rhs.index_add(0, J["iL"], contrib).index_add(0, J["iR"], contrib)
+        dw = wall_vals - w[J["cw"]]
+        rhs = rhs.index_add(0, J["cw"], J["w_w"][:, None, None] * dw[:, :, None] * J["r_w"][:, None, :])
+        return torch.einsum("cij,ckj->cki", J["Minv"], rhs)
+
+    def wall_values(self, w):
+        """No slip, chi = 0; rho and p of the wall cell (821-828, SA)."""
+        wc = w[self.J["cw"]]
+        z = torch.zeros_like(wc[:, 0])
+        return torch.stack([wc[:, 0], z, z, wc[:, 3], z], 1)
+
+    def far_state(self, winf, gam):
+        """Freestream plus a Prandtl-Glauert point vortex of circulation gam at xref,
+        isentropic rho and p from the local speed (830-858)."""
+        J = self.J
+        winf = torch.as_tensor(winf, dtype=DT)
+        x = J["fc_f"][:, 0] - self.xref[0]
+        y = J["fc_f"][:, 1] - self.xref[1]
+        M = self.mach
+        b = math.sqrt(1 - M**2)
+        vm = torch.sqrt(winf[1] ** 2 + winf[2] ** 2) + 1e-30
+        ca, sa = winf[1] / vm, winf[2] / vm
+        xr = x * ca + y * sa
+        yr = -x * sa + y * ca
+        th = torch.atan2(yr, xr)
+        r = torch.sqrt(xr**2 + yr**2)
+        fac = gam * b / (2 * math.pi * r * (1 - M**2 * torch.sin(th) ** 2))
+        du_r, dv_r = fac * torch.sin(th), -fac * torch.cos(th)
+        du = du_r * ca - dv_r * sa
+        dv = du_r * sa + dv_r * ca
+        U = winf[1] + du
+        V = winf[2] + dv
+        q2 = U**2 + V**2
+        T = 1 + 0.5 * (GAM - 1) * (M**2 - q2)
+        rho = T ** (1 / (GAM - 1))
+        p = rho * T / GAM
+        return torch.stack([rho, U, V, p, torch.full_like(rho, float(winf[4]))], 1)
+
+    def grad_T(self, w, g):
+        """grad T = gamma (grad p / rho - p grad rho / rho^2) (1024-1028)."""
+        rho, p = w[:, 0], w[:, 3]
+        return GAM * (g[:, 3] / rho[:, None] - (p / rho**2)[:, None] * g[:, 0])
+
+    # ----------------------------------------------------------------- limiter
+    def venkat(self, w, g, wv):
+        """Venkatakrishnan limiter of all five primitives (983-1022). The max and
+        min over a cell, its interior neighbours and its wall values follow JAX's
+        scatter max/min: equal candidates share the derivative evenly."""
+        J = self.J
+        iL, iR = J["iL"], J["iR"]
+        nbr, wtab = self.nbr, self.wtab
+        big = torch.tensor(float("inf"), dtype=DT)
+        wn = torch.where(torch.as_tensor(nbr >= 0)[:, :, None], w[torch.as_tensor(np.maximum(nbr, 0))], -big)
+        wvx = torch.where(torch.as_tensor(wtab >= 0)[:, :, None], wv[torch.as_tensor(np.maximum(wtab, 0))],
+                          -big)
+        cand_max = torch.cat([w[:, None, :], wn, wvx], 1)
+        wn2 = torch.where(torch.as_tensor(nbr >= 0)[:, :, None], w[torch.as_tensor(np.maximum(nbr, 0))], big)
+        wvn = torch.where(torch.as_tensor(wtab >= 0)[:, :, None], wv[torch.as_tensor(np.maximum(wtab, 0))],
+                          big)
+        cand_min = torch.cat([w[:, None, :], wn2, wvn], 1)
+        wmax = torch.amax(cand_max, 1)
+        wmin = torch.amin(cand_min, 1)
+        dmax, dmin = wmax - w, wmin - w
+        h = torch.sqrt(J["vol"])
+        scale = torch.tensor([1.0, self.mach, self.mach, 1.0 / GAM, max(self.chi_inf, 1.0)], dtype=DT)
+        eps2 = ((self.limiter_k * h) ** 3)[:, None] * scale[None, :] ** 2
+
+        def psi(d2, i):
+            d1 = torch.where(d2 > 0, dmax[i], dmin[i])
+            num = (d1**2 + eps2[i]) * d2 + 2 * d2**2 * d1
+            den = d2 * (d1**2 + 2 * d2**2 + d1 * d2 + eps2[i]) + 1e-300
+            return torch.where(torch.abs(d2) < 1e-14, torch.ones_like(d2),
+                               torch.minimum(torch.maximum(num / den, torch.zeros_like(d2)), torch.ones_like(d2)))
+        pL = psi(torch.einsum("fkj,fj->fk", g[iL], J["rL"]), iL)
+        pR = psi(torch.einsum("fkj,fj->fk", g[iR], J["rR"]), iR)
+        # min over {1, psi of every face side of the cell}
+        ftab = self._ftab()
+        allp = torch.cat([pL, pR, torch.full((1, w.shape[1]), float("inf"), dtype=DT)], 0)
+        cand = torch.cat([torch.ones((self.nc, 1, w.shape[1]), dtype=DT), allp[torch.as_tensor(ftab)]], 1)
+        return torch.amin(cand, 1)
+
+    def _ftab(self):
+        if getattr(self, "_ftab_c", None) is None:
+            nfi = len(self.iL)
+            t = _table(np.r_[self.iL, self.iR], np.r_[np.arange(nfi), nfi + np.arange(nfi)], self.nc)
+            self._ftab_c = np.where(t < 0, 2 * nfi, t)
+        return self._ftab_c
+
+    # ---------------------------------------------------------------- residual
+    def sa_source(self, w, g, wd):
+        """SA-neg source per unit volume in rho chi: rho (P - D)/nu_inf + cb2/sigma rho nu_inf |grad chi|^2 (609-629)."""
+        rho, u, v, p, chi = (w[:, k] for k in range(5))
+        T = GAM * p / rho
+        mu, mut, X, fv1, fn = self.sa_terms(rho, chi, T)
+        Om = torch.abs(g[:, 2, 0] - g[:, 1, 1])
+        nut = chi * self.nu_inf
+        fv2 = 1 - X / (1 + X * fv1)
+        Sbar = nut * fv2 / (KAP**2 * wd**2)
+        St = torch.where(Sbar >= -CV2 * Om, Om + Sbar,
+                         Om + Om * (CV2**2 * Om + CV3 * Sbar) / ((CV3 - 2 * CV2) * Om - Sbar + 1e-30))
+        St = torch.clamp_min(St, 1e-16)
+        r = torch.clamp_max(nut / (St * KAP**2 * wd**2), 10.0)
+        gg = r + CW2 * (r**6 - r)
+        fw = gg * ((1 + CW3**6) / (gg**6 + CW3**6)) ** (1.0 / 6.0)
+        prod = torch.where(chi >= 0, CB1 * St * chi, CB1 * Om * chi)
+        dest = torch.where(chi >= 0, CW1 * fw * nut * chi / wd**2, -CW1 * nut * chi / wd**2)
+        cross = CB2 / SIG * self.nu_inf * (g[:, 4, 0] ** 2 + g[:, 4, 1] ** 2)
+        return rho * (prod - dest + cross)
+
+    def residual(self, Q, winf, gam=0.0, order=None, phi_frozen=None):
+        """R (nc, 5): outward fluxes minus source times volume (860-964, SA path)."""
+        J = self.J
+        order = self.order if order is None else order
+        winf = torch.as_tensor(winf, dtype=DT)
+        w = self.prim(Q)
+        wv = self.wall_values(w)
+        g = self.gradients(w, wv)
+        iL, iR = J["iL"], J["iR"]
+        if order >= 2 and self.limiter_k > 0 and self.limiter_zone != "off":
+            phi = self.venkat(w, g, wv) if phi_frozen is None else torch.as_tensor(phi_frozen, dtype=DT)
+            if self.limiter_zone == "te":
+                phi = torch.where(J["lim_te"][:, None], phi, torch.ones_like(phi))
+            g = g * phi[:, :, None]
+        if order >= 2:
+            g = g.index_put((J["cf"],), torch.zeros((len(self.cf), 5, 2), dtype=DT))
+            wL = w[iL] + torch.einsum("fkj,fj->fk", g[iL], J["rL"])
+            wR = w[iR] + torch.einsum("fkj,fj->fk", g[iR], J["rR"])
+            okL = (wL[:, [0]] > 0.2 * w[iL][:, [0]]) & (wL[:, [3]] > 0.2 * w[iL][:, [3]])
+            okR = (wR[:, [0]] > 0.2 * w[iR][:, [0]]) & (wR[:, [3]] > 0.2 * w[iR][:, [3]])
+            wL = torch.where(okL, wL, w[iL])
+            wR = torch.where(okR, wR, w[iR])
+            wL = torch.cat([wL[:, :4], w[iL, 4:]], 1)         # turb_order 1
+            wR = torch.cat([wR[:, :4], w[iR, 4:]], 1)
+        else:
+            wL, wR = w[iL], w[iR]
+        Fc = self.roe(wL, wR, J["n_i"])
+        wf = 0.5 * (w[iL] + w[iR])
+        T = GAM * w[:, 3] / w[:, 0]
+        gvv = g
+        gT = self.grad_T(w, gvv)
+        e = J["d_i"] / J["dist_i"][:, None]
+
+        def fgrad(phiL, phiR, gl, gr):
+            ga = 0.5 * (gl + gr)
+            corr = (phiR - phiL) / J["dist_i"] - torch.sum(ga * e, 1)
+            return ga + corr[:, None] * e
+        gu = fgrad(w[iL, 1], w[iR, 1], gvv[iL, 1], gvv[iR, 1])
+        gv = fgrad(w[iL, 2], w[iR, 2], gvv[iL, 2], gvv[iR, 2])
+        gTf = fgrad(T[iL], T[iR], gT[iL], gT[iR])
+        gc = fgrad(w[iL, 4], w[iR, 4], gvv[iL, 4], gvv[iR, 4])
+        Fv = self.visc_flux(wf, gu, gv, gTf, gc, J["n_i"])
+        F = (Fc - Fv) * J["A_i"][:, None]
+        Rs = torch.zeros((self.nc, 5), dtype=DT).index_add(0, iL, F).index_add(0, iR, -F)
+        # wall
+        cw = J["cw"]
+        ww = w[cw]
+        pw = ww[:, 3] + (torch.sum(g[cw, 3] * J["r_w"], 1) if order >= 2 else 0.0)
+        pw = torch.where(pw > 0.2 * ww[:, 3], pw, ww[:, 3])
+        n = J["n_w"]
+        z = torch.zeros_like(pw)
+        Fcw = torch.stack([z, pw * n[:, 0], pw * n[:, 1], z, z], 1)
+        dn = J["dn_w"]
+
+        def wgrad(phiw, phii, gi):
+            corr = (phiw - phii) / dn - torch.sum(gi * n, 1)
+            return gi + corr[:, None] * n
+        gu_w = wgrad(z, ww[:, 1], gvv[cw, 1])
+        gv_w = wgrad(z, ww[:, 2], gvv[cw, 2])
+        gc_w = wgrad(z, ww[:, 4], gvv[cw, 4])
+        gT_w = gT[cw] - torch.sum(gT[cw] * n, 1)[:, None] * n
+        Fvw = self.visc_flux(wv, gu_w, gv_w, gT_w, gc_w, n)
+        Rs = Rs.index_add(0, cw, (Fcw - Fvw) * J["A_w"][:, None])
+        # far field
+        cf = J["cf"]
+        wi = w[cf] + (torch.einsum("fkj,fj->fk", g[cf], J["r_f"]) if order >= 2 else 0.0)
+        ok = (wi[:, [0]] > 0.2 * w[cf][:, [0]]) & (wi[:, [3]] > 0.2 * w[cf][:, [3]])
+        wi = torch.where(ok, wi, w[cf])
+        wg = self.far_state(winf, gam)
+        Rs = Rs.index_add(0, cf, self.roe(wi, wg, J["n_f"]) * J["A_f"][:, None])
+        if not self.turbulent:
+            return torch.cat([Rs[:, :4], Q[:, 4:] * J["vol"][:, None]], 1)
+        S = self.sa_source(w, g, J["wd"])
+        return torch.cat([Rs[:, :4], Rs[:, 4:] - S[:, None] * J["vol"][:, None]], 1)
+
+    def phi(self, Q):
+        """The limiter of a state (the _fns "phi" entry, 1192-1194), zone not applied."""
+        w = self.prim(Q)
+        wv = self.wall_values(w)
+        return self.venkat(w, self.gradients(w, wv), wv)
+
+    def jvp(self, Q, V, winf, gam=0.0, order=None, phi_frozen=None):
+        """jax.jvp of the residual (the jvp1/jvp2/jvp2f entries of _fns, 1186-1191)."""
+        return torch.func.jvp(lambda q: self.residual(q, winf, gam, order, phi_frozen), (Q,), (V,))[1]
+
+    # ---------------------------------------------------- first-order Jacobian
+    def _face_int(self, qL, qR, n, d, dist):
+        wL, wR = self.prim(qL[None])[0], self.prim(qR[None])[0]
+        Fc = self.roe(wL[None], wR[None], n[None])[0]
+        e = d / dist
+        TL, TR = GAM * wL[3] / wL[0], GAM * wR[3] / wR[0]
+        g2 = lambda a, b: ((b - a) / dist * e)[None]      # noqa: E731
+        wf = 0.5 * (wL + wR)
+        Fv = self.visc_flux(wf[None], g2(wL[1], wR[1]), g2(wL[2], wR[2]), g2(TL, TR),
+                            g2(wL[4], wR[4]), n[None])[0]
+        return Fc - Fv
+
+    def _face_wall(self, qi, n, dn):
+        wi = self.prim(qi[None])[0]
+        p = wi[3]
+        Fc = torch.stack([0.0 * p, p * n[0], p * n[1], 0.0 * p, 0.0 * p])
+        gw = lambda phi: (-phi / dn * n)[None]            # noqa: E731
+        wf = torch.stack([wi[0], 0.0 * p, 0.0 * p, p, 0.0 * p])
+        Fv = self.visc_flux(wf[None], gw(wi[1]), gw(wi[2]), torch.zeros((1, 2), dtype=DT), gw(wi[4]),
+                            n[None])[0]
+        return Fc - Fv
+
+    def _face_far(self, qi, wg, n):
+        wi = self.prim(qi[None])[0]
+        return self.roe(wi[None], wg[None], n[None])[0]
+
+    def jac1_parts(self, Q, winf, gam):
+        """Per-face first-order flux Jacobians and d(source)/dQ at frozen gradients (1032-1105, SA)."""
+        from torch.func import jacfwd, vmap
+        J = self.J
+        iL, iR = J["iL"], J["iR"]
+        jl, jr = vmap(jacfwd(self._face_int, argnums=(0, 1)))(Q[iL], Q[iR], J["n_i"], J["d_i"], J["dist_i"])
+        jw = vmap(jacfwd(self._face_wall))(Q[J["cw"]], J["n_w"], J["dn_w"])
+        wg = self.far_state(winf, gam)
+        jf = vmap(jacfwd(self._face_far))(Q[J["cf"]], wg, J["n_f"])
+        w = self.prim(Q)
+        g = self.gradients(w, self.wall_values(w))
+
+        def src(q, gi, wd):
+            return self.sa_source(self.prim(q[None]), gi[None], wd[None])[0]
+        js = vmap(jacfwd(src))(Q, g, J["wd"])[:, None, :]
+        return jl, jr, jw, jf, js
+
+    def jac1_blocks(self, Q, winf, gam):
+        """Diagonal blocks (nc,5,5) without V/dt and the off-diagonal blocks for the
+        pairs (iL,iR) then (iR,iL) (1107-1126)."""
+        J = self.J
+        jl, jr, jw, jf, js = self.jac1_parts(Q, winf, gam)
+        jl, jr = jl * J["A_i"][:, None, None], jr * J["A_i"][:, None, None]
+        jw, jf = jw * J["A_w"][:, None, None], jf * J["A_f"][:, None, None]
+        D = torch.zeros((self.nc, 5, 5), dtype=DT)
+        D = D.index_add(0, J["iL"], jl).index_add(0, J["iR"], -jr).index_add(0, J["cw"], jw)
+        D = D.index_add(0, J["cf"], jf)
+        D[:, 4, 4] += J["vol"] * torch.clamp_min(-js[:, 0, 4], 0.0)
+        if not self.turbulent:
+            D[:, 4:, :] = 0.0
+            D[:, 4, 4] = J["vol"]
+            jl = jl.clone()
+            jr = jr.clone()
+            jl[:, 4:, :] = 0.0
+            jr[:, 4:, :] = 0.0
+        return D, torch.cat([jr, -jl], 0)
+
+    def pattern(self):
+        """Block rows/cols of the first-order Jacobian: diagonal, (iL,iR), (iR,iL) (1128-1144)."""
+        nc = self.nc
+        r = np.concatenate([np.arange(nc), self.iL, self.iR])
+        c = np.concatenate([np.arange(nc), self.iR, self.iL])
+        return r, c
+
+    # ---------------------------------------------------------- time step, forces
+    def local_dt(self, Q, cfl):
+        """Local pseudo-time step with convective and viscous spectral radii (1206-1249, SA)."""
+        w = self.prim(torch.as_tensor(Q, dtype=DT)).numpy()
+        a = np.sqrt(GAM * w[:, 3] / w[:, 0])
+        lam = np.zeros(self.nc)
+        un = np.abs(0.5 * ((w[self.iL, 1] + w[self.iR, 1]) * self.n_i[:, 0]
+                           + (w[self.iL, 2] + w[self.iR, 2]) * self.n_i[:, 1]))
+        aa = 0.5 * (a[self.iL] + a[self.iR])
+        s = (un + aa) * self.A_i
+        np.add.at(lam, self.iL, s)
+        np.add.at(lam, self.iR, s)
+        np.add.at(lam, self.cf, (np.abs(w[self.cf, 1] * self.n_f[:, 0] + w[self.cf, 2] * self.n_f[:, 1])
+                                 + a[self.cf]) * self.A_f)
+        np.add.at(lam, self.cw, a[self.cw] * self.A_w)
+        T = GAM * w[:, 3] / w[:, 0]
+        mu = self.mu_inf * T**1.5 * (1 + SUTH) / (T + SUTH)
+        if self.turbulent:
+            nt = np.maximum(w[:, 4], 0) * self.nu_inf
+            X3 = (nt * w[:, 0] / mu) ** 3
+            nut = nt * X3 / (X3 + CV1**3)
+        else:
+            nut = 0.0 * mu
+        lv = np.zeros(self.nc)
+        sv = self.A_i**2
+        np.add.at(lv, self.iL, sv)
+        np.add.at(lv, self.iR, sv)
+        np.add.at(lv, self.cw, self.A_w**2)
+        lam += self.dt_visc_c * max(4.0 / 3.0, GAM) * (mu / w[:, 0] / PR + nut / PRT) * lv / self.vol
+        return cfl * self.vol / lam
+
+    def forces(self, Q, alpha):
+        """Pressure (second-order wall pressure) and friction forces (1251-1296)."""
+        Q = torch.as_tensor(Q, dtype=DT)
+        w = self.prim(Q)
+        g = self.gradients(w, self.wall_values(w)).numpy()
+        w = w.numpy()
+        cw, n, A, dn = self.cw, self.n_w, self.A_w, self.dn_w
+        ww = w[cw]
+        pw = ww[:, 3] + np.sum(g[cw, 3] * self.r_w, 1)
+        T = GAM * ww[:, 3] / ww[:, 0]
+        mu = self.mu_inf * T**1.5 * (1 + SUTH) / (T + SUTH)
+        ut = ww[:, 1] * (-n[:, 1]) + ww[:, 2] * n[:, 0]
+        tau = mu * ut / dn
+        q = 0.5 * self.mach**2
+        fxp = (pw - 1 / GAM) * n[:, 0] * A
+        fyp = (pw - 1 / GAM) * n[:, 1] * A
+        fxv = tau * (-n[:, 1]) * A
+        fyv = tau * n[:, 0] * A
+        a = math.radians(alpha)
+        c = self.chord
+        Lp = (-fxp * math.sin(a) + fyp * math.cos(a)).sum() / (q * c)
+        Dp = (fxp * math.cos(a) + fyp * math.sin(a)).sum() / (q * c)
+        Lv = (-fxv * math.sin(a) + fyv * math.cos(a)).sum() / (q * c)
+        Dv = (fxv * math.cos(a) + fyv * math.sin(a)).sum() / (q * c)
+        xm = self.fc_w[:, 0] - self.xref[0]
+        ym = self.fc_w[:, 1] - self.xref[1]
+        cm = -((xm * (fyp + fyv) - ym * (fxp + fxv)).sum()) / (q * c**2)
+        return dict(cl=Lp + Lv, cd=Dp + Dv, cdp=Dp, cdf=Dv, cm=cm, cp=(pw - 1 / GAM) / q, cf=tau / q)
+
+    def newton_update(self, Q, dq):
+        """The step limit of solve (1822-1847): rho and p may fall by at most 20 %;
+        when at most 100 cells ask for more, only those cells are scaled back,
+        else the whole step. Returns (Q + scaled dq, lam, number of limited cells)."""
+        Q = np.asarray(Q, float)
+        dq = np.asarray(dq, float)
+        w = self.prim(_t(Q)).numpy()
+        wn = self.prim(_t(Q + dq)).numpy()
+        reli = np.maximum(-(wn[:, 0] - w[:, 0]) / w[:, 0], -(wn[:, 3] - w[:, 3]) / w[:, 3])
+        rel = float(np.max(reli))
+        lam = 1.0 if rel < 0.2 else 0.2 / rel
+        nbig = int(np.count_nonzero(reli >= 0.2))
+        if lam < 1.0 and nbig <= 100:
+            lam_c = np.where(reli < 0.2, 1.0, 0.2 / np.maximum(reli, 1e-300))[:, None]
+            return Q + lam_c * dq, lam, nbig
+        return Q + lam * dq, lam, nbig
+
+
+def _table(cells, vals, nc):
+    """Row table: for each cell the values listed against it, in order, padded with -1."""
+    cells = np.asarray(cells)
+    vals = np.asarray(vals)
+    o = np.argsort(cells, kind="stable")
+    c_, v_ = cells[o], vals[o]
+    cnt = np.bincount(c_, minlength=nc)
+    start = np.r_[0, np.cumsum(cnt)[:-1]]
+    pos = np.arange(len(c_)) - start[c_]
+    T = np.full((nc, max(1, int(cnt.max()) if len(cnt) else 1)), -1, np.int64)
+    T[c_, pos] = v_
+    return T
+
+
+# ---------------------------------------------------------------------------
+# block ILU (aero2d/bilu.py:13-166) and GMRES (aero2d/krylov.py:14-87)
+# ---------------------------------------------------------------------------
+class BlockILU:
+    """Block ILU with fill, in a reverse Cuthill-McKee order (aero2d/bilu.py:13-166).
+
+    The pattern is the matrix's, grown `fill` times by P <- P + tril(P) triu(P)
+    (strict parts); the factorisation is row by row in the new order: each lower
+    block (i,k), in ascending k, becomes B_ik invU_k and is subtracted times
+    row k's upper blocks from the pattern blocks (i,j); invU_i = B_ii^-1. The
+    solve is a unit-lower forward sweep and an upper backward sweep with invU."""
+
+    def __init__(self, n, rows, cols, k=5, perm=None, fill=0):
+        import scipy.sparse as sp
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        rows, cols = np.asarray(rows), np.asarray(cols)
+        if perm is None:
+            G = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+            perm = reverse_cuthill_mckee((G + G.T).tocsr(), symmetric_mode=True)
+        inv = np.empty(n, int)
+        inv[perm] = np.arange(n)
+        self.n, self.k, self.perm, self.iperm, self.fill = n, k, np.asarray(perm), inv, int(fill)
+        r0, c0 = inv[rows], inv[cols]
+        P = sp.csr_matrix((np.ones(len(r0)), (r0, c0)), shape=(n, n))
+        for _ in range(int(fill)):
+            P = (P + sp.tril(P, -1).tocsr() @ sp.triu(P, 1).tocsr()).tocsr()
+        P = P.tocoo()
+        r, c = P.row.astype(np.int64), P.col.astype(np.int64)
+        o = np.lexsort((c, r))
+        self.r, self.c = r[o], c[o]
+        key = self.r * n + self.c
+        self.key = key
+        self.src_pos = np.searchsorted(key, r0.astype(np.int64) * n + c0)
+        self.start = np.searchsorted(self.r, np.arange(n))
+        self.end = np.searchsorted(self.r, np.arange(n), side="right")
+        self.diag = np.searchsorted(key, np.arange(n, dtype=np.int64) * (n + 1))
+
+    def factor(self, blocks):
+        n = self.n
+        B = np.zeros((len(self.r), self.k, self.k))
+        B[self.src_pos] = np.asarray(blocks, float)
+        invU = np.zeros((n, self.k, self.k))
+        pos = {}
+        for i in range(n):
+            a, b, d = self.start[i], self.end[i], self.diag[i]
+            cols = {int(self.c[e]): e for e in range(a, b)}
+            for e in range(a, d):                       # lower blocks, ascending column
+                kk = int(self.c[e])
+                B[e] = B[e] @ invU[kk]
+                for e2 in range(self.diag[kk] + 1, self.end[kk]):
+                    j = int(self.c[e2])
+                    t = cols.get(j)
+                    if t is not None:
+                        B[t] -= B[e] @ B[e2]
+            invU[i] = np.linalg.inv(B[d])
+        del pos
+        self.B, self.invU = B, invU
+        return self
+
+    def solve(self, b):
+        k = self.k
+        y = np.asarray(b, float).reshape(-1, k)[self.perm].copy()
+        B, c = self.B, self.c
+        for i in range(self.n):
+            for e in range(self.start[i], self.diag[i]):
+                y[i] -= B[e] @ y[c[e]]
+        x = y
+        for i in range(self.n - 1, -1, -1):
+            for e in range(self.diag[i] + 1, self.end[i]):
+                x[i] -= B[e] @ x[c[e]]
+            x[i] = self.invU[i] @ x[i]
+        out = np.empty_like(x)
+        out[self.perm] = x
+        return out.ravel()
+
+
+def gmres(matvec, b, prec=None, x0=None, rtol=1e-2, maxiter=60, right=False):
+    """GMRES without restarts, CGS2, Givens rotations; left preconditioning by
+    default, right with right=True (aero2d/krylov.py:14-87). Returns (x, info,
+    relative residual estimates)."""
+    hist = []
+    if right and prec is not None:
+        if x0 is not None:
+            r0 = b - matvec(x0)
+            y, info, h = gmres(lambda v: matvec(prec(v)), r0, None, None, rtol, maxiter)
+            return x0 + prec(y), info, h
+        y, info, h = gmres(lambda v: matvec(prec(v)), b, None, None, rtol, maxiter)
+        return prec(y), info, h
+    P = prec if prec is not None else (lambda v: v)
+    n = b.shape[0]
+    x = np.zeros(n) if x0 is None else np.array(x0, float)
+    r = P(b - matvec(x)) if x0 is not None else P(b)
+    beta = float(np.linalg.norm(r))
+    if beta == 0.0 or not np.isfinite(beta):
+        return x, 0, hist
+    m = int(maxiter)
+    V = np.empty((m + 1, n))
+    V[0] = r / beta
+    H = np.zeros((m + 1, m))
+    cs, sn = np.zeros(m), np.zeros(m)
+    g = np.zeros(m + 1)
+    g[0] = beta
+    k_done, info = 0, m
+    for j in range(m):
+        w = P(matvec(V[j]))
+        h = V[:j + 1] @ w
+        w -= h @ V[:j + 1]
+        h2 = V[:j + 1] @ w
+        w -= h2 @ V[:j + 1]
+        h += h2
+        hn = float(np.linalg.norm(w))
+        H[:j + 1, j] = h
+        H[j + 1, j] = hn
+        for i in range(j):
+            t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+            H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+            H[i, j] = t
+        d = np.hypot(H[j, j], H[j + 1, j])
+        if d == 0.0:
+            k_done = j
+            break
+        cs[j], sn[j] = H[j, j] / d, H[j + 1, j] / d
+        H[j, j] = d
+        H[j + 1, j] = 0.0
+        g[j + 1] = -sn[j] * g[j]
+        g[j] = cs[j] * g[j]
+        k_done = j + 1
+        rel = abs(g[j + 1]) / beta
+        hist.append(rel)
+        if not np.isfinite(rel):
+            break
+        if rel <= rtol or hn == 0.0:
+            info = 0
+            break
+        V[j + 1] = w / hn
+    k = k_done
+    if k == 0:
+        return x, info, hist
+    y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+    x += y @ V[:k]
+    return x, info, hist

@@ -1,0 +1,176 @@
+"""RANS (SA-neg) HIP kernels vs the CPU oracle (oracle/rans.py), FP64.
+
+Meshes: hybrid meshes from small O-grids (quadrilateral wall layers, triangles
+outside) and an all-quadrilateral one. States: the freestream with a smooth
+seeded perturbation of every variable, so limiter, Roe branches and the SA
+source take their generic branches. Tolerances: the kernels evaluate the
+oracle's formulas in another order, so values agree at round-off: 1e-11 of
+each field's magnitude for residual, Jacobian-vector product, Jacobian blocks,
+local step and forces; 1e-9 for the ILU solve (round-off amplified by the
+conditioning of the factors); 1e-8 for a Newton step's GMRES update after 20
+Arnoldi steps."""
+import numpy as np
+import pytest
+import torch
+
+from aero2d_amd import naca4, ogrid
+from aero2d_amd.rans import RansSolver, ilu_structure
+from aero2d_amd.rans_mesh import ogrid_mesh
+from oracle.rans import BlockILU, RansOracle, gmres
+
+pytestmark = pytest.mark.gpu
+
+MESHES = {  # name -> (n_side, nj, quad_layers)
+    "hybrid": (17, 13, 6),
+    "quads": (13, 9, None),
+}
+
+
+def _mesh(name):
+    n_side, nj, nq = MESHES[name]
+    g = ogrid(naca4("0012", n_side), nj=nj, ds0=2e-2 if n_side < 17 else 1e-2, radius=6.0)
+    return ogrid_mesh(g, quad_layers=nq)
+
+
+def _state(o, alpha, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    winf = o.freestream(alpha)
+    xc = o.m.centroid
+    W = np.tile(winf, (o.nc, 1))
+    for k in range(5):
+        c = rng.normal(size=4)
+        f = c[0] * np.sin(2.1 * xc[:, 0] + c[1]) * np.cos(1.7 * xc[:, 1] + c[2]) + 0.3 * rng.normal(size=o.nc)
+        scale = amp * (abs(winf[k]) if k != 2 else o.mach) + (0.3 if k == 4 else 0.0)
+        W[:, k] = W[:, k] + scale * f
+    W[:, 1:3] *= np.clip(o.m.wall_dist / 0.05, 0.2, 1.0)[:, None]
+    return o.cons(torch.tensor(W)).numpy()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return float((np.abs(a - b) / scale).max())
+
+
+def _pair(name, mach=0.5, re=2e4, turbulent=True, zone="te", order=2):
+    m = _mesh(name)
+    s = RansSolver(m, mach, re, order=order, turbulent=turbulent)
+    s.limiter_zone = zone
+    o = RansOracle(m, mach, re, order=order, turbulent=turbulent, limiter_zone=zone)
+    return m, s, o
+
+
+@pytest.mark.parametrize("name,turbulent,zone,order", [
+    ("hybrid", True, "te", 2), ("hybrid", True, "all", 2), ("hybrid", True, "off", 2),
+    ("hybrid", True, "te", 1), ("quads", False, "all", 2), ("quads", True, "all", 1),
+])
+def test_residual_matches_oracle(name, turbulent, zone, order):
+    m, s, o = _pair(name, turbulent=turbulent, zone=zone, order=order)
+    Q = _state(o, 2.0, 11)
+    winf = o.freestream(2.0)
+    for gam in (0.0, 0.21):
+        Rg = s.residual(Q, winf, gam)
+        Ro = o.residual(torch.tensor(Q), winf, gam).numpy()
+        assert _rel(Rg, Ro) < 1e-11, (name, zone, order, gam, _rel(Rg, Ro))
+
+
+@pytest.mark.parametrize("name,zone,order,frozen", [
+    ("hybrid", "te", 2, False), ("hybrid", "all", 2, False), ("hybrid", "all", 2, True),
+    ("hybrid", "te", 1, False), ("quads", "off", 2, False),
+])
+def test_jvp_matches_oracle(name, zone, order, frozen):
+    m, s, o = _pair(name, zone=zone, order=order)
+    Q = _state(o, 1.0, 12)
+    winf = o.freestream(1.0)
+    v = np.random.default_rng(3).normal(size=Q.shape)
+    phi = o.phi(torch.tensor(Q)).numpy() if frozen else None
+    Jg = s.jvp(Q, v, winf, 0.17, phi_frozen=phi)
+    Jo = o.jvp(torch.tensor(Q), torch.tensor(v), winf, 0.17, phi_frozen=phi).numpy()
+    assert _rel(Jg, Jo) < 1e-11, _rel(Jg, Jo)
+
+
+def test_limiter_local_dt_forces_match_oracle():
+    m, s, o = _pair("hybrid")
+    Q = _state(o, 3.0, 13)
+    assert _rel(s.phi(Q), o.phi(torch.tensor(Q)).numpy()) < 1e-12
+    assert _rel(s.local_dt(Q, 7.0), o.local_dt(Q, 7.0)) < 1e-12
+    fg, fo = s.forces(Q, 3.0), o.forces(Q, 3.0)
+    for k in ("cl", "cd", "cdp", "cdf", "cm"):
+        assert abs(fg[k] - fo[k]) < 1e-11 * max(1.0, abs(fo[k])), (k, fg[k], fo[k])
+    assert _rel(fg["cp"], fo["cp"]) < 1e-11 and _rel(fg["cf"], fo["cf"]) < 1e-11
+
+
+@pytest.mark.parametrize("turbulent", [True, False])
+def test_jacobian_blocks_match_oracle(turbulent):
+    m, s, o = _pair("hybrid", turbulent=turbulent)
+    Q = _state(o, 2.0, 14)
+    winf = o.freestream(2.0)
+    Dg, Og = s.jac1_blocks(Q, winf, 0.1)
+    Do, Oo = o.jac1_blocks(torch.tensor(Q), winf, 0.1)
+    assert _rel(Dg.reshape(len(Dg), -1), Do.numpy().reshape(len(Dg), -1)) < 1e-11
+    assert _rel(Og.reshape(len(Og), -1), Oo.numpy().reshape(len(Og), -1)) < 1e-11
+
+
+def _oracle_ilu(o, Q, winf, gam, cfl, fill):
+    D, off = o.jac1_blocks(torch.tensor(Q), winf, gam)
+    D = D.numpy().copy()
+    dt = o.local_dt(Q, cfl)
+    D[:, np.arange(5), np.arange(5)] += (o.vol / dt)[:, None]
+    r, c = o.pattern()
+    ilu = BlockILU(o.nc, r, c, k=5, fill=fill)
+    return ilu.factor(np.concatenate([D, off.numpy()])), dt
+
+
+@pytest.mark.parametrize("fill", [0, 3])
+def test_ilu_factor_and_solve_match_oracle(fill):
+    m, s, o = _pair("hybrid")
+    s.prec_fill = fill
+    Q = _state(o, 2.0, 15)
+    winf = o.freestream(2.0)
+    perm, r, c, pos = ilu_structure(s.nc, *s.pattern(), fill=fill)
+    ilu, _ = _oracle_ilu(o, Q, winf, 0.1, 20.0, fill)
+    assert np.array_equal(ilu.perm, perm) and np.array_equal(ilu.r, r) and np.array_equal(ilu.c, c)
+    assert s.ilu_factor(Q, winf, 0.1, 20.0) == 0
+    b = np.random.default_rng(4).normal(size=(s.nc, 5))
+    xg = s.ilu_apply(b)
+    xo = ilu.solve(b.ravel()).reshape(-1, 5)
+    assert _rel(xg, xo) < 1e-9, _rel(xg, xo)
+    info = s.ilu_info()
+    assert info["levels"] > 1 and info["nnz"] == len(r)
+
+
+def test_newton_step_matches_oracle_gmres():
+    m, s, o = _pair("hybrid")
+    s.prec_fill = 1
+    Q = _state(o, 2.0, 16, amp=0.02)
+    winf = o.freestream(2.0)
+    gam, cfl = 0.05, 10.0
+    ilu, dt = _oracle_ilu(o, Q, winf, gam, cfl, 1)
+    Qt = torch.tensor(Q)
+    R = o.residual(Qt, winf, gam).numpy()
+    vdt = np.repeat((o.vol / dt)[:, None], 5, 1).ravel()
+
+    def mv(x):
+        return o.jvp(Qt, torch.tensor(x.reshape(-1, 5)), winf, gam).numpy().ravel() + vdt * x
+    dq_o, info, hist = gmres(mv, -R.ravel(), prec=ilu.solve, rtol=0.0, maxiter=20)
+    st = s.newton_step(2, winf, gam, cfl, 0.0, 20, Q=Q)
+    assert st["gmres_iterations"] == 20 and st["ilu_nonfinite"] == 0
+    dq_g = s.newton_dq()
+    assert _rel(dq_g, dq_o.reshape(-1, 5)) < 1e-8, _rel(dq_g, dq_o.reshape(-1, 5))
+    lin_o = np.linalg.norm(mv(dq_o) + R.ravel()) / np.linalg.norm(R)
+    assert abs(st["lin"] - lin_o) < 1e-8 * max(1.0, lin_o)
+    if st["applied"]:
+        Qn, lam, nbig = o.newton_update(Q, dq_o.reshape(-1, 5))
+        assert abs(st["lam"] - lam) < 1e-9 and st["nbig"] == nbig
+        assert _rel(s._get(), Qn) < 1e-8
+
+
+def test_refuses_bad_arguments():
+    from aero2d_amd import _abi
+    m, s, o = _pair("quads")
+    Q = _state(o, 0.0, 1)
+    winf = o.freestream(0.0)
+    with pytest.raises(_abi.HipError):
+        s.residual(Q, winf, 0.0, order=3)
+    with pytest.raises(_abi.HipError):
+        s.newton_step(2, winf, 0.0, 5.0, 0.01, gmres_iter=500, Q=Q)
