@@ -722,78 +722,102 @@ __global__ void k_jac_diag(MeshDev m, Params P, const double* Q, const double* g
 }
 
 // ---- block ILU in level order ------------------------------------------------------
-// One workgroup walks the levels; rows of a level are independent. In the
-// factorisation a group of 32 lanes owns a row, lane t < 25 owning element
-// (t/5, t%5) of every 5x5 block product; row elements are exchanged by lane
-// shuffles. Lane 0 of the group inverts the diagonal block (Gauss-Jordan,
-// partial pivoting).
+// One workgroup walks the levels; rows of a level are independent and the
+// workgroup barrier separates levels. Level bounds are staged in LDS and each
+// row's work is described by one 16-byte record (row, first entry, end entry),
+// so a level costs a record load, the gathers and the barrier.
+//
+// Factorisation: one wavefront per row. Each half-wavefront holds the current
+// lower block L_ik = B_ik invU_k, lane t < 25 owning element (t/5, t%5) (row
+// elements exchanged by lane shuffles), and the two halves take alternate
+// update pairs (i,j) -= L_ik (k,j). A fence orders the lanes' block updates
+// before the next lower block and the diagonal are read; lane 0 inverts the
+// diagonal block (Gauss-Jordan, partial pivoting).
 struct IluDev {
   int n, nnz;
-  const int *rstart, *rdiag, *rend, *col;
+  const int *rdiag, *col;
   const int *pptr, *pt, *pe2;        // per lower entry: its update pairs (target, row-k entry)
   int nlev, nlevu;
-  const int *lptr, *lrows, *uptr, *urows;
+  const int *lptr, *uptr;            // level bounds into frec / brec
+  const int4 *frec, *brec;           // (row, first entry, end entry, 0): lower entries by
+                                     // forward level, upper entries by backward level
   const int* perm;                   // perm[new] = old
+  int lds_levels;                    // level bounds fit in LDS
 };
 
-constexpr int ILU_TPB = 512;
+constexpr int ILU_TPB = 1024;
+constexpr int ILU_LDS_LEVELS = 12288;   // level bounds staged in LDS up to this many
+
+__device__ __forceinline__ const int* stage_bounds(const int* g, int n, int* sh, bool use) {
+  if (!use) return g;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) sh[t] = g[t];
+  __syncthreads();
+  return sh;
+}
 
 __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor(IluDev L, double* B, double* invU) {
-  const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31, ngrp = blockDim.x >> 5;
-  const bool act = lane < 25;
-  const int r = act ? lane / 5 : 4, cc = act ? lane % 5 : 4;
+  extern __shared__ int sh_lev[];
+  const int* lp = stage_bounds(L.lptr, L.nlev + 1, sh_lev, L.lds_levels);
+  const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int half = (threadIdx.x >> 5) & 1, hl = threadIdx.x & 31;
+  const bool act = hl < 25;
+  const int r = act ? hl / 5 : 4, cc = act ? hl % 5 : 4;
   for (int l = 0; l < L.nlev; ++l) {
-    for (int p = L.lptr[l] + grp; p < L.lptr[l + 1]; p += ngrp) {
-      const int i = L.lrows[p];
-      const int d = L.rdiag[i];
-      for (int e = L.rstart[i]; e < d; ++e) {
+    const int p1 = lp[l + 1];
+    for (int p = lp[l] + wave; p < p1; p += nwave) {
+      const int4 rec = L.frec[p];
+      const int i = rec.x, d = L.rdiag[i];
+      for (int e = rec.y; e < rec.z; ++e) {
         const int k = L.col[e];
         const double be = B[size_t(e) * 25 + r * 5 + cc];
         double s = 0.0;
 #pragma unroll
         for (int mm = 0; mm < 5; ++mm) s += __shfl(be, r * 5 + mm, 32) * invU[size_t(k) * 25 + mm * 5 + cc];
-        if (act) B[size_t(e) * 25 + lane] = s;
-        for (int q = L.pptr[e]; q < L.pptr[e + 1]; ++q) {
+        if (act && half == 0) B[size_t(e) * 25 + hl] = s;
+        const int q1 = L.pptr[e + 1];
+        for (int q = L.pptr[e] + half; q < q1; q += 2) {
           const size_t t = size_t(L.pt[q]) * 25, e2 = size_t(L.pe2[q]) * 25;
           double acc = 0.0;
 #pragma unroll
           for (int mm = 0; mm < 5; ++mm) acc += __shfl(s, r * 5 + mm, 32) * B[e2 + mm * 5 + cc];
-          if (act) B[t + lane] -= acc;
+          if (act) B[t + hl] -= acc;
         }
+        __threadfence_block();   // this row's block updates, by either half, before the next reads
       }
-      __threadfence_block();   // the row's block updates by the other lanes are visible to lane 0
-      if (lane == 0) {
-        double a[5][10];
-        for (int x = 0; x < 5; ++x)
-          for (int y = 0; y < 5; ++y) {
-            a[x][y] = B[size_t(d) * 25 + x * 5 + y];
-            a[x][5 + y] = x == y ? 1.0 : 0.0;
+      // invU_i = B_ii^-1: Gauss-Jordan with partial pivoting, lane (r, cc)
+      // holding element (r, cc) of the block and of the inverse being built
+      double av = B[size_t(d) * 25 + r * 5 + cc];
+      double iv = r == cc ? 1.0 : 0.0;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        int piv = c;
+        double best = fabs(__shfl(av, c * 5 + c, 32));
+#pragma unroll
+        for (int x = c + 1; x < 5; ++x) {
+          const double v = fabs(__shfl(av, x * 5 + c, 32));
+          if (v > best) {
+            best = v;
+            piv = x;
           }
-        for (int col = 0; col < 5; ++col) {
-          int piv = col;
-          double best = fabs(a[col][col]);
-          for (int x = col + 1; x < 5; ++x)
-            if (fabs(a[x][col]) > best) {
-              best = fabs(a[x][col]);
-              piv = x;
-            }
-          if (piv != col)
-            for (int y = 0; y < 10; ++y) {
-              double t = a[col][y];
-              a[col][y] = a[piv][y];
-              a[piv][y] = t;
-            }
-          const double inv = 1.0 / a[col][col];
-          for (int y = 0; y < 10; ++y) a[col][y] *= inv;
-          for (int x = 0; x < 5; ++x)
-            if (x != col) {
-              const double fct = a[x][col];
-              for (int y = 0; y < 10; ++y) a[x][y] -= fct * a[col][y];
-            }
         }
-        for (int x = 0; x < 5; ++x)
-          for (int y = 0; y < 5; ++y) invU[size_t(i) * 25 + x * 5 + y] = a[x][5 + y];
+        if (piv != c) {
+          const int src = r == c ? piv : (r == piv ? c : r);
+          av = __shfl(av, src * 5 + cc, 32);
+          iv = __shfl(iv, src * 5 + cc, 32);
+        }
+        const double inv = 1.0 / __shfl(av, c * 5 + c, 32);
+        if (r == c) {
+          av *= inv;
+          iv *= inv;
+        }
+        const double f = __shfl(av, r * 5 + c, 32);
+        const double ac = __shfl(av, c * 5 + cc, 32), ic = __shfl(iv, c * 5 + cc, 32);
+        if (r != c) {
+          av -= f * ac;
+          iv -= f * ic;
+        }
       }
+      if (act && half == 0) invU[size_t(i) * 25 + hl] = iv;
     }
     __syncthreads();
   }
@@ -816,45 +840,61 @@ __global__ void k_perm_out(int n, const int* perm, const double* y, double* x) {
 }
 
 // Forward (unit lower) and backward (upper, then invU) sweeps over the levels,
-// in place on y; eight lanes per row, lane k < 5 owning component k
-// (aero2d/bilu.py:148-166).
+// in place on y (aero2d/bilu.py:148-166). Eight lanes share a row: each takes
+// every eighth of the row's blocks and forms its 5x5 block-vector products,
+// the eight partial sums are combined by a butterfly of lane shuffles, and
+// lane k < 5 writes component k.
+__device__ __forceinline__ void row_products(const double* B, const int* col, const double* y, int e0, int e1,
+                                             int lane, double a[5]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) a[k] = 0.0;
+  for (int e = e0 + lane; e < e1; e += 8) {
+    const double* yc = y + size_t(col[e]) * 5;
+    const double* b = B + size_t(e) * 25;
+    const double y0 = yc[0], y1 = yc[1], y2 = yc[2], y3 = yc[3], y4 = yc[4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      a[k] += b[k * 5] * y0 + b[k * 5 + 1] * y1 + b[k * 5 + 2] * y2 + b[k * 5 + 3] * y3 + b[k * 5 + 4] * y4;
+  }
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a[k] += __shfl_xor(a[k], off, 8);
+}
+
 __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B, const double* invU,
                                                        double* y) {
+  extern __shared__ int sh_lev[];
+  const bool use = L.lds_levels;
+  const int* lp = stage_bounds(L.lptr, L.nlev + 1, sh_lev, use);
+  const int* up = stage_bounds(L.uptr, L.nlevu + 1, sh_lev + (use ? L.nlev + 1 : 0), use);
   const int grp = threadIdx.x >> 3, lane = threadIdx.x & 7, ngrp = blockDim.x >> 3;
-  const int k = lane < 5 ? lane : 4;
+  double a[5];
   for (int l = 0; l < L.nlev; ++l) {
-    for (int p = L.lptr[l] + grp; p < L.lptr[l + 1]; p += ngrp) {
-      const int i = L.lrows[p];
-      double acc = 0.0;
-      for (int e = L.rstart[i]; e < L.rdiag[i]; ++e) {
-        const double* b = B + size_t(e) * 25 + k * 5;
-        const double* yc = y + size_t(L.col[e]) * 5;
-        double s = 0.0;
+    const int p1 = lp[l + 1];
+    for (int p = lp[l] + grp; p < p1; p += ngrp) {
+      const int4 rec = L.frec[p];
+      row_products(B, L.col, y, rec.y, rec.z, lane, a);
+      double mine = a[0];
 #pragma unroll
-        for (int mm = 0; mm < 5; ++mm) s += b[mm] * yc[mm];
-        acc += s;
-      }
-      if (lane < 5) y[size_t(i) * 5 + k] -= acc;
+      for (int k = 1; k < 5; ++k) mine = lane == k ? a[k] : mine;
+      if (lane < 5) y[size_t(rec.x) * 5 + lane] -= mine;
     }
     __syncthreads();
   }
   for (int l = 0; l < L.nlevu; ++l) {
-    for (int p = L.uptr[l] + grp; p < L.uptr[l + 1]; p += ngrp) {
-      const int i = L.urows[p];
-      double acc = 0.0;
-      for (int e = L.rdiag[i] + 1; e < L.rend[i]; ++e) {
-        const double* b = B + size_t(e) * 25 + k * 5;
-        const double* xc = y + size_t(L.col[e]) * 5;
-        double s = 0.0;
+    const int p1 = up[l + 1];
+    for (int p = up[l] + grp; p < p1; p += ngrp) {
+      const int4 rec = L.brec[p];
+      const int i = rec.x;
+      row_products(B, L.col, y, rec.y, rec.z, lane, a);
+      double t[5];
 #pragma unroll
-        for (int mm = 0; mm < 5; ++mm) s += b[mm] * xc[mm];
-        acc += s;
-      }
-      const double t = y[size_t(i) * 5 + k] - acc;
-      double xi = 0.0;
-#pragma unroll
-      for (int mm = 0; mm < 5; ++mm) xi += invU[size_t(i) * 25 + k * 5 + mm] * __shfl(t, mm, 8);
-      if (lane < 5) y[size_t(i) * 5 + k] = xi;
+      for (int k = 0; k < 5; ++k) t[k] = y[size_t(i) * 5 + k] - a[k];
+      const int kk = lane < 5 ? lane : 4;
+      const double* iu = invU + size_t(i) * 25 + kk * 5;
+      const double xi = iu[0] * t[0] + iu[1] * t[1] + iu[2] * t[2] + iu[3] * t[3] + iu[4] * t[4];
+      if (lane < 5) y[size_t(i) * 5 + lane] = xi;
     }
     __syncthreads();
   }
@@ -1154,7 +1194,8 @@ void mdot(Solver& s, const double* Vb, size_t ldv, int nvec, const double* w, in
 void ilu_solve(Solver& s, const double* b, double* x) {
   const int n = s.m.nc;
   hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s.st, n, s.ilu.perm, b, s.y);
-  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, 0, s.st, s.ilu, s.B, s.invU, s.y);
+  const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + s.ilu.nlevu + 2) * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, s.y);
   hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s.st, n, s.ilu.perm, s.y, x);
 }
 
@@ -1303,7 +1344,8 @@ void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Do
       }
     return;
   }
-  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, 0, s.st, s.ilu, s.B, s.invU);
+  const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU);
   HC(hipGetLastError());
 }
 
@@ -1666,14 +1708,18 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     std::vector<int> lptr, lrows, uptr, urows;
     group(lev, nlev, lptr, lrows);
     group(levu, nlevu, uptr, urows);
+    std::vector<int4> frec(n), brec(n);
+    for (int p = 0; p < n; ++p) {
+      const int i = lrows[p], j = urows[p];
+      frec[p] = make_int4(i, rstart[i], rdiag[i], 0);
+      brec[p] = make_int4(j, rdiag[j] + 1, rend[j], 0);
+    }
     // release an earlier setup
     if (s.has_ilu) fail(-6, "the ILU pattern is already set");
     IluDev& L = s.ilu;
     L.n = n;
     L.nnz = nnz;
-    L.rstart = s.own(dupload(rstart));
     L.rdiag = s.own(dupload(rdiag));
-    L.rend = s.own(dupload(rend));
     L.col = s.own(dupload(std::vector<int>(cols, cols + nnz)));
     L.pptr = s.own(dupload(pptr));
     L.pt = s.own(dupload(pt));
@@ -1681,10 +1727,11 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     L.nlev = nlev;
     L.nlevu = nlevu;
     L.lptr = s.own(dupload(lptr));
-    L.lrows = s.own(dupload(lrows));
     L.uptr = s.own(dupload(uptr));
-    L.urows = s.own(dupload(urows));
+    L.frec = s.own(dupload(frec));
+    L.brec = s.own(dupload(brec));
     L.perm = s.own(dupload(permv));
+    L.lds_levels = nlev + nlevu + 2 <= ILU_LDS_LEVELS;
     s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
     s.posLR = s.own(dupload(std::vector<int>(pos_src + n, pos_src + n + nfi)));
     s.posRL = s.own(dupload(std::vector<int>(pos_src + n + nfi, pos_src + n + 2 * nfi)));

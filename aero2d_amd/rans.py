@@ -669,7 +669,11 @@ class RansSolver:
                     tb = max(1.0, tb / 4.0)
             ct = (self.cfl_turb if self.cfl_turb is not None else 1.0) * tb
             gtol = gmres_tol if (warm and n_back == 0) else min(gmres_tol, 0.01)
-            st = self.newton_step(order, winf, gm, cfl, gtol, gmres_iter, turb_factor=ct)
+            # laminar solves take right preconditioning: left preconditioning
+            # "converges" on |M^-1 r| while the true residual stays near 0.2
+            # (aero2d/rans.py:1781-1790)
+            st = self.newton_step(order, winf, gm, cfl, gtol, gmres_iter, turb_factor=ct,
+                                  right=self.right_prec or not self.turbulent)
             if st["ilu_nonfinite"]:
                 if verbose:
                     print(f"  it {it:4d}: ILU not finite, CFL {cfl:.3g} -> {max(cfl0, cfl / 10):.3g}", flush=True)

@@ -121,6 +121,38 @@ def connectivity(nodes, cells, wall_edges: dict, far_edges: set) -> Mesh:
     return Mesh(nodes, ccw, cen, vol, fc, fn, normal, area, fcen, tag, body, d)
 
 
+def refine_wall(grid, ds_wall: float, layers: int = 1, ratio: float = 1.2):
+    """O-grid whose first `layers` wall layers are re-sampled so the first cell
+    is ds_wall high (y+ ~ 1 meshes, which the marching generator cannot start
+    with directly). Each wall-normal node line is cut at the same fractions of
+    its arc length up to node `layers`: a geometric sequence from ds_wall / H
+    (H the mean arc length to that node) whose count gives a ratio close to
+    `ratio`. Returns an OGrid with the added node layers."""
+    from .grid import OGrid, growth_ratio
+    x, y = np.asarray(grid.x, float), np.asarray(grid.y, float)
+    seg = np.hypot(np.diff(x, axis=1), np.diff(y, axis=1))          # (ni, nj-1)
+    s = np.concatenate([np.zeros((x.shape[0], 1)), np.cumsum(seg, 1)], 1)
+    H = float(s[:, layers].mean())
+    if not 0 < ds_wall < H:
+        raise ValueError("ds_wall must be positive and below the height of the re-sampled layers")
+    n = max(layers, int(np.ceil(np.log(1 + H * (ratio - 1) / ds_wall) / np.log(ratio))))
+    q = growth_ratio(ds_wall / H, 1.0, n)
+    t = np.concatenate([[0.0], np.cumsum((ds_wall / H) * q ** np.arange(n))])
+    t /= t[-1]
+    nx = np.empty((x.shape[0], n + 1))
+    ny = np.empty_like(nx)
+    for i in range(x.shape[0]):
+        st = t * s[i, layers]
+        nx[i] = np.interp(st, s[i, :layers + 1], x[i, :layers + 1])
+        ny[i] = np.interp(st, s[i, :layers + 1], y[i, :layers + 1])
+    X = np.concatenate([nx, x[:, layers + 1:]], 1)
+    Y = np.concatenate([ny, y[:, layers + 1:]], 1)
+    g = OGrid(X, Y)
+    if g.cell_areas().min() <= 0:
+        raise RuntimeError("refined O-grid has inverted cells")
+    return g
+
+
 def ogrid_mesh(grid, quad_layers: int | None = None) -> Mesh:
     """Hybrid mesh from an O-grid's nodes (grid.x, grid.y of shape (ni, nj)).
 
