@@ -1842,6 +1842,56 @@ int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, d
   });
 }
 
+int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, double gam, int reps, double* ms) {
+  return guarded([&] {
+    Solver& s = get(h);
+    if (reps < 1) fail(-6, "reps must be >= 1");
+    if (which < 0 || which > 4) fail(-6, "which must be 0..4");
+    if ((which == 0 || which == 1) && !s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
+    far_state(s, winf, gam);
+    // inputs: the state's residual as the vector, a factorisation in place
+    eval(s, order, nullptr, s.R, nullptr, nullptr, 1.0);
+    if (which == 0 || which == 1) {
+      hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, 10.0, s.dt, s.vdt);
+      factor(s, winf, gam, 1.0, nullptr, nullptr);
+    }
+    auto once = [&] {
+      switch (which) {
+        case 0: ilu_solve(s, s.R, s.tmp); break;
+        case 1: {
+          const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
+          hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU);
+          break;
+        }
+        case 2: eval(s, order, s.R, nullptr, s.tmp, s.vdt, 1.0); break;
+        case 3: eval(s, order, nullptr, s.tmp, nullptr, nullptr, 1.0); break;
+        case 4: {
+          hipLaunchKernelGGL(k_jac_int, nblk(s.m.nfi), TPB, 0, s.st, s.m, s.P, s.Q, s.J, nullptr, s.posLR, s.posRL);
+          hipLaunchKernelGGL(k_jac_wall, nblk(s.m.nfw), TPB, 0, s.st, s.m, s.P, s.Q, s.JW);
+          hipLaunchKernelGGL(k_jac_far, nblk(s.m.nff), TPB, 0, s.st, s.m, s.P, s.Q, s.wgf, s.JF);
+          hipLaunchKernelGGL(k_jac_diag, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, s.g, s.J, s.JW, s.JF, s.vdt,
+                             1.0, nullptr, s.posD, s.tmp2);
+          break;
+        }
+      }
+    };
+    once();
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipEventRecord(e0, s.st));
+    for (int r = 0; r < reps; ++r) once();
+    HC(hipEventRecord(e1, s.st));
+    HC(hipEventSynchronize(e1));
+    float t = 0.f;
+    HC(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HC(hipGetLastError());
+    *ms = double(t) / reps;
+  });
+}
+
 int a2d_rans_newton_dq(int64_t h, double* dq) {
   return guarded([&] {
     Solver& s = get(h);

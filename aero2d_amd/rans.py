@@ -436,6 +436,19 @@ class RansSolver:
             d[k] = int(d[k])
         return d
 
+    KERNELS = ("ilu_solve", "ilu_factor", "matvec", "residual", "jacobian")
+
+    def time_kernel(self, which, alpha=0.0, order=None, gam=0.0, reps=20):
+        """Mean device time in ms of one of KERNELS at the device state
+        (a2d_rans_time_kernel)."""
+        self._ensure_ilu()
+        k = self.KERNELS.index(which)
+        ms = ctypes.c_double(0.0)
+        w = np.ascontiguousarray(self.freestream(alpha), np.float64)
+        _abi.call("a2d_rans_time_kernel", self._device(), k, int(self.order if order is None else order),
+                  _abi.dptr(w), float(gam), int(reps), ctypes.byref(ms))
+        return ms.value
+
     def newton_dq(self):
         dq = np.empty((self.nc, 5))
         _abi.call("a2d_rans_newton_dq", self._device(), _abi.dptr(dq))
@@ -496,7 +509,7 @@ class RansSolver:
         r0 = None
         cfl = cfl0
         self._last_ok, self._last_lin = True, 0.0
-        H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[])
+        H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[])
         events = []
         best = dict(rn=np.inf, has=turning, it=0, cfl=cfl0, order=0)
         if turning:
@@ -680,6 +693,8 @@ class RansSolver:
                 cfl = max(cfl0, cfl / 10)
                 continue
             lin = st["lin"]
+            H["gmres"].append(st["gmres_iterations"])
+            H["lin"].append(lin)
             self._last_ok = bool(np.isfinite(lin) and lin < 0.5)
             self._last_lin = lin
             if not st["applied"]:

@@ -670,11 +670,117 @@ def run_config5(a):
     return out
 
 
+def rans_mesh(n_side=97, nj=65, ds_wall=4e-6):
+    """NACA 0012 hybrid mesh for the RANS workload: an O-grid (20 chords, first
+    layer 1e-3 re-sampled to a wall cell of ds_wall, y+ ~ 1 at Re 6e6), quads in
+    the first 60 layers, triangles beyond."""
+    from aero2d_amd import naca4, ogrid
+    from aero2d_amd.rans_mesh import ogrid_mesh, refine_wall
+    g = refine_wall(ogrid(naca4("0012", n_side), nj=nj, ds0=1e-3, radius=20.0), ds_wall, layers=1, ratio=1.2)
+    return ogrid_mesh(g, quad_layers=60)
+
+
+def rans_cpu_baseline(m, mach, re, alpha, gmres_per_iter, fill=3):
+    """The oracle (PyTorch CPU FP64, oracle/rans.py) timed on the host cores:
+    residual, Jacobian-vector product, first-order Jacobian, BlockILU factor and
+    solve, each once on the same mesh and state, composed into one Newton
+    iteration with the GPU run's mean GMRES iterations per Newton iteration."""
+    import torch
+    from oracle.rans import BlockILU, RansOracle
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    o = RansOracle(m, mach, re)
+    winf = o.freestream(alpha)
+    f = np.clip(o.wall_d / 0.02, 0, 1)[:, None]
+    w0 = np.tile(winf, (o.nc, 1))
+    w0[:, 1:3] *= f
+    Q = o.cons(torch.tensor(w0))
+    v = torch.tensor(np.random.default_rng(0).normal(size=(o.nc, 5)))
+    t = {}
+    t0 = time.perf_counter()
+    R = o.residual(Q, winf, 0.0)
+    t["residual"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.jvp(Q, v, winf, 0.0)
+    t["jvp"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    D, off = o.jac1_blocks(Q, winf, 0.0)
+    D = D.numpy().copy()
+    D[:, np.arange(5), np.arange(5)] += (o.vol / o.local_dt(Q.numpy(), 10.0))[:, None]
+    t["jacobian"] = time.perf_counter() - t0
+    r, c = o.pattern()
+    t0 = time.perf_counter()
+    ilu = BlockILU(o.nc, r, c, fill=fill).factor(np.concatenate([D, off.numpy()]))
+    t["ilu_factor"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ilu.solve(R.numpy().ravel())
+    t["ilu_solve"] = time.perf_counter() - t0
+    per_iter = (t["residual"] + t["jacobian"] + t["ilu_factor"]
+                + (gmres_per_iter + 2) * (t["jvp"] + t["ilu_solve"]))
+    return {"value": 1.0 / per_iter, "unit": "Newton iterations/s", "cores": cores, "kind": "port",
+            "sample": "one oracle residual, JVP, first-order Jacobian, ILU(3) factor and solve on the same "
+                      f"{o.nc}-cell mesh ({', '.join(f'{k} {v:.2f} s' for k, v in t.items())}), composed into "
+                      f"a Newton iteration with {gmres_per_iter:.1f} GMRES iterations"}
+
+
+def run_rans(a):
+    """RANS (SA-neg) Newton-Krylov on the device: Newton iterations per second
+    of a cold-start solve (NACA 0012, M 0.15, Re 6e6, alpha 0, the reference's
+    default settings), the device time of its kernels and the roofline of the
+    dominant one (the ILU triangular solve)."""
+    from aero2d_amd.rans import RansSolver
+    m = rans_mesh()
+    mach, re, alpha = 0.15, 6e6, 0.0
+    s = RansSolver(m, mach, re)
+    info = s.ilu_info()
+    iters = a.n_iter or 80
+    for _ in range(a.warmup):
+        s.solve(alpha, max_iter=5, verbose=False)
+    t0 = time.perf_counter()
+    r = s.solve(alpha, max_iter=iters, verbose=False)
+    wall = time.perf_counter() - t0
+    gm = [int(x) for x in r.history.get("gmres", [])] if "gmres" in r.history else []
+    its = r.iterations
+    line = {"metric": "RANS Newton iterations/s (SA-neg, exact JVP, ILU(3)-GMRES)", "value": its / wall,
+            "unit": "Newton iterations/s", "n_gpus": 1, "steps": its, "warmup": a.warmup,
+            "ms_per_step": 1e3 * wall / its, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"RANS SA NACA 0012 M {mach} Re {re:g} alpha {alpha}, {m.n_cells}-cell hybrid "
+                                   "mesh (y+ ~ 1), cold start, the reference's solver defaults",
+                       "cells": m.n_cells, "ilu_nnz_blocks": info["nnz"], "ilu_levels": info["levels"],
+                       "max_iter": iters},
+            "solve": {"status": r.status, "iterations": its, "cl": r.cl, "cd": r.cd, "cdp": r.cdp, "cdf": r.cdf,
+                      "residual_first": r.residual[0] if r.residual else None,
+                      "residual_last": r.residual[-1] if r.residual else None,
+                      "gmres_iterations_mean": float(np.mean(r.history["gmres"])) if "gmres" in r.history else None}}
+    kt = {k: s.time_kernel(k, alpha, reps=10 if k == "ilu_factor" else 30) for k in s.KERNELS}
+    line["kernel_ms"] = kt
+    n, nnz = m.n_cells, info["nnz"]
+    nfi = int((m.face_tag == 0).sum())
+    solve_bytes = 200 * (nnz + n) + 160 * n
+    line["roofline"] = {"bound": "hbm", "achieved": solve_bytes / (kt["ilu_solve"] * 1e-3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "traffic": None, "bytes_per_launch": solve_bytes,
+                        "ms_per_launch": kt["ilu_solve"],
+                        "frac": solve_bytes / (kt["ilu_solve"] * 1e-3) / 8e12,
+                        "kernel": "k_ilu_solve (block ILU(3) forward and backward sweeps, one workgroup)",
+                        "note": f"latency bound: {info['levels'] + info['levels_back']} dependent levels, "
+                                f"{1e3 * kt['ilu_solve'] / (info['levels'] + info['levels_back']):.2f} us each"}
+    mv_bytes = 8 * (22 * n + 12 * nfi)
+    line["roofline_matvec"] = {"bound": "hbm", "achieved": mv_bytes / (kt["matvec"] * 1e-3) / 1e9, "peak": 8000.0,
+                               "unit": "GB/s", "bytes_per_launch": mv_bytes, "ms_per_launch": kt["matvec"],
+                               "frac": mv_bytes / (kt["matvec"] * 1e-3) / 8e12,
+                               "kernel": "Jacobian-vector product (k_grad, k_limit, k_face, k_cell in dual numbers)"}
+    if not a.no_cpu_baseline:
+        gpi = line["solve"]["gmres_iterations_mean"] or 10.0
+        line["cpu_baseline"] = rans_cpu_baseline(m, mach, re, alpha, gpi)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="config3",
                     choices=["config1", "config2", "config3", "config4", "config5", "kernels", "kernels3",
-                             "explicit", "explicit_kernels"])
+                             "explicit", "explicit_kernels", "rans"])
     ap.add_argument("--rates", default="1,2,4,10,25,100,400", help="config5: mixed-form request rates")
     ap.add_argument("--rates64", default="0.25,0.5,1,10,100", help="config5: FP64 request rates")
     ap.add_argument("--horizon", type=float, default=60.0)
@@ -704,7 +810,7 @@ def main():
     if a.workload != "config2":
         fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,
               "kernels": run_kernels, "kernels3": run_kernels3, "explicit": run_explicit,
-              "explicit_kernels": run_explicit_kernels}[a.workload]
+              "explicit_kernels": run_explicit_kernels, "rans": run_rans}[a.workload]
         print(json.dumps(fn(a)), flush=True)
         return
 

@@ -285,6 +285,12 @@ int a2d_rans_ilu_apply(int64_t h, const double* b, double* x);
  * cells beyond 0.2, per-cell limit used, 0, 0, 0. */
 int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
                          double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out);
+/* Bench hook: mean device time (ms, HIP events on the solver's stream) of one
+ * of: 0 the ILU solve (BlockILU0.solve), 1 the ILU factorisation, 2 the
+ * Newton matrix-vector product (Jacobian-vector product plus vol/dt), 3 the
+ * residual, 4 the first-order Jacobian blocks; reps repetitions at the state.
+ * 0 and 1 factor the Jacobian at CFL 10 first. No counterpart in the reference. */
+int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, double gam, int reps, double* ms);
 /* The last Newton step's dq before the step limit (nc, 5). */
 int a2d_rans_newton_dq(int64_t h, double* dq);
 /* 0: keep the state as the best; 1: return to the best; 2: undo the last step. */

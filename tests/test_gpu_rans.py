@@ -53,6 +53,7 @@ def _rel(a, b):
 
 
 def _pair(name, mach=0.5, re=2e4, turbulent=True, zone="te", order=2):
+    """(mesh, device solver, oracle) with the same constants."""
     m = _mesh(name)
     s = RansSolver(m, mach, re, order=order, turbulent=turbulent)
     s.limiter_zone = zone
@@ -174,3 +175,39 @@ def test_refuses_bad_arguments():
         s.residual(Q, winf, 0.0, order=3)
     with pytest.raises(_abi.HipError):
         s.newton_step(2, winf, 0.0, 5.0, 0.01, gmres_iter=500, Q=Q)
+
+
+def test_newton_trajectory_follows_the_oracle():
+    """Ten first-order pseudo-transient Newton steps from the faded freestream
+    (CFL 5 x 1.25^k, ILU(1)-preconditioned left GMRES to 1e-2, the step limit)
+    on the device and in the oracle: the residual norms agree step by step, so
+    the device's linear algebra, step limit and update are the oracle's."""
+    m, s, o = _pair("quads", turbulent=True, re=5e5)
+    s.prec_fill = 1
+    winf = o.freestream(2.0)
+    Q = s.initial_state(winf)
+    Qo = Q.copy()
+    r, c = o.pattern()
+    cfl = 5.0
+    for it in range(10):
+        Rg = s.residual(Q, winf, 0.0, order=1)
+        st = s.newton_step(1, winf, 0.0, cfl, 1e-2, 60, Q=Q, right=False)
+        Q = s._get()
+        Qt = torch.tensor(Qo)
+        Ro = o.residual(Qt, winf, 0.0, order=1).numpy()
+        assert _rel(Rg, Ro) < 1e-9, (it, _rel(Rg, Ro))
+        D, off = o.jac1_blocks(Qt, winf, 0.0)
+        D = D.numpy().copy()
+        dt = o.local_dt(Qo, cfl)
+        D[:, np.arange(5), np.arange(5)] += (o.vol / dt)[:, None]
+        ilu = BlockILU(o.nc, r, c, fill=1).factor(np.concatenate([D, off.numpy()]))
+        vdt = np.repeat((o.vol / dt)[:, None], 5, 1).ravel()
+
+        def mv(x):
+            return o.jvp(Qt, torch.tensor(x.reshape(-1, 5)), winf, 0.0, order=1).numpy().ravel() + vdt * x
+        dq, info, h = gmres(mv, -Ro.ravel(), prec=ilu.solve, rtol=1e-2, maxiter=60)
+        assert st["gmres_iterations"] == len(h), (it, st["gmres_iterations"], len(h))
+        Qo, lam, nbig = o.newton_update(Qo, dq.reshape(-1, 5))
+        assert abs(st["lam"] - lam) < 1e-8
+        cfl *= 1.25
+    assert _rel(Q, Qo) < 1e-7, _rel(Q, Qo)
