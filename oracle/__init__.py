@@ -14,6 +14,11 @@ Modules:
           (aero2d/implicit.py, jax.scipy.sparse.linalg.gmres)
   explicit  four-stage RK with local time steps, implicit residual smoothing,
           FAS V-cycle and the chunked host loop (aero2d/euler.py:219-435)
+  rans    SA-neg RANS on an unstructured mesh: LM-Roe, viscous fluxes, least-
+          squares gradients, Venkatakrishnan limiter, SA source, far field with
+          the point vortex, first-order Jacobian blocks, local step, forces,
+          step limit (aero2d/rans.py); block ILU(k) (aero2d/bilu.py); GMRES
+          with CGS2 (aero2d/krylov.py)
 
 Pinning (what checks each function, all in tests/test_oracle_*.py):
   cons_to_prim/prim_to_cons  round trip (tests/test_euler.py:27-29 of the reference)
@@ -43,6 +48,20 @@ Pinning (what checks each function, all in tests/test_oracle_*.py):
   explicit.iterate/cycle/    the solve drives R(U) to zero; that state is a
   solve                      fixed point of both; Newton steps of `newton`
                              from it leave it (same zero of the residual)
+  rans.roe                   consistency F(w,w) = physical flux, antisymmetry,
+                             supersonic upwinding (tests/test_oracle_rans.py)
+  rans.visc_flux             closed-form shear layer stress and conduction
+  rans.sa_terms/sa_source    fv1(chi = cv1) = 1/2, fw(r = 1) = 1, source at r = 1
+  rans.gradients             exact for linear fields away from walls
+  rans.residual              freestream preservation, mirror symmetry at alpha 0,
+                             telescoping to boundary fluxes
+  rans.jvp / jac1_parts      central differences, linearity
+  rans.BlockILU              exact for block-tridiagonal matrices (the
+                             reference's tests/test_rans.py:13-31 pattern) and
+                             with complete fill; fill grows the pattern
+  rans.gmres                 left and right preconditioning converge on a dense
+                             system (tests/test_rans.py:201-209 pattern)
+  rans.local_dt / forces     linear in CFL; uniform pressure gives no force
 Parity unpinned: the per-iteration trajectory of JAX's GMRES (its internal
 re-orthogonalisation test and normal-equation least squares are restated from
 JAX's published source, not run); only converged answers are pinned.
