@@ -211,3 +211,21 @@ def test_newton_trajectory_follows_the_oracle():
         assert abs(st["lam"] - lam) < 1e-8
         cfl *= 1.25
     assert _rel(Q, Qo) < 1e-7, _rel(Q, Qo)
+
+
+def test_laminar_naca0012_drag_matches_the_literature():
+    """Laminar NACA 0012 at M 0.5, Re 5000, alpha 0 (the classic viscous test
+    case; published total drag 0.0556-0.0576, pressure part ~0.022-0.024,
+    friction ~0.033-0.034) on a 22k-cell hybrid mesh: the solve converges by the
+    reference's rule, the lift is zero by symmetry and the drag components fall
+    in the published band. zmin = 0.1 (RansSolver.zmin) keeps LM-Roe's velocity
+    dissipation from vanishing at the stagnation point."""
+    from aero2d_amd.rans_mesh import refine_wall
+    g = refine_wall(ogrid(naca4("0012", 97), nj=65, ds0=1e-3, radius=20.0), 2e-4, layers=1, ratio=1.2)
+    m = ogrid_mesh(g, quad_layers=60)
+    s = RansSolver(m, 0.5, 5000.0, turbulent=False)
+    s.zmin = 0.1
+    r = s.solve(0.0, max_iter=200)
+    assert r.converged, r.status
+    assert abs(r.cl) < 1e-3
+    assert 0.053 < r.cd < 0.060 and 0.021 < r.cdp < 0.026 and 0.031 < r.cdf < 0.036, (r.cd, r.cdp, r.cdf)
