@@ -755,7 +755,15 @@ __device__ __forceinline__ const int* stage_bounds(const int* g, int n, int* sh,
   return sh;
 }
 
-__global__ __launch_bounds__(ILU_TPB) void k_ilu_factor(IluDev L, double* B, double* invU) {
+// Batched form: workgroup b works on case cases[b] (the cases share the
+// structure; their blocks, inverses and vectors are sB, sI, sY apart), or on
+// the arrays as given when cases is null.
+__global__ __launch_bounds__(ILU_TPB) void k_ilu_factor(IluDev L, double* B, double* invU, const int* cases,
+                                                        size_t sB, size_t sI) {
+  if (cases) {
+    B += size_t(cases[blockIdx.x]) * sB;
+    invU += size_t(cases[blockIdx.x]) * sI;
+  }
   extern __shared__ int sh_lev[];
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, sh_lev, L.lds_levels);
   const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -863,7 +871,13 @@ __device__ __forceinline__ void row_products(const double* B, const int* col, co
 }
 
 __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B, const double* invU,
-                                                       double* y) {
+                                                       double* y, const int* cases, size_t sB, size_t sI,
+                                                       size_t sY) {
+  if (cases) {
+    B += size_t(cases[blockIdx.x]) * sB;
+    invU += size_t(cases[blockIdx.x]) * sI;
+    y += size_t(cases[blockIdx.x]) * sY;
+  }
   extern __shared__ int sh_lev[];
   const bool use = L.lds_levels;
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, sh_lev, use);
@@ -1111,20 +1125,49 @@ struct Solver {
     owned.push_back((void*)p);
     return p;
   }
+  bool owns_stream = false;
   ~Solver() {
+    if (Vb) (void)hipFree(Vb);
     for (void* p : owned) (void)hipFree(p);
-    if (st) (void)hipStreamDestroy(st);
+    if (st && owns_stream) (void)hipStreamDestroy(st);
   }
 };
 
-std::map<int64_t, std::unique_ptr<Solver>> g_solvers;
+// A solver handle: case slots sharing one mesh, its ILU structure and one
+// stream. Slot 0 owns the mesh arrays; the ILU blocks, inverses and work
+// vectors of all slots are contiguous, so the triangular kernels can take
+// many cases in one launch (one workgroup each); the Arnoldi scalars of all
+// slots are contiguous too (one copy to the host per Arnoldi step).
+constexpr int HSTRIDE = 2 * (GMRES_MAX + 1) + 8;
+struct Group {
+  std::vector<std::unique_ptr<Solver>> cases;
+  int cur = 0;
+  double *Ball = nullptr, *Iall = nullptr, *Yall = nullptr, *hall = nullptr, *smallall = nullptr;
+  int *nfall = nullptr, *dcases = nullptr;
+  size_t sB = 0, sI = 0, sY = 0;
+  std::vector<int> dcases_host;
+  ~Group() {
+    while (cases.size() > 1) cases.pop_back();   // slot 0 (mesh, stream) last
+    cases.clear();
+    for (void* p : {(void*)Ball, (void*)Iall, (void*)Yall, (void*)hall, (void*)smallall, (void*)nfall,
+                    (void*)dcases})
+      if (p) (void)hipFree(p);
+  }
+};
+
+std::map<int64_t, std::unique_ptr<Group>> g_solvers;
 int64_t g_next = 1;
 
-Solver& get(int64_t h) {
+Group& get_group(int64_t h) {
   auto it = g_solvers.find(h);
   if (it == g_solvers.end()) fail(-3, "unknown RANS solver handle");
   require_device();
   return *it->second;
+}
+
+Solver& get(int64_t h) {
+  Group& g = get_group(h);
+  return *g.cases[g.cur];
 }
 
 inline int nblk(int n, int t = TPB) { return (n + t - 1) / t; }
@@ -1195,7 +1238,8 @@ void ilu_solve(Solver& s, const double* b, double* x) {
   const int n = s.m.nc;
   hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s.st, n, s.ilu.perm, b, s.y);
   const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + s.ilu.nlevu + 2) * sizeof(int) : 0;
-  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, s.y);
+  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, s.y, nullptr, size_t(0), size_t(0),
+                     size_t(0));
   hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s.st, n, s.ilu.perm, s.y, x);
 }
 
@@ -1319,7 +1363,8 @@ double lin_residual(Solver& s, int order, double ct_inv, const double* dq, doubl
   return norm2(s, s.tmp2) / std::max(rnorm, 1e-300);
 }
 
-void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Dout, double* OffOut) {
+void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Dout, double* OffOut,
+            bool launch_factor = true) {
   if (!Dout && !s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
   const MeshDev& m = s.m;
   double* B = Dout ? nullptr : s.B;
@@ -1344,8 +1389,9 @@ void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Do
       }
     return;
   }
+  if (!launch_factor) return;
   const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
-  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU);
+  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, nullptr, size_t(0), size_t(0));
   HC(hipGetLastError());
 }
 
@@ -1379,6 +1425,298 @@ void download_cells(Solver& s, const double* src, double* aos) {
   HC(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s.st));
   HC(hipStreamSynchronize(s.st));
   to_aos(s.m.nc, soa, aos);
+}
+
+// Device arrays of one case slot (everything but the group-level ones).
+void alloc_case(Solver& S) {
+  const int nc = S.m.nc, nfi = S.m.nfi, nfw = S.m.nfw, nff = S.m.nff;
+  const size_t n5 = size_t(nc) * 5;
+  S.Q = S.own(dalloc<double>(n5));
+  S.Qprev = S.own(dalloc<double>(n5));
+  S.Qbest = S.own(dalloc<double>(n5));
+  S.R = S.own(dalloc<double>(n5));
+  S.dq = S.own(dalloc<double>(n5));
+  S.dq2 = S.own(dalloc<double>(n5));
+  S.tmp = S.own(dalloc<double>(n5));
+  S.tmp2 = S.own(dalloc<double>(std::max(n5, size_t(nc) * 25)));
+  S.bvec = S.own(dalloc<double>(n5));
+  S.g = S.own(dalloc<double>(2 * n5));
+  S.gt = S.own(dalloc<double>(2 * n5));
+  S.phi = S.own(dalloc<double>(n5));
+  S.phit = S.own(dalloc<double>(n5));
+  S.phi_frozen = S.own(dalloc<double>(n5));
+  S.zero5 = S.own(dalloc<double>(n5));
+  HC(hipMemset(S.zero5, 0, n5 * sizeof(double)));
+  S.F = S.own(dalloc<double>(size_t(nfi) * 5));
+  S.Ft = S.own(dalloc<double>(size_t(nfi) * 5));
+  S.wgf = S.own(dalloc<double>(size_t(nff) * 5));
+  S.dt = S.own(dalloc<double>(nc));
+  S.vdt = S.own(dalloc<double>(nc));
+  S.reli = S.own(dalloc<double>(nc));
+  S.cpbuf = S.own(dalloc<double>(2 * size_t(nfw)));
+  S.nparts_cell = nblk(nc);
+  const int nbv = (5 * nc + VCHUNK - 1) / VCHUNK;
+  S.part = S.own(dalloc<double>(std::max(size_t(5) * S.nparts_cell, size_t(GMRES_MAX + 1) * nbv) +
+                                2 * S.nparts_cell));
+  S.J = S.own(dalloc<double>(size_t(nfi) * 50));
+  S.JW = S.own(dalloc<double>(size_t(nfw) * 25));
+  S.JF = S.own(dalloc<double>(size_t(nff) * 25));
+}
+
+// (Re)allocates the group-level arrays for the current slots and points every
+// slot at its part: Arnoldi scalars, small results, non-finite counters and,
+// once the ILU structure is set, the blocks, inverses and work vectors.
+void relink(Group& g) {
+  const int nc = int(g.cases.size());
+  Solver& s0 = *g.cases[0];
+  for (void** p : {(void**)&g.hall, (void**)&g.smallall, (void**)&g.nfall, (void**)&g.dcases})
+    if (*p) {
+      HC(hipFree(*p));
+      *p = nullptr;
+    }
+  g.hall = dalloc<double>(size_t(nc) * HSTRIDE);
+  g.smallall = dalloc<double>(size_t(nc) * 64);
+  g.nfall = dalloc<int>(nc);
+  g.dcases = dalloc<int>(nc);
+  g.dcases_host.clear();
+  for (int c = 0; c < nc; ++c) {
+    Solver& s = *g.cases[c];
+    s.hdev = g.hall + size_t(c) * HSTRIDE;
+    s.small = g.smallall + size_t(c) * 64;
+    s.nonfinite = g.nfall + c;
+  }
+  if (!s0.has_ilu) return;
+  for (void** p : {(void**)&g.Ball, (void**)&g.Iall, (void**)&g.Yall})
+    if (*p) {
+      HC(hipFree(*p));
+      *p = nullptr;
+    }
+  g.sB = size_t(s0.ilu.nnz) * 25;
+  g.sI = size_t(s0.ilu.n) * 25;
+  g.sY = size_t(s0.ilu.n) * 5;
+  g.Ball = dalloc<double>(g.sB * nc);
+  g.Iall = dalloc<double>(g.sI * nc);
+  g.Yall = dalloc<double>(g.sY * nc);
+  for (int c = 0; c < nc; ++c) {
+    Solver& s = *g.cases[c];
+    s.ilu = s0.ilu;
+    s.posD = s0.posD;
+    s.posLR = s0.posLR;
+    s.posRL = s0.posRL;
+    s.has_ilu = true;
+    s.B = g.Ball + size_t(c) * g.sB;
+    s.invU = g.Iall + size_t(c) * g.sI;
+    s.y = g.Yall + size_t(c) * g.sY;
+  }
+}
+
+// The device list of case slots for a batched launch.
+const int* case_list(Group& g, const std::vector<int>& ids) {
+  if (ids != g.dcases_host) {
+    HC(hipStreamSynchronize(g.cases[0]->st));
+    HC(hipMemcpy(g.dcases, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    g.dcases_host = ids;
+  }
+  return g.dcases;
+}
+
+// ---- batched linear algebra over case slots -----------------------------------------
+// The same operations as ilu_solve / gmres / the Newton step for several
+// slots at once: per-slot kernels are launched in turn on the group's stream,
+// the triangular sweeps and the factorisation of all slots in one launch (a
+// workgroup per slot), and the Arnoldi scalars of all slots come back in one
+// copy per step. Each slot's arithmetic is exactly that of a solve alone.
+
+void ilu_solve_batch(Group& g, const std::vector<Solver*>& cs, const std::vector<int>& ids,
+                     const std::vector<const double*>& b, const std::vector<double*>& x) {
+  Solver& s0 = *g.cases[0];
+  const int n = s0.m.nc;
+  for (size_t i = 0; i < cs.size(); ++i)
+    hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s0.st, n, s0.ilu.perm, b[i], cs[i]->y);
+  const size_t lds = s0.ilu.lds_levels ? size_t(s0.ilu.nlev + s0.ilu.nlevu + 2) * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_ilu_solve, int(ids.size()), ILU_TPB, lds, s0.st, s0.ilu, (const double*)g.Ball,
+                     (const double*)g.Iall, g.Yall, case_list(g, ids), g.sB, g.sI, g.sY);
+  for (size_t i = 0; i < cs.size(); ++i)
+    hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s0.st, n, s0.ilu.perm, cs[i]->y, x[i]);
+}
+
+struct GmresState {
+  std::vector<double> H, cs, sn, gv;
+  double beta = 0.0;
+  int k_done = 0, info = 0;
+  bool active = false;
+};
+
+// gmres() for the slots in ids, all with the same preconditioning side; b[i],
+// x[i] per slot; returns the iterations done per slot in its[i], info in info[i].
+void gmres_batch(Group& g, const std::vector<int>& ids, const std::vector<int>& order,
+                 const std::vector<double>& ct_inv, const std::vector<const double*>& b,
+                 const std::vector<double*>& x, const std::vector<double>& rtol, int m, bool right,
+                 std::vector<int>& its, std::vector<int>& info) {
+  const size_t nb = ids.size();
+  Solver& s0 = *g.cases[0];
+  const int N = s0.N();
+  const size_t ld = size_t(N);
+  std::vector<Solver*> cs(nb);
+  for (size_t i = 0; i < nb; ++i) {
+    cs[i] = g.cases[ids[i]].get();
+    Solver& s = *cs[i];
+    if (m + 1 > s.vcap) {
+      if (s.Vb) (void)hipFree(s.Vb);
+      s.Vb = nullptr;
+      s.Vb = dalloc<double>(size_t(m + 1) * N);
+      s.vcap = m + 1;
+    }
+  }
+  auto V = [&](size_t i, int j) { return cs[i]->Vb + size_t(j) * ld; };
+  std::vector<const double*> pb(nb);
+  std::vector<double*> px(nb);
+  for (size_t i = 0; i < nb; ++i) px[i] = V(i, 0);
+  if (right) {
+    for (size_t i = 0; i < nb; ++i)
+      HC(hipMemcpyAsync(V(i, 0), b[i], N * sizeof(double), hipMemcpyDeviceToDevice, s0.st));
+  } else {
+    ilu_solve_batch(g, cs, ids, b, px);
+  }
+  for (size_t i = 0; i < nb; ++i) mdot(*cs[i], V(i, 0), ld, 1, V(i, 0), 0);
+  std::vector<double> hall(size_t(g.cases.size()) * HSTRIDE);
+  auto pull = [&] {
+    HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    HC(hipStreamSynchronize(s0.st));
+  };
+  pull();
+  std::vector<GmresState> G(nb);
+  its.assign(nb, 0);
+  info.assign(nb, 0);
+  for (size_t i = 0; i < nb; ++i) {
+    const double* h = hall.data() + size_t(ids[i]) * HSTRIDE;
+    GmresState& q = G[i];
+    q.beta = std::sqrt(h[0]);
+    HC(hipMemsetAsync(x[i], 0, N * sizeof(double), s0.st));
+    if (q.beta == 0.0 || !std::isfinite(q.beta)) continue;
+    q.active = true;
+    q.info = m;
+    q.H.assign(size_t(m + 1) * m, 0.0);
+    q.cs.assign(m, 0.0);
+    q.sn.assign(m, 0.0);
+    q.gv.assign(m + 1, 0.0);
+    q.gv[0] = q.beta;
+    hipLaunchKernelGGL(k_scale, nblk(N), TPB, 0, s0.st, N, 1.0 / q.beta, V(i, 0), V(i, 0));
+  }
+  for (int j = 0; j < m; ++j) {
+    std::vector<size_t> act;
+    for (size_t i = 0; i < nb; ++i)
+      if (G[i].active) act.push_back(i);
+    if (act.empty()) break;
+    std::vector<Solver*> ac;
+    std::vector<int> aid;
+    std::vector<const double*> in;
+    std::vector<double*> out;
+    for (size_t i : act) {
+      ac.push_back(cs[i]);
+      aid.push_back(ids[i]);
+    }
+    if (right) {
+      for (size_t i : act) {
+        in.push_back(V(i, j));
+        out.push_back(cs[i]->tmp2);
+      }
+      ilu_solve_batch(g, ac, aid, in, out);
+      for (size_t i : act) matvec(*cs[i], order[i], cs[i]->tmp2, cs[i]->tmp, ct_inv[i]);
+    } else {
+      for (size_t i : act) {
+        matvec(*cs[i], order[i], V(i, j), cs[i]->tmp2, ct_inv[i]);
+        in.push_back(cs[i]->tmp2);
+        out.push_back(cs[i]->tmp);
+      }
+      ilu_solve_batch(g, ac, aid, in, out);
+    }
+    for (size_t i : act) {
+      Solver& s = *cs[i];
+      double* w = s.tmp;
+      mdot(s, s.Vb, ld, j + 1, w, 0);
+      hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s0.st, N, s.Vb, ld, j + 1, s.hdev, w);
+      mdot(s, s.Vb, ld, j + 1, w, j + 1);
+      hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s0.st, N, s.Vb, ld, j + 1, s.hdev + j + 1, w);
+      mdot(s, w, ld, 1, w, 2 * (j + 1));
+    }
+    pull();
+    for (size_t i : act) {
+      GmresState& q = G[i];
+      const double* hb = hall.data() + size_t(ids[i]) * HSTRIDE;
+      auto Hij = [&](int a, int c) -> double& { return q.H[size_t(a) * m + c]; };
+      const double hn = std::sqrt(hb[2 * (j + 1)]);
+      for (int a = 0; a <= j; ++a) Hij(a, j) = hb[a] + hb[j + 1 + a];
+      Hij(j + 1, j) = hn;
+      for (int a = 0; a < j; ++a) {
+        const double t = q.cs[a] * Hij(a, j) + q.sn[a] * Hij(a + 1, j);
+        Hij(a + 1, j) = -q.sn[a] * Hij(a, j) + q.cs[a] * Hij(a + 1, j);
+        Hij(a, j) = t;
+      }
+      const double d = std::hypot(Hij(j, j), Hij(j + 1, j));
+      if (d == 0.0) {
+        q.k_done = j;
+        q.active = false;
+        continue;
+      }
+      q.cs[j] = Hij(j, j) / d;
+      q.sn[j] = Hij(j + 1, j) / d;
+      Hij(j, j) = d;
+      Hij(j + 1, j) = 0.0;
+      q.gv[j + 1] = -q.sn[j] * q.gv[j];
+      q.gv[j] = q.cs[j] * q.gv[j];
+      q.k_done = j + 1;
+      const double rel = std::fabs(q.gv[j + 1]) / q.beta;
+      if (!std::isfinite(rel)) {
+        q.active = false;
+        continue;
+      }
+      if (rel <= rtol[i] || hn == 0.0) {
+        q.info = 0;
+        q.active = false;
+        continue;
+      }
+      hipLaunchKernelGGL(k_scale, nblk(N), TPB, 0, s0.st, N, 1.0 / hn, cs[i]->tmp, V(i, j + 1));
+    }
+  }
+  // x = V y (right: x = M^-1 V y)
+  std::vector<Solver*> rc;
+  std::vector<int> rid;
+  std::vector<const double*> rin;
+  std::vector<double*> rout;
+  std::vector<std::vector<double>> ys(nb);
+  for (size_t i = 0; i < nb; ++i) {
+    GmresState& q = G[i];
+    info[i] = q.beta == 0.0 || !std::isfinite(q.beta) ? 0 : q.info;
+    its[i] = q.k_done;
+    const int k = q.k_done;
+    if (k == 0) continue;
+    auto Hij = [&](int a, int c) { return q.H[size_t(a) * m + c]; };
+    std::vector<double>& yv = ys[i];
+    yv.assign(k, 0.0);
+    for (int a = k - 1; a >= 0; --a) {
+      double t = q.gv[a];
+      for (int c = a + 1; c < k; ++c) t -= Hij(a, c) * yv[c];
+      yv[a] = t / Hij(a, a);
+    }
+    for (auto& v : yv) v = -v;
+    Solver& s = *cs[i];
+    HC(hipMemcpyAsync(s.hdev, yv.data(), k * sizeof(double), hipMemcpyHostToDevice, s0.st));
+    if (right) {
+      HC(hipMemsetAsync(s.tmp2, 0, N * sizeof(double), s0.st));
+      hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s0.st, N, s.Vb, ld, k, s.hdev, s.tmp2);
+      rc.push_back(cs[i]);
+      rid.push_back(ids[i]);
+      rin.push_back(s.tmp2);
+      rout.push_back(x[i]);
+    } else {
+      hipLaunchKernelGGL(k_gs_update, nblk(N), TPB, 0, s0.st, N, s.Vb, ld, k, s.hdev, x[i]);
+    }
+    // the host vector must outlive the asynchronous copy
+    HC(hipStreamSynchronize(s0.st));
+  }
+  if (!rc.empty()) ilu_solve_batch(g, rc, rid, rin, rout);
+  HC(hipStreamSynchronize(s0.st));
 }
 
 }  // namespace
@@ -1477,54 +1815,49 @@ int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const 
     P.turbulent = prm->turbulent;
     P.limiter_zone = prm->limiter_zone;
     s->order = prm->order;
-    const size_t n5 = size_t(nc) * 5;
-    auto& S = *s;
-    S.Q = S.own(dalloc<double>(n5));
-    S.Qprev = S.own(dalloc<double>(n5));
-    S.Qbest = S.own(dalloc<double>(n5));
-    S.R = S.own(dalloc<double>(n5));
-    S.dq = S.own(dalloc<double>(n5));
-    S.tmp = S.own(dalloc<double>(n5));
-    S.tmp2 = S.own(dalloc<double>(std::max(n5, size_t(nc) * 25)));
-    S.bvec = S.own(dalloc<double>(n5));
-    S.g = S.own(dalloc<double>(2 * n5));
-    S.gt = S.own(dalloc<double>(2 * n5));
-    S.phi = S.own(dalloc<double>(n5));
-    S.phit = S.own(dalloc<double>(n5));
-    S.phi_frozen = S.own(dalloc<double>(n5));
-    S.zero5 = S.own(dalloc<double>(n5));
-    HC(hipMemset(S.zero5, 0, n5 * sizeof(double)));
-    S.F = S.own(dalloc<double>(size_t(nfi) * 5));
-    S.Ft = S.own(dalloc<double>(size_t(nfi) * 5));
-    S.wgf = S.own(dalloc<double>(size_t(nff) * 5));
-    S.dt = S.own(dalloc<double>(nc));
-    S.vdt = S.own(dalloc<double>(nc));
-    S.reli = S.own(dalloc<double>(nc));
-    S.cpbuf = S.own(dalloc<double>(2 * size_t(nfw)));
-    S.nparts_cell = nblk(nc);
-    const int nbv = (5 * nc + VCHUNK - 1) / VCHUNK;
-    S.part = S.own(dalloc<double>(std::max(size_t(5) * S.nparts_cell, size_t(GMRES_MAX + 1) * nbv) +
-                                  2 * S.nparts_cell));
-    S.small = S.own(dalloc<double>(64));
-    S.hdev = S.own(dalloc<double>(2 * (GMRES_MAX + 1) + 8));
-    S.dq2 = S.own(dalloc<double>(n5));
-    S.J = S.own(dalloc<double>(size_t(nfi) * 50));
-    S.JW = S.own(dalloc<double>(size_t(nfw) * 25));
-    S.JF = S.own(dalloc<double>(size_t(nff) * 25));
-    S.nonfinite = S.own(dalloc<int>(1));
+    s->owns_stream = true;
+    alloc_case(*s);
+    auto grp = std::make_unique<Group>();
+    grp->cases.push_back(std::move(s));
+    relink(*grp);
     const int64_t h = g_next++;
-    g_solvers[h] = std::move(s);
+    g_solvers[h] = std::move(grp);
     *handle_out = h;
   });
 }
 
 int a2d_rans_destroy(int64_t h) {
   return guarded([&] {
-    get(h);
-    auto& s = *g_solvers[h];
-    if (s.Vb) (void)hipFree(s.Vb);
-    s.Vb = nullptr;
+    get_group(h);
     g_solvers.erase(h);
+  });
+}
+
+int a2d_rans_set_cases(int64_t h, int ncases) {
+  return guarded([&] {
+    Group& g = get_group(h);
+    if (ncases < 1 || ncases > 1024) fail(-6, "ncases must be in [1, 1024]");
+    if (ncases <= int(g.cases.size())) return;
+    Solver& s0 = *g.cases[0];
+    while (int(g.cases.size()) < ncases) {
+      auto s = std::make_unique<Solver>();
+      s->m = s0.m;
+      s->P = s0.P;
+      s->order = s0.order;
+      s->st = s0.st;
+      s->in_zone = s0.in_zone;
+      alloc_case(*s);
+      g.cases.push_back(std::move(s));
+    }
+    relink(g);
+  });
+}
+
+int a2d_rans_select(int64_t h, int c) {
+  return guarded([&] {
+    Group& g = get_group(h);
+    if (c < 0 || c >= int(g.cases.size())) fail(-4, "no such case slot");
+    g.cur = c;
   });
 }
 
@@ -1633,7 +1966,7 @@ int a2d_rans_jac1(int64_t h, const double* winf, double gam, double* D, double* 
 
 int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, const int* cols, const int* pos_src) {
   return guarded([&] {
-    Solver& s = get(h);
+    Solver& s = *get_group(h).cases[0];
     const int n = s.m.nc;
     const int nfi = s.m.nfi;
     if (nnz < n) fail(-6, "pattern smaller than its diagonal");
@@ -1735,10 +2068,8 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
     s.posLR = s.own(dupload(std::vector<int>(pos_src + n, pos_src + n + nfi)));
     s.posRL = s.own(dupload(std::vector<int>(pos_src + n + nfi, pos_src + n + 2 * nfi)));
-    s.B = s.own(dalloc<double>(size_t(nnz) * 25));
-    s.invU = s.own(dalloc<double>(size_t(n) * 25));
-    s.y = s.own(dalloc<double>(size_t(n) * 5));
     s.has_ilu = true;
+    relink(get_group(h));
   });
 }
 
@@ -1860,7 +2191,8 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
         case 0: ilu_solve(s, s.R, s.tmp); break;
         case 1: {
           const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
-          hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU);
+          hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, nullptr, size_t(0),
+                             size_t(0));
           break;
         }
         case 2: eval(s, order, s.R, nullptr, s.tmp, s.vdt, 1.0); break;
@@ -1889,6 +2221,172 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
     (void)hipEventDestroy(e1);
     HC(hipGetLastError());
     *ms = double(t) / reps;
+  });
+}
+
+int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* order, const double* winf,
+                          const double* gam, const double* cfl, const double* turb_factor, const double* gmres_tol,
+                          int gmres_iter, const int* right, int step_limit_cells, double* out) {
+  return guarded([&] {
+    Group& g = get_group(h);
+    Solver& s0 = *g.cases[0];
+    if (!s0.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
+    if (n < 1) fail(-6, "n must be >= 1");
+    if (gmres_iter < 1 || gmres_iter > GMRES_MAX) fail(-6, "gmres_iter must be in [1, 120]");
+    std::vector<int> ids(case_ids, case_ids + n), sorted(ids);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) fail(-6, "a batch must not name a case twice");
+    for (int i = 0; i < n; ++i) {
+      if (ids[i] < 0 || ids[i] >= int(g.cases.size())) fail(-4, "no such case slot");
+      if (order[i] != 1 && order[i] != 2) fail(-6, "order must be 1 or 2");
+      if (!(turb_factor[i] > 0.0)) fail(-6, "turb_factor must be positive");
+    }
+    const int N = s0.N();
+    std::vector<Solver*> cs(n);
+    std::vector<double> ct(n);
+    for (int i = 0; i < n; ++i) {
+      cs[i] = g.cases[ids[i]].get();
+      ct[i] = 1.0 / turb_factor[i];
+      for (int k = 0; k < 12; ++k) out[size_t(i) * 12 + k] = 0.0;
+    }
+    // residual, pseudo-time step and first-order Jacobian of every case, then
+    // one factorisation launch for all
+    for (int i = 0; i < n; ++i) {
+      Solver& s = *cs[i];
+      far_state(s, winf + 5 * i, gam[i]);
+      eval(s, order[i], nullptr, s.R, nullptr, nullptr, 1.0);
+      hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl[i], s.dt, s.vdt);
+      factor(s, winf + 5 * i, gam[i], ct[i], nullptr, nullptr, false);
+    }
+    {
+      const size_t lds = s0.ilu.lds_levels ? size_t(s0.ilu.nlev + 1) * sizeof(int) : 0;
+      hipLaunchKernelGGL(k_ilu_factor, n, ILU_TPB, lds, s0.st, s0.ilu, g.Ball, g.Iall, case_list(g, ids), g.sB,
+                         g.sI);
+      HC(hipGetLastError());
+    }
+    HC(hipMemsetAsync(g.nfall, 0, g.cases.size() * sizeof(int), s0.st));
+    for (int i = 0; i < n; ++i)
+      hipLaunchKernelGGL(k_count_nonfinite, nblk(s0.m.nc * 25), TPB, 0, s0.st, s0.m.nc * 25, cs[i]->invU,
+                         cs[i]->nonfinite);
+    for (int i = 0; i < n; ++i) mdot(*cs[i], cs[i]->R, size_t(N), 1, cs[i]->R, 0);
+    std::vector<int> nf(g.cases.size());
+    std::vector<double> hall(g.cases.size() * HSTRIDE);
+    HC(hipMemcpyAsync(nf.data(), g.nfall, nf.size() * sizeof(int), hipMemcpyDeviceToHost, s0.st));
+    HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    HC(hipStreamSynchronize(s0.st));
+    std::vector<double> rnorm(n);
+    std::vector<int> live;
+    for (int i = 0; i < n; ++i) {
+      rnorm[i] = std::sqrt(hall[size_t(ids[i]) * HSTRIDE]);
+      if (nf[ids[i]] > 0) {
+        out[size_t(i) * 12 + 5] = 1.0;   // ILU not finite: no step
+        continue;
+      }
+      live.push_back(i);
+      hipLaunchKernelGGL(k_negate, nblk(N), TPB, 0, s0.st, N, cs[i]->R, cs[i]->bvec);
+    }
+    // GMRES, grouped by preconditioning side
+    std::vector<double> lin(n, NAN);
+    std::vector<int> total(n, 0), infos(n, 0);
+    auto run = [&](const std::vector<int>& sel, bool rt, bool into_dq2, std::vector<int>& itsv,
+                   std::vector<int>& infv) {
+      std::vector<int> sid, so;
+      std::vector<double> sct, stol;
+      std::vector<const double*> sb;
+      std::vector<double*> sx;
+      for (int i : sel) {
+        sid.push_back(ids[i]);
+        so.push_back(order[i]);
+        sct.push_back(ct[i]);
+        stol.push_back(gmres_tol[i]);
+        sb.push_back(cs[i]->bvec);
+        sx.push_back(into_dq2 ? cs[i]->dq2 : cs[i]->dq);
+      }
+      for (int i : sel) far_state(*cs[i], winf + 5 * i, gam[i]);
+      gmres_batch(g, sid, so, sct, sb, sx, stol, gmres_iter, rt, itsv, infv);
+    };
+    auto lins = [&](const std::vector<int>& sel, bool use_dq2) {
+      for (int i : sel) {
+        Solver& s = *cs[i];
+        matvec(s, order[i], use_dq2 ? s.dq2 : s.dq, s.tmp2, ct[i]);
+        hipLaunchKernelGGL(k_add, nblk(N), TPB, 0, s0.st, N, s.tmp2, s.R, s.tmp2);
+        mdot(s, s.tmp2, size_t(N), 1, s.tmp2, 0);
+      }
+      HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+      HC(hipStreamSynchronize(s0.st));
+      std::vector<double> r(sel.size());
+      for (size_t q = 0; q < sel.size(); ++q)
+        r[q] = std::sqrt(hall[size_t(ids[sel[q]]) * HSTRIDE]) / std::max(rnorm[sel[q]], 1e-300);
+      return r;
+    };
+    for (int side = 0; side < 2; ++side) {
+      std::vector<int> sel;
+      for (int i : live)
+        if ((right[i] != 0) == (side == 1)) sel.push_back(i);
+      if (sel.empty()) continue;
+      std::vector<int> itsv, infv;
+      run(sel, side == 1, false, itsv, infv);
+      const std::vector<double> l = lins(sel, false);
+      for (size_t q = 0; q < sel.size(); ++q) {
+        lin[sel[q]] = l[q];
+        total[sel[q]] = itsv[q];
+        infos[sel[q]] = infv[q];
+      }
+    }
+    // left preconditioning that did not reach a true residual below 0.5:
+    // retry with right preconditioning, keep the better (aero2d/rans.py:1791-1801)
+    std::vector<int> retry;
+    for (int i : live)
+      if (!right[i] && !(std::isfinite(lin[i]) && lin[i] < 0.5)) retry.push_back(i);
+    if (!retry.empty()) {
+      std::vector<int> itsv, infv;
+      run(retry, true, true, itsv, infv);
+      const std::vector<double> l2 = lins(retry, true);
+      for (size_t q = 0; q < retry.size(); ++q) {
+        const int i = retry[q];
+        total[i] += itsv[q];
+        if (std::isfinite(l2[q]) && (!std::isfinite(lin[i]) || l2[q] < lin[i])) {
+          HC(hipMemcpyAsync(cs[i]->dq, cs[i]->dq2, N * sizeof(double), hipMemcpyDeviceToDevice, s0.st));
+          lin[i] = l2[q];
+          infos[i] = infv[q];
+        }
+      }
+    }
+    // step limit and update of the cases whose linear solve succeeded
+    std::vector<int> upd;
+    for (int i : live) {
+      double* o = out + size_t(i) * 12;
+      o[0] = lin[i];
+      o[1] = double(total[i]);
+      o[2] = double(infos[i]);
+      if (!std::isfinite(lin[i]) || lin[i] > 0.9) continue;
+      upd.push_back(i);
+      Solver& s = *cs[i];
+      const int nbp = s.nparts_cell;
+      hipLaunchKernelGGL(k_step_rel, nbp, TPB, 0, s0.st, s.m.nc, s.Q, s.dq, s.reli, s.part, s.part + nbp);
+      hipLaunchKernelGGL(k_max_rows, 1, 64, 0, s0.st, nbp, s.part, s.part + nbp, s.small);
+    }
+    if (upd.empty()) return;
+    std::vector<double> sm(g.cases.size() * 64);
+    HC(hipMemcpyAsync(sm.data(), g.smallall, sm.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    HC(hipStreamSynchronize(s0.st));
+    for (int i : upd) {
+      Solver& s = *cs[i];
+      const double rel = sm[size_t(ids[i]) * 64];
+      const double lam = rel < 0.2 ? 1.0 : 0.2 / rel;
+      const int nbig = int(sm[size_t(ids[i]) * 64 + 1]);
+      const bool per_cell = lam < 1.0 && step_limit_cells > 0 && nbig <= step_limit_cells;
+      HC(hipMemcpyAsync(s.Qprev, s.Q, N * sizeof(double), hipMemcpyDeviceToDevice, s0.st));
+      hipLaunchKernelGGL(k_apply, nblk(s.m.nc), TPB, 0, s0.st, s.m.nc, s.Q, s.dq, s.reli, lam, per_cell ? 1 : 0);
+      double* o = out + size_t(i) * 12;
+      o[3] = lam;
+      o[4] = rel;
+      o[6] = 1.0;
+      o[7] = double(nbig);
+      o[8] = per_cell ? 1.0 : 0.0;
+    }
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(s0.st));
   });
 }
 

@@ -155,6 +155,7 @@ class RansSolver:
         self.nv = 5
         self._h = None
         self._ilu_fill = None
+        self._slot = 0
         self._setup()
 
     # ------------------------------------------------------------------ set-up
@@ -477,263 +478,89 @@ class RansSolver:
               alpha0=None) -> RansResult:
         """Pseudo-transient Newton-Krylov solve to the Convergence rule or
         max_iter iterations (aero2d/rans.py:1298-1900, the SA path)."""
-        crit = conv or self.conv
-        gmres_tol = self.gmres_tol if gmres_tol is None else gmres_tol
-        cfl0 = self.cfl_start if cfl0 is None else cfl0
-        cfl_max = self.cfl_limit if cfl_max is None else cfl_max
-        first_order_iters = self.first_order if first_order_iters is None else first_order_iters
-        rb_factor, st_reject = self.rollback_factor, self.step_reject
-        winf = self.freestream(alpha)
-        self._ensure_ilu()
-        if Q0 is None:
-            Q = self.initial_state(winf)
-        else:
-            Q = np.array(Q0, float)
-            if alpha0 is not None and abs(float(alpha) - float(alpha0)) > 1e-12:
-                Q = self._rotate_state(Q, float(alpha) - float(alpha0))
-        # reference residual of the rule: that of the faded freestream (aero2d/rans.py:1408-1414)
-        R_fs = self.residual(self.initial_state(winf), winf, 0.0, order=2)
-        rfs_eq = np.sqrt(np.mean((R_fs / self.vol[:, None]) ** 2, 0))
-        r_fs = float(rfs_eq.sum())
-        self._set(Q)
-        self._freeze(None)
-        warm = Q0 is not None
-        turning = warm and alpha0 is not None and abs(float(alpha) - float(alpha0)) > 1e-12
-        peak = [0.0, 0, None]
-        if warm and self.vortex_bc:
-            self.gamma_vortex = 0.5 * self._forces_dict(alpha)["cl"] * self.mach * self.chord
-        have_prev = False
-        prev_rn, prev_order = None, None
-        n_rej = ok_streak = n_tiny = 0
-        hist, fh = [], []
-        r0 = None
-        cfl = cfl0
-        self._last_ok, self._last_lin = True, 0.0
-        H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[])
-        events = []
-        best = dict(rn=np.inf, has=turning, it=0, cfl=cfl0, order=0)
-        if turning:
-            self._keep_state(0)
-        tb = 1.0
-        first_until = 0
-        n_back = 0
-        frozen = False
-        prev_order_used = None
-        conv_ok = False
-        status = ""
-        it = 0
-        for it in range(1, max_iter + 1):
-            if control is not None:
-                v_ = control.cfl_set
-                if v_ is not None and float(v_) > 0:
-                    control.cfl_set = None
-                    cfl = float(v_)
-                    self.cfl_hold_value = cfl if control.cfl_hold else None
-                sk, stp = bool(control.skip), bool(control.stop)
-                if sk or stp:
-                    status = "accepted by hand" if sk else "stopped"
-                    control.skip = False
-                    break
-                f_ = float(control.cfl_factor)
-                if f_ != 1.0:
-                    cfl = max(1.0, cfl * f_)
-                    control.cfl_factor = 1.0
-            order = 1 if (it <= first_order_iters and Q0 is None) or it <= first_until else self.order
-            if (order == 2 and prev_order_used == 1 and self.cfl_reset_on_order
-                    and getattr(self, "cfl_hold_value", None) is None):
-                cfl = cfl0
-            prev_order_used = order
-            gm = self.gamma_vortex if self.vortex_bc else 0.0
-            rn5 = self._rn(order, winf, gm)
-            rn = float(rn5.sum())
-            manual = control is not None and control.rollback
-            if manual:
-                control.rollback = False
-            if best["order"] != order:
-                best.update(rn=np.inf, order=order)
-            blowup = (not np.isfinite(rn)) or (self.rollback_auto and rn > rb_factor * best["rn"]
-                                               and it - best["it"] >= 3 and not (warm and it <= 8))
-            if (blowup or manual) and best["has"]:
-                n_back += int(not manual)
-                tb = 1.0
-                why = "by hand" if manual else ("NaN" if not np.isfinite(rn) else
-                                                 f"residual x{rn / best['rn']:.3g} of the lowest")
-                events.append((it, f"back to iteration {best['it']} ({why})", rn, []))
-                if verbose:
-                    print(f"  it {it:4d}: back to iteration {best['it']} ({why})", flush=True)
-                self._keep_state(1)
-                if n_back > self.max_rollbacks:
-                    status = f"unstable: {n_back - 1} rollbacks did not help (best state kept)"
-                    break
-                cfl = max(cfl0, min(cfl, best["cfl"]) * 0.1)
-                first_until = it + (self.rollback_first_order if not np.isfinite(rn) else 0)
-                if frozen:
-                    self._freeze(None)
-                    frozen = False
-                best.update(rn=np.inf, it=it)
-                continue
-            if (self.rollback_auto and have_prev and prev_order == order and n_rej < self.max_rejects
-                    and (not np.isfinite(rn) or rn > st_reject * prev_rn)
-                    and not (warm and it <= 8) and not manual):
-                n_rej += 1
-                self._keep_state(2)
-                cfl = max(cfl0, cfl * 0.3)
-                events.append((it, f"step undone (residual x{rn / prev_rn:.3g}), CFL -> {cfl:.3g}", rn, []))
-                if verbose:
-                    print(f"  it {it:4d}: step undone (residual x{rn / prev_rn:.3g}), CFL -> {cfl:.3g}",
-                          flush=True)
-                continue
-            n_rej = 0
-            ok_streak += 1
-            if not np.isfinite(rn):
-                status = "diverged"
-                break
-            if peak[2] is None:
-                peak[2] = rn
-            if rn > peak[0]:
-                peak[:2] = [rn, it]
-            past_peak = (peak[0] >= 5.0 * peak[2] and it - peak[1] >= 5) or it > 40
-            if rn < 0.95 * best["rn"] and (not turning or past_peak):
-                best.update(rn=rn, has=True, it=it, cfl=cfl, order=order)
-                self._keep_state(0)
-            if r0 is None or (order == 2 and len(hist) and hist[-1][1] == 1):
-                r0 = rn
-            hist.append((rn, order))
-            fr_ = self._forces_dict(alpha)
-            for k_, v_ in (("it", it), ("res", rn5), ("rn", rn), ("cl", fr_["cl"]), ("cd", fr_["cd"]),
-                           ("cfl", cfl)):
-                H[k_].append(v_)
-            if order == 2:
-                self._rref = max(getattr(self, "_rref", 0.0), rn)
-            if control is not None and control.monitor is not None:
-                control.monitor(dict(alpha=alpha, it=it, res=rn5, rn=rn, cl=fr_["cl"], cd=fr_["cd"], cfl=cfl,
-                                     rref=getattr(self, "_rref", rn), order=order, r_fs=r_fs, lin=self._last_lin))
-            if order == 2 and rn < tol * r0:
-                conv_ok = True
-                break
-            if order == 2:
-                fh.append((fr_["cl"], fr_["cd"]))
-                o2 = [h[0] for h in hist if h[1] == 2]
-                rref = r_fs
-                stalled = len(o2) > 60 and min(o2[-30:]) > 0.5 * min(o2[:-30])
+        run = _Run(self, 0, alpha, Q0, max_iter, tol, cfl0, cfl_max, first_order_iters, verbose, callback,
+                   gmres_tol, gmres_iter, conv, control, alpha0)
+        return self._drive([run])[0]
 
-                def steady(win, tol_cl, tol_cd):
-                    if len(fh) <= win:
-                        return False
-                    cls = [f[0] for f in fh[-win - 1:]]
-                    cds = [f[1] for f in fh[-win - 1:]]
-                    return (max(cls) - min(cls) < tol_cl * max(0.2, abs(fh[-1][0]))
-                            and max(cds) - min(cds) < tol_cd)
-                hold = max(1, int(crit.hold))
-                lvl = 10.0 ** (-crit.orders) * rref
-                own = max(o2)
-                dropped = (not warm) or rn <= 1e-2 * own
-                if (len(o2) >= hold and max(o2[-hold:]) <= lvl and dropped
-                        and steady(hold, crit.cl_tol, crit.cd_tol)):
-                    conv_ok, status = True, "converged"
+    def solve_polar(self, alphas, Q0=None, max_iter=300, tol=1e-7, cfl0=None, cfl_max=None,
+                    first_order_iters=None, verbose=False, gmres_tol=None, gmres_iter=60, conv=None):
+        """Every angle of a polar solved at once, one case slot per angle, each
+        with the loop of `solve` from its own start; their Newton steps run as
+        one batched device call per iteration (the factorisation and
+        triangular sweeps of all angles in single launches). The reference runs
+        a polar's angles in worker processes (rans_polar, aero2d/rans.py:2619);
+        each result here equals that angle's `solve` alone. Q0: None or one
+        state per angle. Returns the RansResults in the order of alphas."""
+        alphas = [float(a) for a in alphas]
+        self._ensure_ilu()
+        _abi.call("a2d_rans_set_cases", self._device(), len(alphas))
+        runs = [_Run(self, k, a, None if Q0 is None else Q0[k], max_iter, tol, cfl0, cfl_max, first_order_iters,
+                     verbose, None, gmres_tol, gmres_iter, conv, None, None) for k, a in enumerate(alphas)]
+        return self._drive(runs)
+
+    def _select(self, slot):
+        if self._slot != slot:
+            _abi.call("a2d_rans_select", self._device(), int(slot))
+            self._slot = slot
+
+    def _drive(self, runs):
+        """Advance the runs in lockstep: each run's iteration up to its Newton
+        step on the host, then one batched Newton step for all that asked."""
+        self._ensure_ilu()
+        for r in runs:
+            self._select(r.slot)
+            r.start()
+        while True:
+            reqs = []
+            for r in runs:
+                if r.done:
+                    continue
+                self._select(r.slot)
+                q = r.plan()
+                if q is not None:
+                    reqs.append((r, q))
+            if not reqs:
+                if all(r.done for r in runs):
                     break
-                if (len(o2) >= 8 and max(o2[-3:]) <= 10.0 ** (-1.5) * lvl and rn <= 1e-3 * own
-                        and steady(3, crit.cl_tol, crit.cd_tol)):
-                    conv_ok, status = True, "converged"
-                    break
-                if (crit.stall_ok and len(o2) > 40 and stalled and rn < 10.0 ** (1.0 - crit.orders) * rref
-                        and steady(max(hold, 30), crit.cl_tol, crit.cd_tol)):
-                    conv_ok, status = True, "converged (residual stalled, forces steady)"
-                    break
-                if (crit.stall_ok and len(o2) > 60 and stalled and rn < 10.0 ** -2.5 * own
-                        and steady(max(hold, 40), crit.cl_tol, crit.cd_tol)):
-                    conv_ok, status = True, "converged (forces steady)"
-                    break
-                if (self.freeze_limiter and not frozen and self.limiter_k > 0 and self.limiter_zone != "off"
-                        and stalled and steady(30, 0.005, 1e-3)):
-                    Qc = self._get()
-                    self._freeze(self.phi(Qc))
-                    self._set(Qc)
-                    frozen = True
-                    if verbose:
-                        print(f"  it {it:4d}: residual stalled at {rn:.2e}: limiter frozen", flush=True)
-            # CFL (aero2d/rans.py:1617-1680)
-            if self.cfl_mode == "formula":
-                raise NotImplementedError("cfl_mode 'formula' is not built; use 'grow', 'fixed' or 'ser'")
-            if len(hist) >= 2 and hist[-2][1] == order:
-                ok_ = self._last_ok
-                lin_ = self._last_lin
-                if self.cfl_mode == "fixed":
-                    cfl = min(cfl_max, cfl * self.cfl_growth) if ok_ and rn < 3.0 * hist[-2][0] \
-                        else max(cfl0, 0.5 * cfl)
-                elif self.cfl_mode == "ser":
-                    cfl = min(cfl_max, max(cfl0, cfl * min(2.0, hist[-2][0] / rn)))
-                else:
-                    grow = (self.cfl_growth if rn < 1.2 * hist[-2][0] and ok_ and lin_ < 0.25 else
-                            self.cfl_lin_back if lin_ >= 0.25 and ok_ and rn < 3.0 * hist[-2][0] else
-                            1.0 if rn < 3.0 * hist[-2][0] and ok_ else 0.5)
-                    cfl = min(cfl_max, max(cfl0, cfl * grow))
-            if getattr(self, "cfl_hold_value", None) is not None:
-                cfl = float(self.cfl_hold_value)
-            if self.turb_boost > 1.0:
-                ok_ = self._last_ok
-                fw_ = fh[-11:] if order >= 2 else []
-                calm = (len(fw_) == 11 and max(f[0] for f in fw_) - min(f[0] for f in fw_)
-                        < 2e-3 * max(0.2, abs(fw_[-1][0]))
-                        and max(f[1] for f in fw_) - min(f[1] for f in fw_) < 5e-4)
-                if calm and ok_ and cfl >= self.turb_boost_cfl and len(hist) >= 2 and rn < 1.2 * hist[-2][0]:
-                    tb = min(self.turb_boost, 1.5 * tb)
-                elif not ok_ or (len(hist) >= 2 and rn >= 1.2 * hist[-2][0]):
-                    tb = max(1.0, tb / 4.0)
-            ct = (self.cfl_turb if self.cfl_turb is not None else 1.0) * tb
-            gtol = gmres_tol if (warm and n_back == 0) else min(gmres_tol, 0.01)
-            # laminar solves take right preconditioning: left preconditioning
-            # "converges" on |M^-1 r| while the true residual stays near 0.2
-            # (aero2d/rans.py:1781-1790)
-            st = self.newton_step(order, winf, gm, cfl, gtol, gmres_iter, turb_factor=ct,
-                                  right=self.right_prec or not self.turbulent)
-            if st["ilu_nonfinite"]:
-                if verbose:
-                    print(f"  it {it:4d}: ILU not finite, CFL {cfl:.3g} -> {max(cfl0, cfl / 10):.3g}", flush=True)
-                cfl = max(cfl0, cfl / 10)
                 continue
-            lin = st["lin"]
-            H["gmres"].append(st["gmres_iterations"])
-            H["lin"].append(lin)
-            self._last_ok = bool(np.isfinite(lin) and lin < 0.5)
-            self._last_lin = lin
-            if not st["applied"]:
-                if verbose:
-                    print(f"  it {it:4d}: GMRES did not converge ({lin:.2g}), residual {rn:.3e}, CFL {cfl:.3g} -> "
-                          f"{max(cfl0, cfl / 4):.3g}", flush=True)
-                cfl = max(cfl0, cfl / 4)
-                continue
-            lam = st["lam"]
-            if lam < 1.0 and not st["per_cell"]:
-                cfl = max(cfl0 if n_tiny < 10 else 0.02 * cfl0, cfl * 0.5)
-            n_tiny = n_tiny + 1 if lam < 0.05 and not st["per_cell"] else 0
-            if n_tiny >= 60:
-                status = "stuck: the Newton step was cut to almost nothing for 60 iterations"
-                break
-            have_prev, prev_rn, prev_order = True, rn, order
-            if self.vortex_bc and it % 5 == 0:
-                self.gamma_vortex = 0.5 * self._forces_dict(alpha)["cl"] * self.mach * self.chord
-            if verbose and (it % 10 == 0 or it < 5):
-                fr = self._forces_dict(alpha)
-                print(f"  it {it:4d} ord {order} res {rn:.3e} cfl {cfl:.2e} lam {lam:.2f} "
-                      f"gmres {st['gmres_iterations']}:{lin:.1e} Cl {fr['cl']:.4f} Cd {fr['cd']:.5f}", flush=True)
-            if callback and callback(it, rn):
-                break
-        if status == "diverged" and best["has"]:
-            self._keep_state(1)
-            status = "diverged (best state kept)"
-        fr = self._forces_dict(alpha)
-        Hn = {k_: np.array(v_) for k_, v_ in H.items()}
-        if not status:
-            status = "converged" if conv_ok else "not converged within the iterations allowed"
-        Hn["rref"] = float(getattr(self, "_rref", 0.0) or (max(Hn["rn"]) if len(Hn["rn"]) else 1.0))
-        Hn["r_fs"] = r_fs
-        Hn["events"] = events
-        if n_back:
-            status += f"; rollbacks: {n_back}"
-        Q = self._get()
-        return RansResult(alpha, fr["cl"], fr["cd"], fr["cdp"], fr["cdf"], fr["cm"], conv_ok, it,
-                          [h[0] for h in hist], fr["elements"], Q, status, Hn)
+            outs = self._newton_steps([q for _, q in reqs])
+            for (r, _), st in zip(reqs, outs):
+                self._select(r.slot)
+                r.after(st)
+        res = []
+        for r in runs:
+            self._select(r.slot)
+            res.append(r.finish())
+        self._select(0)
+        return res
+
+    def _newton_steps(self, qs):
+        """a2d_rans_newton_steps over the requests (one per slot)."""
+        n = len(qs)
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)     # noqa: E731
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)   # noqa: E731
+        ids = i32([q["slot"] for q in qs])
+        order = i32([q["order"] for q in qs])
+        winf = f64(np.stack([q["winf"] for q in qs]))
+        gam = f64([q["gam"] for q in qs])
+        cfl = f64([q["cfl"] for q in qs])
+        ct = f64([q["ct"] for q in qs])
+        gtol = f64([q["gtol"] for q in qs])
+        right = i32([int(bool(q["right"])) for q in qs])
+        out = np.zeros((n, 12))
+        gi = int(qs[0]["gmres_iter"])
+        if any(int(q["gmres_iter"]) != gi for q in qs):
+            raise ValueError("a batch of Newton steps shares one GMRES size")
+        _abi.call("a2d_rans_newton_steps", self._device(), n, _abi.iptr(ids), _abi.iptr(order), _abi.dptr(winf),
+                  _abi.dptr(gam), _abi.dptr(cfl), _abi.dptr(ct), _abi.dptr(gtol), gi, _abi.iptr(right),
+                  int(self.step_limit_cells), _abi.dptr(out))
+        res = []
+        for row in out:
+            d = {k: float(v) for k, v in zip(self.STEP_FIELDS, row)}
+            for k in ("gmres_iterations", "info", "ilu_nonfinite", "applied", "nbig", "per_cell"):
+                d[k] = int(d[k])
+            res.append(d)
+        return res
 
     def _rotate_state(self, Q, dalpha, inner=0.05, outer=0.5):
         """Velocity turned by dalpha degrees, fully beyond `outer` chords from the
@@ -746,3 +573,299 @@ class RansSolver:
         Q[:, 1] = c * mu - s_ * mv
         Q[:, 2] = s_ * mu + c * mv
         return Q
+
+
+class _Run:
+    """One case's pseudo-transient continuation loop (aero2d/rans.py:1298-1900,
+    the SA path), split at its Newton step so that several cases can share one
+    batched device step: plan() runs an iteration up to the step and returns the
+    step's parameters (None when the iteration ends before a step: a rollback,
+    an undone step, or the end of the loop), after() takes the step's figures."""
+
+    def __init__(self, S, slot, alpha, Q0, max_iter, tol, cfl0, cfl_max, first_order_iters, verbose, callback,
+                 gmres_tol, gmres_iter, conv, control, alpha0):
+        self.S, self.slot, self.alpha = S, slot, float(alpha)
+        self.Q0, self.max_iter, self.tol = Q0, max_iter, tol
+        self.crit = conv or S.conv
+        self.gmres_tol = S.gmres_tol if gmres_tol is None else gmres_tol
+        self.gmres_iter = gmres_iter
+        self.cfl0 = S.cfl_start if cfl0 is None else cfl0
+        self.cfl_max = S.cfl_limit if cfl_max is None else cfl_max
+        self.first_order_iters = S.first_order if first_order_iters is None else first_order_iters
+        self.verbose, self.callback, self.control, self.alpha0 = verbose, callback, control, alpha0
+        self.done = False
+
+    def start(self):
+        S = self.S
+        self.winf = S.freestream(self.alpha)
+        if self.Q0 is None:
+            Q = S.initial_state(self.winf)
+        else:
+            Q = np.array(self.Q0, float)
+            if self.alpha0 is not None and abs(self.alpha - float(self.alpha0)) > 1e-12:
+                Q = S._rotate_state(Q, self.alpha - float(self.alpha0))
+        # reference residual of the rule: that of the faded freestream (aero2d/rans.py:1408-1414)
+        R_fs = S.residual(S.initial_state(self.winf), self.winf, 0.0, order=2)
+        self.r_fs = float(np.sqrt(np.mean((R_fs / S.vol[:, None]) ** 2, 0)).sum())
+        S._set(Q)
+        S._freeze(None)
+        self.warm = self.Q0 is not None
+        self.turning = self.warm and self.alpha0 is not None and abs(self.alpha - float(self.alpha0)) > 1e-12
+        self.peak = [0.0, 0, None]
+        self.gamma = 0.0
+        if self.warm and S.vortex_bc:
+            self.gamma = 0.5 * S._forces_dict(self.alpha)["cl"] * S.mach * S.chord
+        self.have_prev, self.prev_rn, self.prev_order = False, None, None
+        self.n_rej = self.ok_streak = self.n_tiny = 0
+        self.hist, self.fh = [], []
+        self.r0 = None
+        self.cfl = self.cfl0
+        self.last_ok, self.last_lin = True, 0.0
+        self.H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[])
+        self.events = []
+        self.best = dict(rn=np.inf, has=self.turning, it=0, cfl=self.cfl0, order=0)
+        if self.turning:
+            S._keep_state(0)
+        self.tb = 1.0
+        self.first_until = 0
+        self.n_back = 0
+        self.frozen = False
+        self.prev_order_used = None
+        self.conv_ok = False
+        self.status = ""
+        self.it = 0
+        self.cfl_hold = None
+
+    def _say(self, text):
+        if self.verbose:
+            print(f"  it {self.it:4d}: {text}", flush=True)
+
+    def plan(self):
+        S, c = self.S, self.control
+        if self.it >= self.max_iter:
+            self.done = True
+            return None
+        self.it += 1
+        it = self.it
+        if c is not None:
+            v_ = c.cfl_set
+            if v_ is not None and float(v_) > 0:
+                c.cfl_set = None
+                self.cfl = float(v_)
+                self.cfl_hold = self.cfl if c.cfl_hold else None
+            sk, stp = bool(c.skip), bool(c.stop)
+            if sk or stp:
+                self.status = "accepted by hand" if sk else "stopped"
+                c.skip = False
+                self.done = True
+                return None
+            f_ = float(c.cfl_factor)
+            if f_ != 1.0:
+                self.cfl = max(1.0, self.cfl * f_)
+                c.cfl_factor = 1.0
+        order = 1 if (it <= self.first_order_iters and self.Q0 is None) or it <= self.first_until else S.order
+        if (order == 2 and self.prev_order_used == 1 and S.cfl_reset_on_order and self.cfl_hold is None):
+            self.cfl = self.cfl0
+        self.prev_order_used = order
+        self.order = order
+        gm = self.gamma if S.vortex_bc else 0.0
+        rn5 = S._rn(order, self.winf, gm)
+        rn = float(rn5.sum())
+        self.rn = rn
+        manual = c is not None and c.rollback
+        if manual:
+            c.rollback = False
+        best = self.best
+        if best["order"] != order:
+            best.update(rn=np.inf, order=order)
+        blowup = (not np.isfinite(rn)) or (S.rollback_auto and rn > S.rollback_factor * best["rn"]
+                                           and it - best["it"] >= 3 and not (self.warm and it <= 8))
+        if (blowup or manual) and best["has"]:
+            self.n_back += int(not manual)
+            self.tb = 1.0
+            why = "by hand" if manual else ("NaN" if not np.isfinite(rn) else
+                                             f"residual x{rn / best['rn']:.3g} of the lowest")
+            self.events.append((it, f"back to iteration {best['it']} ({why})", rn, []))
+            self._say(f"back to iteration {best['it']} ({why})")
+            S._keep_state(1)
+            if self.n_back > S.max_rollbacks:
+                self.status = f"unstable: {self.n_back - 1} rollbacks did not help (best state kept)"
+                self.done = True
+                return None
+            self.cfl = max(self.cfl0, min(self.cfl, best["cfl"]) * 0.1)
+            self.first_until = it + (S.rollback_first_order if not np.isfinite(rn) else 0)
+            if self.frozen:
+                S._freeze(None)
+                self.frozen = False
+            best.update(rn=np.inf, it=it)
+            return None
+        if (S.rollback_auto and self.have_prev and self.prev_order == order and self.n_rej < S.max_rejects
+                and (not np.isfinite(rn) or rn > S.step_reject * self.prev_rn)
+                and not (self.warm and it <= 8) and not manual):
+            self.n_rej += 1
+            S._keep_state(2)
+            self.cfl = max(self.cfl0, self.cfl * 0.3)
+            self.events.append((it, f"step undone (residual x{rn / self.prev_rn:.3g}), CFL -> {self.cfl:.3g}",
+                                rn, []))
+            self._say(f"step undone (residual x{rn / self.prev_rn:.3g}), CFL -> {self.cfl:.3g}")
+            return None
+        self.n_rej = 0
+        self.ok_streak += 1
+        if not np.isfinite(rn):
+            self.status = "diverged"
+            self.done = True
+            return None
+        peak = self.peak
+        if peak[2] is None:
+            peak[2] = rn
+        if rn > peak[0]:
+            peak[:2] = [rn, it]
+        past_peak = (peak[0] >= 5.0 * peak[2] and it - peak[1] >= 5) or it > 40
+        if rn < 0.95 * best["rn"] and (not self.turning or past_peak):
+            best.update(rn=rn, has=True, it=it, cfl=self.cfl, order=order)
+            S._keep_state(0)
+        hist = self.hist
+        if self.r0 is None or (order == 2 and len(hist) and hist[-1][1] == 1):
+            self.r0 = rn
+        hist.append((rn, order))
+        fr_ = S._forces_dict(self.alpha)
+        for k_, v_ in (("it", it), ("res", rn5), ("rn", rn), ("cl", fr_["cl"]), ("cd", fr_["cd"]),
+                       ("cfl", self.cfl)):
+            self.H[k_].append(v_)
+        if order == 2:
+            S._rref = max(getattr(S, "_rref", 0.0), rn)
+        if c is not None and c.monitor is not None:
+            c.monitor(dict(alpha=self.alpha, it=it, res=rn5, rn=rn, cl=fr_["cl"], cd=fr_["cd"], cfl=self.cfl,
+                           rref=getattr(S, "_rref", rn), order=order, r_fs=self.r_fs, lin=self.last_lin))
+        if order == 2 and rn < self.tol * self.r0:
+            self.conv_ok = True
+            self.done = True
+            return None
+        crit = self.crit
+        if order == 2:
+            fh = self.fh
+            fh.append((fr_["cl"], fr_["cd"]))
+            o2 = [h[0] for h in hist if h[1] == 2]
+            rref = self.r_fs
+            stalled = len(o2) > 60 and min(o2[-30:]) > 0.5 * min(o2[:-30])
+
+            def steady(win, tol_cl, tol_cd):
+                if len(fh) <= win:
+                    return False
+                cls = [f[0] for f in fh[-win - 1:]]
+                cds = [f[1] for f in fh[-win - 1:]]
+                return (max(cls) - min(cls) < tol_cl * max(0.2, abs(fh[-1][0]))
+                        and max(cds) - min(cds) < tol_cd)
+            hold = max(1, int(crit.hold))
+            lvl = 10.0 ** (-crit.orders) * rref
+            own = max(o2)
+            dropped = (not self.warm) or rn <= 1e-2 * own
+            end = None
+            if (len(o2) >= hold and max(o2[-hold:]) <= lvl and dropped
+                    and steady(hold, crit.cl_tol, crit.cd_tol)):
+                end = "converged"
+            elif (len(o2) >= 8 and max(o2[-3:]) <= 10.0 ** (-1.5) * lvl and rn <= 1e-3 * own
+                    and steady(3, crit.cl_tol, crit.cd_tol)):
+                end = "converged"
+            elif (crit.stall_ok and len(o2) > 40 and stalled and rn < 10.0 ** (1.0 - crit.orders) * rref
+                    and steady(max(hold, 30), crit.cl_tol, crit.cd_tol)):
+                end = "converged (residual stalled, forces steady)"
+            elif (crit.stall_ok and len(o2) > 60 and stalled and rn < 10.0 ** -2.5 * own
+                    and steady(max(hold, 40), crit.cl_tol, crit.cd_tol)):
+                end = "converged (forces steady)"
+            if end is not None:
+                self.conv_ok, self.status, self.done = True, end, True
+                return None
+            if (S.freeze_limiter and not self.frozen and S.limiter_k > 0 and S.limiter_zone != "off"
+                    and stalled and steady(30, 0.005, 1e-3)):
+                Qc = S._get()
+                S._freeze(S.phi(Qc))
+                S._set(Qc)
+                self.frozen = True
+                self._say(f"residual stalled at {rn:.2e}: limiter frozen")
+        # CFL (aero2d/rans.py:1617-1680)
+        if S.cfl_mode == "formula":
+            raise NotImplementedError("cfl_mode 'formula' is not built; use 'grow', 'fixed' or 'ser'")
+        if len(hist) >= 2 and hist[-2][1] == order:
+            ok_, lin_ = self.last_ok, self.last_lin
+            if S.cfl_mode == "fixed":
+                self.cfl = min(self.cfl_max, self.cfl * S.cfl_growth) if ok_ and rn < 3.0 * hist[-2][0] \
+                    else max(self.cfl0, 0.5 * self.cfl)
+            elif S.cfl_mode == "ser":
+                self.cfl = min(self.cfl_max, max(self.cfl0, self.cfl * min(2.0, hist[-2][0] / rn)))
+            else:
+                grow = (S.cfl_growth if rn < 1.2 * hist[-2][0] and ok_ and lin_ < 0.25 else
+                        S.cfl_lin_back if lin_ >= 0.25 and ok_ and rn < 3.0 * hist[-2][0] else
+                        1.0 if rn < 3.0 * hist[-2][0] and ok_ else 0.5)
+                self.cfl = min(self.cfl_max, max(self.cfl0, self.cfl * grow))
+        if self.cfl_hold is not None:
+            self.cfl = float(self.cfl_hold)
+        if S.turb_boost > 1.0:
+            ok_ = self.last_ok
+            fw_ = self.fh[-11:] if order >= 2 else []
+            calm = (len(fw_) == 11 and max(f[0] for f in fw_) - min(f[0] for f in fw_)
+                    < 2e-3 * max(0.2, abs(fw_[-1][0]))
+                    and max(f[1] for f in fw_) - min(f[1] for f in fw_) < 5e-4)
+            if calm and ok_ and self.cfl >= S.turb_boost_cfl and len(hist) >= 2 and rn < 1.2 * hist[-2][0]:
+                self.tb = min(S.turb_boost, 1.5 * self.tb)
+            elif not ok_ or (len(hist) >= 2 and rn >= 1.2 * hist[-2][0]):
+                self.tb = max(1.0, self.tb / 4.0)
+        ct = (S.cfl_turb if S.cfl_turb is not None else 1.0) * self.tb
+        gtol = self.gmres_tol if (self.warm and self.n_back == 0) else min(self.gmres_tol, 0.01)
+        # laminar solves take right preconditioning: left preconditioning
+        # "converges" on |M^-1 r| while the true residual stays near 0.2
+        # (aero2d/rans.py:1781-1790)
+        return dict(slot=self.slot, order=order, winf=self.winf, gam=gm, cfl=self.cfl, ct=ct, gtol=gtol,
+                    right=S.right_prec or not S.turbulent, gmres_iter=self.gmres_iter)
+
+    def after(self, st):
+        S, it = self.S, self.it
+        if st["ilu_nonfinite"]:
+            self._say(f"ILU not finite, CFL {self.cfl:.3g} -> {max(self.cfl0, self.cfl / 10):.3g}")
+            self.cfl = max(self.cfl0, self.cfl / 10)
+            return
+        lin = st["lin"]
+        self.H["gmres"].append(st["gmres_iterations"])
+        self.H["lin"].append(lin)
+        self.last_ok = bool(np.isfinite(lin) and lin < 0.5)
+        self.last_lin = lin
+        if not st["applied"]:
+            self._say(f"GMRES did not converge ({lin:.2g}), residual {self.rn:.3e}, CFL {self.cfl:.3g} -> "
+                      f"{max(self.cfl0, self.cfl / 4):.3g}")
+            self.cfl = max(self.cfl0, self.cfl / 4)
+            return
+        lam = st["lam"]
+        if lam < 1.0 and not st["per_cell"]:
+            self.cfl = max(self.cfl0 if self.n_tiny < 10 else 0.02 * self.cfl0, self.cfl * 0.5)
+        self.n_tiny = self.n_tiny + 1 if lam < 0.05 and not st["per_cell"] else 0
+        if self.n_tiny >= 60:
+            self.status = "stuck: the Newton step was cut to almost nothing for 60 iterations"
+            self.done = True
+            return
+        self.have_prev, self.prev_rn, self.prev_order = True, self.rn, self.order
+        if S.vortex_bc and it % 5 == 0:
+            self.gamma = 0.5 * S._forces_dict(self.alpha)["cl"] * S.mach * S.chord
+        if self.verbose and (it % 10 == 0 or it < 5):
+            fr = S._forces_dict(self.alpha)
+            print(f"  it {it:4d} ord {self.order} res {self.rn:.3e} cfl {self.cfl:.2e} lam {lam:.2f} "
+                  f"gmres {st['gmres_iterations']}:{lin:.1e} Cl {fr['cl']:.4f} Cd {fr['cd']:.5f}", flush=True)
+        if self.callback and self.callback(it, self.rn):
+            self.done = True
+
+    def finish(self):
+        S = self.S
+        if self.status == "diverged" and self.best["has"]:
+            S._keep_state(1)
+            self.status = "diverged (best state kept)"
+        fr = S._forces_dict(self.alpha)
+        Hn = {k_: np.array(v_) for k_, v_ in self.H.items()}
+        if not self.status:
+            self.status = "converged" if self.conv_ok else "not converged within the iterations allowed"
+        Hn["rref"] = float(getattr(S, "_rref", 0.0) or (max(Hn["rn"]) if len(Hn["rn"]) else 1.0))
+        Hn["r_fs"] = self.r_fs
+        Hn["events"] = self.events
+        if self.n_back:
+            self.status += f"; rollbacks: {self.n_back}"
+        S.gamma_vortex = self.gamma
+        return RansResult(self.alpha, fr["cl"], fr["cd"], fr["cdp"], fr["cdf"], fr["cm"], self.conv_ok, self.it,
+                          [h[0] for h in self.hist], fr["elements"], S._get(), self.status, Hn)

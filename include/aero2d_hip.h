@@ -291,6 +291,21 @@ int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, d
  * residual, 4 the first-order Jacobian blocks; reps repetitions at the state.
  * 0 and 1 factor the Jacobian at CFL 10 first. No counterpart in the reference. */
 int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, double gam, int reps, double* ms);
+/* Case slots (no counterpart in the reference, whose rans_polar runs angles
+ * in separate processes): a solver holds ncases slots (grown, never shrunk)
+ * that share its mesh, ILU structure and stream, each with its own state,
+ * Jacobian, factors and Krylov basis. a2d_rans_select makes slot c the one
+ * the single-case entry points above act on (slot 0 at creation). */
+int a2d_rans_set_cases(int64_t h, int ncases);
+int a2d_rans_select(int64_t h, int c);
+/* a2d_rans_newton_step for n distinct slots at once: per slot its order,
+ * winf (5), gam, CFL, turbulence step factor, GMRES tolerance and side; out
+ * (n, 12) as a2d_rans_newton_step's. The factorisation and every triangular
+ * sweep of all slots run as one launch (a workgroup per slot) and the
+ * Arnoldi steps go in lockstep; each slot's result equals its step alone. */
+int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* order, const double* winf,
+                          const double* gam, const double* cfl, const double* turb_factor, const double* gmres_tol,
+                          int gmres_iter, const int* right, int step_limit_cells, double* out);
 /* The last Newton step's dq before the step limit (nc, 5). */
 int a2d_rans_newton_dq(int64_t h, double* dq);
 /* 0: keep the state as the best; 1: return to the best; 2: undo the last step. */

@@ -229,3 +229,52 @@ def test_laminar_naca0012_drag_matches_the_literature():
     assert r.converged, r.status
     assert abs(r.cl) < 1e-3
     assert 0.053 < r.cd < 0.060 and 0.021 < r.cdp < 0.026 and 0.031 < r.cdf < 0.036, (r.cd, r.cdp, r.cdf)
+
+
+def test_batched_newton_steps_equal_single_steps():
+    """Two case slots stepped by one a2d_rans_newton_steps call (one
+    factorisation launch and shared triangular launches) give the bits of each
+    slot's a2d_rans_newton_step alone."""
+    m, s, o = _pair("hybrid")
+    s.prec_fill = 1
+    winf = [o.freestream(a) for a in (1.0, 3.0)]
+    Qs = [_state(o, a, 20 + k, amp=0.02) for k, a in enumerate((1.0, 3.0))]
+    alone = []
+    for k in range(2):
+        st = s.newton_step(2, winf[k], 0.03 * k, 10.0 + 5 * k, 1e-2, 20, Q=Qs[k])
+        alone.append((st, s._get(), s.newton_dq()))
+    _abi_set = __import__("aero2d_amd._abi", fromlist=["call"]).call
+    _abi_set("a2d_rans_set_cases", s._device(), 2)
+    for k in range(2):
+        s._select(k)
+        s._set(Qs[k])
+    outs = s._newton_steps([dict(slot=k, order=2, winf=winf[k], gam=0.03 * k, cfl=10.0 + 5 * k, ct=1.0, gtol=1e-2,
+                                 right=False, gmres_iter=20) for k in range(2)])
+    for k in range(2):
+        s._select(k)
+        st, Qa, dqa = alone[k]
+        assert outs[k] == st, (k, outs[k], st)
+        assert np.array_equal(s.newton_dq(), dqa) and np.array_equal(s._get(), Qa)
+    s._select(0)
+
+
+def test_polar_batch_equals_single_solves():
+    """solve_polar over three angles (one slot each, lockstep batched steps)
+    returns each angle's solve alone, bit for bit, with the same history."""
+    m = _mesh("quads")
+    alphas = (0.0, 2.0, 4.0)
+    kw = dict(max_iter=12)
+    single = []
+    for a in alphas:
+        s1 = RansSolver(m, 0.5, 2e4)
+        s1.prec_fill = 1
+        single.append(s1.solve(a, **kw))
+        s1.close()
+    sp = RansSolver(m, 0.5, 2e4)
+    sp.prec_fill = 1
+    batch = sp.solve_polar(alphas, **kw)
+    for r1, rb in zip(single, batch):
+        assert rb.alpha == r1.alpha and rb.iterations == r1.iterations and rb.status == r1.status
+        assert np.array_equal(rb.state, r1.state)
+        assert rb.cl == r1.cl and rb.cd == r1.cd
+        assert np.array_equal(np.asarray(rb.residual), np.asarray(r1.residual))
