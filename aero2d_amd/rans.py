@@ -483,19 +483,23 @@ class RansSolver:
         return self._drive([run])[0]
 
     def solve_polar(self, alphas, Q0=None, max_iter=300, tol=1e-7, cfl0=None, cfl_max=None,
-                    first_order_iters=None, verbose=False, gmres_tol=None, gmres_iter=60, conv=None):
+                    first_order_iters=None, verbose=False, gmres_tol=None, gmres_iter=60, conv=None, alpha0=None):
         """Every angle of a polar solved at once, one case slot per angle, each
         with the loop of `solve` from its own start; their Newton steps run as
         one batched device call per iteration (the factorisation and
         triangular sweeps of all angles in single launches). The reference runs
         a polar's angles in worker processes (rans_polar, aero2d/rans.py:2619);
         each result here equals that angle's `solve` alone. Q0: None or one
-        state per angle. Returns the RansResults in the order of alphas."""
+        state per angle; alpha0: the angle each Q0 was solved at (a number or
+        one per angle), so its far field is turned to the new angle as
+        `solve(..., alpha0=)` does. Returns the RansResults in the order of
+        alphas."""
         alphas = [float(a) for a in alphas]
         self._ensure_ilu()
         _abi.call("a2d_rans_set_cases", self._device(), len(alphas))
+        a0 = [alpha0] * len(alphas) if alpha0 is None or np.isscalar(alpha0) else list(alpha0)
         runs = [_Run(self, k, a, None if Q0 is None else Q0[k], max_iter, tol, cfl0, cfl_max, first_order_iters,
-                     verbose, None, gmres_tol, gmres_iter, conv, None, None) for k, a in enumerate(alphas)]
+                     verbose, None, gmres_tol, gmres_iter, conv, None, a0[k]) for k, a in enumerate(alphas)]
         return self._drive(runs)
 
     def _select(self, slot):

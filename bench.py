@@ -774,6 +774,28 @@ def run_rans(a):
                                "unit": "GB/s", "bytes_per_launch": mv_bytes, "ms_per_launch": kt["matvec"],
                                "frac": mv_bytes / (kt["matvec"] * 1e-3) / 8e12,
                                "kernel": "Jacobian-vector product (k_grad, k_limit, k_face, k_cell in dual numbers)"}
+    if a.cases != 0:
+        # a polar: every angle in its own case slot, all stepped together,
+        # warm-started from the converged alpha-0 state turned to each angle
+        # (the continuation the reference's polars use, aero2d/rans.py:2375-2436)
+        nang = a.cases or 16
+        alphas = list(np.linspace(-4.0, 8.0, nang))
+        sp = RansSolver(m, mach, re)
+        sp.zmin = 0.1
+        t0 = time.perf_counter()
+        rp = sp.solve_polar(alphas, Q0=[r.state] * nang, alpha0=alpha, max_iter=iters)
+        wp = time.perf_counter() - t0
+        conv = int(sum(x.converged for x in rp))
+        line["polar"] = {"angles": nang, "alphas": [float(x) for x in alphas], "wall_s": wp,
+                         "start": "the converged alpha-0 state, turned to each angle",
+                         "newton_iterations_per_s": float(sum(x.iterations for x in rp)) / wp,
+                         "converged": conv, "converged_angles_per_s": conv / wp,
+                         "single_angle_converged_per_s": (1.0 / wall) if r.converged else None,
+                         "speedup_vs_one_at_a_time": (conv / wp) * wall if r.converged else None,
+                         "newton_iterations": [int(x.iterations) for x in rp],
+                         "cl": [float(x.cl) for x in rp], "cd": [float(x.cd) for x in rp],
+                         "status": [x.status for x in rp]}
+        sp.close()
     if not a.no_cpu_baseline:
         gpi = line["solve"]["gmres_iterations_mean"] or 10.0
         line["cpu_baseline"] = rans_cpu_baseline(m, mach, re, alpha, gpi)
