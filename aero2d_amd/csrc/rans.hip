@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -739,10 +740,12 @@ struct IluDev {
   const int *pptr, *pt, *pe2;        // per lower entry: its update pairs (target, row-k entry)
   int nlev, nlevu;
   const int *lptr, *uptr;            // level bounds into frec / brec
-  const int4 *frec, *brec;           // (row, first entry, end entry, 0): lower entries by
-                                     // forward level, upper entries by backward level
+  const int4 *frec, *brec;           // frec (row, row start, diagonal, row end) by forward
+                                     // level; brec (row, first upper, row end, 0) by backward level
   const int* perm;                   // perm[new] = old
   int lds_levels;                    // level bounds fit in LDS
+  int rowcap, paircap;               // longest row (blocks) and most update pairs of one row
+  int lds_waves;                     // wavefronts of the LDS factorisation (0: use the global one)
 };
 
 constexpr int ILU_TPB = 1024;
@@ -831,6 +834,111 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor(IluDev L, double* B, dou
   }
 }
 
+// Ordering of this wavefront's LDS accesses between lanes (LDS operations of
+// one wavefront complete in order; the fences keep the compiler from moving
+// accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The factorisation with each row staged in LDS: a wavefront copies its row's
+// blocks and the row's update-pair list into LDS (coalesced), runs the
+// elimination there (the same products in the same order as k_ilu_factor),
+// and writes the row back. The only global loads inside the elimination are
+// the finished blocks and inverses of earlier rows, which no store of the
+// row can alias, so they are issued early instead of waiting on the row's
+// own updates.
+__global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B, double* invU, const int* cases,
+                                                            size_t sB, size_t sI) {
+  if (cases) {
+    B += size_t(cases[blockIdx.x]) * sB;
+    invU += size_t(cases[blockIdx.x]) * sI;
+  }
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6, lane = threadIdx.x & 63;
+  double* rowb = smem + size_t(wave) * L.rowcap * 25;
+  int* pairs_all = reinterpret_cast<int*>(smem + size_t(nwave) * L.rowcap * 25);
+  int* rowp = pairs_all + size_t(wave) * L.paircap * 2;
+  int* lev_sh = pairs_all + size_t(nwave) * L.paircap * 2;
+  const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, L.lds_levels);
+  const int half = lane >> 5, hl = lane & 31;
+  const bool act = hl < 25;
+  const int r = act ? hl / 5 : 4, cc = act ? hl % 5 : 4;
+  for (int l = 0; l < L.nlev; ++l) {
+    const int p1 = lp[l + 1];
+    for (int p = lp[l] + wave; p < p1; p += nwave) {
+      const int4 rec = L.frec[p];
+      const int i = rec.x, rs = rec.y, d = rec.z, re = rec.w;
+      const int nb25 = (re - rs) * 25;
+      for (int t = lane; t < nb25; t += 64) rowb[t] = B[size_t(rs) * 25 + t];
+      const int q0 = L.pptr[rs], q1 = L.pptr[d];
+      for (int t = lane; t < q1 - q0; t += 64) {
+        rowp[2 * t] = L.pt[q0 + t] - rs;
+        rowp[2 * t + 1] = L.pe2[q0 + t];
+      }
+      wave_sync();
+      for (int e = rs; e < d; ++e) {
+        const int k = L.col[e];
+        const double be = rowb[(e - rs) * 25 + r * 5 + cc];
+        double sv = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) sv += __shfl(be, r * 5 + mm, 32) * invU[size_t(k) * 25 + mm * 5 + cc];
+        double lr[5];
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) lr[mm] = __shfl(sv, r * 5 + mm, 32);
+        wave_sync();
+        if (act && half == 0) rowb[(e - rs) * 25 + hl] = sv;
+        const int qb = L.pptr[e + 1] - q0;
+        for (int q = L.pptr[e] - q0 + half; q < qb; q += 2) {
+          const int tl = rowp[2 * q], e2 = rowp[2 * q + 1];
+          double acc = 0.0;
+#pragma unroll
+          for (int mm = 0; mm < 5; ++mm) acc += lr[mm] * B[size_t(e2) * 25 + mm * 5 + cc];
+          if (act) rowb[tl * 25 + hl] -= acc;
+        }
+        wave_sync();
+      }
+      double av = rowb[(d - rs) * 25 + r * 5 + cc];
+      double iv = r == cc ? 1.0 : 0.0;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        int piv = c;
+        double best = fabs(__shfl(av, c * 5 + c, 32));
+#pragma unroll
+        for (int x = c + 1; x < 5; ++x) {
+          const double v = fabs(__shfl(av, x * 5 + c, 32));
+          if (v > best) {
+            best = v;
+            piv = x;
+          }
+        }
+        if (piv != c) {
+          const int src = r == c ? piv : (r == piv ? c : r);
+          av = __shfl(av, src * 5 + cc, 32);
+          iv = __shfl(iv, src * 5 + cc, 32);
+        }
+        const double inv = 1.0 / __shfl(av, c * 5 + c, 32);
+        if (r == c) {
+          av *= inv;
+          iv *= inv;
+        }
+        const double f = __shfl(av, r * 5 + c, 32);
+        const double ac = __shfl(av, c * 5 + cc, 32), ic = __shfl(iv, c * 5 + cc, 32);
+        if (r != c) {
+          av -= f * ac;
+          iv -= f * ic;
+        }
+      }
+      if (act && half == 0) invU[size_t(i) * 25 + hl] = iv;
+      for (int t = lane; t < nb25; t += 64) B[size_t(rs) * 25 + t] = rowb[t];
+      wave_sync();
+    }
+    __syncthreads();
+  }
+}
+
 // x (SoA [5][n], original order) -> y ([n][5], permuted order).
 __global__ void k_perm_in(int n, const int* perm, const double* b, double* y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -911,6 +1019,19 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
       if (lane < 5) y[size_t(i) * 5 + lane] = xi;
     }
     __syncthreads();
+  }
+}
+
+// One factorisation launch over ncase workgroups (ncase = 1 with cases null):
+// the LDS-staged kernel when the rows fit, else the global one.
+void launch_ilu_factor(const IluDev& L, double* B, double* invU, const int* cases, int ncase, size_t sB,
+                       size_t sI, hipStream_t st) {
+  const size_t lev = L.lds_levels ? size_t(L.nlev + 1) * sizeof(int) : 0;
+  if (L.lds_waves > 0) {
+    const size_t lds = size_t(L.lds_waves) * (size_t(L.rowcap) * 25 * sizeof(double) + size_t(L.paircap) * 8) + lev;
+    hipLaunchKernelGGL(k_ilu_factor_lds, ncase, L.lds_waves * 64, lds, st, L, B, invU, cases, sB, sI);
+  } else {
+    hipLaunchKernelGGL(k_ilu_factor, ncase, ILU_TPB, lev, st, L, B, invU, cases, sB, sI);
   }
 }
 
@@ -1390,8 +1511,7 @@ void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Do
     return;
   }
   if (!launch_factor) return;
-  const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
-  hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, nullptr, size_t(0), size_t(0));
+  launch_ilu_factor(s.ilu, s.B, s.invU, nullptr, 1, 0, 0, s.st);
   HC(hipGetLastError());
 }
 
@@ -2044,7 +2164,7 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     std::vector<int4> frec(n), brec(n);
     for (int p = 0; p < n; ++p) {
       const int i = lrows[p], j = urows[p];
-      frec[p] = make_int4(i, rstart[i], rdiag[i], 0);
+      frec[p] = make_int4(i, rstart[i], rdiag[i], rend[i]);
       brec[p] = make_int4(j, rdiag[j] + 1, rend[j], 0);
     }
     // release an earlier setup
@@ -2065,6 +2185,22 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     L.brec = s.own(dupload(brec));
     L.perm = s.own(dupload(permv));
     L.lds_levels = nlev + nlevu + 2 <= ILU_LDS_LEVELS;
+    int rowcap = 1, paircap = 1;
+    for (int i = 0; i < n; ++i) {
+      rowcap = std::max(rowcap, rend[i] - rstart[i]);
+      paircap = std::max(paircap, pptr[rdiag[i]] - pptr[rstart[i]]);
+    }
+    L.rowcap = rowcap;
+    L.paircap = paircap;
+    {
+      int lds_max = 65536;
+      HC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, a2d::require_device()));
+      const size_t lev = L.lds_levels ? size_t(nlev + 1) * sizeof(int) : 0;
+      const size_t per = size_t(rowcap) * 25 * sizeof(double) + size_t(paircap) * 8;
+      const long fit = (long(lds_max) - long(lev)) / long(per);
+      L.lds_waves = fit >= 4 ? int(std::min<long>(fit, ILU_TPB / 64)) : 0;
+      if (getenv("A2D_RANS_FACTOR_GLOBAL")) L.lds_waves = 0;
+    }
     s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
     s.posLR = s.own(dupload(std::vector<int>(pos_src + n, pos_src + n + nfi)));
     s.posRL = s.own(dupload(std::vector<int>(pos_src + n + nfi, pos_src + n + 2 * nfi)));
@@ -2189,12 +2325,7 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
     auto once = [&] {
       switch (which) {
         case 0: ilu_solve(s, s.R, s.tmp); break;
-        case 1: {
-          const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + 1) * sizeof(int) : 0;
-          hipLaunchKernelGGL(k_ilu_factor, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, nullptr, size_t(0),
-                             size_t(0));
-          break;
-        }
+        case 1: launch_ilu_factor(s.ilu, s.B, s.invU, nullptr, 1, 0, 0, s.st); break;
         case 2: eval(s, order, s.R, nullptr, s.tmp, s.vdt, 1.0); break;
         case 3: eval(s, order, nullptr, s.tmp, nullptr, nullptr, 1.0); break;
         case 4: {
@@ -2258,12 +2389,8 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
       hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl[i], s.dt, s.vdt);
       factor(s, winf + 5 * i, gam[i], ct[i], nullptr, nullptr, false);
     }
-    {
-      const size_t lds = s0.ilu.lds_levels ? size_t(s0.ilu.nlev + 1) * sizeof(int) : 0;
-      hipLaunchKernelGGL(k_ilu_factor, n, ILU_TPB, lds, s0.st, s0.ilu, g.Ball, g.Iall, case_list(g, ids), g.sB,
-                         g.sI);
-      HC(hipGetLastError());
-    }
+    launch_ilu_factor(s0.ilu, g.Ball, g.Iall, case_list(g, ids), n, g.sB, g.sI, s0.st);
+    HC(hipGetLastError());
     HC(hipMemsetAsync(g.nfall, 0, g.cases.size() * sizeof(int), s0.st));
     for (int i = 0; i < n; ++i)
       hipLaunchKernelGGL(k_count_nonfinite, nblk(s0.m.nc * 25), TPB, 0, s0.st, s0.m.nc * 25, cs[i]->invU,
