@@ -744,11 +744,24 @@ struct IluDev {
                                      // level; brec (row, first upper, row end, 0) by backward level
   const int* perm;                   // perm[new] = old
   int lds_levels;                    // level bounds fit in LDS
-  int rowcap, paircap;               // longest row (blocks) and most update pairs of one row
+  int rowcap, paircap, lowcap;       // longest row (blocks), most update pairs and most lower
+                                     // blocks of one row
   int lds_waves;                     // wavefronts of the LDS factorisation (0: use the global one)
+  // Staged copy S of the factors for the LDS triangular sweeps: per level, the
+  // blocks its rows read, contiguous (forward: each row's lower blocks;
+  // backward: each row's upper blocks, then its inverse diagonal block).
+  const int4 *fsrec, *bsrec;         // (row, first entry, end entry, the row's first block
+                                     // in its level's run) by forward / backward level
+  const int *flev, *blev;            // first staged block of each level (blev after flev's)
+  const int* ssrc;                   // source of staged block b: entry e < nnz of B, or
+                                     // nnz + i for invU of row i
+  int nst;                           // staged blocks (nnz: every block once, inverses included)
+  int scap;                          // longest level run, in doubles
+  int lds_solve;                     // 1: the sweeps stage levels in LDS (k_ilu_solve_lds)
 };
 
 constexpr int ILU_TPB = 1024;
+constexpr int PAIR_SLOTS = 12;          // update pairs in flight per wavefront (5 lanes each)
 constexpr int ILU_LDS_LEVELS = 12288;   // level bounds staged in LDS up to this many
 
 __device__ __forceinline__ const int* stage_bounds(const int* g, int n, int* sh, bool use) {
@@ -845,8 +858,9 @@ __device__ __forceinline__ void wave_sync() {
 
 // The factorisation with each row staged in LDS: a wavefront copies its row's
 // blocks and the row's update-pair list into LDS (coalesced), runs the
-// elimination there (the same products in the same order as k_ilu_factor),
-// and writes the row back. The only global loads inside the elimination are
+// elimination there (the same products in the same order as k_ilu_factor,
+// with twelve update pairs in flight per wavefront: five lanes per pair, one
+// column each), and writes the row back. The only global loads inside the elimination are
 // the finished blocks and inverses of earlier rows, which no store of the
 // row can alias, so they are issued early instead of waiting on the row's
 // own updates.
@@ -859,13 +873,15 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
   extern __shared__ double smem[];
   const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6, lane = threadIdx.x & 63;
   double* rowb = smem + size_t(wave) * L.rowcap * 25;
-  int* pairs_all = reinterpret_cast<int*>(smem + size_t(nwave) * L.rowcap * 25);
+  double* rowi = smem + size_t(nwave) * L.rowcap * 25 + size_t(wave) * L.lowcap * 25;
+  int* pairs_all = reinterpret_cast<int*>(smem + size_t(nwave) * (L.rowcap + L.lowcap) * 25);
   int* rowp = pairs_all + size_t(wave) * L.paircap * 2;
   int* lev_sh = pairs_all + size_t(nwave) * L.paircap * 2;
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, L.lds_levels);
   const int half = lane >> 5, hl = lane & 31;
   const bool act = hl < 25;
   const int r = act ? hl / 5 : 4, cc = act ? hl % 5 : 4;
+  const int pslot = lane / 5, pc = lane - 5 * pslot;
   for (int l = 0; l < L.nlev; ++l) {
     const int p1 = lp[l + 1];
     for (int p = lp[l] + wave; p < p1; p += nwave) {
@@ -873,6 +889,8 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
       const int i = rec.x, rs = rec.y, d = rec.z, re = rec.w;
       const int nb25 = (re - rs) * 25;
       for (int t = lane; t < nb25; t += 64) rowb[t] = B[size_t(rs) * 25 + t];
+      // the inverses of the lower entries' rows (finished at earlier levels)
+      for (int t = lane; t < (d - rs) * 25; t += 64) rowi[t] = invU[size_t(L.col[rs + t / 25]) * 25 + t % 25];
       const int q0 = L.pptr[rs], q1 = L.pptr[d];
       for (int t = lane; t < q1 - q0; t += 64) {
         rowp[2 * t] = L.pt[q0 + t] - rs;
@@ -880,23 +898,32 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
       }
       wave_sync();
       for (int e = rs; e < d; ++e) {
-        const int k = L.col[e];
         const double be = rowb[(e - rs) * 25 + r * 5 + cc];
         double sv = 0.0;
 #pragma unroll
-        for (int mm = 0; mm < 5; ++mm) sv += __shfl(be, r * 5 + mm, 32) * invU[size_t(k) * 25 + mm * 5 + cc];
-        double lr[5];
-#pragma unroll
-        for (int mm = 0; mm < 5; ++mm) lr[mm] = __shfl(sv, r * 5 + mm, 32);
+        for (int mm = 0; mm < 5; ++mm) sv += __shfl(be, r * 5 + mm, 32) * rowi[(e - rs) * 25 + mm * 5 + cc];
         wave_sync();
         if (act && half == 0) rowb[(e - rs) * 25 + hl] = sv;
+        wave_sync();
+        // update pairs twelve at a time: lane (slot, column pc) forms column pc of
+        // L_ik U_kj for its pair (L_ik read back from the row in LDS, the same
+        // address for every lane), each element in the order of the plain kernel
+        const double* lk = rowb + (e - rs) * 25;
         const int qb = L.pptr[e + 1] - q0;
-        for (int q = L.pptr[e] - q0 + half; q < qb; q += 2) {
-          const int tl = rowp[2 * q], e2 = rowp[2 * q + 1];
-          double acc = 0.0;
+        if (lane < 5 * PAIR_SLOTS) {
+          for (int q = L.pptr[e] - q0 + pslot; q < qb; q += PAIR_SLOTS) {
+            const int tl = rowp[2 * q], e2 = rowp[2 * q + 1];
+            double u[5];
 #pragma unroll
-          for (int mm = 0; mm < 5; ++mm) acc += lr[mm] * B[size_t(e2) * 25 + mm * 5 + cc];
-          if (act) rowb[tl * 25 + hl] -= acc;
+            for (int mm = 0; mm < 5; ++mm) u[mm] = B[size_t(e2) * 25 + mm * 5 + pc];
+#pragma unroll
+            for (int rr = 0; rr < 5; ++rr) {
+              double acc = 0.0;
+#pragma unroll
+              for (int mm = 0; mm < 5; ++mm) acc += lk[rr * 5 + mm] * u[mm];
+              rowb[tl * 25 + rr * 5 + pc] -= acc;
+            }
+          }
         }
         wave_sync();
       }
@@ -960,13 +987,14 @@ __global__ void k_perm_out(int n, const int* perm, const double* y, double* x) {
 // every eighth of the row's blocks and forms its 5x5 block-vector products,
 // the eight partial sums are combined by a butterfly of lane shuffles, and
 // lane k < 5 writes component k.
-__device__ __forceinline__ void row_products(const double* B, const int* col, const double* y, int e0, int e1,
+// Bb: the block of entry e0 (the row's blocks follow it).
+__device__ __forceinline__ void row_products(const double* Bb, const int* col, const double* y, int e0, int e1,
                                              int lane, double a[5]) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) a[k] = 0.0;
   for (int e = e0 + lane; e < e1; e += 8) {
     const double* yc = y + size_t(col[e]) * 5;
-    const double* b = B + size_t(e) * 25;
+    const double* b = Bb + size_t(e - e0) * 25;
     const double y0 = yc[0], y1 = yc[1], y2 = yc[2], y3 = yc[3], y4 = yc[4];
 #pragma unroll
     for (int k = 0; k < 5; ++k)
@@ -996,7 +1024,7 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
     const int p1 = lp[l + 1];
     for (int p = lp[l] + grp; p < p1; p += ngrp) {
       const int4 rec = L.frec[p];
-      row_products(B, L.col, y, rec.y, rec.z, lane, a);
+      row_products(B + size_t(rec.y) * 25, L.col, y, rec.y, rec.z, lane, a);
       double mine = a[0];
 #pragma unroll
       for (int k = 1; k < 5; ++k) mine = lane == k ? a[k] : mine;
@@ -1009,7 +1037,7 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
     for (int p = up[l] + grp; p < p1; p += ngrp) {
       const int4 rec = L.brec[p];
       const int i = rec.x;
-      row_products(B, L.col, y, rec.y, rec.z, lane, a);
+      row_products(B + size_t(rec.y) * 25, L.col, y, rec.y, rec.z, lane, a);
       double t[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) t[k] = y[size_t(i) * 5 + k] - a[k];
@@ -1022,16 +1050,141 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
   }
 }
 
+// S <- the factors' blocks in level order (a gather; grid.y over the cases).
+__global__ void k_ilu_stage(IluDev L, const double* B, const double* invU, double* S, const int* cases,
+                            size_t sB, size_t sI, size_t sS) {
+  if (cases) {
+    const size_t c = size_t(cases[blockIdx.y]);
+    B += c * sB;
+    invU += c * sI;
+    S += c * sS;
+  }
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= size_t(L.nst) * 25) return;
+  const int b = int(t / 25), k = int(t - size_t(b) * 25);
+  const int id = L.ssrc[b];
+  S[t] = id < L.nnz ? B[size_t(id) * 25 + k] : invU[size_t(id - L.nnz) * 25 + k];
+}
+
+constexpr int SOLVE_K = 16;   // doubles a staging thread carries: a level run is at most
+                              // SOLVE_K * ILU_TPB / 2 doubles
+
+// Copy n doubles src -> dst by nt threads (t this thread's index): all loads
+// first, then the LDS stores, so a thread has its loads in flight together.
+__device__ __forceinline__ void stage_run(const double* __restrict__ src, int n, double* dst, int t, int nt) {
+  double v[SOLVE_K];
+#pragma unroll
+  for (int k = 0; k < SOLVE_K; ++k) {
+    const int idx = t + k * nt;
+    v[k] = idx < n ? src[idx] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < SOLVE_K; ++k) {
+    const int idx = t + k * nt;
+    if (idx < n) dst[idx] = v[k];
+  }
+}
+
+// The sweeps of k_ilu_solve with the factors read from LDS: the upper half of
+// the workgroup copies the next level's run of S into one LDS buffer while the
+// lower half works on the current level from the other (eight lanes per row,
+// the same products in the same order as k_ilu_solve), so a level costs the
+// loads of y and the barrier; the factors' loads (from HBM, the larger
+// latency) overlap the previous level. Separate wavefronts keep the copy's
+// loads out of the workers' wait counts.
+__global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const double* S, double* y, const int* cases,
+                                                           size_t sS, size_t sY) {
+  if (cases) {
+    S += size_t(cases[blockIdx.x]) * sS;
+    y += size_t(cases[blockIdx.x]) * sY;
+  }
+  extern __shared__ double smem[];
+  double* buf0 = smem;
+  double* buf1 = smem + L.scap;
+  int* lev_sh = reinterpret_cast<int*>(smem + 2 * size_t(L.scap));
+  const bool use = L.lds_levels;
+  const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, use);
+  const int* up = stage_bounds(L.uptr, L.nlevu + 1, lev_sh + (use ? L.nlev + 1 : 0), use);
+  const int nw = blockDim.x >> 1;                 // workers: threads [0, nw); copiers: [nw, 2 nw)
+  const bool copier = threadIdx.x >= nw;
+  const int ct = threadIdx.x - nw;
+  const int grp = threadIdx.x >> 3, lane = threadIdx.x & 7, ngrp = nw >> 3;
+  const int nl = L.nlev + L.nlevu;
+  // run of level q of the whole sequence (forward levels, then backward ones)
+  auto run = [&](int q, int& b0, int& b1) {
+    if (q < L.nlev) {
+      b0 = L.flev[q];
+      b1 = L.flev[q + 1];
+    } else {
+      b0 = L.blev[q - L.nlev];
+      b1 = L.blev[q - L.nlev + 1];
+    }
+  };
+  if (copier) {
+    int b0, b1;
+    run(0, b0, b1);
+    stage_run(S + size_t(b0) * 25, (b1 - b0) * 25, buf0, ct, nw);
+  }
+  __syncthreads();
+  double a[5];
+  for (int q = 0; q < nl; ++q) {
+    const double* cur = (q & 1) ? buf1 : buf0;
+    if (copier) {
+      if (q + 1 < nl) {
+        int b0, b1;
+        run(q + 1, b0, b1);
+        stage_run(S + size_t(b0) * 25, (b1 - b0) * 25, (q & 1) ? buf0 : buf1, ct, nw);
+      }
+    } else if (q < L.nlev) {
+      const int p1 = lp[q + 1];
+      for (int p = lp[q] + grp; p < p1; p += ngrp) {
+        const int4 rec = L.fsrec[p];
+        row_products(cur + size_t(rec.w) * 25, L.col, y, rec.y, rec.z, lane, a);
+        double mine = a[0];
+#pragma unroll
+        for (int k = 1; k < 5; ++k) mine = lane == k ? a[k] : mine;
+        if (lane < 5) y[size_t(rec.x) * 5 + lane] -= mine;
+      }
+    } else {
+      const int l = q - L.nlev, p1 = up[l + 1];
+      for (int p = up[l] + grp; p < p1; p += ngrp) {
+        const int4 rec = L.bsrec[p];
+        const int i = rec.x;
+        row_products(cur + size_t(rec.w) * 25, L.col, y, rec.y, rec.z, lane, a);
+        double t[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) t[k] = y[size_t(i) * 5 + k] - a[k];
+        const int kk = lane < 5 ? lane : 4;
+        const double* iu = cur + (size_t(rec.w) + size_t(rec.z - rec.y)) * 25 + kk * 5;
+        const double xi = iu[0] * t[0] + iu[1] * t[1] + iu[2] * t[2] + iu[3] * t[3] + iu[4] * t[4];
+        if (lane < 5) y[size_t(i) * 5 + lane] = xi;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+size_t ilu_solve_lds_bytes(const IluDev& L) {
+  return 2 * size_t(L.scap) * sizeof(double) + (L.lds_levels ? size_t(L.nlev + L.nlevu + 2) * sizeof(int) : 0);
+}
+
 // One factorisation launch over ncase workgroups (ncase = 1 with cases null):
-// the LDS-staged kernel when the rows fit, else the global one.
-void launch_ilu_factor(const IluDev& L, double* B, double* invU, const int* cases, int ncase, size_t sB,
-                       size_t sI, hipStream_t st) {
+// the LDS-staged kernel when the rows fit, else the global one; then, for the
+// LDS sweeps, the staged copy S of the factors.
+void launch_ilu_factor(const IluDev& L, double* B, double* invU, double* S, const int* cases, int ncase,
+                       size_t sB, size_t sI, size_t sS, hipStream_t st) {
   const size_t lev = L.lds_levels ? size_t(L.nlev + 1) * sizeof(int) : 0;
   if (L.lds_waves > 0) {
-    const size_t lds = size_t(L.lds_waves) * (size_t(L.rowcap) * 25 * sizeof(double) + size_t(L.paircap) * 8) + lev;
+    const size_t lds =
+        size_t(L.lds_waves) * (size_t(L.rowcap + L.lowcap) * 25 * sizeof(double) + size_t(L.paircap) * 8) + lev;
     hipLaunchKernelGGL(k_ilu_factor_lds, ncase, L.lds_waves * 64, lds, st, L, B, invU, cases, sB, sI);
   } else {
     hipLaunchKernelGGL(k_ilu_factor, ncase, ILU_TPB, lev, st, L, B, invU, cases, sB, sI);
+  }
+  if (L.lds_solve) {
+    const size_t nt = size_t(L.nst) * 25;
+    hipLaunchKernelGGL(k_ilu_stage, dim3(unsigned((nt + 255) / 256), unsigned(ncase)), 256, 0, st, L,
+                       (const double*)B, (const double*)invU, S, cases, sB, sI, sS);
   }
 }
 
@@ -1230,7 +1383,7 @@ struct Solver {
   std::vector<unsigned char> in_zone;   // host copy of the trailing-edge limiter zone
   // Jacobian and ILU
   double *J = nullptr, *JW = nullptr, *JF = nullptr;
-  double *B = nullptr, *invU = nullptr, *y = nullptr;
+  double *B = nullptr, *invU = nullptr, *y = nullptr, *S = nullptr;
   int *posD = nullptr, *posLR = nullptr, *posRL = nullptr;
   IluDev ilu{};
   bool has_ilu = false;
@@ -1263,15 +1416,15 @@ constexpr int HSTRIDE = 2 * (GMRES_MAX + 1) + 8;
 struct Group {
   std::vector<std::unique_ptr<Solver>> cases;
   int cur = 0;
-  double *Ball = nullptr, *Iall = nullptr, *Yall = nullptr, *hall = nullptr, *smallall = nullptr;
+  double *Ball = nullptr, *Iall = nullptr, *Yall = nullptr, *Sall = nullptr, *hall = nullptr, *smallall = nullptr;
   int *nfall = nullptr, *dcases = nullptr;
-  size_t sB = 0, sI = 0, sY = 0;
+  size_t sB = 0, sI = 0, sY = 0, sS = 0;
   std::vector<int> dcases_host;
   ~Group() {
     while (cases.size() > 1) cases.pop_back();   // slot 0 (mesh, stream) last
     cases.clear();
-    for (void* p : {(void*)Ball, (void*)Iall, (void*)Yall, (void*)hall, (void*)smallall, (void*)nfall,
-                    (void*)dcases})
+    for (void* p : {(void*)Ball, (void*)Iall, (void*)Yall, (void*)Sall, (void*)hall, (void*)smallall,
+                    (void*)nfall, (void*)dcases})
       if (p) (void)hipFree(p);
   }
 };
@@ -1358,9 +1511,14 @@ void mdot(Solver& s, const double* Vb, size_t ldv, int nvec, const double* w, in
 void ilu_solve(Solver& s, const double* b, double* x) {
   const int n = s.m.nc;
   hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s.st, n, s.ilu.perm, b, s.y);
-  const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + s.ilu.nlevu + 2) * sizeof(int) : 0;
-  hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, s.y, nullptr, size_t(0), size_t(0),
-                     size_t(0));
+  if (s.ilu.lds_solve) {
+    hipLaunchKernelGGL(k_ilu_solve_lds, 1, ILU_TPB, ilu_solve_lds_bytes(s.ilu), s.st, s.ilu, (const double*)s.S,
+                       s.y, nullptr, size_t(0), size_t(0));
+  } else {
+    const size_t lds = s.ilu.lds_levels ? size_t(s.ilu.nlev + s.ilu.nlevu + 2) * sizeof(int) : 0;
+    hipLaunchKernelGGL(k_ilu_solve, 1, ILU_TPB, lds, s.st, s.ilu, s.B, s.invU, s.y, nullptr, size_t(0),
+                       size_t(0), size_t(0));
+  }
   hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s.st, n, s.ilu.perm, s.y, x);
 }
 
@@ -1511,7 +1669,7 @@ void factor(Solver& s, const double* winf, double gam, double ct_inv, double* Do
     return;
   }
   if (!launch_factor) return;
-  launch_ilu_factor(s.ilu, s.B, s.invU, nullptr, 1, 0, 0, s.st);
+  launch_ilu_factor(s.ilu, s.B, s.invU, s.S, nullptr, 1, 0, 0, 0, s.st);
   HC(hipGetLastError());
 }
 
@@ -1606,7 +1764,7 @@ void relink(Group& g) {
     s.nonfinite = g.nfall + c;
   }
   if (!s0.has_ilu) return;
-  for (void** p : {(void**)&g.Ball, (void**)&g.Iall, (void**)&g.Yall})
+  for (void** p : {(void**)&g.Ball, (void**)&g.Iall, (void**)&g.Yall, (void**)&g.Sall})
     if (*p) {
       HC(hipFree(*p));
       *p = nullptr;
@@ -1614,9 +1772,11 @@ void relink(Group& g) {
   g.sB = size_t(s0.ilu.nnz) * 25;
   g.sI = size_t(s0.ilu.n) * 25;
   g.sY = size_t(s0.ilu.n) * 5;
+  g.sS = s0.ilu.lds_solve ? size_t(s0.ilu.nst) * 25 : 0;
   g.Ball = dalloc<double>(g.sB * nc);
   g.Iall = dalloc<double>(g.sI * nc);
   g.Yall = dalloc<double>(g.sY * nc);
+  if (g.sS) g.Sall = dalloc<double>(g.sS * nc);
   for (int c = 0; c < nc; ++c) {
     Solver& s = *g.cases[c];
     s.ilu = s0.ilu;
@@ -1627,6 +1787,7 @@ void relink(Group& g) {
     s.B = g.Ball + size_t(c) * g.sB;
     s.invU = g.Iall + size_t(c) * g.sI;
     s.y = g.Yall + size_t(c) * g.sY;
+    s.S = g.Sall ? g.Sall + size_t(c) * g.sS : nullptr;
   }
 }
 
@@ -1653,9 +1814,14 @@ void ilu_solve_batch(Group& g, const std::vector<Solver*>& cs, const std::vector
   const int n = s0.m.nc;
   for (size_t i = 0; i < cs.size(); ++i)
     hipLaunchKernelGGL(k_perm_in, nblk(n), TPB, 0, s0.st, n, s0.ilu.perm, b[i], cs[i]->y);
-  const size_t lds = s0.ilu.lds_levels ? size_t(s0.ilu.nlev + s0.ilu.nlevu + 2) * sizeof(int) : 0;
-  hipLaunchKernelGGL(k_ilu_solve, int(ids.size()), ILU_TPB, lds, s0.st, s0.ilu, (const double*)g.Ball,
-                     (const double*)g.Iall, g.Yall, case_list(g, ids), g.sB, g.sI, g.sY);
+  if (s0.ilu.lds_solve) {
+    hipLaunchKernelGGL(k_ilu_solve_lds, int(ids.size()), ILU_TPB, ilu_solve_lds_bytes(s0.ilu), s0.st, s0.ilu,
+                       (const double*)g.Sall, g.Yall, case_list(g, ids), g.sS, g.sY);
+  } else {
+    const size_t lds = s0.ilu.lds_levels ? size_t(s0.ilu.nlev + s0.ilu.nlevu + 2) * sizeof(int) : 0;
+    hipLaunchKernelGGL(k_ilu_solve, int(ids.size()), ILU_TPB, lds, s0.st, s0.ilu, (const double*)g.Ball,
+                       (const double*)g.Iall, g.Yall, case_list(g, ids), g.sB, g.sI, g.sY);
+  }
   for (size_t i = 0; i < cs.size(); ++i)
     hipLaunchKernelGGL(k_perm_out, nblk(n), TPB, 0, s0.st, n, s0.ilu.perm, cs[i]->y, x[i]);
 }
@@ -2185,21 +2351,64 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     L.brec = s.own(dupload(brec));
     L.perm = s.own(dupload(permv));
     L.lds_levels = nlev + nlevu + 2 <= ILU_LDS_LEVELS;
-    int rowcap = 1, paircap = 1;
+    int rowcap = 1, paircap = 1, lowcap = 1;
     for (int i = 0; i < n; ++i) {
       rowcap = std::max(rowcap, rend[i] - rstart[i]);
       paircap = std::max(paircap, pptr[rdiag[i]] - pptr[rstart[i]]);
+      lowcap = std::max(lowcap, rdiag[i] - rstart[i]);
     }
     L.rowcap = rowcap;
     L.paircap = paircap;
+    L.lowcap = lowcap;
     {
       int lds_max = 65536;
       HC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, a2d::require_device()));
       const size_t lev = L.lds_levels ? size_t(nlev + 1) * sizeof(int) : 0;
-      const size_t per = size_t(rowcap) * 25 * sizeof(double) + size_t(paircap) * 8;
+      const size_t per = size_t(rowcap + lowcap) * 25 * sizeof(double) + size_t(paircap) * 8;
       const long fit = (long(lds_max) - long(lev)) / long(per);
       L.lds_waves = fit >= 4 ? int(std::min<long>(fit, ILU_TPB / 64)) : 0;
       if (getenv("A2D_RANS_FACTOR_GLOBAL")) L.lds_waves = 0;
+    }
+    {
+      // staged copy for the LDS sweeps: each level's blocks contiguous
+      std::vector<int> ssrc, flev(nlev + 1), blev(nlevu + 1);
+      std::vector<int4> fsrec(n), bsrec(n);
+      ssrc.reserve(size_t(nnz));
+      int scap = 0;
+      for (int l = 0; l < nlev; ++l) {
+        flev[l] = int(ssrc.size());
+        for (int p = lptr[l]; p < lptr[l + 1]; ++p) {
+          const int i = lrows[p];
+          fsrec[p] = make_int4(i, rstart[i], rdiag[i], int(ssrc.size()) - flev[l]);
+          for (int e = rstart[i]; e < rdiag[i]; ++e) ssrc.push_back(e);
+        }
+        scap = std::max(scap, (int(ssrc.size()) - flev[l]) * 25);
+      }
+      flev[nlev] = int(ssrc.size());
+      for (int l = 0; l < nlevu; ++l) {
+        blev[l] = int(ssrc.size());
+        for (int p = uptr[l]; p < uptr[l + 1]; ++p) {
+          const int j = urows[p];
+          bsrec[p] = make_int4(j, rdiag[j] + 1, rend[j], int(ssrc.size()) - blev[l]);
+          for (int e = rdiag[j] + 1; e < rend[j]; ++e) ssrc.push_back(e);
+          ssrc.push_back(nnz + j);
+        }
+        scap = std::max(scap, (int(ssrc.size()) - blev[l]) * 25);
+      }
+      blev[nlevu] = int(ssrc.size());
+      L.nst = int(ssrc.size());
+      L.scap = scap;
+      int lds_max = 65536;
+      HC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, a2d::require_device()));
+      L.lds_solve = scap <= SOLVE_K * (ILU_TPB / 2) && ilu_solve_lds_bytes(L) <= size_t(lds_max) &&
+                    !getenv("A2D_RANS_SOLVE_GLOBAL");
+      if (L.lds_solve) {
+        L.fsrec = s.own(dupload(fsrec));
+        L.bsrec = s.own(dupload(bsrec));
+        L.flev = s.own(dupload(flev));
+        L.blev = s.own(dupload(blev));
+        L.ssrc = s.own(dupload(ssrc));
+      }
     }
     s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
     s.posLR = s.own(dupload(std::vector<int>(pos_src + n, pos_src + n + nfi)));
@@ -2216,7 +2425,7 @@ int a2d_rans_ilu_info(int64_t h, int* out4) {
     out4[0] = s.ilu.nnz;
     out4[1] = s.ilu.nlev;
     out4[2] = s.ilu.nlevu;
-    out4[3] = 0;
+    out4[3] = s.ilu.lds_waves;
   });
 }
 
@@ -2325,7 +2534,7 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
     auto once = [&] {
       switch (which) {
         case 0: ilu_solve(s, s.R, s.tmp); break;
-        case 1: launch_ilu_factor(s.ilu, s.B, s.invU, nullptr, 1, 0, 0, s.st); break;
+        case 1: launch_ilu_factor(s.ilu, s.B, s.invU, s.S, nullptr, 1, 0, 0, 0, s.st); break;
         case 2: eval(s, order, s.R, nullptr, s.tmp, s.vdt, 1.0); break;
         case 3: eval(s, order, nullptr, s.tmp, nullptr, nullptr, 1.0); break;
         case 4: {
@@ -2389,7 +2598,7 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
       hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl[i], s.dt, s.vdt);
       factor(s, winf + 5 * i, gam[i], ct[i], nullptr, nullptr, false);
     }
-    launch_ilu_factor(s0.ilu, g.Ball, g.Iall, case_list(g, ids), n, g.sB, g.sI, s0.st);
+    launch_ilu_factor(s0.ilu, g.Ball, g.Iall, g.Sall, case_list(g, ids), n, g.sB, g.sI, g.sS, s0.st);
     HC(hipGetLastError());
     HC(hipMemsetAsync(g.nfall, 0, g.cases.size() * sizeof(int), s0.st));
     for (int i = 0; i < n; ++i)

@@ -279,7 +279,7 @@ class RansSolver:
         self._ensure_ilu()
         out = np.zeros(4, np.int32)
         _abi.call("a2d_rans_ilu_info", self._device(), _abi.iptr(out))
-        return dict(nnz=int(out[0]), levels=int(out[1]), levels_back=int(out[2]))
+        return dict(nnz=int(out[0]), levels=int(out[1]), levels_back=int(out[2]), factor_waves=int(out[3]))
 
     # ---------------------------------------------------------------- physics
     def prim(self, Q):

@@ -266,7 +266,8 @@ int a2d_rans_jac1(int64_t h, const double* winf, double gam, double* D, double* 
  * nnz), and pos_src (nc + 2 nfi): the pattern entry of each Jacobian block in
  * the order of a2d_rans_jac1 (diagonal, (iL,iR), (iR,iL)). Once per solver. */
 int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, const int* cols, const int* pos_src);
-/* nnz, forward levels, backward levels, 0. */
+/* nnz, forward levels, backward levels, wavefronts of the LDS-staged
+ * factorisation (0: rows too long for LDS, the global kernel runs). */
 int a2d_rans_ilu_info(int64_t h, int* out4);
 /* Assemble vol/dt + J1 at the state and CFL (turbulence equation's step
  * times turb_factor) and factor it (BlockILU0.factor); nonfinite = count of
