@@ -253,15 +253,14 @@ def test_newton_trajectory_follows_the_oracle():
 def test_laminar_naca0012_drag_matches_the_literature():
     """Laminar NACA 0012 at M 0.5, Re 5000, alpha 0 (the classic viscous test
     case; published total drag 0.0556-0.0576, pressure part ~0.022-0.024,
-    friction ~0.033-0.034) on a 22k-cell hybrid mesh: the solve converges by the
-    reference's rule, the lift is zero by symmetry and the drag components fall
-    in the published band. zmin = 0.1 (RansSolver.zmin) keeps LM-Roe's velocity
-    dissipation from vanishing at the stagnation point."""
-    from aero2d_amd.rans_mesh import refine_wall
-    g = refine_wall(ogrid(naca4("0012", 97), nj=65, ds0=1e-3, radius=20.0), 2e-4, layers=1, ratio=1.2)
-    m = ogrid_mesh(g, quad_layers=60)
+    friction ~0.033-0.034) with the reference's solver defaults on a 27k-cell
+    C-grid (first cell 2e-4): the solve converges by the reference's rule, the
+    lift is zero by symmetry and the drag components fall in the published
+    band (this mesh: 0.0554, 0.0228, 0.0326)."""
+    from aero2d_amd.rans_mesh import refine_cwall
+    g = refine_cwall(cgrid(naca4("0012", 201), n_side=97, nj=65, ds0=1e-3, radius=20.0), 2e-4)
+    m = cgrid_mesh(g)
     s = RansSolver(m, 0.5, 5000.0, turbulent=False)
-    s.zmin = 0.1
     r = s.solve(0.0, max_iter=200)
     assert r.converged, r.status
     assert abs(r.cl) < 1e-3
