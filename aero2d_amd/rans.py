@@ -80,6 +80,70 @@ class RansControl:
         self.cfl_hold = True
 
 
+_CFL_FUNCS = {"min": min, "max": max, "exp": math.exp, "log": math.log, "log10": math.log10,
+              "sqrt": math.sqrt, "abs": abs, "pow": pow, "tanh": math.tanh, "floor": math.floor,
+              "ceil": math.ceil, "where": lambda c, a, b: a if c else b, "pi": math.pi, "e": math.e}
+_CFL_FUNCS["\u0433\u0434\u0435"] = _CFL_FUNCS["where"]     # the reference's name of where(c, a, b)
+_CFL_VARS = ("n", "orders", "res", "dres", "cfl")
+
+
+def _cfl_ast(expr):
+    import ast
+    text = str(expr).replace("^", "**").strip()
+    try:
+        tree = ast.parse(text, mode="eval")
+    except SyntaxError as err:
+        raise ValueError(f"CFL formula: syntax error ({err.msg})") from None
+    allowed = (ast.Expression, ast.BinOp, ast.UnaryOp, ast.BoolOp, ast.Compare, ast.IfExp, ast.Call,
+               ast.Name, ast.Load, ast.Constant, ast.operator, ast.unaryop, ast.boolop, ast.cmpop)
+    for node in ast.walk(tree):
+        if not isinstance(node, allowed):
+            raise ValueError(f"CFL formula: {type(node).__name__} is not allowed")
+        if isinstance(node, ast.Name) and node.id not in _CFL_FUNCS and node.id not in _CFL_VARS:
+            raise ValueError(f"CFL formula: unknown name '{node.id}' (allowed: {', '.join(_CFL_VARS)} and "
+                             "min, max, exp, log, log10, sqrt, abs, pow, tanh, floor, ceil, where, pi, e)")
+        if isinstance(node, ast.Call) and not (isinstance(node.func, ast.Name) and node.func.id in _CFL_FUNCS
+                                               and not node.keywords):
+            raise ValueError("CFL formula: only the listed functions may be called")
+        if isinstance(node, ast.Constant) and not isinstance(node.value, (int, float)):
+            raise ValueError("CFL formula: only numbers are allowed as constants")
+    return tree
+
+
+def cfl_formula(expr, **v) -> float:
+    """The CFL number given by a user expression (aero2d/rans.py:102-123): the
+    variables n (iteration at this angle), orders (decades the residual has
+    fallen since the first iteration), res (residual / first residual), dres
+    (residual / previous residual) and cfl (the current CFL); the functions
+    min, max, exp, log, log10, sqrt, abs, pow, tanh, floor, ceil and
+    where(c, a, b) (also under the reference's name), the constants pi and e;
+    '^' is a power. The expression is parsed and checked node by node (numbers,
+    arithmetic, comparisons, the listed names and calls only). Raises
+    ValueError on a bad expression or a value that is not a finite positive
+    number."""
+    tree = _cfl_ast(expr)
+    names = dict(_CFL_FUNCS)
+    names.update({k: v[k] for k in _CFL_VARS if k in v})
+    try:
+        val = float(eval(compile(tree, "<CFL>", "eval"), {"__builtins__": {}}, names))
+    except (ArithmeticError, ValueError, TypeError, NameError) as err:
+        raise ValueError(f"CFL formula: cannot evaluate ({err})") from None
+    if not math.isfinite(val) or val <= 0:
+        raise ValueError(f"CFL formula gave {val:g}: a positive number is needed")
+    return val
+
+
+def cfl_formula_recursive(expr) -> bool:
+    """Whether the formula reads the current CFL or the residual ratio of
+    neighbouring iterations (aero2d/rans.py:126-132)."""
+    import ast
+    try:
+        tree = _cfl_ast(expr)
+    except ValueError:
+        return False
+    return any(isinstance(nd, ast.Name) and nd.id in ("cfl", "dres") for nd in ast.walk(tree))
+
+
 def ilu_structure(n, rows, cols, fill=0, perm=None):
     """Block ILU(k) structure as BlockILU0.__init__ forms it (aero2d/bilu.py:14-59):
     perm (reverse Cuthill-McKee of the symmetrised pattern unless given,
@@ -624,6 +688,10 @@ class _Run:
         self.hist, self.fh = [], []
         self.r0 = None
         self.cfl = self.cfl0
+        self.cfl_target, self.cfl_pen = None, 1.0
+        if S.cfl_mode == "formula":      # checks the formula before the first step (aero2d/rans.py:1430-1431)
+            self.cfl = min(self.cfl_max, cfl_formula(S.cfl_formula, n=0, orders=0.0, res=1.0, dres=1.0,
+                                                     cfl=self.cfl0))
         self.last_ok, self.last_lin = True, 0.0
         self.H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[])
         self.events = []
@@ -789,8 +857,24 @@ class _Run:
                 self._say(f"residual stalled at {rn:.2e}: limiter frozen")
         # CFL (aero2d/rans.py:1617-1680)
         if S.cfl_mode == "formula":
-            raise NotImplementedError("cfl_mode 'formula' is not built; use 'grow', 'fixed' or 'ser'")
-        if len(hist) >= 2 and hist[-2][1] == order:
+            # the user's expression; after a protective cut (rejected step,
+            # rollback) the CFL returns to it x1.25 per good iteration unless the
+            # expression itself reads cfl or dres (aero2d/rans.py:1643-1665)
+            dres = rn / max(hist[-2][0], 1e-300) if len(hist) >= 2 else 1.0
+            target = cfl_formula(S.cfl_formula, n=it, orders=float(np.log10(max(hist[0][0], 1e-300)
+                                                                            / max(rn, 1e-300))),
+                                 res=rn / max(hist[0][0], 1e-300), dres=dres, cfl=self.cfl)
+            target = float(min(self.cfl_max, max(target, 1e-3)))
+            if cfl_formula_recursive(S.cfl_formula):
+                self.cfl_target, self.cfl_pen, self.cfl = target, 1.0, target
+            else:
+                last, pen = self.cfl_target, self.cfl_pen
+                if last is not None and self.cfl < 0.99 * last * pen:
+                    pen = self.cfl / last
+                pen = min(1.0, pen * 1.25) if self.last_ok else pen
+                self.cfl_target, self.cfl_pen = target, pen
+                self.cfl = target * pen
+        elif len(hist) >= 2 and hist[-2][1] == order:
             ok_, lin_ = self.last_ok, self.last_lin
             if S.cfl_mode == "fixed":
                 self.cfl = min(self.cfl_max, self.cfl * S.cfl_growth) if ok_ and rn < 3.0 * hist[-2][0] \

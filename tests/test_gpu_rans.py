@@ -290,6 +290,27 @@ def test_turbulent_naca0012_matches_nasa_tmr():
     assert abs(r10.cl / 1.0909 - 1) < 0.015 and abs(r10.cd / 0.01231 - 1) < 0.10, (r10.cl, r10.cd)
 
 
+def test_cfl_formula_mode_follows_the_expression():
+    """cfl_mode "formula" (aero2d/rans.py:1643-1665): each iteration's CFL is
+    the user's expression of the iteration number while no step is undone
+    (the history records the CFL each iteration starts with: the expression
+    at n = it - 1)."""
+    m, s, o = _pair("hybrid")
+    s.cfl_mode = "formula"
+    s.cfl_formula = "min(5*1.2^n, 40)"
+    r = s.solve(2.0, max_iter=10)
+    first_event = min([e[0] for e in r.history["events"]] + [10**9])
+    checked = 0
+    for it, cfl in zip(r.history["it"], r.history["cfl"]):
+        if it < first_event:
+            assert abs(cfl - min(5 * 1.2 ** (it - 1), 40)) < 1e-12, (it, cfl)
+            checked += 1
+    assert checked >= 5
+    s.cfl_formula = "n +"
+    with pytest.raises(ValueError):
+        s.solve(2.0, max_iter=2)
+
+
 def test_batched_newton_steps_equal_single_steps():
     """Two case slots stepped by one a2d_rans_newton_steps call (one
     factorisation launch and shared triangular launches) give the bits of each

@@ -129,3 +129,29 @@ def test_censored_latency_counts_unfinished_requests():
     assert out["censored_at"] == [] and np.isclose(out["p50_ms"], 2000.0)
     out = censored_latency(done, [0.0], wall=10.0)            # censored at 10 s: above p50
     assert out["censored_at"] == [99]
+
+
+def test_rans_cfl_formula():
+    """The RANS CFL expression (cfl_formula): the reference's default grows
+    5 x 1.2^n to 1000; variables, functions, where(c, a, b) (and its name in
+    the reference) and '^' evaluate; unknown names, attribute access, other
+    calls, syntax errors and non-positive values raise ValueError; a formula
+    reading cfl or dres is recursive."""
+    import math
+    from aero2d_amd.rans import cfl_formula, cfl_formula_recursive
+    f = "min(5*1.2**n, 1000)"
+    assert cfl_formula(f, n=0) == 5.0 and abs(cfl_formula(f, n=3) - 5 * 1.2 ** 3) < 1e-12
+    assert cfl_formula(f, n=100) == 1000.0
+    assert abs(cfl_formula("10^orders", orders=2.5) - 10 ** 2.5) < 1e-9
+    assert cfl_formula("where(n < 10, 5, 50)", n=3) == 5 and cfl_formula("where(n < 10, 5, 50)", n=30) == 50
+    assert cfl_formula("\u0433\u0434\u0435(res > 0.1, 2, 20)", res=0.01) == 20
+    v = cfl_formula("max(cfl * min(2, 1/dres), 1) + sqrt(4) + log10(100) + tanh(0) + pi - e", cfl=10, dres=0.5)
+    assert abs(v - (20 + 2 + 2 + 0 + math.pi - math.e)) < 1e-12
+    for bad in ("n +", "foo(n)", "n.real", "__import__('os')", "(lambda: 1)()", "[1][0]", "'a'", "-n"):
+        try:
+            cfl_formula(bad, n=1)
+        except ValueError:
+            continue
+        raise AssertionError(bad)
+    assert cfl_formula_recursive("cfl*1.1") and cfl_formula_recursive("min(1/dres, 2)")
+    assert not cfl_formula_recursive(f) and not cfl_formula_recursive("n +")
