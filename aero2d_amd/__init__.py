@@ -19,7 +19,7 @@ from .euler import (GAMMA, EulerResult, EulerSolver, compute_metrics, cons_to_pr
                     prim_to_cons)
 from .implicit import ImplicitEuler
 from .rans_mesh import CGrid, Mesh, cgrid, cgrid_mesh, connectivity, ogrid_mesh, refine_cwall, refine_wall
-from .rans import Convergence, RansControl, RansResult, RansSolver
+from .rans import Convergence, RansControl, RansResult, RansSolver, cfl_formula, transfer_state
 
 __version__ = "0.1.0"
 
@@ -28,4 +28,4 @@ __all__ = ["Airfoil", "naca4", "naca4_params", "cosine_spacing", "normalize", "r
            "self_similar_spacing", "GAMMA", "EulerResult", "EulerSolver", "compute_metrics",
            "cons_to_prim", "hllc_flux", "prim_to_cons", "ImplicitEuler", "Mesh", "connectivity", "ogrid_mesh",
            "refine_wall", "CGrid", "cgrid", "cgrid_mesh", "refine_cwall", "Convergence", "RansControl",
-           "RansResult", "RansSolver"]
+           "RansResult", "RansSolver", "cfl_formula", "transfer_state"]

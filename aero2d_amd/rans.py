@@ -172,6 +172,25 @@ def ilu_structure(n, rows, cols, fill=0, perm=None):
     return perm, r, c, src_pos
 
 
+def transfer_state(mesh_from: Mesh, Q, mesh_to: Mesh) -> np.ndarray:
+    """A state (nc, 5) of mesh_from carried to mesh_to (aero2d/rans.py:2438-2454):
+    each new cell takes the values of the nearest old cell centroid; where the
+    new cell is closer to a wall than that old cell, its velocity and rho chi
+    are scaled by the ratio of the wall distances (capped at 1) and the energy
+    loses the removed kinetic energy, so density and pressure carry over."""
+    from scipy.spatial import cKDTree
+    Q = np.asarray(Q, np.float64)
+    _, near = cKDTree(mesh_from.centroid).query(mesh_to.centroid)
+    out = Q[near].copy()
+    scale = np.clip(mesh_to.wall_dist / np.maximum(mesh_from.wall_dist[near], 1e-30), 0.0, 1.0)
+    kinetic = 0.5 * (out[:, 1] ** 2 + out[:, 2] ** 2) / out[:, 0]
+    out[:, 1] *= scale
+    out[:, 2] *= scale
+    out[:, 3] -= kinetic * (1.0 - scale ** 2)
+    out[:, 4] *= scale
+    return out
+
+
 class RansSolver:
     """SA RANS on a Mesh (aero2d/rans.py:187-326 for the constructor and the
     defaults kept here)."""

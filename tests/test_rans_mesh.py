@@ -120,3 +120,32 @@ def test_cgrid_mesh_invariants(grid, quad_layers):
     np.add.at(acc, m.face_cells[right, 1], -nA[right])
     assert np.abs(acc).max() < 1e-12
     assert m.wall_dist.min() > 0 and np.all(m.wall_dist <= 7.5)
+
+
+def test_transfer_state_between_meshes(grid):
+    """transfer_state: the identity on the same mesh; onto a mesh with a finer
+    wall layer the nearest-cell values keep density and pressure, the
+    velocity and rho chi shrink by the wall-distance ratio, and a uniform
+    state away from walls carries over exactly (the reference's test pattern,
+    tests/test_rans.py:128-137: finite, positive density, the new mesh's size)."""
+    from aero2d_amd.rans import transfer_state
+    GAM = 1.4
+    coarse = cgrid_mesh(grid)
+    fine = cgrid_mesh(refine_cwall(grid, 1e-4))
+    rng = np.random.default_rng(3)
+    w = np.column_stack([1 + 0.1 * rng.random(coarse.n_cells), 0.5 + 0.1 * rng.random(coarse.n_cells),
+                         0.1 * rng.random(coarse.n_cells), 1 / GAM + 0.05 * rng.random(coarse.n_cells),
+                         3 * rng.random(coarse.n_cells)])
+    Q = np.column_stack([w[:, 0], w[:, 0] * w[:, 1], w[:, 0] * w[:, 2],
+                         w[:, 3] / (GAM - 1) + 0.5 * w[:, 0] * (w[:, 1] ** 2 + w[:, 2] ** 2), w[:, 0] * w[:, 4]])
+    assert np.array_equal(transfer_state(coarse, Q, coarse), Q)
+    Qf = transfer_state(coarse, Q, fine)
+    assert Qf.shape == (fine.n_cells, 5) and np.all(np.isfinite(Qf)) and np.all(Qf[:, 0] > 0)
+    p = lambda q: (GAM - 1) * (q[:, 3] - 0.5 * (q[:, 1] ** 2 + q[:, 2] ** 2) / q[:, 0])   # noqa: E731
+    from scipy.spatial import cKDTree
+    _, near = cKDTree(coarse.centroid).query(fine.centroid)
+    assert np.allclose(p(Qf), p(Q)[near], rtol=1e-12, atol=1e-14)
+    assert np.array_equal(Qf[:, 0], Q[near, 0])
+    s = np.clip(fine.wall_dist / coarse.wall_dist[near], 0, 1)
+    assert np.allclose(Qf[:, 1], Q[near, 1] * s) and np.allclose(Qf[:, 4], Q[near, 4] * s)
+    assert (s < 1).any()
