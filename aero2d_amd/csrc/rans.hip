@@ -762,6 +762,7 @@ struct IluDev {
 
 constexpr int ILU_TPB = 1024;
 constexpr int PAIR_SLOTS = 12;          // update pairs in flight per wavefront (5 lanes each)
+constexpr int STAGE_K = 12;             // loads per lane in one batch of the row staging
 constexpr int ILU_LDS_LEVELS = 12288;   // level bounds staged in LDS up to this many
 
 __device__ __forceinline__ const int* stage_bounds(const int* g, int n, int* sh, bool use) {
@@ -876,7 +877,8 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
   double* rowi = smem + size_t(nwave) * L.rowcap * 25 + size_t(wave) * L.lowcap * 25;
   int* pairs_all = reinterpret_cast<int*>(smem + size_t(nwave) * (L.rowcap + L.lowcap) * 25);
   int* rowp = pairs_all + size_t(wave) * L.paircap * 2;
-  int* lev_sh = pairs_all + size_t(nwave) * L.paircap * 2;
+  int* rowq = pairs_all + size_t(nwave) * L.paircap * 2 + size_t(wave) * (L.lowcap + 1);
+  int* lev_sh = pairs_all + size_t(nwave) * (L.paircap * 2 + L.lowcap + 1);
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, L.lds_levels);
   const int half = lane >> 5, hl = lane & 31;
   const bool act = hl < 25;
@@ -887,21 +889,59 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
     for (int p = lp[l] + wave; p < p1; p += nwave) {
       const int4 rec = L.frec[p];
       const int i = rec.x, rs = rec.y, d = rec.z, re = rec.w;
-      const int nb25 = (re - rs) * 25;
-      for (int t = lane; t < nb25; t += 64) rowb[t] = B[size_t(rs) * 25 + t];
-      // the inverses of the lower entries' rows (finished at earlier levels)
-      for (int t = lane; t < (d - rs) * 25; t += 64) rowi[t] = invU[size_t(L.col[rs + t / 25]) * 25 + t % 25];
+      const int nb25 = (re - rs) * 25, nl25 = (d - rs) * 25;
       const int q0 = L.pptr[rs], q1 = L.pptr[d];
-      for (int t = lane; t < q1 - q0; t += 64) {
-        rowp[2 * t] = L.pt[q0 + t] - rs;
-        rowp[2 * t + 1] = L.pe2[q0 + t];
+      // stage the row: its blocks, the inverses of its lower entries' rows
+      // (finished at earlier levels), its update pairs and their bounds. Each
+      // batch issues all of a lane's loads (clamped indices, unconditional)
+      // before its LDS stores, so a batch costs one memory latency.
+      for (int t0 = 0; t0 < nb25; t0 += 64 * STAGE_K) {
+        double v[STAGE_K];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k) v[k] = B[size_t(rs) * 25 + min(t0 + lane + 64 * k, nb25 - 1)];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k)
+          if (t0 + lane + 64 * k < nb25) rowb[t0 + lane + 64 * k] = v[k];
       }
+      for (int t0 = 0; t0 < nl25; t0 += 64 * STAGE_K) {
+        int src[STAGE_K];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k) {
+          const int t = min(t0 + lane + 64 * k, nl25 - 1);
+          src[k] = L.col[rs + t / 25] * 25 + t % 25;
+        }
+        double v[STAGE_K];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k) v[k] = invU[size_t(src[k])];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k)
+          if (t0 + lane + 64 * k < nl25) rowi[t0 + lane + 64 * k] = v[k];
+      }
+      for (int t0 = 0; t0 < q1 - q0; t0 += 64 * STAGE_K) {
+        int a[STAGE_K], b[STAGE_K];
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k) {
+          const int t = min(t0 + lane + 64 * k, q1 - q0 - 1);
+          a[k] = L.pt[q0 + t];
+          b[k] = L.pe2[q0 + t];
+        }
+#pragma unroll
+        for (int k = 0; k < STAGE_K; ++k)
+          if (t0 + lane + 64 * k < q1 - q0) {
+            rowp[2 * (t0 + lane + 64 * k)] = a[k] - rs;
+            rowp[2 * (t0 + lane + 64 * k) + 1] = b[k];
+          }
+      }
+      for (int t = lane; t <= d - rs; t += 64) rowq[t] = L.pptr[rs + t] - q0;
       wave_sync();
       for (int e = rs; e < d; ++e) {
-        const double be = rowb[(e - rs) * 25 + r * 5 + cc];
+        // row r of B_ik and column cc of invU_k straight from LDS (the values
+        // the plain kernel moves by shuffles; all ten reads in flight at once)
+        const double* bik = rowb + (e - rs) * 25 + r * 5;
+        const double* ik = rowi + (e - rs) * 25 + cc;
         double sv = 0.0;
 #pragma unroll
-        for (int mm = 0; mm < 5; ++mm) sv += __shfl(be, r * 5 + mm, 32) * rowi[(e - rs) * 25 + mm * 5 + cc];
+        for (int mm = 0; mm < 5; ++mm) sv += bik[mm] * ik[mm * 5];
         wave_sync();
         if (act && half == 0) rowb[(e - rs) * 25 + hl] = sv;
         wave_sync();
@@ -909,9 +949,9 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_factor_lds(IluDev L, double* B,
         // L_ik U_kj for its pair (L_ik read back from the row in LDS, the same
         // address for every lane), each element in the order of the plain kernel
         const double* lk = rowb + (e - rs) * 25;
-        const int qb = L.pptr[e + 1] - q0;
+        const int qb = rowq[e - rs + 1];
         if (lane < 5 * PAIR_SLOTS) {
-          for (int q = L.pptr[e] - q0 + pslot; q < qb; q += PAIR_SLOTS) {
+          for (int q = rowq[e - rs] + pslot; q < qb; q += PAIR_SLOTS) {
             const int tl = rowp[2 * q], e2 = rowp[2 * q + 1];
             double u[5];
 #pragma unroll
@@ -1175,8 +1215,8 @@ void launch_ilu_factor(const IluDev& L, double* B, double* invU, double* S, cons
                        size_t sB, size_t sI, size_t sS, hipStream_t st) {
   const size_t lev = L.lds_levels ? size_t(L.nlev + 1) * sizeof(int) : 0;
   if (L.lds_waves > 0) {
-    const size_t lds =
-        size_t(L.lds_waves) * (size_t(L.rowcap + L.lowcap) * 25 * sizeof(double) + size_t(L.paircap) * 8) + lev;
+    const size_t lds = size_t(L.lds_waves) * (size_t(L.rowcap + L.lowcap) * 25 * sizeof(double) +
+                                              size_t(L.paircap) * 8 + size_t(L.lowcap + 1) * 4) + lev;
     hipLaunchKernelGGL(k_ilu_factor_lds, ncase, L.lds_waves * 64, lds, st, L, B, invU, cases, sB, sI);
   } else {
     hipLaunchKernelGGL(k_ilu_factor, ncase, ILU_TPB, lev, st, L, B, invU, cases, sB, sI);
@@ -2327,6 +2367,16 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
     std::vector<int> lptr, lrows, uptr, urows;
     group(lev, nlev, lptr, lrows);
     group(levu, nlevu, uptr, urows);
+    // within a level, the rows with the most elimination work first: the
+    // factorisation deals rows to wavefronts in this order, so the longest
+    // rows start in the first round (rows of a level are independent: the
+    // results do not depend on the order)
+    for (int l = 0; l < nlev; ++l)
+      std::stable_sort(lrows.begin() + lptr[l], lrows.begin() + lptr[l + 1], [&](int a, int b) {
+        const int wa = pptr[rdiag[a]] - pptr[rstart[a]] + (rdiag[a] - rstart[a]);
+        const int wb = pptr[rdiag[b]] - pptr[rstart[b]] + (rdiag[b] - rstart[b]);
+        return wa > wb;
+      });
     std::vector<int4> frec(n), brec(n);
     for (int p = 0; p < n; ++p) {
       const int i = lrows[p], j = urows[p];
@@ -2364,7 +2414,8 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
       int lds_max = 65536;
       HC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, a2d::require_device()));
       const size_t lev = L.lds_levels ? size_t(nlev + 1) * sizeof(int) : 0;
-      const size_t per = size_t(rowcap + lowcap) * 25 * sizeof(double) + size_t(paircap) * 8;
+      const size_t per = size_t(rowcap + lowcap) * 25 * sizeof(double) + size_t(paircap) * 8 +
+                         size_t(lowcap + 1) * 4;
       const long fit = (long(lds_max) - long(lev)) / long(per);
       L.lds_waves = fit >= 4 ? int(std::min<long>(fit, ILU_TPB / 64)) : 0;
       if (getenv("A2D_RANS_FACTOR_GLOBAL")) L.lds_waves = 0;
