@@ -46,6 +46,20 @@ void hc(hipError_t e, const char* what) {
 }
 #define HC(x) hc((x), #x)
 
+// Creation: no handle yet and no library-wide lock (the map insert locks the map).
+template <class F>
+int guarded_new(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error_text(e.what());
+    return -99;
+  }
+}
+
 template <class F>
 int guarded(F&& f) {
   std::lock_guard<std::recursive_mutex> lk(api_mutex());
@@ -1456,8 +1470,14 @@ constexpr int HSTRIDE = 2 * (GMRES_MAX + 1) + 8;
 struct Group {
   std::vector<std::unique_ptr<Solver>> cases;
   int cur = 0;
+  std::recursive_mutex mu;   // one caller at a time per handle
   double *Ball = nullptr, *Iall = nullptr, *Yall = nullptr, *Sall = nullptr, *hall = nullptr, *smallall = nullptr;
   int *nfall = nullptr, *dcases = nullptr;
+  // pinned host mirrors of the per-slot scalars and of the case list: copies
+  // between them and the device are DMA on the group's stream (no staging
+  // through the runtime's pageable-copy buffers, no null-stream copy)
+  double *hall_h = nullptr, *small_h = nullptr;
+  int *nf_h = nullptr, *cases_h = nullptr;
   size_t sB = 0, sI = 0, sY = 0, sS = 0;
   std::vector<int> dcases_host;
   ~Group() {
@@ -1466,17 +1486,47 @@ struct Group {
     for (void* p : {(void*)Ball, (void*)Iall, (void*)Yall, (void*)Sall, (void*)hall, (void*)smallall,
                     (void*)nfall, (void*)dcases})
       if (p) (void)hipFree(p);
+    for (void* p : {(void*)hall_h, (void*)small_h, (void*)nf_h, (void*)cases_h})
+      if (p) (void)hipHostFree(p);
   }
 };
 
-std::map<int64_t, std::unique_ptr<Group>> g_solvers;
+// Handles. The map has its own lock; each call on a handle holds that
+// handle's lock (guarded_h) and a reference to its group, so calls on
+// different handles run concurrently from different host threads (each
+// handle has its own stream), and a handle destroyed meanwhile lives until
+// the calls on it end.
+std::mutex g_map_mu;
+std::map<int64_t, std::shared_ptr<Group>> g_solvers;
 int64_t g_next = 1;
 
-Group& get_group(int64_t h) {
+std::shared_ptr<Group> find_group(int64_t h) {
+  std::lock_guard<std::mutex> lk(g_map_mu);
   auto it = g_solvers.find(h);
   if (it == g_solvers.end()) fail(-3, "unknown RANS solver handle");
+  return it->second;
+}
+
+// The group of h, for a call inside guarded_h(h, ...) (which keeps it alive).
+Group& get_group(int64_t h) {
+  Group& g = *find_group(h);
   require_device();
-  return *it->second;
+  return g;
+}
+
+template <class F>
+int guarded_h(int64_t h, F&& f) {
+  try {
+    std::shared_ptr<Group> keep = find_group(h);
+    std::lock_guard<std::recursive_mutex> lk(keep->mu);
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error_text(e.what());
+    return -99;
+  }
 }
 
 Solver& get(int64_t h) {
@@ -1792,10 +1842,19 @@ void relink(Group& g) {
       HC(hipFree(*p));
       *p = nullptr;
     }
+  for (void** p : {(void**)&g.hall_h, (void**)&g.small_h, (void**)&g.nf_h, (void**)&g.cases_h})
+    if (*p) {
+      HC(hipHostFree(*p));
+      *p = nullptr;
+    }
   g.hall = dalloc<double>(size_t(nc) * HSTRIDE);
   g.smallall = dalloc<double>(size_t(nc) * 64);
   g.nfall = dalloc<int>(nc);
   g.dcases = dalloc<int>(nc);
+  HC(hipHostMalloc((void**)&g.hall_h, size_t(nc) * HSTRIDE * sizeof(double), hipHostMallocDefault));
+  HC(hipHostMalloc((void**)&g.small_h, size_t(nc) * 64 * sizeof(double), hipHostMallocDefault));
+  HC(hipHostMalloc((void**)&g.nf_h, size_t(nc) * sizeof(int), hipHostMallocDefault));
+  HC(hipHostMalloc((void**)&g.cases_h, size_t(nc) * sizeof(int), hipHostMallocDefault));
   g.dcases_host.clear();
   for (int c = 0; c < nc; ++c) {
     Solver& s = *g.cases[c];
@@ -1834,8 +1893,10 @@ void relink(Group& g) {
 // The device list of case slots for a batched launch.
 const int* case_list(Group& g, const std::vector<int>& ids) {
   if (ids != g.dcases_host) {
+    // the stream's earlier copy out of cases_h has run once the stream is idle
     HC(hipStreamSynchronize(g.cases[0]->st));
-    HC(hipMemcpy(g.dcases, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    std::copy(ids.begin(), ids.end(), g.cases_h);
+    HC(hipMemcpyAsync(g.dcases, g.cases_h, ids.size() * sizeof(int), hipMemcpyHostToDevice, g.cases[0]->st));
     g.dcases_host = ids;
   }
   return g.dcases;
@@ -1905,9 +1966,9 @@ void gmres_batch(Group& g, const std::vector<int>& ids, const std::vector<int>& 
     ilu_solve_batch(g, cs, ids, b, px);
   }
   for (size_t i = 0; i < nb; ++i) mdot(*cs[i], V(i, 0), ld, 1, V(i, 0), 0);
-  std::vector<double> hall(size_t(g.cases.size()) * HSTRIDE);
+  double* const hall = g.hall_h;
   auto pull = [&] {
-    HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    HC(hipMemcpyAsync(hall, g.hall, g.cases.size() * HSTRIDE * sizeof(double), hipMemcpyDeviceToHost, s0.st));
     HC(hipStreamSynchronize(s0.st));
   };
   pull();
@@ -1915,7 +1976,7 @@ void gmres_batch(Group& g, const std::vector<int>& ids, const std::vector<int>& 
   its.assign(nb, 0);
   info.assign(nb, 0);
   for (size_t i = 0; i < nb; ++i) {
-    const double* h = hall.data() + size_t(ids[i]) * HSTRIDE;
+    const double* h = hall + size_t(ids[i]) * HSTRIDE;
     GmresState& q = G[i];
     q.beta = std::sqrt(h[0]);
     HC(hipMemsetAsync(x[i], 0, N * sizeof(double), s0.st));
@@ -1969,7 +2030,7 @@ void gmres_batch(Group& g, const std::vector<int>& ids, const std::vector<int>& 
     pull();
     for (size_t i : act) {
       GmresState& q = G[i];
-      const double* hb = hall.data() + size_t(ids[i]) * HSTRIDE;
+      const double* hb = hall + size_t(ids[i]) * HSTRIDE;
       auto Hij = [&](int a, int c) -> double& { return q.H[size_t(a) * m + c]; };
       const double hn = std::sqrt(hb[2 * (j + 1)]);
       for (int a = 0; a <= j; ++a) Hij(a, j) = hb[a] + hb[j + 1 + a];
@@ -2057,7 +2118,7 @@ int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const 
                     const double* face_geom, const int* wall_cell, const double* wall_geom, const int* far_cell,
                     const double* far_geom, const double* cell_geom, const a2d_rans_params* prm,
                     int64_t* handle_out) {
-  return guarded([&] {
+  return guarded_new([&] {
     a2d::require_device();
     if (nc < 1 || nfi < 1 || nfw < 1 || nff < 1) fail(-6, "a RANS mesh needs cells, interior, wall and far faces");
     if (!prm || !handle_out) fail(-6, "null argument");
@@ -2143,9 +2204,10 @@ int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const 
     s->order = prm->order;
     s->owns_stream = true;
     alloc_case(*s);
-    auto grp = std::make_unique<Group>();
+    auto grp = std::make_shared<Group>();
     grp->cases.push_back(std::move(s));
     relink(*grp);
+    std::lock_guard<std::mutex> lk(g_map_mu);
     const int64_t h = g_next++;
     g_solvers[h] = std::move(grp);
     *handle_out = h;
@@ -2153,14 +2215,19 @@ int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const 
 }
 
 int a2d_rans_destroy(int64_t h) {
-  return guarded([&] {
-    get_group(h);
+  try {
+    std::shared_ptr<Group> keep = find_group(h);
+    std::lock_guard<std::recursive_mutex> lk(keep->mu);   // waits for calls in flight on h
+    std::lock_guard<std::mutex> lm(g_map_mu);
     g_solvers.erase(h);
-  });
+    return 0;   // the group is freed with the last reference, after the locks
+  } catch (const Fail& e) {
+    return e.code;
+  }
 }
 
 int a2d_rans_set_cases(int64_t h, int ncases) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Group& g = get_group(h);
     if (ncases < 1 || ncases > 1024) fail(-6, "ncases must be in [1, 1024]");
     if (ncases <= int(g.cases.size())) return;
@@ -2180,7 +2247,7 @@ int a2d_rans_set_cases(int64_t h, int ncases) {
 }
 
 int a2d_rans_select(int64_t h, int c) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Group& g = get_group(h);
     if (c < 0 || c >= int(g.cases.size())) fail(-4, "no such case slot");
     g.cur = c;
@@ -2188,21 +2255,21 @@ int a2d_rans_select(int64_t h, int c) {
 }
 
 int a2d_rans_set_state(int64_t h, const double* Q) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     upload_cells(s, Q, s.Q);
   });
 }
 
 int a2d_rans_get_state(int64_t h, double* Q) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     download_cells(s, s.Q, Q);
   });
 }
 
 int a2d_rans_freeze_limiter(int64_t h, const double* phi) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (!phi) {
       s.frozen = false;
@@ -2221,7 +2288,7 @@ int a2d_rans_freeze_limiter(int64_t h, const double* phi) {
 }
 
 int a2d_rans_limiter(int64_t h, double* phi) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     hipLaunchKernelGGL(k_grad<double>, nblk(s.m.nc), TPB, 0, s.st, s.m, s.Q, nullptr, s.g, nullptr);
     Params P = s.P;
@@ -2234,7 +2301,7 @@ int a2d_rans_limiter(int64_t h, double* phi) {
 }
 
 int a2d_rans_residual(int64_t h, int order, const double* winf, double gam, double* R, double* rn5) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
     far_state(s, winf, gam);
@@ -2246,7 +2313,7 @@ int a2d_rans_residual(int64_t h, int order, const double* winf, double gam, doub
 }
 
 int a2d_rans_jvp(int64_t h, int order, const double* winf, double gam, const double* v, double* out) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
     upload_cells(s, v, s.bvec);
@@ -2257,7 +2324,7 @@ int a2d_rans_jvp(int64_t h, int order, const double* winf, double gam, const dou
 }
 
 int a2d_rans_local_dt(int64_t h, double cfl, double* dt) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
     HC(hipGetLastError());
@@ -2267,7 +2334,7 @@ int a2d_rans_local_dt(int64_t h, double cfl, double* dt) {
 }
 
 int a2d_rans_forces(int64_t h, double alpha_deg, double* out5, double* cp, double* cf) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     hipLaunchKernelGGL(k_grad<double>, nblk(s.m.nc), TPB, 0, s.st, s.m, s.Q, nullptr, s.g, nullptr);
     const double a = alpha_deg * M_PI / 180.0;
@@ -2284,14 +2351,14 @@ int a2d_rans_forces(int64_t h, double alpha_deg, double* out5, double* cp, doubl
 }
 
 int a2d_rans_jac1(int64_t h, const double* winf, double gam, double* D, double* off) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     factor(s, winf, gam, 1.0, D, off);
   });
 }
 
 int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, const int* cols, const int* pos_src) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = *get_group(h).cases[0];
     const int n = s.m.nc;
     const int nfi = s.m.nfi;
@@ -2470,7 +2537,7 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
 }
 
 int a2d_rans_ilu_info(int64_t h, int* out4) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (!s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
     out4[0] = s.ilu.nnz;
@@ -2481,7 +2548,7 @@ int a2d_rans_ilu_info(int64_t h, int* out4) {
 }
 
 int a2d_rans_ilu_factor(int64_t h, const double* winf, double gam, double cfl, double turb_factor, int* nonfinite) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
     factor(s, winf, gam, 1.0 / turb_factor, nullptr, nullptr);
@@ -2491,7 +2558,7 @@ int a2d_rans_ilu_factor(int64_t h, const double* winf, double gam, double cfl, d
 }
 
 int a2d_rans_ilu_apply(int64_t h, const double* b, double* x) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (!s.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
     upload_cells(s, b, s.bvec);
@@ -2502,7 +2569,7 @@ int a2d_rans_ilu_apply(int64_t h, const double* b, double* x) {
 
 int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
                          double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
     if (gmres_iter < 1 || gmres_iter > GMRES_MAX) fail(-6, "gmres_iter must be in [1, 120]");
@@ -2570,7 +2637,7 @@ int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, d
 }
 
 int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, double gam, int reps, double* ms) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     if (reps < 1) fail(-6, "reps must be >= 1");
     if (which < 0 || which > 4) fail(-6, "which must be 0..4");
@@ -2618,7 +2685,7 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
 int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* order, const double* winf,
                           const double* gam, const double* cfl, const double* turb_factor, const double* gmres_tol,
                           int gmres_iter, const int* right, int step_limit_cells, double* out) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Group& g = get_group(h);
     Solver& s0 = *g.cases[0];
     if (!s0.has_ilu) fail(-6, "a2d_rans_ilu_setup has not been called");
@@ -2656,10 +2723,11 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
       hipLaunchKernelGGL(k_count_nonfinite, nblk(s0.m.nc * 25), TPB, 0, s0.st, s0.m.nc * 25, cs[i]->invU,
                          cs[i]->nonfinite);
     for (int i = 0; i < n; ++i) mdot(*cs[i], cs[i]->R, size_t(N), 1, cs[i]->R, 0);
-    std::vector<int> nf(g.cases.size());
-    std::vector<double> hall(g.cases.size() * HSTRIDE);
-    HC(hipMemcpyAsync(nf.data(), g.nfall, nf.size() * sizeof(int), hipMemcpyDeviceToHost, s0.st));
-    HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    int* const nf = g.nf_h;
+    double* const hall = g.hall_h;
+    const size_t hall_bytes = g.cases.size() * HSTRIDE * sizeof(double);
+    HC(hipMemcpyAsync(nf, g.nfall, g.cases.size() * sizeof(int), hipMemcpyDeviceToHost, s0.st));
+    HC(hipMemcpyAsync(hall, g.hall, hall_bytes, hipMemcpyDeviceToHost, s0.st));
     HC(hipStreamSynchronize(s0.st));
     std::vector<double> rnorm(n);
     std::vector<int> live;
@@ -2699,7 +2767,7 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
         hipLaunchKernelGGL(k_add, nblk(N), TPB, 0, s0.st, N, s.tmp2, s.R, s.tmp2);
         mdot(s, s.tmp2, size_t(N), 1, s.tmp2, 0);
       }
-      HC(hipMemcpyAsync(hall.data(), g.hall, hall.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+      HC(hipMemcpyAsync(hall, g.hall, hall_bytes, hipMemcpyDeviceToHost, s0.st));
       HC(hipStreamSynchronize(s0.st));
       std::vector<double> r(sel.size());
       for (size_t q = 0; q < sel.size(); ++q)
@@ -2754,8 +2822,8 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
       hipLaunchKernelGGL(k_max_rows, 1, 64, 0, s0.st, nbp, s.part, s.part + nbp, s.small);
     }
     if (upd.empty()) return;
-    std::vector<double> sm(g.cases.size() * 64);
-    HC(hipMemcpyAsync(sm.data(), g.smallall, sm.size() * sizeof(double), hipMemcpyDeviceToHost, s0.st));
+    double* const sm = g.small_h;
+    HC(hipMemcpyAsync(sm, g.smallall, g.cases.size() * 64 * sizeof(double), hipMemcpyDeviceToHost, s0.st));
     HC(hipStreamSynchronize(s0.st));
     for (int i : upd) {
       Solver& s = *cs[i];
@@ -2778,14 +2846,14 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
 }
 
 int a2d_rans_newton_dq(int64_t h, double* dq) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     download_cells(s, s.dq, dq);
   });
 }
 
 int a2d_rans_keep(int64_t h, int what) {
-  return guarded([&] {
+  return guarded_h(h, [&] {
     Solver& s = get(h);
     const size_t bytes = size_t(s.N()) * sizeof(double);
     switch (what) {

@@ -565,8 +565,24 @@ class RansSolver:
                    gmres_tol, gmres_iter, conv, control, alpha0)
         return self._drive([run])[0]
 
+    _SETTINGS = ("limiter_k", "cfl_reset_on_order", "limiter_zone", "limiter_te_radius", "cfl_mode",
+                 "cfl_growth", "cfl_formula", "cfl_turb", "turb_boost", "turb_boost_cfl", "step_limit_cells",
+                 "dt_visc_c", "conv", "cfl_start", "cfl_limit", "first_order", "rollback_factor",
+                 "rollback_auto", "cfl_lin_back", "rollback_first_order", "step_reject", "max_rejects",
+                 "gmres_tol", "max_rollbacks", "freeze_limiter", "prec_fill", "right_prec", "zmin")
+
+    def clone(self) -> "RansSolver":
+        """A new solver on the same mesh with the same constants and settings
+        (its own device memory and stream)."""
+        c = RansSolver(self.m, self.mach, self.re, self.chord, self.xref, self.chi_inf, self.order,
+                       self.vortex_bc, self.turbulent)
+        for k in self._SETTINGS:
+            setattr(c, k, getattr(self, k))
+        return c
+
     def solve_polar(self, alphas, Q0=None, max_iter=300, tol=1e-7, cfl0=None, cfl_max=None,
-                    first_order_iters=None, verbose=False, gmres_tol=None, gmres_iter=60, conv=None, alpha0=None):
+                    first_order_iters=None, verbose=False, gmres_tol=None, gmres_iter=60, conv=None, alpha0=None,
+                    workers=1):
         """Every angle of a polar solved at once, one case slot per angle, each
         with the loop of `solve` from its own start; their Newton steps run as
         one batched device call per iteration (the factorisation and
@@ -575,9 +591,35 @@ class RansSolver:
         each result here equals that angle's `solve` alone. Q0: None or one
         state per angle; alpha0: the angle each Q0 was solved at (a number or
         one per angle), so its far field is turned to the new angle as
-        `solve(..., alpha0=)` does. Returns the RansResults in the order of
-        alphas."""
+        `solve(..., alpha0=)` does. workers > 1: the angles are dealt round
+        robin to that many solvers (clones of this one, each with its own
+        stream), driven from as many host threads, so their launches run side
+        by side on the device and each lockstep waits only for its own angles.
+        Returns the RansResults in the order of alphas."""
         alphas = [float(a) for a in alphas]
+        workers = max(1, min(int(workers), len(alphas)))
+        if workers > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            a0s = [alpha0] * len(alphas) if alpha0 is None or np.isscalar(alpha0) else list(alpha0)
+            parts = [list(range(w, len(alphas), workers)) for w in range(workers)]
+            solvers = [self] + [self.clone() for _ in range(workers - 1)]
+
+            def one(w):
+                idx = parts[w]
+                return solvers[w].solve_polar([alphas[k] for k in idx], None if Q0 is None else [Q0[k] for k in idx],
+                                              max_iter, tol, cfl0, cfl_max, first_order_iters, verbose, gmres_tol,
+                                              gmres_iter, conv, [a0s[k] for k in idx])
+            try:
+                with ThreadPoolExecutor(max_workers=workers) as ex:
+                    outs = list(ex.map(one, range(workers)))
+            finally:
+                for sv in solvers[1:]:
+                    sv.close()
+            res = [None] * len(alphas)
+            for w, idx in enumerate(parts):
+                for k, r in zip(idx, outs[w]):
+                    res[k] = r
+            return res
         self._ensure_ilu()
         _abi.call("a2d_rans_set_cases", self._device(), len(alphas))
         a0 = [alpha0] * len(alphas) if alpha0 is None or np.isscalar(alpha0) else list(alpha0)

@@ -812,6 +812,16 @@ def run_rans(a):
                          "newton_iterations": [int(x.iterations) for x in rp],
                          "cl": [float(x.cl) for x in rp], "cd": [float(x.cd) for x in rp],
                          "status": [x.status for x in rp]}
+        if a.polar_workers > 1:
+            t0 = time.perf_counter()
+            rw = sp.solve_polar(alphas, Q0=[r.state] * nang, alpha0=alpha, max_iter=iters, workers=a.polar_workers)
+            ww = time.perf_counter() - t0
+            cw = int(sum(x.converged for x in rw))
+            line["polar"]["workers"] = {
+                "workers": a.polar_workers, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                "wall_s": ww, "converged": cw, "converged_angles_per_s": cw / ww,
+                "speedup_vs_one_solver": wp / ww,
+                "same_states_as_one_solver": all(np.array_equal(x.state, y.state) for x, y in zip(rp, rw))}
         sp.close()
     if not a.no_cpu_baseline:
         gpi = line["solve"]["gmres_iterations_mean"] or 10.0
@@ -842,6 +852,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="config3: the mixed solve alone")
     ap.add_argument("--no-solves", action="store_true", help="explicit: cycle timing only")
+    ap.add_argument("--polar-workers", type=int, default=4,
+                    help="rans: solvers (each with its own stream and host thread) the polar's angles are dealt to")
     ap.add_argument("--no-restart-checks", action="store_true",
                     help="config3: skip the cross-restart checks of cases outside the accuracy bar")
     a = ap.parse_args()
@@ -849,6 +861,12 @@ def main():
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (cases are independent; "
               "a multi-GPU run would shard the batch across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "rans" and a.polar_workers > 1:
+        # the polar's worker solvers each drive their own stream; they run side
+        # by side only on hardware queues of their own (HIP's default is 4),
+        # which the runtime fixes when it starts, before any HIP call here
+        q = max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), 2 * a.polar_workers)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(q, 32))
 
     if a.workload != "config2":
         fn = {"config1": run_config1, "config3": run_config3, "config4": run_config4, "config5": run_config5,

@@ -340,7 +340,8 @@ def test_batched_newton_steps_equal_single_steps():
 
 def test_polar_batch_equals_single_solves():
     """solve_polar over three angles (one slot each, lockstep batched steps)
-    returns each angle's solve alone, bit for bit, with the same history."""
+    returns each angle's solve alone, bit for bit, with the same history; so
+    does the polar dealt to two worker solvers on two host threads."""
     m = _mesh("quads")
     alphas = (0.0, 2.0, 4.0)
     kw = dict(max_iter=12)
@@ -358,3 +359,10 @@ def test_polar_batch_equals_single_solves():
         assert np.array_equal(rb.state, r1.state)
         assert rb.cl == r1.cl and rb.cd == r1.cd
         assert np.array_equal(np.asarray(rb.residual), np.asarray(r1.residual))
+    # the same polar dealt to two worker solvers driven from two host threads
+    # (each handle its own stream and lock): the same states, bit for bit
+    sw = RansSolver(m, 0.5, 2e4)
+    sw.prec_fill = 1
+    work = sw.solve_polar(alphas, workers=2, **kw)
+    for r1, rw in zip(single, work):
+        assert rw.alpha == r1.alpha and rw.iterations == r1.iterations and np.array_equal(rw.state, r1.state)
