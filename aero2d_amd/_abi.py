@@ -98,12 +98,12 @@ SIGNATURES = {
     "a2d_rans_ilu_apply": (ctypes.c_int, [_i64, _dp, _dp]),
     "a2d_rans_newton_step": (ctypes.c_int, [_i64, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                            ctypes.c_int, _dp]),
+                                            ctypes.c_int, ctypes.c_int, _dp]),
     "a2d_rans_newton_dq": (ctypes.c_int, [_i64, _dp]),
     "a2d_rans_set_cases": (ctypes.c_int, [_i64, ctypes.c_int]),
     "a2d_rans_select": (ctypes.c_int, [_i64, ctypes.c_int]),
     "a2d_rans_newton_steps": (ctypes.c_int, [_i64, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
-                                             _ip, ctypes.c_int, _dp]),
+                                             _ip, ctypes.c_int, _ip, _dp]),
     "a2d_rans_time_kernel": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
                                             _dp]),
     "a2d_rans_keep": (ctypes.c_int, [_i64, ctypes.c_int]),

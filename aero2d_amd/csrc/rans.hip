@@ -1441,6 +1441,10 @@ struct Solver {
   int *posD = nullptr, *posLR = nullptr, *posRL = nullptr;
   IluDev ilu{};
   bool has_ilu = false;
+  // the factors in B/invU/S are a Newton step's finite factorisation of this
+  // slot that no other entry point has overwritten since (what reuse_ilu may
+  // keep)
+  bool ilu_reusable = false;
   int* nonfinite = nullptr;
   // Krylov basis
   double* Vb = nullptr;
@@ -1883,6 +1887,7 @@ void relink(Group& g) {
     s.posLR = s0.posLR;
     s.posRL = s0.posRL;
     s.has_ilu = true;
+    s.ilu_reusable = false;      // fresh arrays hold no factors
     s.B = g.Ball + size_t(c) * g.sB;
     s.invU = g.Iall + size_t(c) * g.sI;
     s.y = g.Yall + size_t(c) * g.sY;
@@ -2550,6 +2555,7 @@ int a2d_rans_ilu_info(int64_t h, int* out4) {
 int a2d_rans_ilu_factor(int64_t h, const double* winf, double gam, double cfl, double turb_factor, int* nonfinite) {
   return guarded_h(h, [&] {
     Solver& s = get(h);
+    s.ilu_reusable = false;
     hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
     factor(s, winf, gam, 1.0 / turb_factor, nullptr, nullptr);
     if (nonfinite) *nonfinite = count_nonfinite(s, s.invU, s.m.nc * 25);
@@ -2568,7 +2574,8 @@ int a2d_rans_ilu_apply(int64_t h, const double* b, double* x) {
 }
 
 int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
-                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out) {
+                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, int reuse_ilu,
+                         double* out) {
   return guarded_h(h, [&] {
     Solver& s = get(h);
     if (order != 1 && order != 2) fail(-6, "order must be 1 or 2");
@@ -2580,12 +2587,19 @@ int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, d
     // residual of the state (the Newton right-hand side)
     far_state(s, winf, gam);
     eval(s, order, nullptr, s.R, nullptr, nullptr, 1.0);
-    // pseudo-time step, first-order Jacobian + vol/dt, ILU
+    // pseudo-time step, first-order Jacobian + vol/dt, ILU (or the last
+    // factors kept: only the preconditioner lags, the Newton matrix-vector
+    // product uses this step's vol/dt)
     hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl, s.dt, s.vdt);
-    factor(s, winf, gam, ct_inv, nullptr, nullptr);
-    if (count_nonfinite(s, s.invU, s.m.nc * 25) > 0) {
-      out[5] = 1.0;                       // ILU not finite: no step
-      return;
+    if (!(reuse_ilu && s.ilu_reusable)) {
+      s.ilu_reusable = false;
+      factor(s, winf, gam, ct_inv, nullptr, nullptr);
+      out[9] = 1.0;
+      if (count_nonfinite(s, s.invU, s.m.nc * 25) > 0) {
+        out[5] = 1.0;                     // ILU not finite: no step
+        return;
+      }
+      s.ilu_reusable = true;
     }
     const double rnorm = norm2(s, s.R);
     hipLaunchKernelGGL(k_negate, nblk(N), TPB, 0, s.st, N, s.R, s.bvec);
@@ -2646,6 +2660,7 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
     // inputs: the state's residual as the vector, a factorisation in place
     eval(s, order, nullptr, s.R, nullptr, nullptr, 1.0);
     if (which == 0 || which == 1) {
+      s.ilu_reusable = false;
       hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, 10.0, s.dt, s.vdt);
       factor(s, winf, gam, 1.0, nullptr, nullptr);
     }
@@ -2684,7 +2699,8 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
 
 int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* order, const double* winf,
                           const double* gam, const double* cfl, const double* turb_factor, const double* gmres_tol,
-                          int gmres_iter, const int* right, int step_limit_cells, double* out) {
+                          int gmres_iter, const int* right, int step_limit_cells, const int* reuse_ilu,
+                          double* out) {
   return guarded_h(h, [&] {
     Group& g = get_group(h);
     Solver& s0 = *g.cases[0];
@@ -2708,20 +2724,32 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
       for (int k = 0; k < 12; ++k) out[size_t(i) * 12 + k] = 0.0;
     }
     // residual, pseudo-time step and first-order Jacobian of every case, then
-    // one factorisation launch for all
+    // one factorisation launch for all that factor (the others keep their
+    // last factors)
+    std::vector<int> fids;
+    std::vector<char> fresh(n);
     for (int i = 0; i < n; ++i) {
       Solver& s = *cs[i];
       far_state(s, winf + 5 * i, gam[i]);
       eval(s, order[i], nullptr, s.R, nullptr, nullptr, 1.0);
       hipLaunchKernelGGL(k_local_dt, nblk(s.m.nc), TPB, 0, s.st, s.m, s.P, s.Q, cfl[i], s.dt, s.vdt);
+      fresh[i] = !(reuse_ilu && reuse_ilu[i] && s.ilu_reusable);
+      if (!fresh[i]) continue;
+      s.ilu_reusable = false;
       factor(s, winf + 5 * i, gam[i], ct[i], nullptr, nullptr, false);
+      fids.push_back(ids[i]);
+      out[size_t(i) * 12 + 9] = 1.0;
     }
-    launch_ilu_factor(s0.ilu, g.Ball, g.Iall, g.Sall, case_list(g, ids), n, g.sB, g.sI, g.sS, s0.st);
-    HC(hipGetLastError());
+    if (!fids.empty()) {
+      launch_ilu_factor(s0.ilu, g.Ball, g.Iall, g.Sall, case_list(g, fids), int(fids.size()), g.sB, g.sI, g.sS,
+                        s0.st);
+      HC(hipGetLastError());
+    }
     HC(hipMemsetAsync(g.nfall, 0, g.cases.size() * sizeof(int), s0.st));
     for (int i = 0; i < n; ++i)
-      hipLaunchKernelGGL(k_count_nonfinite, nblk(s0.m.nc * 25), TPB, 0, s0.st, s0.m.nc * 25, cs[i]->invU,
-                         cs[i]->nonfinite);
+      if (fresh[i])
+        hipLaunchKernelGGL(k_count_nonfinite, nblk(s0.m.nc * 25), TPB, 0, s0.st, s0.m.nc * 25, cs[i]->invU,
+                           cs[i]->nonfinite);
     for (int i = 0; i < n; ++i) mdot(*cs[i], cs[i]->R, size_t(N), 1, cs[i]->R, 0);
     int* const nf = g.nf_h;
     double* const hall = g.hall_h;
@@ -2737,6 +2765,7 @@ int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* orde
         out[size_t(i) * 12 + 5] = 1.0;   // ILU not finite: no step
         continue;
       }
+      if (fresh[i]) cs[i]->ilu_reusable = true;
       live.push_back(i);
       hipLaunchKernelGGL(k_negate, nblk(N), TPB, 0, s0.st, N, cs[i]->R, cs[i]->bvec);
     }

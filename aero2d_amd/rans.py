@@ -232,6 +232,7 @@ class RansSolver:
         self.max_rollbacks = 4
         self.freeze_limiter = True
         self.prec_fill = 3
+        self.prec_lag = 1
         self.right_prec = False
         self.zmin = 0.0
         self.gamma_vortex = 0.0
@@ -499,13 +500,16 @@ class RansSolver:
         return x
 
     STEP_FIELDS = ("lin", "gmres_iterations", "info", "lam", "rel", "ilu_nonfinite", "applied", "nbig",
-                   "per_cell")
+                   "per_cell", "factored")
+    _INT_FIELDS = ("gmres_iterations", "info", "ilu_nonfinite", "applied", "nbig", "per_cell", "factored")
 
     def newton_step(self, order, winf, gam, cfl, gmres_tol, gmres_iter=60, turb_factor=1.0, right=None,
-                    Q=None):
+                    Q=None, reuse_ilu=False):
         """One Newton step at the device state (or at Q): the linear solve and
-        update of RansSolver.solve (aero2d/rans.py:1738-1858). Returns a dict of
-        the step's figures."""
+        update of RansSolver.solve (aero2d/rans.py:1738-1858). reuse_ilu keeps
+        the slot's last factorisation as the preconditioner (prec_lag,
+        aero2d/rans.py:1736-1752) when it is a Newton step's finite one.
+        Returns a dict of the step's figures ("factored": whether it factored)."""
         self._ensure_ilu()
         if Q is not None:
             self._set(Q)
@@ -514,9 +518,9 @@ class RansSolver:
         r = self.right_prec if right is None else right
         _abi.call("a2d_rans_newton_step", self._device(), int(order), _abi.dptr(w), float(gam), float(cfl),
                   float(turb_factor), float(gmres_tol), int(gmres_iter), int(bool(r)), int(self.step_limit_cells),
-                  _abi.dptr(out))
+                  int(bool(reuse_ilu)), _abi.dptr(out))
         d = {k: float(v) for k, v in zip(self.STEP_FIELDS, out)}
-        for k in ("gmres_iterations", "info", "ilu_nonfinite", "applied", "nbig", "per_cell"):
+        for k in self._INT_FIELDS:
             d[k] = int(d[k])
         return d
 
@@ -569,7 +573,7 @@ class RansSolver:
                  "cfl_growth", "cfl_formula", "cfl_turb", "turb_boost", "turb_boost_cfl", "step_limit_cells",
                  "dt_visc_c", "conv", "cfl_start", "cfl_limit", "first_order", "rollback_factor",
                  "rollback_auto", "cfl_lin_back", "rollback_first_order", "step_reject", "max_rejects",
-                 "gmres_tol", "max_rollbacks", "freeze_limiter", "prec_fill", "right_prec", "zmin")
+                 "gmres_tol", "max_rollbacks", "freeze_limiter", "prec_fill", "prec_lag", "right_prec", "zmin")
 
     def clone(self) -> "RansSolver":
         """A new solver on the same mesh with the same constants and settings
@@ -676,17 +680,18 @@ class RansSolver:
         ct = f64([q["ct"] for q in qs])
         gtol = f64([q["gtol"] for q in qs])
         right = i32([int(bool(q["right"])) for q in qs])
+        reuse = i32([int(bool(q.get("reuse", False))) for q in qs])
         out = np.zeros((n, 12))
         gi = int(qs[0]["gmres_iter"])
         if any(int(q["gmres_iter"]) != gi for q in qs):
             raise ValueError("a batch of Newton steps shares one GMRES size")
         _abi.call("a2d_rans_newton_steps", self._device(), n, _abi.iptr(ids), _abi.iptr(order), _abi.dptr(winf),
                   _abi.dptr(gam), _abi.dptr(cfl), _abi.dptr(ct), _abi.dptr(gtol), gi, _abi.iptr(right),
-                  int(self.step_limit_cells), _abi.dptr(out))
+                  int(self.step_limit_cells), _abi.iptr(reuse), _abi.dptr(out))
         res = []
         for row in out:
             d = {k: float(v) for k, v in zip(self.STEP_FIELDS, row)}
-            for k in ("gmres_iterations", "info", "ilu_nonfinite", "applied", "nbig", "per_cell"):
+            for k in self._INT_FIELDS:
                 d[k] = int(d[k])
             res.append(d)
         return res
@@ -754,7 +759,7 @@ class _Run:
             self.cfl = min(self.cfl_max, cfl_formula(S.cfl_formula, n=0, orders=0.0, res=1.0, dres=1.0,
                                                      cfl=self.cfl0))
         self.last_ok, self.last_lin = True, 0.0
-        self.H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[])
+        self.H = dict(it=[], res=[], rn=[], cl=[], cd=[], cfl=[], gmres=[], lin=[], factored=[])
         self.events = []
         self.best = dict(rn=np.inf, has=self.turning, it=0, cfl=self.cfl0, order=0)
         if self.turning:
@@ -768,6 +773,7 @@ class _Run:
         self.status = ""
         self.it = 0
         self.cfl_hold = None
+        self.ilu_state = None            # (iteration, CFL, order) of the last factorisation
 
     def _say(self, text):
         if self.verbose:
@@ -799,6 +805,7 @@ class _Run:
         order = 1 if (it <= self.first_order_iters and self.Q0 is None) or it <= self.first_until else S.order
         if (order == 2 and self.prev_order_used == 1 and S.cfl_reset_on_order and self.cfl_hold is None):
             self.cfl = self.cfl0
+            self.ilu_state = None
         self.prev_order_used = order
         self.order = order
         gm = self.gamma if S.vortex_bc else 0.0
@@ -826,6 +833,7 @@ class _Run:
                 self.done = True
                 return None
             self.cfl = max(self.cfl0, min(self.cfl, best["cfl"]) * 0.1)
+            self.ilu_state = None
             self.first_until = it + (S.rollback_first_order if not np.isfinite(rn) else 0)
             if self.frozen:
                 S._freeze(None)
@@ -836,6 +844,7 @@ class _Run:
                 and (not np.isfinite(rn) or rn > S.step_reject * self.prev_rn)
                 and not (self.warm and it <= 8) and not manual):
             self.n_rej += 1
+            self.ilu_state = None
             S._keep_state(2)
             self.cfl = max(self.cfl0, self.cfl * 0.3)
             self.events.append((it, f"step undone (residual x{rn / self.prev_rn:.3g}), CFL -> {self.cfl:.3g}",
@@ -964,18 +973,30 @@ class _Run:
         # laminar solves take right preconditioning: left preconditioning
         # "converges" on |M^-1 r| while the true residual stays near 0.2
         # (aero2d/rans.py:1781-1790)
+        # the last factorisation stays the preconditioner for up to prec_lag
+        # iterations while the CFL is within a factor 2 of its own and the
+        # order is the same (aero2d/rans.py:1736-1752); prec_lag 1 factors
+        # every iteration
+        st_ = self.ilu_state
+        fresh = (st_ is None or it - st_[0] >= max(1, int(S.prec_lag)) or not (0.5 < self.cfl / st_[1] < 2.0)
+                 or st_[2] != order)
+        self.step_cfl = self.cfl
         return dict(slot=self.slot, order=order, winf=self.winf, gam=gm, cfl=self.cfl, ct=ct, gtol=gtol,
-                    right=S.right_prec or not S.turbulent, gmres_iter=self.gmres_iter)
+                    right=S.right_prec or not S.turbulent, gmres_iter=self.gmres_iter, reuse=not fresh)
 
     def after(self, st):
         S, it = self.S, self.it
         if st["ilu_nonfinite"]:
             self._say(f"ILU not finite, CFL {self.cfl:.3g} -> {max(self.cfl0, self.cfl / 10):.3g}")
             self.cfl = max(self.cfl0, self.cfl / 10)
+            self.ilu_state = None
             return
+        if st["factored"]:
+            self.ilu_state = (it, self.step_cfl, self.order)
         lin = st["lin"]
         self.H["gmres"].append(st["gmres_iterations"])
         self.H["lin"].append(lin)
+        self.H["factored"].append(st["factored"])
         self.last_ok = bool(np.isfinite(lin) and lin < 0.5)
         self.last_lin = lin
         if not st["applied"]:

@@ -775,6 +775,23 @@ def run_rans(a):
                        "cells": m.n_cells, "ilu_nnz_blocks": info["nnz"], "ilu_levels": info["levels"],
                        "max_iter": iters},
             "solve": solve_rec(r, wall, 0.0), "solve_alpha10": solve_rec(r10, wall10, 10.0)}
+    if a.prec_lag > 1:
+        # the reference's prec_lag: the last factorisation kept up to prec_lag
+        # iterations while the CFL stays within a factor 2 (its default, kept
+        # above, factors every iteration)
+        sl = RansSolver(m, mach, re)
+        sl.prec_lag = a.prec_lag
+        t0 = time.perf_counter()
+        rl = sl.solve(alpha, max_iter=iters, verbose=False)
+        wl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rl10 = sl.solve(10.0, Q0=rl.state, alpha0=alpha, max_iter=iters, verbose=False)
+        wl10 = time.perf_counter() - t0
+        sl.close()
+        line["prec_lag"] = {"prec_lag": a.prec_lag, "solve": solve_rec(rl, wl, 0.0),
+                            "solve_alpha10": solve_rec(rl10, wl10, 10.0),
+                            "factorisations": [int(np.sum(x.history["factored"])) for x in (rl, rl10)],
+                            "newton_iterations_per_s": (rl.iterations + rl10.iterations) / (wl + wl10)}
     kt = {k: s.time_kernel(k, alpha, reps=10 if k == "ilu_factor" else 30) for k in s.KERNELS}
     line["kernel_ms"] = kt
     n, nnz = m.n_cells, info["nnz"]
@@ -854,6 +871,8 @@ def main():
     ap.add_argument("--no-solves", action="store_true", help="explicit: cycle timing only")
     ap.add_argument("--polar-workers", type=int, default=4,
                     help="rans: solvers (each with its own stream and host thread) the polar's angles are dealt to")
+    ap.add_argument("--prec-lag", type=int, default=3,
+                    help="rans: also solve with the ILU factorisation kept up to this many iterations (1: skip)")
     ap.add_argument("--no-restart-checks", action="store_true",
                     help="config3: skip the cross-restart checks of cases outside the accuracy bar")
     a = ap.parse_args()

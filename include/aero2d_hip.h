@@ -282,10 +282,17 @@ int a2d_rans_ilu_apply(int64_t h, const double* b, double* x);
  * linear residual is finite and <= 0.9, the step limit (rho, p fall by at
  * most 20 %; only the cells that ask for more when there are at most
  * step_limit_cells of them) and the update; the previous state is kept.
+ * reuse_ilu = 1 keeps the slot's last factorisation as the preconditioner
+ * (no Jacobian assembly, no factorisation: the reference's prec_lag,
+ * aero2d/rans.py:1736-1752); it is honoured only when that factorisation was
+ * made by a Newton step of this slot, was finite, and no other entry point
+ * has factored since; otherwise the step factors.
  * out (12): lin, GMRES iterations, info, lam, rel, ilu_nonfinite, applied,
- * cells beyond 0.2, per-cell limit used, 0, 0, 0. */
+ * cells beyond 0.2, per-cell limit used, factored (1: this step factored,
+ * 0: it reused the last factors), 0, 0. */
 int a2d_rans_newton_step(int64_t h, int order, const double* winf, double gam, double cfl, double turb_factor,
-                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, double* out);
+                         double gmres_tol, int gmres_iter, int right, int step_limit_cells, int reuse_ilu,
+                         double* out);
 /* Bench hook: mean device time (ms, HIP events on the solver's stream) of one
  * of: 0 the ILU solve (BlockILU0.solve), 1 the ILU factorisation, 2 the
  * Newton matrix-vector product (Jacobian-vector product plus vol/dt), 3 the
@@ -300,13 +307,16 @@ int a2d_rans_time_kernel(int64_t h, int which, int order, const double* winf, do
 int a2d_rans_set_cases(int64_t h, int ncases);
 int a2d_rans_select(int64_t h, int c);
 /* a2d_rans_newton_step for n distinct slots at once: per slot its order,
- * winf (5), gam, CFL, turbulence step factor, GMRES tolerance and side; out
- * (n, 12) as a2d_rans_newton_step's. The factorisation and every triangular
- * sweep of all slots run as one launch (a workgroup per slot) and the
- * Arnoldi steps go in lockstep; each slot's result equals its step alone. */
+ * winf (5), gam, CFL, turbulence step factor, GMRES tolerance, side and
+ * reuse_ilu flag (reuse_ilu may be NULL: every slot factors); out (n, 12) as
+ * a2d_rans_newton_step's. The factorisation of the slots that factor and
+ * every triangular sweep of all slots run as one launch (a workgroup per
+ * slot) and the Arnoldi steps go in lockstep; each slot's result equals its
+ * step alone. */
 int a2d_rans_newton_steps(int64_t h, int n, const int* case_ids, const int* order, const double* winf,
                           const double* gam, const double* cfl, const double* turb_factor, const double* gmres_tol,
-                          int gmres_iter, const int* right, int step_limit_cells, double* out);
+                          int gmres_iter, const int* right, int step_limit_cells, const int* reuse_ilu,
+                          double* out);
 /* The last Newton step's dq before the step limit (nc, 5). */
 int a2d_rans_newton_dq(int64_t h, double* dq);
 /* 0: keep the state as the best; 1: return to the best; 2: undo the last step. */

@@ -338,6 +338,89 @@ def test_batched_newton_steps_equal_single_steps():
     s._select(0)
 
 
+def test_reused_factorisation_lags_only_the_preconditioner():
+    """reuse_ilu, the reference's prec_lag (aero2d/rans.py:1736-1752): a step
+    that keeps the last factorisation at the same state and CFL gives the bits
+    of the step that factored, without factoring; at another CFL the kept
+    factors only precondition, the Newton matrix takes the new vol/dt (the
+    step matches the oracle's GMRES with the CFL-10 ILU and the CFL-15
+    matrix); a slot with nothing to keep (after a2d_rans_ilu_factor, or new
+    slots) factors; batched steps with mixed flags give each slot's bits."""
+    m, s, o = _pair("hybrid")
+    s.prec_fill = 1
+    Q = _state(o, 2.0, 17, amp=0.02)
+    winf = o.freestream(2.0)
+    gam = 0.05
+    st1 = s.newton_step(2, winf, gam, 10.0, 0.0, 20, Q=Q, reuse_ilu=True)    # nothing kept yet: factors
+    Q1, dq1 = s._get(), s.newton_dq()
+    assert st1["factored"] == 1 and st1["ilu_nonfinite"] == 0
+    st2 = s.newton_step(2, winf, gam, 10.0, 0.0, 20, Q=Q, reuse_ilu=True)
+    assert st2["factored"] == 0 and {**st2, "factored": 1} == st1
+    assert np.array_equal(s._get(), Q1) and np.array_equal(s.newton_dq(), dq1)
+    # CFL 15 with the CFL-10 factors
+    st3 = s.newton_step(2, winf, gam, 15.0, 0.0, 20, Q=Q, reuse_ilu=True)
+    assert st3["factored"] == 0
+    ilu10, _ = _oracle_ilu(o, Q, winf, gam, 10.0, 1)
+    dt15 = o.local_dt(Q, 15.0)
+    Qt = torch.tensor(Q)
+    R = o.residual(Qt, winf, gam).numpy()
+    vdt = np.repeat((o.vol / dt15)[:, None], 5, 1).ravel()
+
+    def mv(x):
+        return o.jvp(Qt, torch.tensor(x.reshape(-1, 5)), winf, gam).numpy().ravel() + vdt * x
+    dq_o, _, _ = gmres(mv, -R.ravel(), prec=ilu10.solve, rtol=0.0, maxiter=20)
+    assert _rel(s.newton_dq(), dq_o.reshape(-1, 5)) < 1e-8, _rel(s.newton_dq(), dq_o.reshape(-1, 5))
+    lin_o = np.linalg.norm(mv(dq_o) + R.ravel()) / np.linalg.norm(R)
+    assert abs(st3["lin"] - lin_o) < 1e-8 * max(1.0, lin_o)
+    # another entry point factored: nothing to keep
+    assert s.ilu_factor(Q, winf, gam, 10.0) == 0
+    assert s.newton_step(2, winf, gam, 10.0, 0.0, 20, Q=Q, reuse_ilu=True)["factored"] == 1
+    # batched: new slots factor; then slot 0 keeps its factors, slot 1 factors again
+    __import__("aero2d_amd._abi", fromlist=["call"]).call("a2d_rans_set_cases", s._device(), 2)
+    Qs = [Q, _state(o, 3.0, 18, amp=0.02)]
+    req = [dict(slot=k, order=2, winf=winf, gam=gam, cfl=10.0 + 5 * k, ct=1.0, gtol=1e-2, right=False,
+                gmres_iter=20, reuse=True) for k in range(2)]
+    res = []
+    for reuse in ((True, True), (True, False)):
+        for k in range(2):
+            s._select(k)
+            s._set(Qs[k])
+            req[k]["reuse"] = reuse[k]
+        outs = s._newton_steps(req)
+        got = []
+        for k in range(2):
+            s._select(k)
+            got.append((s._get(), s.newton_dq()))
+        res.append((outs, got))
+    (o1, g1), (o2, g2) = res
+    assert [d["factored"] for d in o1] == [1, 1] and [d["factored"] for d in o2] == [0, 1]
+    for k in range(2):
+        assert {**o2[k], "factored": 1} == o1[k]
+        assert np.array_equal(g2[k][0], g1[k][0]) and np.array_equal(g2[k][1], g1[k][1])
+    s._select(0)
+
+
+def test_prec_lag_solve_converges_with_fewer_factorisations():
+    """A solve with prec_lag 3 (the last factorisation kept up to three
+    iterations while the CFL stays within a factor 2 and the order holds)
+    converges to the prec_lag 1 answer, factoring on fewer iterations: the
+    laminar NACA 0012 case (M 0.5, Re 5000) on the 27k-cell C-grid."""
+    from aero2d_amd.rans_mesh import refine_cwall
+    m = cgrid_mesh(refine_cwall(cgrid(naca4("0012", 201), n_side=97, nj=65, ds0=1e-3, radius=20.0), 2e-4))
+    out = {}
+    for lag in (1, 3):
+        s = RansSolver(m, 0.5, 5000.0, turbulent=False)
+        s.prec_lag = lag
+        r = s.solve(0.0, max_iter=200)
+        f = np.asarray(r.history["factored"])
+        out[lag] = (r, f)
+        s.close()
+    (r1, f1), (r3, f3) = out[1], out[3]
+    assert r1.converged and r3.converged, (r1.status, r3.status)
+    assert f1.all() and 0 < f3.sum() < len(f3)
+    assert abs(r3.cd - r1.cd) < 1e-4 * abs(r1.cd) and abs(r3.cl - r1.cl) < 1e-5
+
+
 def test_polar_batch_equals_single_solves():
     """solve_polar over three angles (one slot each, lockstep batched steps)
     returns each angle's solve alone, bit for bit, with the same history; so
