@@ -27,7 +27,8 @@ class RansParams(ctypes.Structure):
     _fields_ = [("mach", ctypes.c_double), ("re", ctypes.c_double), ("chord", ctypes.c_double),
                 ("xref_x", ctypes.c_double), ("xref_y", ctypes.c_double), ("chi_inf", ctypes.c_double),
                 ("limiter_k", ctypes.c_double), ("zmin", ctypes.c_double), ("order", ctypes.c_int),
-                ("turbulent", ctypes.c_int), ("vortex_bc", ctypes.c_int), ("limiter_zone", ctypes.c_int)]
+                ("turbulent", ctypes.c_int), ("vortex_bc", ctypes.c_int), ("limiter_zone", ctypes.c_int),
+                ("sa_raw_grad", ctypes.c_int), ("visc_raw_grad", ctypes.c_int)]
 
 
 class Knobs(ctypes.Structure):

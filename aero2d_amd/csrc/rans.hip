@@ -295,7 +295,8 @@ __global__ void k_far_state(MeshDev m, Params P, double w0, double w1, double w2
 // Interior face flux: reconstruction, LM-Roe, viscous flux with the face
 // gradient (cell mean plus the correction along the line of centres), times
 // the face length (aero2d/rans.py:878-919).
-template <class S>
+// VR (visc_raw_grad): the viscous part from the unlimited gradients.
+template <class S, bool VR>
 __global__ void k_face(MeshDev m, Params P, int order, const double* Q, const double* V, const double* g,
                        const double* gt, const double* phi, const double* phit, double* F, double* Ft) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -332,6 +333,12 @@ __global__ void k_face(MeshDev m, Params P, int order, const double* Q, const do
   }
   S Fc[5];
   roe(P, wL, wR, nx, ny, Fc);
+  if constexpr (VR) {
+    if (order >= 2) {               // the gradients before limiting and far-field zeroing
+      load_g<S>(m, g, gt, nullptr, nullptr, 1, L, GL);
+      load_g<S>(m, g, gt, nullptr, nullptr, 1, R, GR);
+    }
+  }
   S wf[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) wf[k] = K<S>(0.5) * (wL0[k] + wR0[k]);
@@ -364,8 +371,11 @@ __global__ void k_face(MeshDev m, Params P, int order, const double* Q, const do
 // Cell residual: signed sum of interior face fluxes, wall and far-field faces,
 // minus the SA source times the volume (aero2d/rans.py:920-964). With vdt
 // non-null the output is the Newton matrix times V: the tangent plus
-// vol/dt V (vol/(dt ct) on the turbulence equation).
-template <class S>
+// vol/dt V (vol/(dt ct) on the turbulence equation). VR (visc_raw_grad): the
+// wall's viscous flux from the unlimited gradients; SR (sa_raw_grad): the SA
+// source from them (the wall pressure and the far-field state always take
+// the limited ones).
+template <class S, bool VR, bool SR>
 __global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const double* V, const double* g,
                        const double* gt, const double* phi, const double* phit, const double* F,
                        const double* Ft, const double* wgf, double* Rv, double* Rt, const double* vdt,
@@ -390,6 +400,12 @@ __global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const do
   }
   const int nw = m.nfw;
   if (m.wptr[c] < m.wptr[c + 1]) {
+    // VR: the limited G[3] kept for the wall pressure's reconstruction, G
+    // reloaded unlimited for the viscous flux
+    S g3[2] = {G[3][0], G[3][1]};
+    if constexpr (VR) {
+      if (order >= 2) load_g<S>(m, g, gt, nullptr, nullptr, 1, c, G);
+    }
     S gT[2];
 #pragma unroll
     for (int d = 0; d < 2; ++d) gT[d] = K<S>(GAM) * (G[3][d] / w[0] - (w[3] / (w[0] * w[0])) * G[0][d]);
@@ -398,7 +414,11 @@ __global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const do
       const double nx = m.wg[f], ny = m.wg[nw + f], A = m.wg[2 * nw + f];
       const double rx = m.wg[3 * nw + f], ry = m.wg[4 * nw + f], dn = m.wg[5 * nw + f];
       S pw = w[3];
-      if (order >= 2) pw = w[3] + (G[3][0] * rx + G[3][1] * ry);
+      if constexpr (VR) {
+        if (order >= 2) pw = w[3] + (g3[0] * rx + g3[1] * ry);
+      } else {
+        if (order >= 2) pw = w[3] + (G[3][0] * rx + G[3][1] * ry);
+      }
       if (!(val(pw) > 0.2 * val(w[3]))) pw = w[3];
       auto wgrad = [&](S phii, const S gi[2], S out[2]) {
         S corr = (K<S>(0.0) - phii) / K<S>(dn) - (gi[0] * nx + gi[1] * ny);
@@ -420,6 +440,9 @@ __global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const do
       R[2] = R[2] + (pw * ny - Fv[2]) * K<S>(A);
       R[3] = R[3] + (K<S>(0.0) - Fv[3]) * K<S>(A);
       R[4] = R[4] + (K<S>(0.0) - Fv[4]) * K<S>(A);
+    }
+    if constexpr (VR) {
+      if (order >= 2) load_g<S>(m, g, gt, phi, phit, order, c, G);
     }
   }
   const int nff = m.nff;
@@ -448,6 +471,9 @@ __global__ void k_cell(MeshDev m, Params P, int order, const double* Q, const do
   }
   const double vol = m.cg[c];
   if (P.turbulent) {
+    if constexpr (SR) {
+      if (order >= 2) load_g<S>(m, g, gt, nullptr, nullptr, 1, c, G);
+    }
     R[4] = R[4] - sa_source(P, w, G, m.cg[m.nc + c]) * K<S>(vol);
   } else {
     R[4] = ld<S>(Q, V, 4 * m.nc + c) * K<S>(vol);
@@ -1410,9 +1436,16 @@ __global__ void k_count_nonfinite(int n, const double* x, int* out) {
 // ---- host side ----------------------------------------------------------------------
 template <class T>
 T* dalloc(size_t n) {
+  // A2D_RANS_POISON=1 fills every new array with all-ones bytes (NaN doubles,
+  // -1 ints): a kernel that reads memory nothing wrote then shows NaN
+  static const bool poison = [] {
+    const char* e = std::getenv("A2D_RANS_POISON");
+    return e && *e && *e != '0';
+  }();
   T* p = nullptr;
   if (n == 0) n = 1;
   HC(hipMalloc(&p, n * sizeof(T)));
+  if (poison) HC(hipMemset(p, 0xFF, n * sizeof(T)));
   return p;
 }
 template <class T>
@@ -1543,6 +1576,23 @@ inline int nblk(int n, int t = TPB) { return (n + t - 1) / t; }
 // Residual (or, with V, its tangent along V) of the state Q into Rv/Rt.
 // Limiter: none at first order or with the zone off; frozen when set; else
 // evaluated (and differentiated) from Q.
+// k_face / k_cell in the instantiation the solver's raw-gradient options select.
+template <class S, class... A>
+void launch_face(Solver& s, int blocks, A... a) {
+  if (s.P.visc_raw) hipLaunchKernelGGL((k_face<S, true>), blocks, TPB, 0, s.st, a...);
+  else hipLaunchKernelGGL((k_face<S, false>), blocks, TPB, 0, s.st, a...);
+}
+template <class S, class... A>
+void launch_cell(Solver& s, int blocks, A... a) {
+  const int k = (s.P.visc_raw ? 1 : 0) + (s.P.sa_raw ? 2 : 0);
+  switch (k) {
+    case 0: hipLaunchKernelGGL((k_cell<S, false, false>), blocks, TPB, 0, s.st, a...); break;
+    case 1: hipLaunchKernelGGL((k_cell<S, true, false>), blocks, TPB, 0, s.st, a...); break;
+    case 2: hipLaunchKernelGGL((k_cell<S, false, true>), blocks, TPB, 0, s.st, a...); break;
+    default: hipLaunchKernelGGL((k_cell<S, true, true>), blocks, TPB, 0, s.st, a...); break;
+  }
+}
+
 void eval(Solver& s, int order, const double* V, double* Rv, double* Rt, const double* vdt, double ct_inv) {
   const MeshDev& m = s.m;
   const bool lim = order >= 2 && s.P.limiter_k > 0 && s.P.limiter_zone != 0;
@@ -1559,10 +1609,9 @@ void eval(Solver& s, int order, const double* V, double* Rv, double* Rt, const d
       phi = s.phi;
       phit = s.phit;
     }
-    hipLaunchKernelGGL(k_face<Dual<double>>, nblk(m.nfi), TPB, 0, s.st, m, s.P, order, s.Q, V, s.g, s.gt, phi,
-                       phit, s.F, s.Ft);
-    hipLaunchKernelGGL(k_cell<Dual<double>>, nblk(m.nc), TPB, 0, s.st, m, s.P, order, s.Q, V, s.g, s.gt, phi,
-                       phit, s.F, s.Ft, s.wgf, Rv, Rt, vdt, ct_inv);
+    launch_face<Dual<double>>(s, nblk(m.nfi), m, s.P, order, s.Q, V, s.g, s.gt, phi, phit, s.F, s.Ft);
+    launch_cell<Dual<double>>(s, nblk(m.nc), m, s.P, order, s.Q, V, s.g, s.gt, phi, phit, s.F, s.Ft, s.wgf, Rv,
+                              Rt, vdt, ct_inv);
   } else {
     hipLaunchKernelGGL(k_grad<double>, nblk(m.nc), TPB, 0, s.st, m, s.Q, nullptr, s.g, nullptr);
     if (lim && s.frozen) {
@@ -1572,10 +1621,11 @@ void eval(Solver& s, int order, const double* V, double* Rv, double* Rt, const d
                          nullptr);
       phi = s.phi;
     }
-    hipLaunchKernelGGL(k_face<double>, nblk(m.nfi), TPB, 0, s.st, m, s.P, order, s.Q, nullptr, s.g, nullptr,
-                       phi, nullptr, s.F, nullptr);
-    hipLaunchKernelGGL(k_cell<double>, nblk(m.nc), TPB, 0, s.st, m, s.P, order, s.Q, nullptr, s.g, nullptr,
-                       phi, nullptr, s.F, nullptr, s.wgf, Rv, nullptr, nullptr, 1.0);
+    launch_face<double>(s, nblk(m.nfi), m, s.P, order, s.Q, (const double*)nullptr, s.g,
+                        (const double*)nullptr, phi, (const double*)nullptr, s.F, (double*)nullptr);
+    launch_cell<double>(s, nblk(m.nc), m, s.P, order, s.Q, (const double*)nullptr, s.g, (const double*)nullptr,
+                        phi, (const double*)nullptr, s.F, (const double*)nullptr, s.wgf, Rv, (double*)nullptr,
+                        (const double*)nullptr, 1.0);
   }
   HC(hipGetLastError());
 }
@@ -2206,6 +2256,8 @@ int a2d_rans_create(int nc, int nfi, int nfw, int nff, const int* face_l, const 
     P.chord = prm->chord;
     P.turbulent = prm->turbulent;
     P.limiter_zone = prm->limiter_zone;
+    P.sa_raw = prm->sa_raw_grad != 0;
+    P.visc_raw = prm->visc_raw_grad != 0;
     s->order = prm->order;
     s->owns_stream = true;
     alloc_case(*s);

@@ -37,6 +37,8 @@ struct Params {
   double mach, mu_inf, nu_inf, chi_inf, limiter_k, xref_x, xref_y, zmin, chord;
   int turbulent;      // 0: laminar (rho chi == 0 equation)
   int limiter_zone;   // 0 off, 1 trailing-edge zone, 2 everywhere
+  int sa_raw;         // 1: SA source from the unlimited gradients
+  int visc_raw;       // 1: viscous fluxes from the unlimited gradients
 };
 
 // prim (aero2d/rans.py:455-461)

@@ -233,6 +233,8 @@ class RansSolver:
         self.freeze_limiter = True
         self.prec_fill = 3
         self.prec_lag = 1
+        self.sa_raw_grad = False
+        self.visc_raw_grad = False
         self.right_prec = False
         self.zmin = 0.0
         self.gamma_vortex = 0.0
@@ -314,7 +316,8 @@ class RansSolver:
         zone = {"off": 0, "te": 1, "all": 2}[self.limiter_zone]
         prm = _abi.RansParams(self.mach, self.re, self.chord, self.xref[0], self.xref[1], self.chi_inf,
                               self.limiter_k, self.zmin, int(self.order), int(bool(self.turbulent)),
-                              int(bool(self.vortex_bc)), zone)
+                              int(bool(self.vortex_bc)), zone, int(bool(self.sa_raw_grad)),
+                              int(bool(self.visc_raw_grad)))
         arrs = [i32(self.iL), i32(self.iR), f64(fgeo), i32(self.cw), f64(wgeo), i32(self.cf), f64(ffgeo),
                 f64(cgeo)]
         self._keep = arrs
@@ -573,7 +576,8 @@ class RansSolver:
                  "cfl_growth", "cfl_formula", "cfl_turb", "turb_boost", "turb_boost_cfl", "step_limit_cells",
                  "dt_visc_c", "conv", "cfl_start", "cfl_limit", "first_order", "rollback_factor",
                  "rollback_auto", "cfl_lin_back", "rollback_first_order", "step_reject", "max_rejects",
-                 "gmres_tol", "max_rollbacks", "freeze_limiter", "prec_fill", "prec_lag", "right_prec", "zmin")
+                 "gmres_tol", "max_rollbacks", "freeze_limiter", "prec_fill", "prec_lag", "right_prec", "zmin",
+                 "sa_raw_grad", "visc_raw_grad")
 
     def clone(self) -> "RansSolver":
         """A new solver on the same mesh with the same constants and settings

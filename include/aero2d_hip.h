@@ -227,6 +227,8 @@ typedef struct a2d_rans_params {
   int turbulent;      /* 0: laminar, the rho chi equation is rho chi = 0 */
   int vortex_bc;      /* informational: the host passes gam = 0 without it */
   int limiter_zone;   /* 0 "off", 1 "te", 2 "all" */
+  int sa_raw_grad;    /* 1: the SA source from the unlimited gradients (aero2d/rans.py:309,961) */
+  int visc_raw_grad;  /* 1: the viscous fluxes from the unlimited gradients (aero2d/rans.py:310,897) */
 } a2d_rans_params;
 
 /* Mesh arrays as RansSolver._setup forms them (aero2d/rans.py:328-425):

@@ -41,10 +41,12 @@ class RansOracle:
     aero2d/rans_mesh.py:19-36). Keyword defaults follow RansSolver.__init__
     (aero2d/rans.py:188-326): chi_inf 3, order 2, vortex_bc, turbulent,
     limiter_k 5, limiter_zone "te" with radius 0.05 chord, turb_order 1,
-    dt_visc_c 1, zmin 0."""
+    dt_visc_c 1, zmin 0, sa_raw_grad and visc_raw_grad off (309-310: the SA
+    source and the viscous fluxes from the unlimited gradients when on)."""
 
     def __init__(self, mesh, mach, re, chord=1.0, xref=(0.25, 0.0), chi_inf=3.0, order=2,
-                 vortex_bc=True, turbulent=True, limiter_zone="te", limiter_k=5.0):
+                 vortex_bc=True, turbulent=True, limiter_zone="te", limiter_k=5.0, sa_raw_grad=False,
+                 visc_raw_grad=False):
         self.m = mesh
         self.mach, self.re, self.chord = float(mach), float(re), float(chord)
         self.mu_inf = self.mach * chord / self.re
@@ -56,6 +58,7 @@ class RansOracle:
         self.turbulent = turbulent
         self.limiter_k = limiter_k
         self.limiter_zone = limiter_zone
+        self.sa_raw_grad, self.visc_raw_grad = bool(sa_raw_grad), bool(visc_raw_grad)
         self.limiter_te_radius = 0.05
         self.dt_visc_c = 1.0
         self.zmin = 0.0
@@ -353,6 +356,7 @@ class RansOracle:
         w = self.prim(Q)
         wv = self.wall_values(w)
         g = self.gradients(w, wv)
+        gr = g                                  # unlimited (868): the raw-gradient options read these
         iL, iR = J["iL"], J["iR"]
         if order >= 2 and self.limiter_k > 0 and self.limiter_zone != "off":
             phi = self.venkat(w, g, wv) if phi_frozen is None else torch.as_tensor(phi_frozen, dtype=DT)
@@ -374,7 +378,7 @@ class RansOracle:
         Fc = self.roe(wL, wR, J["n_i"])
         wf = 0.5 * (w[iL] + w[iR])
         T = GAM * w[:, 3] / w[:, 0]
-        gvv = g
+        gvv = gr if self.visc_raw_grad else g   # 897
         gT = self.grad_T(w, gvv)
         e = J["d_i"] / J["dist_i"][:, None]
 
@@ -417,7 +421,7 @@ class RansOracle:
         Rs = Rs.index_add(0, cf, self.roe(wi, wg, J["n_f"]) * J["A_f"][:, None])
         if not self.turbulent:
             return torch.cat([Rs[:, :4], Q[:, 4:] * J["vol"][:, None]], 1)
-        S = self.sa_source(w, g, J["wd"])
+        S = self.sa_source(w, gr if self.sa_raw_grad else g, J["wd"])   # 961
         return torch.cat([Rs[:, :4], Rs[:, 4:] - S[:, None] * J["vol"][:, None]], 1)
 
     def phi(self, Q):

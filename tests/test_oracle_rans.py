@@ -291,3 +291,39 @@ def test_local_dt_scales_with_cfl_and_forces_vanish_for_uniform_pressure(mesh):
     assert np.all(dt1 > 0) and np.allclose(dt3, 3 * dt1, rtol=1e-14)
     f = o.forces(Q, 2.0)
     assert abs(f["cl"]) < 1e-12 and abs(f["cd"]) < 1e-12 and abs(f["cm"]) < 1e-12
+
+
+def test_raw_gradient_options_touch_only_their_terms(mesh):
+    """sa_raw_grad / visc_raw_grad (aero2d/rans.py:309-310, 868, 897, 961)
+    read the gradients before limiting and far-field zeroing. At first order
+    (no limiting, no zeroing) they change nothing. With the limiter in the
+    trailing-edge zone: visc_raw_grad leaves continuity (no viscous flux) alone and changes the
+    other equations only in cells whose gradient is limited or zeroed and in
+    their face neighbours; sa_raw_grad changes only the rho chi equation and
+    only in those cells. A raw gradient leaking into the reconstruction would
+    move continuity; a source read at the neighbours would spread the change."""
+    base = dict(limiter_zone="te")
+    o0 = RansOracle(mesh, 0.5, 2e4, **base)
+    Q = _state(o0, 2.0, 21, amp=0.2)
+    winf = o0.freestream(2.0)
+    ov = RansOracle(mesh, 0.5, 2e4, visc_raw_grad=True, **base)
+    osr = RansOracle(mesh, 0.5, 2e4, sa_raw_grad=True, **base)
+    for order in (1,):
+        R0 = o0.residual(Q, winf, 0.1, order=order)
+        assert torch.equal(ov.residual(Q, winf, 0.1, order=order), R0)
+        assert torch.equal(osr.residual(Q, winf, 0.1, order=order), R0)
+    R0, Rv, Rs = (o.residual(Q, winf, 0.1) for o in (o0, ov, osr))
+    phi = o0.phi(Q).numpy()
+    diff = (phi < 1).any(1) & o0.J["lim_te"].numpy()
+    diff[o0.cf] = True                         # far-field cells: zeroed gradients
+    assert 0 < diff.sum() < o0.nc
+    near = diff.copy()
+    L, R = o0.iL, o0.iR
+    near[L[diff[R]]] = True
+    near[R[diff[L]]] = True
+    dv = (Rv - R0).abs().numpy()
+    assert torch.equal(Rv[:, 0], R0[:, 0])
+    assert dv[~near].max() == 0.0 and dv[near, 1:].max() > 1e-8
+    ds = (Rs - R0).abs().numpy()
+    assert torch.equal(Rs[:, :4], R0[:, :4])
+    assert ds[~diff, 4].max() == 0.0 and ds[diff, 4].max() > 1e-8
