@@ -795,8 +795,11 @@ struct IluDev {
   const int *flev, *blev;            // first staged block of each level (blev after flev's)
   const int* ssrc;                   // source of staged block b: entry e < nnz of B, or
                                      // nnz + i for invU of row i
+  const int* scol;                   // column of staged block b (0 for an inverse)
   int nst;                           // staged blocks (nnz: every block once, inverses included)
   int scap;                          // longest level run, in doubles
+  int ccap, rcap;                    // LDS slots per level for its columns (scap / 25, a
+                                     // multiple of 4) and its row records
   int lds_solve;                     // 1: the sweeps stage levels in LDS (k_ilu_solve_lds)
 };
 
@@ -1067,13 +1070,13 @@ __global__ void k_perm_out(int n, const int* perm, const double* y, double* x) {
 // every eighth of the row's blocks and forms its 5x5 block-vector products,
 // the eight partial sums are combined by a butterfly of lane shuffles, and
 // lane k < 5 writes component k.
-// Bb: the block of entry e0 (the row's blocks follow it).
-__device__ __forceinline__ void row_products(const double* Bb, const int* col, const double* y, int e0, int e1,
+// Bb, colb: the block and the column of entry e0 (the row's follow them).
+__device__ __forceinline__ void row_products(const double* Bb, const int* colb, const double* y, int e0, int e1,
                                              int lane, double a[5]) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) a[k] = 0.0;
   for (int e = e0 + lane; e < e1; e += 8) {
-    const double* yc = y + size_t(col[e]) * 5;
+    const double* yc = y + size_t(colb[e - e0]) * 5;
     const double* b = Bb + size_t(e - e0) * 25;
     const double y0 = yc[0], y1 = yc[1], y2 = yc[2], y3 = yc[3], y4 = yc[4];
 #pragma unroll
@@ -1104,7 +1107,7 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
     const int p1 = lp[l + 1];
     for (int p = lp[l] + grp; p < p1; p += ngrp) {
       const int4 rec = L.frec[p];
-      row_products(B + size_t(rec.y) * 25, L.col, y, rec.y, rec.z, lane, a);
+      row_products(B + size_t(rec.y) * 25, L.col + rec.y, y, rec.y, rec.z, lane, a);
       double mine = a[0];
 #pragma unroll
       for (int k = 1; k < 5; ++k) mine = lane == k ? a[k] : mine;
@@ -1117,7 +1120,7 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve(IluDev L, const double* B
     for (int p = up[l] + grp; p < p1; p += ngrp) {
       const int4 rec = L.brec[p];
       const int i = rec.x;
-      row_products(B + size_t(rec.y) * 25, L.col, y, rec.y, rec.z, lane, a);
+      row_products(B + size_t(rec.y) * 25, L.col + rec.y, y, rec.y, rec.z, lane, a);
       double t[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) t[k] = y[size_t(i) * 5 + k] - a[k];
@@ -1166,11 +1169,13 @@ __device__ __forceinline__ void stage_run(const double* __restrict__ src, int n,
 }
 
 // The sweeps of k_ilu_solve with the factors read from LDS: the upper half of
-// the workgroup copies the next level's run of S into one LDS buffer while the
-// lower half works on the current level from the other (eight lanes per row,
-// the same products in the same order as k_ilu_solve), so a level costs the
-// loads of y and the barrier; the factors' loads (from HBM, the larger
-// latency) overlap the previous level. Separate wavefronts keep the copy's
+// the workgroup copies the next level's run of S, with its row records and
+// block columns, into one set of LDS buffers while the lower half works on
+// the current level from the other (eight lanes per row, the same products in
+// the same order as k_ilu_solve), so a level costs the loads of y and the
+// barrier; the loads of factors, records and columns (the record and column
+// loads would otherwise be two dependent global round trips ahead of the y
+// loads) overlap the previous level. Separate wavefronts keep the copy's
 // loads out of the workers' wait counts.
 __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const double* S, double* y, const int* cases,
                                                            size_t sS, size_t sY) {
@@ -1181,7 +1186,11 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
   extern __shared__ double smem[];
   double* buf0 = smem;
   double* buf1 = smem + L.scap;
-  int* lev_sh = reinterpret_cast<int*>(smem + 2 * size_t(L.scap));
+  int4* rec0 = reinterpret_cast<int4*>(smem + 2 * size_t(L.scap));
+  int4* rec1 = rec0 + L.rcap;
+  int* col0 = reinterpret_cast<int*>(rec1 + L.rcap);
+  int* col1 = col0 + L.ccap;
+  int* lev_sh = col1 + L.ccap;
   const bool use = L.lds_levels;
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, use);
   const int* up = stage_bounds(L.uptr, L.nlevu + 1, lev_sh + (use ? L.nlev + 1 : 0), use);
@@ -1190,47 +1199,68 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
   const int ct = threadIdx.x - nw;
   const int grp = threadIdx.x >> 3, lane = threadIdx.x & 7, ngrp = nw >> 3;
   const int nl = L.nlev + L.nlevu;
-  // run of level q of the whole sequence (forward levels, then backward ones)
-  auto run = [&](int q, int& b0, int& b1) {
+  // run of level q of the whole sequence (forward levels, then backward
+  // ones): staged blocks [b0, b1), row records [p0, p1) of recs
+  auto run = [&](int q, int& b0, int& b1, int& p0, int& p1, const int4*& recs) {
     if (q < L.nlev) {
       b0 = L.flev[q];
       b1 = L.flev[q + 1];
+      p0 = lp[q];
+      p1 = lp[q + 1];
+      recs = L.fsrec;
     } else {
-      b0 = L.blev[q - L.nlev];
-      b1 = L.blev[q - L.nlev + 1];
+      const int l = q - L.nlev;
+      b0 = L.blev[l];
+      b1 = L.blev[l + 1];
+      p0 = up[l];
+      p1 = up[l + 1];
+      recs = L.bsrec;
     }
   };
-  if (copier) {
-    int b0, b1;
-    run(0, b0, b1);
-    stage_run(S + size_t(b0) * 25, (b1 - b0) * 25, buf0, ct, nw);
-  }
+  // copiers: level q's blocks, columns and records into buffer set q & 1
+  auto stage = [&](int q) {
+    int b0, b1, p0, p1;
+    const int4* recs;
+    run(q, b0, b1, p0, p1, recs);
+    int4* rd = (q & 1) ? rec1 : rec0;
+    int* cd = (q & 1) ? col1 : col0;
+    const int nb = b1 - b0, np = p1 - p0;
+    // the small copies' loads first, so they are in flight with the blocks'
+    int cv = 0;
+    int4 rv = make_int4(0, 0, 0, 0);
+    if (ct < nb) cv = L.scol[b0 + ct];
+    if (ct < np) rv = recs[p0 + ct];
+    stage_run(S + size_t(b0) * 25, nb * 25, (q & 1) ? buf1 : buf0, ct, nw);
+    if (ct < nb) cd[ct] = cv;
+    if (ct < np) rd[ct] = rv;
+    for (int t = ct + nw; t < nb; t += nw) cd[t] = L.scol[b0 + t];
+    for (int t = ct + nw; t < np; t += nw) rd[t] = recs[p0 + t];
+  };
+  if (copier) stage(0);
   __syncthreads();
   double a[5];
   for (int q = 0; q < nl; ++q) {
     const double* cur = (q & 1) ? buf1 : buf0;
+    const int4* rcur = (q & 1) ? rec1 : rec0;
+    const int* ccur = (q & 1) ? col1 : col0;
     if (copier) {
-      if (q + 1 < nl) {
-        int b0, b1;
-        run(q + 1, b0, b1);
-        stage_run(S + size_t(b0) * 25, (b1 - b0) * 25, (q & 1) ? buf0 : buf1, ct, nw);
-      }
+      if (q + 1 < nl) stage(q + 1);
     } else if (q < L.nlev) {
-      const int p1 = lp[q + 1];
-      for (int p = lp[q] + grp; p < p1; p += ngrp) {
-        const int4 rec = L.fsrec[p];
-        row_products(cur + size_t(rec.w) * 25, L.col, y, rec.y, rec.z, lane, a);
+      const int p0 = lp[q], p1 = lp[q + 1];
+      for (int p = p0 + grp; p < p1; p += ngrp) {
+        const int4 rec = rcur[p - p0];
+        row_products(cur + size_t(rec.w) * 25, ccur + rec.w, y, rec.y, rec.z, lane, a);
         double mine = a[0];
 #pragma unroll
         for (int k = 1; k < 5; ++k) mine = lane == k ? a[k] : mine;
         if (lane < 5) y[size_t(rec.x) * 5 + lane] -= mine;
       }
     } else {
-      const int l = q - L.nlev, p1 = up[l + 1];
-      for (int p = up[l] + grp; p < p1; p += ngrp) {
-        const int4 rec = L.bsrec[p];
+      const int l = q - L.nlev, p0 = up[l], p1 = up[l + 1];
+      for (int p = p0 + grp; p < p1; p += ngrp) {
+        const int4 rec = rcur[p - p0];
         const int i = rec.x;
-        row_products(cur + size_t(rec.w) * 25, L.col, y, rec.y, rec.z, lane, a);
+        row_products(cur + size_t(rec.w) * 25, ccur + rec.w, y, rec.y, rec.z, lane, a);
         double t[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) t[k] = y[size_t(i) * 5 + k] - a[k];
@@ -1245,7 +1275,8 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
 }
 
 size_t ilu_solve_lds_bytes(const IluDev& L) {
-  return 2 * size_t(L.scap) * sizeof(double) + (L.lds_levels ? size_t(L.nlev + L.nlevu + 2) * sizeof(int) : 0);
+  return 2 * size_t(L.scap) * sizeof(double) + 2 * size_t(L.rcap) * sizeof(int4) +
+         2 * size_t(L.ccap) * sizeof(int) + (L.lds_levels ? size_t(L.nlev + L.nlevu + 2) * sizeof(int) : 0);
 }
 
 // One factorisation launch over ncase workgroups (ncase = 1 with cases null):
@@ -2573,6 +2604,13 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
       blev[nlevu] = int(ssrc.size());
       L.nst = int(ssrc.size());
       L.scap = scap;
+      L.ccap = (scap / 25 + 3) & ~3;
+      int rcap = 0;
+      for (int l = 0; l < nlev; ++l) rcap = std::max(rcap, lptr[l + 1] - lptr[l]);
+      for (int l = 0; l < nlevu; ++l) rcap = std::max(rcap, uptr[l + 1] - uptr[l]);
+      L.rcap = rcap;
+      std::vector<int> scol(ssrc.size());
+      for (size_t b = 0; b < ssrc.size(); ++b) scol[b] = ssrc[b] < nnz ? cols[ssrc[b]] : 0;
       int lds_max = 65536;
       HC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, a2d::require_device()));
       L.lds_solve = scap <= SOLVE_K * (ILU_TPB / 2) && ilu_solve_lds_bytes(L) <= size_t(lds_max) &&
@@ -2583,6 +2621,7 @@ int a2d_rans_ilu_setup(int64_t h, int nnz, const int* perm, const int* rows, con
         L.flev = s.own(dupload(flev));
         L.blev = s.own(dupload(blev));
         L.ssrc = s.own(dupload(ssrc));
+        L.scol = s.own(dupload(scol));
       }
     }
     s.posD = s.own(dupload(std::vector<int>(pos_src, pos_src + n)));
