@@ -1192,8 +1192,13 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
   int* col1 = col0 + L.ccap;
   int* lev_sh = col1 + L.ccap;
   const bool use = L.lds_levels;
+  const int nb2 = use ? L.nlev + L.nlevu + 2 : 0;
   const int* lp = stage_bounds(L.lptr, L.nlev + 1, lev_sh, use);
   const int* up = stage_bounds(L.uptr, L.nlevu + 1, lev_sh + (use ? L.nlev + 1 : 0), use);
+  // the staged runs' bounds too: a copier's first load of a level is then
+  // from LDS, not a dependent global load ahead of its block loads
+  const int* fl = stage_bounds(L.flev, L.nlev + 1, lev_sh + nb2, use);
+  const int* bl = stage_bounds(L.blev, L.nlevu + 1, lev_sh + nb2 + (use ? L.nlev + 1 : 0), use);
   const int nw = blockDim.x >> 1;                 // workers: threads [0, nw); copiers: [nw, 2 nw)
   const bool copier = threadIdx.x >= nw;
   const int ct = threadIdx.x - nw;
@@ -1203,15 +1208,15 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
   // ones): staged blocks [b0, b1), row records [p0, p1) of recs
   auto run = [&](int q, int& b0, int& b1, int& p0, int& p1, const int4*& recs) {
     if (q < L.nlev) {
-      b0 = L.flev[q];
-      b1 = L.flev[q + 1];
+      b0 = fl[q];
+      b1 = fl[q + 1];
       p0 = lp[q];
       p1 = lp[q + 1];
       recs = L.fsrec;
     } else {
       const int l = q - L.nlev;
-      b0 = L.blev[l];
-      b1 = L.blev[l + 1];
+      b0 = bl[l];
+      b1 = bl[l + 1];
       p0 = up[l];
       p1 = up[l + 1];
       recs = L.bsrec;
@@ -1276,7 +1281,7 @@ __global__ __launch_bounds__(ILU_TPB) void k_ilu_solve_lds(IluDev L, const doubl
 
 size_t ilu_solve_lds_bytes(const IluDev& L) {
   return 2 * size_t(L.scap) * sizeof(double) + 2 * size_t(L.rcap) * sizeof(int4) +
-         2 * size_t(L.ccap) * sizeof(int) + (L.lds_levels ? size_t(L.nlev + L.nlevu + 2) * sizeof(int) : 0);
+         2 * size_t(L.ccap) * sizeof(int) + (L.lds_levels ? 2 * size_t(L.nlev + L.nlevu + 2) * sizeof(int) : 0);
 }
 
 // One factorisation launch over ncase workgroups (ncase = 1 with cases null):
